@@ -1,0 +1,210 @@
+/*
+ * pfem.h -- C-ABI of the MI355X-native Newton-step hot path (libpfem.so).
+ *
+ * Replaces two seams of the reference package PlasticityFEM (lagrangian/src):
+ *   assemble! / assemble_threaded!     Assembly.jl:238-257, :340-354
+ *   _linear_solve!                     Solver.jl:235-290
+ * and offers the driver around them (newton! Solver.jl:317-389, solve!
+ * Solver.jl:419-475) as fused entry points.  Plain pointers and sizes only; no
+ * HIP or torch types cross this boundary.  All device memory (mesh, BSR K,
+ * R, U, dU, Gauss state, AMG hierarchy, workspaces) is owned by the context
+ * and stays resident between calls; host arrays are borrowed for the call.
+ *
+ * Conventions: indices 0-based (the reference is 1-based; a binding converts
+ * once).  U, R are node-major (dof = 3*node + comp, Mesh.jl:111).  Gauss state
+ * arrays are 6 x ngp column-major (st[6*gp + k], gp = 8*e + g, Mesh.jl:124-138).
+ * Every call is synchronous on return and returns a pfem_status.
+ * A context is not thread-safe; distinct contexts may be used concurrently.
+ */
+#ifndef PFEM_H
+#define PFEM_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum pfem_status {
+    PFEM_OK = 0,
+    PFEM_ERR_INVERTED = 1,      /* ElementInversionError(e, g, J): pfem_fail filled (FiniteStrain.jl:42-52) */
+    PFEM_ERR_NOT_CONVERGED = 2, /* Krylov not converged after the fallback ladder (Solver.jl:270-286) */
+    PFEM_ERR_NON_SPD = 3,       /* breakdown: p'Ap <= 0 or r'z <= 0 with no fallback left */
+    PFEM_ERR_UNSUPPORTED = 4,   /* material / element branch not implemented (Hkin > 0) */
+    PFEM_ERR_HIP = 5,           /* HIP runtime error: code in pfem_fail.hip */
+    PFEM_ERR_BAD_ARG = 6,       /* invalid argument (guards of Materials.jl:90-103 included) */
+    PFEM_ERR_NO_DEVICE = 7      /* no GPU visible */
+} pfem_status;
+
+typedef enum pfem_element_kind {
+    PFEM_HEX8_FINITE = 0,       /* Hex8Finite      (FiniteStrain.jl:64) */
+    PFEM_HEX8_FINITE_FBAR = 1   /* Hex8FiniteFbar  (FiniteStrain.jl:65) */
+} pfem_element_kind;
+
+/* J2Material inputs (Materials.jl:82-108).  ssat = sy0 and delta = 0 select the
+ * linear isotropic law. */
+typedef struct pfem_material {
+    double E, nu, sy0, Hiso, Hkin, ssat, delta;
+} pfem_material;
+
+/* Mesh and element description (Mesh.jl:17-22, Model.jl:63-75). */
+typedef struct pfem_mesh_desc {
+    int64_t nnodes;
+    int64_t nelem;
+    const double *nodes;        /* 3 x nnodes (x,y,z per node) */
+    const int64_t *conn;        /* 8 x nelem, VTK hex order, 0-based */
+    const int32_t *material_id; /* per element index into materials, or NULL (all 0) */
+    int32_t nmaterials;
+    const pfem_material *materials;
+    int32_t element_kind;       /* pfem_element_kind */
+} pfem_mesh_desc;
+
+typedef enum pfem_precision {
+    PFEM_PREC_FP64 = 0,         /* plain FP64 PCG + SA-AMG (the drop-in) */
+    PFEM_PREC_FP32_IR = 1       /* FP32 PCG + FP32 V-cycle inside FP64 iterative refinement */
+} pfem_precision;
+
+typedef enum pfem_smoother {
+    PFEM_SMOOTH_JACOBI = 0,     /* damped block-Jacobi, one sweep (Solver.jl:138 Jacobi(2/3; iter=1)) */
+    PFEM_SMOOTH_CHEBYSHEV = 1   /* Chebyshev polynomial in D^-1 A (block diagonal D) */
+} pfem_smoother;
+
+/* Linear-solve options (LinearSolveState, Solver.jl:86-111). */
+typedef struct pfem_options {
+    int32_t precision;          /* pfem_precision */
+    int32_t smoother;           /* pfem_smoother */
+    int32_t smoother_degree;    /* sweeps (Jacobi) or polynomial degree (Chebyshev); <=0 => default */
+    int32_t cg_itmax;           /* inner Krylov cap (default 200) */
+    double eta_min, eta_max;    /* Eisenstat-Walker clamp (defaults 1e-8, 0.1) */
+    double ew_gamma, ew_alpha;  /* defaults 0.9, 1.5 */
+    int32_t coarse_max;         /* coarsest-level size target in DOF (default 1500) */
+    int32_t max_levels;         /* AMG level cap (default 10) */
+    int32_t verbose;            /* print per-iteration lines to stderr */
+    int32_t reserved[7];
+} pfem_options;
+
+typedef struct pfem_fail {
+    int64_t e;                  /* element (0-based) */
+    int32_t g;                  /* Gauss point 0..7 */
+    double J;                   /* offending det F */
+    int32_t hip;                /* hipError_t for PFEM_ERR_HIP */
+} pfem_fail;
+
+typedef struct pfem_ctx pfem_ctx;
+
+/* Fills `o` with the reference defaults (Solver.jl:103-111). */
+void pfem_default_options(pfem_options *o);
+
+/* Model(mesh, material; element) set-up (Model.jl:63-75): uploads the mesh,
+ * builds the block pattern (Assembly.jl:62-162, as 3x3 BSR), the greedy element
+ * colouring (Assembly.jl:172-188 on box meshes), scatter slots, and resets the
+ * Gauss state (Mesh.jl:132-138). */
+int pfem_create(const pfem_mesh_desc *mesh, const pfem_options *opts, pfem_ctx **out);
+void pfem_destroy(pfem_ctx *ctx);
+
+/* Dirichlet / Neumann data (Model.jl:102-145, frozen as in _freeze_bcs
+ * Solver.jl:192-218: duplicates resolved last-wins).  ramp[i] != 0 scales by
+ * the load factor. */
+int pfem_set_dirichlet(pfem_ctx *ctx, int64_t n, const int64_t *dofs, const double *values,
+                       const uint8_t *ramp);
+int pfem_set_neumann(pfem_ctx *ctx, int64_t n, const int64_t *dofs, const double *values);
+
+/* reset! (Model.jl:157-162): U = 0, Gauss state undeformed. */
+int pfem_reset(pfem_ctx *ctx);
+
+int pfem_set_displacement(pfem_ctx *ctx, const double *U);
+int pfem_get_displacement(pfem_ctx *ctx, double *U);
+/* Committed Gauss state; any pointer may be NULL.  epsp, cpinv, sig: 6 x ngp;
+ * abar: ngp.  sig holds the last assembled Kirchhoff stress. */
+int pfem_set_state(pfem_ctx *ctx, const double *epsp, const double *abar, const double *cpinv);
+int pfem_get_state(pfem_ctx *ctx, double *epsp, double *abar, double *cpinv, double *sig);
+
+/* assemble! (Assembly.jl:238-257): K (device) and R = F_int overwritten from U
+ * (host array, or NULL to use the device-resident U) and the committed
+ * history; sig overwritten; commit != 0 writes the new history.  R_out may be
+ * NULL.  Returns PFEM_ERR_INVERTED with `fail` filled on det F <= 0. */
+int pfem_assemble(pfem_ctx *ctx, const double *U, int commit, double *R_out, pfem_fail *fail);
+
+/* R -= lambda * Fext (Solver.jl:326-327, :349). */
+int pfem_subtract_external(pfem_ctx *ctx, double lambda);
+
+/* Both impose_dirichlet! calls of newton! (Solver.jl:354-355,
+ * BoundaryConditions.jl:62-109) on the device K and R at load factor lambda. */
+int pfem_impose_dirichlet(pfem_ctx *ctx, double lambda);
+
+/* ||R||_2 (Solver.jl:358). */
+int pfem_residual_norm(pfem_ctx *ctx, double *out);
+
+/* _linear_solve!(dU, K, b = -R, ls, rtol, rebuild_pc) (Solver.jl:235-290).
+ * dU stays on the device; niter = Krylov iterations (all attempts);
+ * solver_used: 0 PCG, 1 PCG after AMG refresh, 2 BiCGStab fallback,
+ * 3 FP32-IR.  true_relres = ||b - K dU|| / ||b||. */
+int pfem_linear_solve(pfem_ctx *ctx, double rtol, int rebuild_pc, int *niter,
+                      int *solver_used, double *true_relres);
+
+/* U += dU (Solver.jl:383). */
+int pfem_update_displacement(pfem_ctx *ctx);
+
+/* Copy the device K out as BSR (rowptr nb+1, colind nnzb, vals 9*nnzb
+ * row-major blocks) or as the reference's CSC (Assembly.jl:99-162).  Any
+ * pointer NULL => only sizes returned. */
+int pfem_get_K_bsr(pfem_ctx *ctx, int64_t *nnzb, int32_t *rowptr, int32_t *colind, double *vals);
+int pfem_get_K_csc(pfem_ctx *ctx, int64_t *nnz, int64_t *colptr, int64_t *rowval, double *nzval);
+int pfem_get_R(pfem_ctx *ctx, double *R);
+int pfem_get_dU(pfem_ctx *ctx, double *dU);
+/* Element colouring: colour[e], returns count in *ncolors. */
+int pfem_get_colors(pfem_ctx *ctx, int32_t *colour, int32_t *ncolors);
+
+typedef struct pfem_newton_result {
+    int32_t converged;
+    int32_t iters;              /* Newton iterations (length of the residual history) */
+    int32_t krylov_iters[64];   /* per Newton iteration */
+    int32_t solver_used[64];
+    double residuals[64];
+    double t_assemble, t_bc, t_setup, t_solve, t_total; /* seconds */
+} pfem_newton_result;
+
+/* One load step of newton! at load factor lambda (Solver.jl:317-389). */
+int pfem_newton(pfem_ctx *ctx, double lambda, double tol, int maxiter,
+                pfem_newton_result *res, pfem_fail *fail);
+
+/* Commit pass of solve! (Solver.jl:470-471). */
+int pfem_commit(pfem_ctx *ctx, pfem_fail *fail);
+
+/* Element-level entry for kernel checks: for nelem independent elements with
+ * node coordinates Xe (24 per element, node-major) and displacements ue (24),
+ * and per-GP history (6x8, 8, 6x8 per element), compute Fe (24), Ke (576,
+ * column-major 24x24) and sig (6x8) on the device with the production
+ * per-GP and tangent-contraction kernels.  commit != 0 updates the history
+ * arrays.  material applies to all elements. */
+int pfem_element_batch(int32_t element_kind, const pfem_material *mat, int64_t nelem,
+                       const double *Xe, const double *ue, double *epsp, double *abar,
+                       double *cpinv, int commit, double *Fe, double *Ke, double *sig,
+                       pfem_fail *fail);
+
+typedef struct pfem_stats {
+    int64_t nnodes, nelem, ndof, nnzb, ncolors;
+    int32_t amg_levels;
+    int64_t amg_ndof[12], amg_nnzb[12];
+    double amg_operator_complexity;
+    double t_assemble, t_setup, t_solve;    /* accumulated seconds */
+    double t_gp_kernel, t_scatter_kernel;   /* accumulated device seconds (events) */
+    double t_spmv;                           /* last pfem_bench_spmv result (s per SpMV) */
+    double t_pcg_spmv;                       /* accumulated device time of the fine FP64 SpMV in PCG (s) */
+    int64_t n_pcg_spmv;                      /* number of those launches */
+    int64_t n_assemble, n_setup, n_solve, n_krylov;
+    int64_t device_bytes;                    /* device memory held by the context */
+} pfem_stats;
+
+int pfem_get_stats(pfem_ctx *ctx, pfem_stats *st);
+
+/* Time `reps` fine-level SpMVs y = K x (FP64 if prec == 0, FP32 if 1) on the
+ * context's stream with events; returns mean seconds per SpMV. */
+int pfem_bench_spmv(pfem_ctx *ctx, int prec, int reps, double *sec_per_spmv);
+
+/* Human-readable text for a status. */
+const char *pfem_status_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

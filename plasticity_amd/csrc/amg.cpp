@@ -1,0 +1,562 @@
+// amg.cpp -- smoothed-aggregation AMG: host symbolic set-up, device numeric set-up
+// orchestration and the V-cycle (see amg.h for the method and the reference seam).
+#include "amg.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+
+#include "context.h"
+
+namespace pfem {
+namespace {
+
+// CSR view of a level's block graph.
+struct Graph {
+    int64_t nb = 0;
+    std::vector<int64_t> ptr;
+    std::vector<int32_t> idx;
+};
+
+Graph fine_graph(const FineMatrix &K) {
+    Graph g;
+    g.nb = K.nb;
+    g.ptr.assign(K.nb + 1, 0);
+    for (int64_t i = 0; i < K.nb; ++i) g.ptr[i + 1] = g.ptr[i] + K.h_rowlen[i];
+    g.idx.resize(g.ptr[K.nb]);
+    for (int64_t i = 0; i < K.nb; ++i) {
+        const int64_t base = K.h_slice_ptr[i / SELL_C] + i % SELL_C;
+        for (int k = 0; k < K.h_rowlen[i]; ++k) g.idx[g.ptr[i] + k] = K.h_colind[base + (int64_t)k * SELL_C];
+    }
+    return g;
+}
+
+// Standard three-phase aggregation (Vanek, Mandel & Brezina, Computing 56, 1996) on
+// the block graph with every structural connection strong (theta = 0).
+int64_t aggregate(const Graph &g, const std::vector<uint8_t> &active, std::vector<int32_t> &agg) {
+    const int64_t nb = g.nb;
+    agg.assign(nb, -1);
+    int32_t na = 0;
+    for (int64_t i = 0; i < nb; ++i) {  // phase 1: whole free neighbourhoods
+        if (!active[i] || agg[i] >= 0) continue;
+        bool free_nbhd = true;
+        int nact = 0;
+        for (int64_t k = g.ptr[i]; k < g.ptr[i + 1]; ++k) {
+            const int32_t j = g.idx[k];
+            if (j == i || !active[j]) continue;
+            ++nact;
+            if (agg[j] >= 0) { free_nbhd = false; break; }
+        }
+        if (!free_nbhd || nact == 0) continue;
+        agg[i] = na;
+        for (int64_t k = g.ptr[i]; k < g.ptr[i + 1]; ++k) {
+            const int32_t j = g.idx[k];
+            if (active[j]) agg[j] = na;
+        }
+        ++na;
+    }
+    std::vector<int32_t> p1(agg);
+    for (int64_t i = 0; i < nb; ++i) {  // phase 2: join a neighbouring phase-1 aggregate
+        if (!active[i] || agg[i] >= 0) continue;
+        for (int64_t k = g.ptr[i]; k < g.ptr[i + 1]; ++k) {
+            const int32_t j = g.idx[k];
+            if (j != i && p1[j] >= 0) { agg[i] = p1[j]; break; }
+        }
+    }
+    for (int64_t i = 0; i < nb; ++i) {  // phase 3: leftovers with their free neighbours
+        if (!active[i] || agg[i] >= 0) continue;
+        agg[i] = na;
+        for (int64_t k = g.ptr[i]; k < g.ptr[i + 1]; ++k) {
+            const int32_t j = g.idx[k];
+            if (active[j] && agg[j] < 0) agg[j] = na;
+        }
+        ++na;
+    }
+    return na;
+}
+
+// Per-aggregate thin QR (modified Gram-Schmidt, two passes) of the near-null-space
+// rows B (bs x 6 per node).  Pt receives Q's rows per node, Bc the 6x6 R factor per
+// aggregate.  A column whose norm collapses below 1e-10 of its original is dropped.
+void tentative(int bs, int64_t nb, int64_t nagg, const std::vector<int32_t> &agg, const std::vector<double> &B,
+               std::vector<double> &Pt, std::vector<double> &Bc) {
+    Pt.assign((size_t)nb * bs * 6, 0.0);
+    Bc.assign((size_t)nagg * 36, 0.0);
+    std::vector<int64_t> ptr(nagg + 1, 0);
+    for (int64_t i = 0; i < nb; ++i)
+        if (agg[i] >= 0) ptr[agg[i] + 1]++;
+    for (int64_t a = 0; a < nagg; ++a) ptr[a + 1] += ptr[a];
+    std::vector<int64_t> members(ptr[nagg]), f(ptr.begin(), ptr.end() - 1);
+    for (int64_t i = 0; i < nb; ++i)
+        if (agg[i] >= 0) members[f[agg[i]]++] = i;
+    std::vector<double> M, Qm;
+    for (int64_t a = 0; a < nagg; ++a) {
+        const int64_t m0 = ptr[a], cnt = ptr[a + 1] - m0;
+        const int64_t rows = cnt * bs;
+        M.assign(rows * 6, 0.0);  // column-major rows x 6
+        for (int64_t t = 0; t < cnt; ++t)
+            for (int r = 0; r < bs; ++r)
+                for (int col = 0; col < 6; ++col)
+                    M[col * rows + t * bs + r] = B[((size_t)members[m0 + t] * bs + r) * 6 + col];
+        Qm.assign(rows * 6, 0.0);
+        double Rf[36] = {0};
+        for (int col = 0; col < 6; ++col) {
+            double *v = &M[col * rows];
+            double n0 = 0.0;
+            for (int64_t r = 0; r < rows; ++r) n0 += v[r] * v[r];
+            n0 = std::sqrt(n0);
+            for (int pass = 0; pass < 2; ++pass)
+                for (int j = 0; j < col; ++j) {
+                    const double *q = &Qm[j * rows];
+                    double d = 0.0;
+                    for (int64_t r = 0; r < rows; ++r) d += q[r] * v[r];
+                    Rf[6 * j + col] += d;
+                    for (int64_t r = 0; r < rows; ++r) v[r] -= d * q[r];
+                }
+            double nn = 0.0;
+            for (int64_t r = 0; r < rows; ++r) nn += v[r] * v[r];
+            nn = std::sqrt(nn);
+            if (n0 == 0.0 || nn <= 1e-10 * n0) continue;  // dropped: zero column
+            Rf[6 * col + col] = nn;
+            for (int64_t r = 0; r < rows; ++r) Qm[col * rows + r] = v[r] / nn;
+        }
+        for (int64_t t = 0; t < cnt; ++t)
+            for (int r = 0; r < bs; ++r)
+                for (int col = 0; col < 6; ++col)
+                    Pt[((size_t)members[m0 + t] * bs + r) * 6 + col] = Qm[col * rows + t * bs + r];
+        std::memcpy(&Bc[36 * a], Rf, sizeof Rf);
+    }
+}
+
+template <class T>
+void up(dbuf<T> &d, const std::vector<T> &h, hipStream_t s) {
+    d.alloc(std::max<size_t>(1, h.size()));
+    d.upload(h.data(), h.size(), s);
+}
+
+void sorted_union_rows(int64_t nrows, const std::function<void(int64_t, std::vector<int32_t> &)> &gather,
+                       std::vector<int32_t> &ptr, std::vector<int32_t> &idx) {
+    ptr.assign(nrows + 1, 0);
+    idx.clear();
+    std::vector<int32_t> tmp;
+    for (int64_t i = 0; i < nrows; ++i) {
+        tmp.clear();
+        gather(i, tmp);
+        std::sort(tmp.begin(), tmp.end());
+        tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+        idx.insert(idx.end(), tmp.begin(), tmp.end());
+        if (idx.size() >= (size_t)INT32_MAX) throw status_failure(PFEM_ERR_BAD_ARG, "AMG pattern too large");
+        ptr[i + 1] = (int32_t)idx.size();
+    }
+}
+
+}  // namespace
+
+Amg *amg_create(pfem_ctx *c) {
+    hipStream_t s = c->stream;
+    auto *a = new Amg();
+    a->smoother = c->opt.smoother;
+    a->degree = c->opt.smoother_degree > 0 ? c->opt.smoother_degree : (a->smoother == PFEM_SMOOTH_CHEBYSHEV ? 2 : 1);
+    const int64_t coarse_max = c->opt.coarse_max > 0 ? c->opt.coarse_max : 1500;
+    const int max_levels = c->opt.max_levels > 1 ? c->opt.max_levels : 10;
+
+    // level 0 graph, activity and near-null space (Solver.jl:117-129, from undeformed
+    // coordinates); rows of constrained dofs are zeroed (their equations are identities)
+    std::vector<uint8_t> flag(c->ndof);
+    c->bcflag.download(flag.data(), c->ndof, s);
+    c->sync();
+    Graph g = fine_graph(c->K);
+    std::vector<uint8_t> active(c->nnodes, 1);
+    std::vector<double> B((size_t)c->nnodes * 18, 0.0);
+    for (int64_t n = 0; n < c->nnodes; ++n) {
+        const double x = c->h_nodes[3 * n], y = c->h_nodes[3 * n + 1], z = c->h_nodes[3 * n + 2];
+        const double rb[3][6] = {{1, 0, 0, 0, z, -y}, {0, 1, 0, -z, 0, x}, {0, 0, 1, y, -x, 0}};
+        int nfix = 0;
+        for (int r = 0; r < 3; ++r) {
+            const bool fixed = flag[3 * n + r] != 0;
+            nfix += fixed;
+            for (int col = 0; col < 6; ++col) B[(3 * n + r) * 6 + col] = fixed ? 0.0 : rb[r][col];
+        }
+        active[n] = nfix < 3;
+    }
+    int bs = 3;
+    const int64_t fine_nnz = c->K.nnzb * 9;
+    int64_t total_nnz = fine_nnz;
+    for (int lev = 0;; ++lev) {
+        a->L.emplace_back();
+        AmgLevel &L = a->L.back();
+        L.bs = bs;
+        L.nb = g.nb;
+        L.n = (int64_t)bs * g.nb;
+        if (lev > 0) {  // coarse operator storage, pattern = previous level's R A P
+            L.nnzb = (int64_t)g.idx.size();
+            total_nnz += L.nnzb * bs * bs;
+            L.h_rowptr.resize(g.nb + 1);
+            for (int64_t i = 0; i <= g.nb; ++i) L.h_rowptr[i] = (int32_t)g.ptr[i];
+            L.h_colind = g.idx;
+            std::vector<int32_t> dpos(g.nb, 0);
+            for (int64_t i = 0; i < g.nb; ++i)
+                for (int64_t k = g.ptr[i]; k < g.ptr[i + 1]; ++k)
+                    if (g.idx[k] == i) dpos[i] = (int32_t)k;
+            up(L.rowptr, L.h_rowptr, s);
+            up(L.colind, L.h_colind, s);
+            up(L.diagpos, dpos, s);
+            L.A.alloc(std::max<int64_t>(1, L.nnzb * 36));
+        }
+        L.x.alloc(L.n); L.b.alloc(L.n); L.r.alloc(L.n); L.d.alloc(L.n); L.t.alloc(L.n);
+        L.dinv.alloc((size_t)L.nb * bs * bs);
+        if (L.n <= coarse_max || lev + 1 >= max_levels) break;
+        std::vector<int32_t> agg;
+        const int64_t nagg = aggregate(g, active, agg);
+        if (nagg == 0 || nagg * 6 >= L.n) break;  // no useful coarsening: this is the coarsest
+        L.nagg = nagg;
+        std::vector<double> Pt, Bc;
+        tentative(bs, L.nb, nagg, agg, B, Pt, Bc);
+        // P pattern: aggregates of the node and its neighbours
+        std::vector<int32_t> Pptr, Pidx;
+        sorted_union_rows(L.nb, [&](int64_t i, std::vector<int32_t> &o) {
+            if (!active[i]) return;
+            for (int64_t k = g.ptr[i]; k < g.ptr[i + 1]; ++k)
+                if (agg[g.idx[k]] >= 0) o.push_back(agg[g.idx[k]]);
+        }, Pptr, Pidx);
+        // Q = A P pattern
+        std::vector<int32_t> Qptr, Qidx;
+        sorted_union_rows(L.nb, [&](int64_t i, std::vector<int32_t> &o) {
+            if (!active[i]) return;
+            for (int64_t k = g.ptr[i]; k < g.ptr[i + 1]; ++k) {
+                const int32_t j = g.idx[k];
+                for (int32_t m = Pptr[j]; m < Pptr[j + 1]; ++m) o.push_back(Pidx[m]);
+            }
+        }, Qptr, Qidx);
+        // R = P^T pattern (+ source block of each R block)
+        std::vector<int32_t> Rptr(nagg + 1, 0), Ridx(Pidx.size()), Rsrc(Pidx.size());
+        for (int32_t J : Pidx) Rptr[J + 1]++;
+        for (int64_t J = 0; J < nagg; ++J) Rptr[J + 1] += Rptr[J];
+        {
+            std::vector<int32_t> fpos(Rptr.begin(), Rptr.end() - 1);
+            for (int64_t i = 0; i < L.nb; ++i)
+                for (int32_t m = Pptr[i]; m < Pptr[i + 1]; ++m) {
+                    const int32_t J = Pidx[m];
+                    Ridx[fpos[J]] = (int32_t)i;
+                    Rsrc[fpos[J]++] = m;
+                }
+        }
+        // coarse pattern C = R Q
+        std::vector<int32_t> Cptr, Cidx;
+        sorted_union_rows(nagg, [&](int64_t I, std::vector<int32_t> &o) {
+            for (int32_t m = Rptr[I]; m < Rptr[I + 1]; ++m) {
+                const int32_t i = Ridx[m];
+                for (int32_t q = Qptr[i]; q < Qptr[i + 1]; ++q) o.push_back(Qidx[q]);
+            }
+        }, Cptr, Cidx);
+        std::vector<int32_t> Qrow(Qidx.size()), Crow(Cidx.size());
+        for (int64_t i = 0; i < L.nb; ++i)
+            for (int32_t q = Qptr[i]; q < Qptr[i + 1]; ++q) Qrow[q] = (int32_t)i;
+        for (int64_t I = 0; I < nagg; ++I)
+            for (int32_t q = Cptr[I]; q < Cptr[I + 1]; ++q) Crow[q] = (int32_t)I;
+        up(L.agg, agg, s);
+        up(L.Pt, Pt, s);
+        up(L.P_rowptr, Pptr, s); up(L.P_colind, Pidx, s); L.P.alloc(std::max<size_t>(1, Pidx.size() * bs * 6));
+        up(L.R_rowptr, Rptr, s); up(L.R_colind, Ridx, s); up(L.R_src, Rsrc, s);
+        L.R.alloc(std::max<size_t>(1, Ridx.size() * bs * 6));
+        up(L.Q_rowptr, Qptr, s); up(L.Q_colind, Qidx, s); up(L.Q_row, Qrow, s);
+        L.Q.alloc(std::max<size_t>(1, Qidx.size() * bs * 6));
+        up(L.C_row, Crow, s);
+        c->sync();
+        // the next level's graph is the coarse pattern
+        Graph gn;
+        gn.nb = nagg;
+        gn.ptr.assign(Cptr.begin(), Cptr.end());
+        gn.idx = std::move(Cidx);
+        g = std::move(gn);
+        active.assign(nagg, 1);
+        B = std::move(Bc);
+        bs = 6;
+    }
+    // power-iteration start vectors (fixed pseudo-random sequence)
+    for (auto &L : a->L) {
+        std::vector<double> v(L.n);
+        uint64_t st = 0x243F6A8885A308D3ull;
+        for (int64_t i = 0; i < L.n; ++i) {
+            st = st * 6364136223846793005ull + 1442695040888963407ull;
+            v[i] = (double)(st >> 11) * 0x1.0p-53 - 0.5;
+        }
+        L.pw.alloc(L.n);
+        L.pw.upload(v.data(), L.n, s);
+    }
+    a->nc = a->L.back().n;
+    a->coarse_inv.alloc((size_t)a->nc * a->nc);
+    a->op_complexity = (double)total_nnz / (double)fine_nnz;
+    c->st.amg_levels = (int32_t)a->L.size();
+    for (size_t l = 0; l < a->L.size() && l < 12; ++l) {
+        c->st.amg_ndof[l] = a->L[l].n;
+        c->st.amg_nnzb[l] = l == 0 ? c->K.nnzb : a->L[l].nnzb;
+    }
+    c->st.amg_operator_complexity = a->op_complexity;
+    c->sync();
+    return a;
+}
+
+void amg_destroy(Amg *a) { delete a; }
+
+// ---------------------------------------------------------------- numeric set-up
+namespace {
+
+// rho(D^-1 A) by power iteration from a fixed start vector.
+double spectral_radius(pfem_ctx *c, Amg *a, int l, int iters) {
+    AmgLevel &L = a->L[l];
+    hipStream_t s = c->stream;
+    double *x = L.d.p, *y = L.r.p;
+    PF_HIP(hipMemcpyAsync(x, L.pw.p, L.n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    dot_dev<double, double>(L.n, x, x, c->partial.p, c->scal.p + 10, s);
+    double nx = std::sqrt(c->sync_read(c->scal.p + 10));
+    double lam = 0.0;
+    for (int it = 0; it < iters; ++it) {
+        axpby<double>(L.n, 1.0 / nx, x, 0.0, x, s);  // normalise
+        if (l == 0) sell_spmv<3, double, double>(c->K, c->K.v64.p, x, L.t.p, s);
+        else bsr_spmv<6, 6, double, double>(L.nb, L.rowptr.p, L.colind.p, L.A.p, x, L.t.p, 0.0, s);
+        if (L.bs == 3) block_diag_apply<3, double, double>(L.nb, L.dinv.p, L.t.p, y, 1.0, s);
+        else block_diag_apply<6, double, double>(L.nb, L.dinv.p, L.t.p, y, 1.0, s);
+        dot_dev<double, double>(L.n, y, y, c->partial.p, c->scal.p + 10, s);
+        const double ny = std::sqrt(c->sync_read(c->scal.p + 10));
+        lam = ny;  // ||D^-1 A x|| with ||x|| = 1
+        std::swap(x, y);
+        nx = ny;
+        if (nx == 0.0) break;
+    }
+    return lam > 0 ? lam : 1.0;
+}
+
+// Dense inverse of the coarsest operator on the host (Gauss-Jordan, partial
+// pivoting); zero rows (dropped rigid-body columns) are decoupled with a unit
+// diagonal.
+void coarse_inverse(pfem_ctx *c, Amg *a) {
+    hipStream_t s = c->stream;
+    AmgLevel &L = a->L.back();
+    const int64_t n = L.n;
+    std::vector<double> M((size_t)n * n, 0.0);
+    if (a->L.size() == 1) {  // the fine level itself is the coarsest
+        std::vector<double> hv(9 * c->K.nslots);
+        c->K.v64.download(hv.data(), hv.size(), s);
+        c->sync();
+        const FineMatrix &K = c->K;
+        for (int64_t i = 0; i < K.nb; ++i) {
+            const int64_t base = K.h_slice_ptr[i / SELL_C] + i % SELL_C;
+            for (int k = 0; k < K.h_rowlen[i]; ++k) {
+                const int64_t pos = base + (int64_t)k * SELL_C;
+                const int64_t j = K.h_colind[pos];
+                for (int r = 0; r < 3; ++r)
+                    for (int q = 0; q < 3; ++q) M[(3 * i + r) * n + 3 * j + q] = hv[9 * pos + 3 * r + q];
+            }
+        }
+    } else {
+        std::vector<double> hv(36 * L.nnzb);
+        L.A.download(hv.data(), hv.size(), s);
+        c->sync();
+        for (int64_t i = 0; i < L.nb; ++i)
+            for (int32_t k = L.h_rowptr[i]; k < L.h_rowptr[i + 1]; ++k) {
+                const int64_t j = L.h_colind[k];
+                for (int r = 0; r < 6; ++r)
+                    for (int q = 0; q < 6; ++q) M[(6 * i + r) * n + 6 * j + q] = hv[36 * k + 6 * r + q];
+            }
+    }
+    for (int64_t r = 0; r < n; ++r) {
+        bool zero = true;
+        for (int64_t q = 0; q < n && zero; ++q) zero = M[r * n + q] == 0.0;
+        if (zero) M[r * n + r] = 1.0;
+    }
+    std::vector<double> I((size_t)n * n, 0.0);
+    for (int64_t r = 0; r < n; ++r) I[r * n + r] = 1.0;
+    for (int64_t col = 0; col < n; ++col) {
+        int64_t piv = col;
+        double best = std::fabs(M[col * n + col]);
+        for (int64_t r = col + 1; r < n; ++r)
+            if (std::fabs(M[r * n + col]) > best) { best = std::fabs(M[r * n + col]); piv = r; }
+        if (piv != col) {
+            std::swap_ranges(M.begin() + col * n, M.begin() + (col + 1) * n, M.begin() + piv * n);
+            std::swap_ranges(I.begin() + col * n, I.begin() + (col + 1) * n, I.begin() + piv * n);
+        }
+        const double d = M[col * n + col];
+        const double id = d != 0.0 ? 1.0 / d : 0.0;
+        double *mc = &M[col * n], *ic = &I[col * n];
+        for (int64_t q = 0; q < n; ++q) { mc[q] *= id; ic[q] *= id; }
+        for (int64_t r = 0; r < n; ++r) {
+            if (r == col) continue;
+            const double f = M[r * n + col];
+            if (f == 0.0) continue;
+            double *mr = &M[r * n], *ir = &I[r * n];
+            for (int64_t q = 0; q < n; ++q) { mr[q] -= f * mc[q]; ir[q] -= f * ic[q]; }
+        }
+    }
+    a->coarse_inv.upload(I.data(), (size_t)n * n, s);
+    c->sync();
+}
+
+}  // namespace
+
+void amg_numeric_setup(pfem_ctx *c, Amg *a) {
+    hipStream_t s = c->stream;
+    const bool want32 = c->opt.precision == PFEM_PREC_FP32_IR;
+    for (size_t l = 0; l < a->L.size(); ++l) {
+        AmgLevel &L = a->L[l];
+        const bool fine = (l == 0);
+        const double *Av = fine ? c->K.v64.p : L.A.p;
+        if (L.bs == 3) amg_block_inverse<3>(L.nb, fine ? c->K.diagpos.p : L.diagpos.p, Av, L.dinv.p, s);
+        else amg_block_inverse<6>(L.nb, L.diagpos.p, Av, L.dinv.p, s);
+        if (l + 1 == a->L.size()) break;  // coarsest: dense solve
+        L.rho = spectral_radius(c, a, (int)l, 12);
+        const double omega = (4.0 / 3.0) / L.rho;
+        AmgLevel &N = a->L[l + 1];
+        if (L.bs == 3) {
+            amg_smooth_P<3, true>(L.nb, c->K.slice_ptr.p, c->K.rowlen.p, nullptr, c->K.colind.p, Av, L.agg.p, L.Pt.p,
+                                  L.dinv.p, omega, L.P_rowptr.p, L.P_colind.p, L.P.p, s);
+            amg_transpose_P<3>((int64_t)L.R_colind.n, L.R_src.p, L.P.p, L.R.p, s);
+            amg_AP<3, true>((int64_t)L.Q_colind.n, L.Q_row.p, L.Q_colind.p, c->K.slice_ptr.p, c->K.rowlen.p, nullptr,
+                            c->K.colind.p, Av, L.P_rowptr.p, L.P_colind.p, L.P.p, L.Q.p, s);
+            amg_RAP<3>(N.nnzb, L.C_row.p, N.colind.p, L.R_rowptr.p, L.R_colind.p, L.R.p, L.Q_rowptr.p, L.Q_colind.p,
+                       L.Q.p, N.A.p, s);
+        } else {
+            amg_smooth_P<6, false>(L.nb, nullptr, nullptr, L.rowptr.p, L.colind.p, Av, L.agg.p, L.Pt.p, L.dinv.p,
+                                   omega, L.P_rowptr.p, L.P_colind.p, L.P.p, s);
+            amg_transpose_P<6>((int64_t)L.R_colind.n, L.R_src.p, L.P.p, L.R.p, s);
+            amg_AP<6, false>((int64_t)L.Q_colind.n, L.Q_row.p, L.Q_colind.p, nullptr, nullptr, L.rowptr.p,
+                             L.colind.p, Av, L.P_rowptr.p, L.P_colind.p, L.P.p, L.Q.p, s);
+            amg_RAP<6>(N.nnzb, L.C_row.p, N.colind.p, L.R_rowptr.p, L.R_colind.p, L.R.p, L.Q_rowptr.p, L.Q_colind.p,
+                       L.Q.p, N.A.p, s);
+        }
+    }
+    coarse_inverse(c, a);
+    if (want32) {
+        for (size_t l = 0; l < a->L.size(); ++l) {
+            AmgLevel &L = a->L[l];
+            if (!L.x32.p) {
+                L.x32.alloc(L.n); L.b32.alloc(L.n); L.r32.alloc(L.n); L.d32.alloc(L.n); L.t32.alloc(L.n);
+            }
+            L.dinv32.alloc(L.dinv.n);
+            convert<double, float>((int64_t)L.dinv.n, L.dinv.p, L.dinv32.p, s);
+            if (l > 0) {
+                if (!L.A32.p) L.A32.alloc(L.A.n);
+                convert<double, float>((int64_t)L.A.n, L.A.p, L.A32.p, s);
+            }
+            if (l + 1 < a->L.size()) {
+                if (!L.P32.p) { L.P32.alloc(L.P.n); L.R32.alloc(L.R.n); }
+                convert<double, float>((int64_t)L.P.n, L.P.p, L.P32.p, s);
+                convert<double, float>((int64_t)L.R.n, L.R.p, L.R32.p, s);
+            }
+        }
+        a->have32 = true;
+    }
+    c->sync();
+}
+
+// ---------------------------------------------------------------- V-cycle
+namespace {
+
+template <class T> struct Vec;
+template <> struct Vec<double> {
+    static double *x(AmgLevel &L) { return L.x.p; }
+    static double *b(AmgLevel &L) { return L.b.p; }
+    static double *r(AmgLevel &L) { return L.r.p; }
+    static double *d(AmgLevel &L) { return L.d.p; }
+    static double *t(AmgLevel &L) { return L.t.p; }
+    static const double *A(AmgLevel &L) { return L.A.p; }
+    static const double *P(AmgLevel &L) { return L.P.p; }
+    static const double *R(AmgLevel &L) { return L.R.p; }
+    static const double *Dinv(AmgLevel &L) { return L.dinv.p; }
+    static const double *K(pfem_ctx *c) { return c->K.v64.p; }
+};
+template <> struct Vec<float> {
+    static float *x(AmgLevel &L) { return L.x32.p; }
+    static float *b(AmgLevel &L) { return L.b32.p; }
+    static float *r(AmgLevel &L) { return L.r32.p; }
+    static float *d(AmgLevel &L) { return L.d32.p; }
+    static float *t(AmgLevel &L) { return L.t32.p; }
+    static const float *A(AmgLevel &L) { return L.A32.p; }
+    static const float *P(AmgLevel &L) { return L.P32.p; }
+    static const float *R(AmgLevel &L) { return L.R32.p; }
+    static const float *Dinv(AmgLevel &L) { return L.dinv32.p; }
+    static const float *K(pfem_ctx *c) { return c->K.v32.p; }
+};
+
+template <class T>
+void level_matvec(pfem_ctx *c, AmgLevel &L, int l, const T *x, T *y) {
+    if (l == 0) sell_spmv<3, T, T>(c->K, Vec<T>::K(c), x, y, c->stream);
+    else bsr_spmv<6, 6, T, T>(L.nb, L.rowptr.p, L.colind.p, Vec<T>::A(L), x, y, 0.0, c->stream);
+}
+template <class T>
+void level_residual(pfem_ctx *c, AmgLevel &L, int l, const T *x, const T *b, T *r) {
+    if (l == 0) sell_residual<3, T, T>(c->K, Vec<T>::K(c), x, b, r, c->stream);
+    else bsr_residual<6, T, T>(L.nb, L.rowptr.p, L.colind.p, Vec<T>::A(L), x, b, r, c->stream);
+}
+template <class T>
+void diag_apply(AmgLevel &L, const T *r, T *z, double scale, hipStream_t s) {
+    if (L.bs == 3) block_diag_apply<3, T, T>(L.nb, Vec<T>::Dinv(L), r, z, scale, s);
+    else block_diag_apply<6, T, T>(L.nb, Vec<T>::Dinv(L), r, z, scale, s);
+}
+
+// Polynomial smoother on level l: x <- x + p(D^-1 A) D^-1 (b - A x).
+// Chebyshev on [lmax/30, lmax] with lmax = 1.1 rho (Adams, Brezina, Hu & Tuminaro,
+// J. Comput. Phys. 188, 2003), or damped block Jacobi with w = 4/(3 rho).
+template <class T>
+void smooth(pfem_ctx *c, Amg *a, int l, const T *b, T *x, bool zero_init) {
+    AmgLevel &L = a->L[l];
+    hipStream_t s = c->stream;
+    T *r = Vec<T>::r(L), *d = Vec<T>::d(L), *t = Vec<T>::t(L);
+    if (a->smoother == PFEM_SMOOTH_JACOBI) {
+        const double w = (4.0 / 3.0) / L.rho;
+        for (int it = 0; it < a->degree; ++it) {
+            if (zero_init && it == 0) {
+                diag_apply<T>(L, b, x, w, s);
+                continue;
+            }
+            level_residual<T>(c, L, l, x, b, r);
+            diag_apply<T>(L, r, d, w, s);
+            axpby<T>(L.n, 1.0, d, 1.0, x, s);
+        }
+        return;
+    }
+    const double lmax = 1.1 * L.rho, lmin = lmax / 30.0;
+    const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
+    const double sigma = theta / delta;
+    double rho_old = 1.0 / sigma;
+    if (zero_init) PF_HIP(hipMemcpyAsync(r, b, L.n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    else level_residual<T>(c, L, l, x, b, r);
+    diag_apply<T>(L, r, d, 1.0 / theta, s);
+    if (zero_init) PF_HIP(hipMemcpyAsync(x, d, L.n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    else axpby<T>(L.n, 1.0, d, 1.0, x, s);
+    for (int k = 2; k <= a->degree; ++k) {
+        level_matvec<T>(c, L, l, d, t);
+        const double rho_new = 1.0 / (2.0 * sigma - rho_old);
+        const double c1 = rho_new * rho_old, c2 = 2.0 * rho_new / delta;
+        if (L.bs == 3) cheb_step<3, T, T>(L.nb, Vec<T>::Dinv(L), t, r, d, x, c1, c2, s);
+        else cheb_step<6, T, T>(L.nb, Vec<T>::Dinv(L), t, r, d, x, c1, c2, s);
+        rho_old = rho_new;
+    }
+}
+
+template <class T>
+void vcycle(pfem_ctx *c, Amg *a, int l, const T *b, T *x) {
+    AmgLevel &L = a->L[l];
+    hipStream_t s = c->stream;
+    if ((size_t)l + 1 == a->L.size()) {  // coarsest: FP64 dense solve
+        dense_gemv<T>(a->nc, a->coarse_inv.p, b, x, s);
+        return;
+    }
+    AmgLevel &N = a->L[l + 1];
+    smooth<T>(c, a, l, b, x, true);
+    level_residual<T>(c, L, l, x, b, Vec<T>::r(L));
+    if (L.bs == 3) bsr_spmv<6, 3, T, T>(L.nagg, L.R_rowptr.p, L.R_colind.p, Vec<T>::R(L), Vec<T>::r(L), Vec<T>::b(N), 0.0, s);
+    else bsr_spmv<6, 6, T, T>(L.nagg, L.R_rowptr.p, L.R_colind.p, Vec<T>::R(L), Vec<T>::r(L), Vec<T>::b(N), 0.0, s);
+    vcycle<T>(c, a, l + 1, Vec<T>::b(N), Vec<T>::x(N));
+    if (L.bs == 3) bsr_spmv<3, 6, T, T>(L.nb, L.P_rowptr.p, L.P_colind.p, Vec<T>::P(L), Vec<T>::x(N), x, 1.0, s);
+    else bsr_spmv<6, 6, T, T>(L.nb, L.P_rowptr.p, L.P_colind.p, Vec<T>::P(L), Vec<T>::x(N), x, 1.0, s);
+    smooth<T>(c, a, l, b, x, false);
+}
+
+}  // namespace
+
+void amg_apply64(pfem_ctx *c, Amg *a, const double *r, double *z) { vcycle<double>(c, a, 0, r, z); }
+void amg_apply32(pfem_ctx *c, Amg *a, const float *r, float *z) { vcycle<float>(c, a, 0, r, z); }
+
+}  // namespace pfem

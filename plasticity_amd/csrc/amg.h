@@ -1,0 +1,107 @@
+// amg.h -- smoothed-aggregation AMG with rigid-body near-null space (internal).
+//
+// Replaces AlgebraicMultigrid.jl's smoothed_aggregation(K; B) + aspreconditioner
+// (Solver.jl:136-154) with a node-block variant: aggregates of mesh nodes (not
+// scalar DOFs), tentative prolongator from a per-aggregate QR of the six rigid-body
+// modes (Solver.jl:117-129), Jacobi-smoothed prolongator P = (I - w D^-1 A) Pt with
+// block-diagonal D and w = 4/(3 rho(D^-1 A)), Galerkin coarse operators R A P with
+// R = P^T.  The symbolic part (aggregates, QR, sparsity of P, A P and R A P) depends
+// only on the mesh graph, coordinates and constraints and is built once on the host;
+// the numeric part runs on the device at every rebuild (once per load step).
+#pragma once
+#include "common.h"
+
+struct pfem_ctx;
+
+namespace pfem {
+
+struct AmgLevel {
+    int bs = 3;                  // dofs per block row on this level (3 fine, 6 coarse)
+    int64_t nb = 0;              // block rows
+    int64_t n = 0;               // dofs = bs * nb
+    // operator (level 0 uses the context's SELL K)
+    int64_t nnzb = 0;
+    dbuf<int32_t> rowptr, colind, diagpos;
+    dbuf<double> A;
+    dbuf<float> A32;
+    std::vector<int32_t> h_rowptr, h_colind;
+    // block-diagonal inverse and spectral radius of D^-1 A
+    dbuf<double> dinv;
+    dbuf<float> dinv32;
+    double rho = 2.0;
+    // prolongator to this level from the next: P (nb x nagg, bs x 6 blocks)
+    int64_t nagg = 0;
+    dbuf<int32_t> agg;           // node -> aggregate (-1 none)
+    dbuf<double> Pt;             // tentative block per node (bs x 6), zero if agg < 0
+    dbuf<int32_t> P_rowptr, P_colind;
+    dbuf<double> P;
+    dbuf<float> P32;
+    // restriction R = P^T (nagg x nb, 6 x bs blocks)
+    dbuf<int32_t> R_rowptr, R_colind, R_src;
+    dbuf<double> R;
+    dbuf<float> R32;
+    // Q = A P (nb x nagg), and the row of every Q block / coarse block
+    dbuf<int32_t> Q_rowptr, Q_colind, Q_row;
+    dbuf<double> Q;
+    dbuf<int32_t> C_row;         // row of every block of the next level's operator
+    // V-cycle vectors
+    dbuf<double> x, b, r, d, t;
+    dbuf<double> pw;             // power-iteration start vector
+    dbuf<float> x32, b32, r32, d32, t32;
+};
+
+struct Amg {
+    std::vector<AmgLevel> L;
+    // coarsest level: dense inverse (FP64)
+    int64_t nc = 0;
+    dbuf<double> coarse_inv;
+    std::vector<double> h_dense;
+    int smoother = PFEM_SMOOTH_CHEBYSHEV;
+    int degree = 2;
+    bool have32 = false;
+    double op_complexity = 1.0;
+};
+
+Amg *amg_create(pfem_ctx *c);                 // symbolic set-up (host)
+void amg_destroy(Amg *a);
+void amg_numeric_setup(pfem_ctx *c, Amg *a);  // numeric set-up (device)
+void amg_apply64(pfem_ctx *c, Amg *a, const double *r, double *z);  // z = V-cycle(r)
+void amg_apply32(pfem_ctx *c, Amg *a, const float *r, float *z);
+
+// amg.hip launchers
+template <int BS>
+void amg_block_inverse(int64_t nb, const int32_t *diagpos, const double *A, double *dinv, hipStream_t s);
+template <int BS, bool SELL>
+void amg_smooth_P(int64_t nb, const int64_t *slice_ptr, const int32_t *rowlen, const int32_t *rowptr,
+                  const int32_t *colind, const double *A, const int32_t *agg, const double *Pt,
+                  const double *dinv, double omega, const int32_t *P_rowptr, const int32_t *P_colind,
+                  double *P, hipStream_t s);
+template <int BS, bool SELL>
+void amg_AP(int64_t nQ, const int32_t *Q_row, const int32_t *Q_colind, const int64_t *slice_ptr,
+            const int32_t *rowlen, const int32_t *rowptr, const int32_t *colind, const double *A,
+            const int32_t *P_rowptr, const int32_t *P_colind, const double *P, double *Q, hipStream_t s);
+template <int BS>
+void amg_RAP(int64_t nC, const int32_t *C_row, const int32_t *C_colind, const int32_t *R_rowptr,
+             const int32_t *R_colind, const double *R, const int32_t *Q_rowptr, const int32_t *Q_colind,
+             const double *Q, double *C, hipStream_t s);
+template <int BS>
+void amg_transpose_P(int64_t nR, const int32_t *R_src, const double *P, double *R, hipStream_t s);
+template <int BR, int BC, class VT, class XT>
+void bsr_spmv(int64_t nb, const int32_t *rowptr, const int32_t *colind, const VT *vals, const XT *x, XT *y,
+              double beta, hipStream_t s);  // y = A x + beta*y
+template <int BS, class VT, class XT>
+void bsr_residual(int64_t nb, const int32_t *rowptr, const int32_t *colind, const VT *vals, const XT *x,
+                  const XT *b, XT *r, hipStream_t s);
+template <int BS, class VT, class XT>
+void sell_residual(const FineMatrix &K, const VT *vals, const XT *x, const XT *b, XT *r, hipStream_t s);
+template <int BS, class VT, class XT>
+void sell_spmv(const FineMatrix &K, const VT *vals, const XT *x, XT *y, hipStream_t s);
+template <int BS, class DT, class XT>
+void block_diag_apply(int64_t nb, const DT *dinv, const XT *r, XT *z, double scale, hipStream_t s);
+template <int BS, class DT, class XT>
+void cheb_step(int64_t nb, const DT *dinv, const XT *Ad, XT *r, XT *d, XT *x, double c1, double c2,
+               hipStream_t s);  // r -= Ad; d = c1 d + c2 Dinv r; x += d
+template <class XT>
+void dense_gemv(int64_t n, const double *Ainv, const XT *b, XT *x, hipStream_t s);
+
+}  // namespace pfem

@@ -1,0 +1,467 @@
+// amg.hip -- device kernels of the smoothed-aggregation AMG (numeric set-up and
+// V-cycle operators).  Every kernel is deterministic: each output block is owned by
+// exactly one thread, so repeated set-ups give bit-identical hierarchies.
+#include <hip/hip_runtime.h>
+
+#include "amg.h"
+
+namespace pfem {
+namespace {
+
+inline unsigned grid_for(int64_t n, int bs) {
+    int64_t g = (n + bs - 1) / bs;
+    return (unsigned)(g < 1 ? 1 : g);
+}
+
+// A-row accessor: SELL-64 (fine level) or CSR of blocks (coarse levels).
+template <bool SELL>
+struct RowIt {
+    int64_t start, stride;
+    int len;
+    __device__ RowIt(int64_t i, const int64_t *slice_ptr, const int32_t *rowlen, const int32_t *rowptr) {
+        if (SELL) {
+            start = slice_ptr[i >> 6] + (i & 63);
+            stride = SELL_C;
+            len = rowlen[i];
+        } else {
+            start = rowptr[i];
+            stride = 1;
+            len = rowptr[i + 1] - rowptr[i];
+        }
+    }
+    __device__ int64_t pos(int k) const { return start + (int64_t)k * stride; }
+};
+
+__device__ __forceinline__ int find_sorted(const int32_t *a, int lo, int hi, int32_t key) {
+    // index in [lo, hi) with a[idx] == key, or -1
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const int32_t v = a[mid];
+        if (v == key) return mid;
+        if (v < key) lo = mid + 1; else hi = mid;
+    }
+    return -1;
+}
+
+// Gauss-Jordan inverse with partial pivoting of a BS x BS diagonal block.  A zero
+// row (a coarse dof whose rigid-body column was dropped by the QR) is treated as
+// decoupled: its diagonal is taken as 1.
+template <int BS>
+__global__ void block_inverse_kernel(int64_t nb, const int32_t *__restrict__ diagpos, const double *__restrict__ A,
+                                     double *__restrict__ dinv) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    double a[BS][BS], v[BS][BS];
+    const double *m = A + (int64_t)BS * BS * diagpos[i];
+    for (int r = 0; r < BS; ++r) {
+        bool zero = true;
+        for (int c = 0; c < BS; ++c) {
+            a[r][c] = m[BS * r + c];
+            v[r][c] = (r == c) ? 1.0 : 0.0;
+            zero = zero && a[r][c] == 0.0;
+        }
+        if (zero) a[r][r] = 1.0;
+    }
+    for (int col = 0; col < BS; ++col) {
+        int piv = col;
+        double best = fabs(a[col][col]);
+        for (int r = col + 1; r < BS; ++r)
+            if (fabs(a[r][col]) > best) { best = fabs(a[r][col]); piv = r; }
+        if (piv != col)
+            for (int c = 0; c < BS; ++c) {
+                double t = a[col][c]; a[col][c] = a[piv][c]; a[piv][c] = t;
+                t = v[col][c]; v[col][c] = v[piv][c]; v[piv][c] = t;
+            }
+        const double d = a[col][col];
+        const double id = d != 0.0 ? 1.0 / d : 0.0;
+        for (int c = 0; c < BS; ++c) { a[col][c] *= id; v[col][c] *= id; }
+        for (int r = 0; r < BS; ++r) {
+            if (r == col) continue;
+            const double f = a[r][col];
+            if (f == 0.0) continue;
+            for (int c = 0; c < BS; ++c) { a[r][c] -= f * a[col][c]; v[r][c] -= f * v[col][c]; }
+        }
+    }
+    double *o = dinv + (int64_t)BS * BS * i;
+    for (int r = 0; r < BS; ++r)
+        for (int c = 0; c < BS; ++c) o[BS * r + c] = v[r][c];
+}
+
+// P row i = Pt_i [at agg(i)] - omega * Dinv_i * sum_k A_ik Pt_{col k} [at agg(col k)]
+template <int BS, bool SELL>
+__global__ void smooth_P_kernel(int64_t nb, const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ rowlen,
+                                const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colind,
+                                const double *__restrict__ A, const int32_t *__restrict__ agg,
+                                const double *__restrict__ Pt, const double *__restrict__ dinv, double omega,
+                                const int32_t *__restrict__ P_rowptr, const int32_t *__restrict__ P_colind,
+                                double *__restrict__ P) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    const int p0 = P_rowptr[i], p1 = P_rowptr[i + 1];
+    if (p0 == p1) return;
+    constexpr int B6 = BS * 6;
+    for (int m = p0; m < p1; ++m)
+        for (int t = 0; t < B6; ++t) P[(int64_t)B6 * m + t] = 0.0;
+    RowIt<SELL> row(i, slice_ptr, rowlen, rowptr);
+    for (int k = 0; k < row.len; ++k) {
+        const int64_t pos = row.pos(k);
+        const int32_t j = colind[pos];
+        const int32_t J = agg[j];
+        if (J < 0) continue;
+        const int m = find_sorted(P_colind, p0, p1, J);
+        if (m < 0) continue;
+        const double *a = A + (int64_t)BS * BS * pos;
+        const double *pt = Pt + (int64_t)B6 * j;
+        double *o = P + (int64_t)B6 * m;
+        for (int r = 0; r < BS; ++r)
+            for (int c = 0; c < 6; ++c) {
+                double s = 0.0;
+                for (int q = 0; q < BS; ++q) s += a[BS * r + q] * pt[6 * q + c];
+                o[6 * r + c] += s;
+            }
+    }
+    const double *di = dinv + (int64_t)BS * BS * i;
+    const int32_t Ji = agg[i];
+    for (int m = p0; m < p1; ++m) {
+        double *o = P + (int64_t)B6 * m;
+        double tmp[B6];
+        for (int r = 0; r < BS; ++r)
+            for (int c = 0; c < 6; ++c) {
+                double s = 0.0;
+                for (int q = 0; q < BS; ++q) s += di[BS * r + q] * o[6 * q + c];
+                tmp[6 * r + c] = s;
+            }
+        const bool own = (P_colind[m] == Ji);
+        for (int t = 0; t < B6; ++t) o[t] = (own ? Pt[(int64_t)B6 * i + t] : 0.0) - omega * tmp[t];
+    }
+}
+
+// Q block (i, J) = sum_k A_ik P_{col k, J}
+template <int BS, bool SELL>
+__global__ void AP_kernel(int64_t nQ, const int32_t *__restrict__ Q_row, const int32_t *__restrict__ Q_colind,
+                          const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ rowlen,
+                          const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colind,
+                          const double *__restrict__ A, const int32_t *__restrict__ P_rowptr,
+                          const int32_t *__restrict__ P_colind, const double *__restrict__ P,
+                          double *__restrict__ Q) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nQ) return;
+    const int64_t i = Q_row[q];
+    const int32_t J = Q_colind[q];
+    constexpr int B6 = BS * 6;
+    double acc[B6];
+    for (int t = 0; t < B6; ++t) acc[t] = 0.0;
+    RowIt<SELL> row(i, slice_ptr, rowlen, rowptr);
+    for (int k = 0; k < row.len; ++k) {
+        const int64_t pos = row.pos(k);
+        const int32_t j = colind[pos];
+        const int m = find_sorted(P_colind, P_rowptr[j], P_rowptr[j + 1], J);
+        if (m < 0) continue;
+        const double *a = A + (int64_t)BS * BS * pos;
+        const double *p = P + (int64_t)B6 * m;
+        for (int r = 0; r < BS; ++r)
+            for (int c = 0; c < 6; ++c) {
+                double s = 0.0;
+                for (int t = 0; t < BS; ++t) s += a[BS * r + t] * p[6 * t + c];
+                acc[6 * r + c] += s;
+            }
+    }
+    double *o = Q + (int64_t)B6 * q;
+    for (int t = 0; t < B6; ++t) o[t] = acc[t];
+}
+
+// Coarse block (I, J) = sum_{m in R row I} R_m Q_{R_col m, J}
+template <int BS>
+__global__ void RAP_kernel(int64_t nC, const int32_t *__restrict__ C_row, const int32_t *__restrict__ C_colind,
+                           const int32_t *__restrict__ R_rowptr, const int32_t *__restrict__ R_colind,
+                           const double *__restrict__ R, const int32_t *__restrict__ Q_rowptr,
+                           const int32_t *__restrict__ Q_colind, const double *__restrict__ Q,
+                           double *__restrict__ C) {
+    const int64_t cb = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cb >= nC) return;
+    const int32_t I = C_row[cb], J = C_colind[cb];
+    double acc[36];
+    for (int t = 0; t < 36; ++t) acc[t] = 0.0;
+    for (int m = R_rowptr[I]; m < R_rowptr[I + 1]; ++m) {
+        const int32_t i = R_colind[m];
+        const int q = find_sorted(Q_colind, Q_rowptr[i], Q_rowptr[i + 1], J);
+        if (q < 0) continue;
+        const double *rb = R + (int64_t)6 * BS * m;
+        const double *qb = Q + (int64_t)BS * 6 * q;
+        for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 6; ++c) {
+                double s = 0.0;
+                for (int t = 0; t < BS; ++t) s += rb[BS * r + t] * qb[6 * t + c];
+                acc[6 * r + c] += s;
+            }
+    }
+    double *o = C + 36 * cb;
+    for (int t = 0; t < 36; ++t) o[t] = acc[t];
+}
+
+template <int BS>
+__global__ void transpose_P_kernel(int64_t nR, const int32_t *__restrict__ R_src, const double *__restrict__ P,
+                                   double *__restrict__ R) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nR) return;
+    const double *p = P + (int64_t)BS * 6 * R_src[m];
+    double *o = R + (int64_t)6 * BS * m;
+    for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < BS; ++c) o[BS * r + c] = p[6 * c + r];
+}
+
+// y = A x + beta*y; one thread per (block row, row-in-block)
+template <int BR, int BC, class VT, class XT>
+__global__ void bsr_spmv_kernel(int64_t nb, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colind,
+                                const VT *__restrict__ vals, const XT *__restrict__ x, XT *__restrict__ y,
+                                double beta) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = t / BR;
+    const int r = (int)(t % BR);
+    if (i >= nb) return;
+    VT s = 0;
+    for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+        const VT *v = vals + (int64_t)BR * BC * k + BC * r;
+        const XT *xx = x + (int64_t)BC * colind[k];
+#pragma unroll
+        for (int c = 0; c < BC; ++c) s += v[c] * (VT)xx[c];
+    }
+    const int64_t o = (int64_t)BR * i + r;
+    y[o] = (XT)(beta == 0.0 ? (double)s : (double)s + beta * (double)y[o]);
+}
+
+template <int BS, class VT, class XT>
+__global__ void bsr_residual_kernel(int64_t nb, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colind,
+                                    const VT *__restrict__ vals, const XT *__restrict__ x, const XT *__restrict__ b,
+                                    XT *__restrict__ r) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = t / BS;
+    const int rr = (int)(t % BS);
+    if (i >= nb) return;
+    VT s = 0;
+    for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+        const VT *v = vals + (int64_t)BS * BS * k + BS * rr;
+        const XT *xx = x + (int64_t)BS * colind[k];
+#pragma unroll
+        for (int c = 0; c < BS; ++c) s += v[c] * (VT)xx[c];
+    }
+    const int64_t o = (int64_t)BS * i + rr;
+    r[o] = (XT)((VT)b[o] - s);
+}
+
+// fine SELL residual / product with a caller-chosen value array (FP64 or FP32 copy)
+template <class VT, class XT, bool RES>
+__global__ __launch_bounds__(256) void sell3_kernel(int64_t nb, const int64_t *__restrict__ slice_ptr,
+                                                    const int32_t *__restrict__ rowlen,
+                                                    const int32_t *__restrict__ colind, const VT *__restrict__ vals,
+                                                    const XT *__restrict__ x, const XT *__restrict__ b,
+                                                    XT *__restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nb) return;
+    const int64_t base = slice_ptr[i >> 6] + (i & 63);
+    const int len = rowlen[i];
+    VT y0 = 0, y1 = 0, y2 = 0;
+#pragma unroll 3
+    for (int k = 0; k < len; ++k) {
+        const int64_t pos = base + (int64_t)k * SELL_C;
+        const int64_t j = colind[pos];
+        const VT *v = vals + 9 * pos;
+        const VT x0 = (VT)x[3 * j], x1 = (VT)x[3 * j + 1], x2 = (VT)x[3 * j + 2];
+        y0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
+        y1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
+        y2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
+    }
+    if (RES) {
+        y[3 * i] = (XT)((VT)b[3 * i] - y0);
+        y[3 * i + 1] = (XT)((VT)b[3 * i + 1] - y1);
+        y[3 * i + 2] = (XT)((VT)b[3 * i + 2] - y2);
+    } else {
+        y[3 * i] = (XT)y0;
+        y[3 * i + 1] = (XT)y1;
+        y[3 * i + 2] = (XT)y2;
+    }
+}
+
+template <int BS, class DT, class XT>
+__global__ void block_diag_apply_kernel(int64_t nb, const DT *__restrict__ dinv, const XT *__restrict__ r,
+                                        XT *__restrict__ z, double scale) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = t / BS;
+    const int rr = (int)(t % BS);
+    if (i >= nb) return;
+    const DT *m = dinv + (int64_t)BS * BS * i + BS * rr;
+    const XT *rv = r + (int64_t)BS * i;
+    DT s = 0;
+#pragma unroll
+    for (int c = 0; c < BS; ++c) s += m[c] * (DT)rv[c];
+    z[(int64_t)BS * i + rr] = (XT)(scale * (double)s);
+}
+
+// r -= Ad; d = c1 d + c2 Dinv r; x += d
+template <int BS, class DT, class XT>
+__global__ void cheb_step_kernel(int64_t nb, const DT *__restrict__ dinv, const XT *__restrict__ Ad,
+                                 XT *__restrict__ r, XT *__restrict__ d, XT *__restrict__ x, double c1, double c2) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    DT rv[BS];
+#pragma unroll
+    for (int c = 0; c < BS; ++c) {
+        const int64_t o = (int64_t)BS * i + c;
+        rv[c] = (DT)r[o] - (DT)Ad[o];
+        r[o] = (XT)rv[c];
+    }
+    const DT *m = dinv + (int64_t)BS * BS * i;
+#pragma unroll
+    for (int rr = 0; rr < BS; ++rr) {
+        DT s = 0;
+#pragma unroll
+        for (int c = 0; c < BS; ++c) s += m[BS * rr + c] * rv[c];
+        const int64_t o = (int64_t)BS * i + rr;
+        const double dn = c1 * (double)d[o] + c2 * (double)s;
+        d[o] = (XT)dn;
+        x[o] = (XT)((double)x[o] + dn);
+    }
+}
+
+template <class XT>
+__global__ void dense_gemv_kernel(int64_t n, const double *__restrict__ Ainv, const XT *__restrict__ b,
+                                  XT *__restrict__ x) {
+    // one 64-lane wave per row
+    const int64_t row = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    double s = 0.0;
+    for (int64_t c = lane; c < n; c += 64) s += Ainv[row * n + c] * (double)b[c];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) x[row] = (XT)s;
+}
+
+}  // namespace
+
+template <int BS>
+void amg_block_inverse(int64_t nb, const int32_t *diagpos, const double *A, double *dinv, hipStream_t s) {
+    hipLaunchKernelGGL((block_inverse_kernel<BS>), dim3(grid_for(nb, 128)), dim3(128), 0, s, nb, diagpos, A, dinv);
+    PF_HIP(hipGetLastError());
+}
+template <int BS, bool SELL>
+void amg_smooth_P(int64_t nb, const int64_t *slice_ptr, const int32_t *rowlen, const int32_t *rowptr,
+                  const int32_t *colind, const double *A, const int32_t *agg, const double *Pt, const double *dinv,
+                  double omega, const int32_t *P_rowptr, const int32_t *P_colind, double *P, hipStream_t s) {
+    hipLaunchKernelGGL((smooth_P_kernel<BS, SELL>), dim3(grid_for(nb, 128)), dim3(128), 0, s, nb, slice_ptr, rowlen,
+                       rowptr, colind, A, agg, Pt, dinv, omega, P_rowptr, P_colind, P);
+    PF_HIP(hipGetLastError());
+}
+template <int BS, bool SELL>
+void amg_AP(int64_t nQ, const int32_t *Q_row, const int32_t *Q_colind, const int64_t *slice_ptr, const int32_t *rowlen,
+            const int32_t *rowptr, const int32_t *colind, const double *A, const int32_t *P_rowptr,
+            const int32_t *P_colind, const double *P, double *Q, hipStream_t s) {
+    hipLaunchKernelGGL((AP_kernel<BS, SELL>), dim3(grid_for(nQ, 128)), dim3(128), 0, s, nQ, Q_row, Q_colind,
+                       slice_ptr, rowlen, rowptr, colind, A, P_rowptr, P_colind, P, Q);
+    PF_HIP(hipGetLastError());
+}
+template <int BS>
+void amg_RAP(int64_t nC, const int32_t *C_row, const int32_t *C_colind, const int32_t *R_rowptr,
+             const int32_t *R_colind, const double *R, const int32_t *Q_rowptr, const int32_t *Q_colind,
+             const double *Q, double *C, hipStream_t s) {
+    hipLaunchKernelGGL((RAP_kernel<BS>), dim3(grid_for(nC, 128)), dim3(128), 0, s, nC, C_row, C_colind, R_rowptr,
+                       R_colind, R, Q_rowptr, Q_colind, Q, C);
+    PF_HIP(hipGetLastError());
+}
+template <int BS>
+void amg_transpose_P(int64_t nR, const int32_t *R_src, const double *P, double *R, hipStream_t s) {
+    hipLaunchKernelGGL((transpose_P_kernel<BS>), dim3(grid_for(nR, 256)), dim3(256), 0, s, nR, R_src, P, R);
+    PF_HIP(hipGetLastError());
+}
+template <int BR, int BC, class VT, class XT>
+void bsr_spmv(int64_t nb, const int32_t *rowptr, const int32_t *colind, const VT *vals, const XT *x, XT *y,
+              double beta, hipStream_t s) {
+    if (nb <= 0) return;
+    hipLaunchKernelGGL((bsr_spmv_kernel<BR, BC, VT, XT>), dim3(grid_for(nb * BR, 256)), dim3(256), 0, s, nb, rowptr,
+                       colind, vals, x, y, beta);
+    PF_HIP(hipGetLastError());
+}
+template <int BS, class VT, class XT>
+void bsr_residual(int64_t nb, const int32_t *rowptr, const int32_t *colind, const VT *vals, const XT *x,
+                  const XT *b, XT *r, hipStream_t s) {
+    hipLaunchKernelGGL((bsr_residual_kernel<BS, VT, XT>), dim3(grid_for(nb * BS, 256)), dim3(256), 0, s, nb, rowptr,
+                       colind, vals, x, b, r);
+    PF_HIP(hipGetLastError());
+}
+template <int BS, class VT, class XT>
+void sell_residual(const FineMatrix &K, const VT *vals, const XT *x, const XT *b, XT *r, hipStream_t s) {
+    static_assert(BS == 3, "fine level is 3x3");
+    hipLaunchKernelGGL((sell3_kernel<VT, XT, true>), dim3(grid_for(K.nb, 256)), dim3(256), 0, s, K.nb,
+                       K.slice_ptr.p, K.rowlen.p, K.colind.p, vals, x, b, r);
+    PF_HIP(hipGetLastError());
+}
+template <int BS, class VT, class XT>
+void sell_spmv(const FineMatrix &K, const VT *vals, const XT *x, XT *y, hipStream_t s) {
+    static_assert(BS == 3, "fine level is 3x3");
+    hipLaunchKernelGGL((sell3_kernel<VT, XT, false>), dim3(grid_for(K.nb, 256)), dim3(256), 0, s, K.nb,
+                       K.slice_ptr.p, K.rowlen.p, K.colind.p, vals, x, (const XT *)nullptr, y);
+    PF_HIP(hipGetLastError());
+}
+template <int BS, class DT, class XT>
+void block_diag_apply(int64_t nb, const DT *dinv, const XT *r, XT *z, double scale, hipStream_t s) {
+    hipLaunchKernelGGL((block_diag_apply_kernel<BS, DT, XT>), dim3(grid_for(nb * BS, 256)), dim3(256), 0, s, nb,
+                       dinv, r, z, scale);
+    PF_HIP(hipGetLastError());
+}
+template <int BS, class DT, class XT>
+void cheb_step(int64_t nb, const DT *dinv, const XT *Ad, XT *r, XT *d, XT *x, double c1, double c2, hipStream_t s) {
+    hipLaunchKernelGGL((cheb_step_kernel<BS, DT, XT>), dim3(grid_for(nb, 256)), dim3(256), 0, s, nb, dinv, Ad, r, d,
+                       x, c1, c2);
+    PF_HIP(hipGetLastError());
+}
+template <class XT>
+void dense_gemv(int64_t n, const double *Ainv, const XT *b, XT *x, hipStream_t s) {
+    hipLaunchKernelGGL((dense_gemv_kernel<XT>), dim3(grid_for(n, 4)), dim3(256), 0, s, n, Ainv, b, x);
+    PF_HIP(hipGetLastError());
+}
+
+// explicit instantiations
+template void amg_block_inverse<3>(int64_t, const int32_t *, const double *, double *, hipStream_t);
+template void amg_block_inverse<6>(int64_t, const int32_t *, const double *, double *, hipStream_t);
+#define PF_SM(BS, SELL)                                                                                            \
+    template void amg_smooth_P<BS, SELL>(int64_t, const int64_t *, const int32_t *, const int32_t *,                \
+                                         const int32_t *, const double *, const int32_t *, const double *,          \
+                                         const double *, double, const int32_t *, const int32_t *, double *,        \
+                                         hipStream_t);                                                              \
+    template void amg_AP<BS, SELL>(int64_t, const int32_t *, const int32_t *, const int64_t *, const int32_t *,     \
+                                   const int32_t *, const int32_t *, const double *, const int32_t *,               \
+                                   const int32_t *, const double *, double *, hipStream_t);
+PF_SM(3, true)
+PF_SM(6, false)
+#undef PF_SM
+template void amg_RAP<3>(int64_t, const int32_t *, const int32_t *, const int32_t *, const int32_t *, const double *,
+                         const int32_t *, const int32_t *, const double *, double *, hipStream_t);
+template void amg_RAP<6>(int64_t, const int32_t *, const int32_t *, const int32_t *, const int32_t *, const double *,
+                         const int32_t *, const int32_t *, const double *, double *, hipStream_t);
+template void amg_transpose_P<3>(int64_t, const int32_t *, const double *, double *, hipStream_t);
+template void amg_transpose_P<6>(int64_t, const int32_t *, const double *, double *, hipStream_t);
+#define PF_OPS(VT, XT)                                                                                             \
+    template void bsr_spmv<6, 6, VT, XT>(int64_t, const int32_t *, const int32_t *, const VT *, const XT *, XT *,  \
+                                         double, hipStream_t);                                                     \
+    template void bsr_spmv<3, 6, VT, XT>(int64_t, const int32_t *, const int32_t *, const VT *, const XT *, XT *,  \
+                                         double, hipStream_t);                                                     \
+    template void bsr_spmv<6, 3, VT, XT>(int64_t, const int32_t *, const int32_t *, const VT *, const XT *, XT *,  \
+                                         double, hipStream_t);                                                     \
+    template void bsr_residual<6, VT, XT>(int64_t, const int32_t *, const int32_t *, const VT *, const XT *,       \
+                                          const XT *, XT *, hipStream_t);                                          \
+    template void sell_residual<3, VT, XT>(const FineMatrix &, const VT *, const XT *, const XT *, XT *,           \
+                                           hipStream_t);                                                           \
+    template void sell_spmv<3, VT, XT>(const FineMatrix &, const VT *, const XT *, XT *, hipStream_t);             \
+    template void block_diag_apply<3, VT, XT>(int64_t, const VT *, const XT *, XT *, double, hipStream_t);         \
+    template void block_diag_apply<6, VT, XT>(int64_t, const VT *, const XT *, XT *, double, hipStream_t);         \
+    template void cheb_step<3, VT, XT>(int64_t, const VT *, const XT *, XT *, XT *, XT *, double, double,          \
+                                       hipStream_t);                                                               \
+    template void cheb_step<6, VT, XT>(int64_t, const VT *, const XT *, XT *, XT *, XT *, double, double,          \
+                                       hipStream_t);
+PF_OPS(double, double)
+PF_OPS(float, float)
+#undef PF_OPS
+template void dense_gemv<double>(int64_t, const double *, const double *, double *, hipStream_t);
+template void dense_gemv<float>(int64_t, const double *, const float *, float *, hipStream_t);
+
+}  // namespace pfem

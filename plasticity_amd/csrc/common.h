@@ -1,0 +1,121 @@
+// common.h -- internal declarations shared by the libpfem translation units.
+// Not part of the C-ABI (include/pfem.h is).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "pfem.h"
+
+namespace pfem {
+
+// HIP error -> exception carrying the code; the C-ABI layer maps it to PFEM_ERR_HIP.
+struct hip_failure : std::runtime_error {
+    hipError_t code;
+    hip_failure(hipError_t c, const char *where)
+        : std::runtime_error(std::string(where) + ": " + hipGetErrorString(c)), code(c) {}
+};
+struct status_failure : std::runtime_error {
+    int status;
+    status_failure(int s, const std::string &m) : std::runtime_error(m), status(s) {}
+};
+
+#define PF_HIP(call)                                                        \
+    do {                                                                    \
+        hipError_t e_ = (call);                                             \
+        if (e_ != hipSuccess) throw ::pfem::hip_failure(e_, #call);         \
+    } while (0)
+
+// Device buffer owned by a context (RAII).
+template <class T>
+struct dbuf {
+    T *p = nullptr;
+    size_t n = 0;
+    dbuf() = default;
+    explicit dbuf(size_t count) { alloc(count); }
+    dbuf(const dbuf &) = delete;
+    dbuf &operator=(const dbuf &) = delete;
+    dbuf(dbuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    dbuf &operator=(dbuf &&o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~dbuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) PF_HIP(hipMalloc(&p, count * sizeof(T)));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+    void upload(const T *h, size_t count, hipStream_t s) {
+        if (count) PF_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void download(T *h, size_t count, hipStream_t s) const {
+        if (count) PF_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+    void zero(hipStream_t s) {
+        if (n) PF_HIP(hipMemsetAsync(p, 0, n * sizeof(T), s));
+    }
+};
+
+// Material constants as used on the device (derived like J2Material, Materials.jl:104-107).
+struct DevMat {
+    double E, nu, sy0, Hiso, Hkin, ssat, delta;
+    double G, K, lam;
+};
+int make_devmat(const pfem_material &m, DevMat &d);  // PFEM_OK or PFEM_ERR_BAD_ARG / UNSUPPORTED
+
+// Fine-level stiffness in sliced block-ELL ("SELL-64") layout: 64 consecutive block
+// rows form a slice; slice s stores its blocks column by column, position
+// pos(i, k) = slice_ptr[i/64] + 64*k + i%64, each block 9 contiguous values (row-major).
+constexpr int SELL_C = 64;
+
+struct FineMatrix {
+    int64_t nb = 0;         // block rows (= nodes)
+    int64_t nslots = 0;     // padded block slots
+    int64_t nnzb = 0;       // true blocks
+    int64_t nslices = 0;
+    dbuf<int64_t> slice_ptr;  // nslices + 1
+    dbuf<int32_t> rowlen;     // nb
+    dbuf<int32_t> colind;     // nslots (padding -> own row, value 0)
+    dbuf<int32_t> diagpos;    // nb: slot of the diagonal block
+    dbuf<double> v64;         // 9 * nslots
+    dbuf<float> v32;          // 9 * nslots (FP32-IR copy)
+    std::vector<int64_t> h_slice_ptr;
+    std::vector<int32_t> h_rowlen, h_colind;  // host copies of the pattern
+};
+
+// Element kernels (element.hip)
+struct ElementArgs {
+    int64_t nelem;
+    const double *X;        // 3 x nnodes
+    const int64_t *conn;    // 8 x nelem
+    const double *U;        // ndof
+    const int32_t *matid;   // nelem or null
+    const DevMat *mats;     // device array
+    int fbar;
+    int commit;
+    double *epsp, *abar, *cpinv, *sig;  // Gauss state (6 x ngp / ngp)
+    double *packet;         // per GP packet (PK doubles)
+    double *g0;             // per element 24
+    unsigned long long *fail_key;  // min (8e+g) of inverted GPs, ~0ull if none
+};
+constexpr int PK = 132;     // doubles per Gauss-point packet
+
+void launch_gp_kernel(const ElementArgs &a, int64_t e_begin, int64_t e_count, hipStream_t s);
+void launch_scatter_kernel(const ElementArgs &a, const int64_t *elem_list, int64_t count,
+                           const int32_t *slots, double *Kv, double *R, hipStream_t s);
+void launch_dense_ke_kernel(const ElementArgs &a, int64_t nelem, double *Fe, double *Ke,
+                            hipStream_t s);
+void launch_detF_kernel(const ElementArgs &a, int64_t gp, double *out, hipStream_t s);
+
+}  // namespace pfem

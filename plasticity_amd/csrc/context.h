@@ -1,0 +1,78 @@
+// context.h -- the pfem_ctx object (internal).
+#pragma once
+#include <chrono>
+#include <memory>
+
+#include "common.h"
+#include "kernels.h"
+
+namespace pfem {
+struct Amg;  // amg.cpp
+void amg_destroy(Amg *);
+}  // namespace pfem
+
+struct pfem_ctx {
+    hipStream_t stream = nullptr;
+    int device = 0;
+    pfem_options opt{};
+    int64_t nnodes = 0, nelem = 0, ndof = 0;
+    int fbar = 1;
+
+    // mesh
+    pfem::dbuf<double> X;
+    pfem::dbuf<int64_t> conn;
+    pfem::dbuf<int32_t> matid;
+    pfem::dbuf<pfem::DevMat> mats;
+    std::vector<double> h_nodes;
+    std::vector<int64_t> h_conn;
+
+    // pattern, colours, scatter slots
+    pfem::FineMatrix K;
+    pfem::dbuf<int32_t> slots;  // 64 per element: slot of block (a, b)
+    int ncolors = 0;
+    std::vector<int64_t> color_ptr;
+    std::vector<int32_t> h_color;
+    pfem::dbuf<int64_t> color_order;
+
+    // Gauss state (committed history + last stress) and element scratch
+    pfem::dbuf<double> epsp, abar, cpinv, sig;
+    pfem::dbuf<double> packet, g0;
+    int64_t packet_elems = 0;  // elements per packet chunk
+    pfem::dbuf<unsigned long long> fail_key;
+
+    // vectors
+    pfem::dbuf<double> U, R, dU, rhs, Fext0, dg;
+    double fext0_norm = 0.0;
+
+    // boundary conditions
+    int64_t nbc = 0, nbcrows = 0;
+    pfem::dbuf<int64_t> bcdofs;
+    pfem::dbuf<uint8_t> bcflag;
+    pfem::dbuf<double> bcval;
+    pfem::dbuf<int32_t> bcrows;
+
+    // Krylov workspace (FP64 and FP32)
+    pfem::dbuf<double> w_r, w_z, w_p, w_q, w_x, w_s, w_t, w_v, w_rh;
+    pfem::dbuf<float> f_r, f_z, f_p, f_q, f_x, f_b;
+    pfem::dbuf<double> partial, scal;
+    double *h_scal = nullptr;  // pinned
+
+    // preconditioner
+    std::unique_ptr<pfem::Amg, void (*)(pfem::Amg *)> amg{nullptr, pfem::amg_destroy};
+    bool pc_ready = false;
+
+    // statistics
+    pfem_stats st{};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    double sync_read(const double *dptr);  // D2H of one device scalar
+    void sync() { PF_HIP(hipStreamSynchronize(stream)); }
+};
+
+namespace pfem {
+inline double now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+int assemble_impl(pfem_ctx *c, int commit, pfem_fail *fail);
+int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *used, double *relres);
+}  // namespace pfem

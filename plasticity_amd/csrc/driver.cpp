@@ -1,0 +1,493 @@
+// driver.cpp -- assembly driver, Krylov solves and the Newton loop.
+//
+//  assemble_impl      assemble! (Assembly.jl:238-257, :361-385): zero K and R, one
+//                     per-GP launch over all elements, then the colour-ordered
+//                     scatter launches (colours serially, as Assembly.jl:378-383).
+//  linear_solve_impl  _linear_solve! (Solver.jl:235-290): PCG with the AMG V-cycle
+//                     (stopping on the preconditioned residual sqrt(r'z) relative to
+//                     its initial value, atol = 0), refresh-and-retry, then a
+//                     BiCGStab fallback with the same preconditioner in place of the
+//                     reference's direct solve (a 3M-DOF LU is not an option on the
+//                     device); or FP32 PCG inside FP64 iterative refinement.
+//  pfem_newton        newton! (Solver.jl:317-389) with the Eisenstat-Walker forcing
+//                     term (:159-171) and the reference-scaled convergence test.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "amg.h"
+#include "context.h"
+
+using namespace pfem;
+
+namespace pfem {
+
+static ElementArgs element_args(pfem_ctx *c, int commit) {
+    ElementArgs a{};
+    a.nelem = c->nelem;
+    a.X = c->X.p;
+    a.conn = c->conn.p;
+    a.U = c->U.p;
+    a.matid = c->matid.p;
+    a.mats = c->mats.p;
+    a.fbar = c->fbar;
+    a.commit = commit;
+    a.epsp = c->epsp.p;
+    a.abar = c->abar.p;
+    a.cpinv = c->cpinv.p;
+    a.sig = c->sig.p;
+    a.packet = c->packet.p;
+    a.g0 = c->g0.p;
+    a.fail_key = c->fail_key.p;
+    return a;
+}
+
+int assemble_impl(pfem_ctx *c, int commit, pfem_fail *fail) {
+    hipStream_t s = c->stream;
+    const double t0 = now();
+    ElementArgs a = element_args(c, commit);
+    PF_HIP(hipMemsetAsync(c->fail_key.p, 0xff, sizeof(unsigned long long), s));
+    c->K.v64.zero(s);  // Assembly.jl:361-362
+    c->R.zero(s);
+    PF_HIP(hipEventRecord(c->ev0, s));
+    launch_gp_kernel(a, 0, c->nelem, s);
+    PF_HIP(hipEventRecord(c->ev1, s));
+    unsigned long long key = 0;
+    PF_HIP(hipMemcpyAsync(&key, c->fail_key.p, sizeof key, hipMemcpyDeviceToHost, s));
+    c->sync();
+    float ms_gp = 0.f;
+    PF_HIP(hipEventElapsedTime(&ms_gp, c->ev0, c->ev1));
+    c->st.t_gp_kernel += 1e-3 * ms_gp;
+    if (key != ~0ull) {  // ElementInversionError(e, g, J)
+        launch_detF_kernel(a, (int64_t)key, c->scal.p, s);
+        double J = c->sync_read(c->scal.p);
+        if (fail) { fail->e = (int64_t)(key / 8); fail->g = (int32_t)(key % 8); fail->J = J; }
+        return PFEM_ERR_INVERTED;
+    }
+    PF_HIP(hipEventRecord(c->ev0, s));
+    for (int col = 0; col < c->ncolors; ++col)
+        launch_scatter_kernel(a, c->color_order.p + c->color_ptr[col], c->color_ptr[col + 1] - c->color_ptr[col],
+                              c->slots.p, c->K.v64.p, c->R.p, s);
+    PF_HIP(hipEventRecord(c->ev1, s));
+    c->sync();
+    float ms_sc = 0.f;
+    PF_HIP(hipEventElapsedTime(&ms_sc, c->ev0, c->ev1));
+    c->st.t_scatter_kernel += 1e-3 * ms_sc;
+    c->st.t_assemble += now() - t0;
+    c->st.n_assemble++;
+    return PFEM_OK;
+}
+
+// ---------------------------------------------------------------- preconditioner
+static void pc_setup(pfem_ctx *c) {
+    const double t0 = now();
+    if (!c->amg) c->amg.reset(amg_create(c));
+    amg_numeric_setup(c, c->amg.get());
+    c->pc_ready = true;
+    c->st.t_setup += now() - t0;
+    c->st.n_setup++;
+}
+
+// PCG on the fine operator, FP64 (Krylov.jl cg! semantics as used at Solver.jl:258:
+// x0 = 0, stop when sqrt(r'z) <= rtol * sqrt(r0'z0)).  Returns 0 solved,
+// 1 not converged (itmax), 2 breakdown.
+static int pcg64(pfem_ctx *c, const double *b, double *x, double rtol, int itmax, int *iters) {
+    hipStream_t s = c->stream;
+    const int64_t n = c->ndof;
+    double *r = c->w_r.p, *z = c->w_z.p, *p = c->w_p.p, *q = c->w_q.p;
+    double *sc = c->scal.p;
+    fill<double>(n, 0.0, x, s);
+    PF_HIP(hipMemcpyAsync(r, b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    amg_apply64(c, c->amg.get(), r, z);
+    dot_dev<double, double>(n, r, z, c->partial.p, sc + 0, s);
+    double rz = c->sync_read(sc + 0);
+    *iters = 0;
+    if (rz < 0) return 2;
+    const double rn0 = std::sqrt(rz);
+    if (rn0 == 0.0) return 0;
+    const double eps = rtol * rn0;
+    PF_HIP(hipMemcpyAsync(p, z, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    for (int it = 1; it <= itmax; ++it) {
+        PF_HIP(hipEventRecord(c->ev0, s));
+        spmv_fine64_dot(c->K, p, q, c->partial.p, sc + 1, s);  // q = K p, sc[1] = p'q
+        PF_HIP(hipEventRecord(c->ev1, s));
+        const double pq = c->sync_read(sc + 1);
+        {
+            float ms = 0.f;
+            PF_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            c->st.t_pcg_spmv += 1e-3 * ms;
+            c->st.n_pcg_spmv++;
+        }
+        if (!(pq > 0)) { *iters = it; return 2; }
+        cg_update<double>(n, sc + 0, sc + 1, p, q, x, r, s);  // x += a p, r -= a q
+        amg_apply64(c, c->amg.get(), r, z);
+        dot_dev<double, double>(n, r, z, c->partial.p, sc + 2, s);
+        const double rzn = c->sync_read(sc + 2);
+        *iters = it;
+        if (rzn < 0) return 2;
+        if (c->opt.verbose > 1) fprintf(stderr, "      pcg %3d  sqrt(rz) = %.3e  (target %.3e)\n", it, std::sqrt(rzn), eps);
+        if (std::sqrt(rzn) <= eps) return 0;
+        xpby_ratio<double>(n, sc + 2, sc + 0, z, p, s);  // p = z + (rzn/rz) p
+        PF_HIP(hipMemcpyAsync(sc + 0, sc + 2, sizeof(double), hipMemcpyDeviceToDevice, s));
+        rz = rzn;
+    }
+    return 1;
+}
+
+// Right-preconditioned BiCGStab (van der Vorst 1992) with the same V-cycle;
+// stops on the true residual ||b - Kx|| <= rtol ||b||.
+static int bicgstab64(pfem_ctx *c, const double *b, double *x, double rtol, int itmax, int *iters) {
+    hipStream_t s = c->stream;
+    const int64_t n = c->ndof;
+    double *r = c->w_r.p, *rh = c->w_rh.p, *p = c->w_p.p, *v = c->w_v.p, *ph = c->w_z.p, *sv = c->w_s.p,
+           *sh = c->w_q.p, *t = c->w_t.p, *sc = c->scal.p;
+    auto dot = [&](const double *a, const double *bb) {
+        dot_dev<double, double>(n, a, bb, c->partial.p, sc + 8, s);
+        return c->sync_read(sc + 8);
+    };
+    fill<double>(n, 0.0, x, s);
+    PF_HIP(hipMemcpyAsync(r, b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    PF_HIP(hipMemcpyAsync(rh, b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    fill<double>(n, 0.0, p, s);
+    fill<double>(n, 0.0, v, s);
+    const double bn = std::sqrt(dot(b, b));
+    *iters = 0;
+    if (bn == 0.0) return 0;
+    double rho = 1, alpha = 1, omega = 1;
+    for (int it = 1; it <= itmax; ++it) {
+        *iters = it;
+        const double rho_n = dot(rh, r);
+        if (rho_n == 0.0) return 2;
+        const double beta = (rho_n / rho) * (alpha / omega);
+        axpby<double>(n, -omega, v, 1.0, p, s);  // p = p - w v
+        axpby<double>(n, 1.0, r, beta, p, s);    // p = r + beta p
+        amg_apply64(c, c->amg.get(), p, ph);
+        spmv_fine64(c->K, ph, v, s);
+        const double rv = dot(rh, v);
+        if (rv == 0.0) return 2;
+        alpha = rho_n / rv;
+        PF_HIP(hipMemcpyAsync(sv, r, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        axpby<double>(n, -alpha, v, 1.0, sv, s);  // s = r - a v
+        axpby<double>(n, alpha, ph, 1.0, x, s);   // x += a ph
+        const double sn = std::sqrt(dot(sv, sv));
+        if (sn <= rtol * bn) return 0;
+        amg_apply64(c, c->amg.get(), sv, sh);
+        spmv_fine64(c->K, sh, t, s);
+        const double tt = dot(t, t);
+        if (tt == 0.0) return 2;
+        omega = dot(t, sv) / tt;
+        axpby<double>(n, omega, sh, 1.0, x, s);  // x += w sh
+        PF_HIP(hipMemcpyAsync(r, sv, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        axpby<double>(n, -omega, t, 1.0, r, s);  // r = s - w t
+        const double rn = std::sqrt(dot(r, r));
+        if (c->opt.verbose > 1) fprintf(stderr, "      bicgstab %3d  |r|/|b| = %.3e\n", it, rn / bn);
+        if (rn <= rtol * bn) return 0;
+        if (omega == 0.0) return 2;
+        rho = rho_n;
+    }
+    return 1;
+}
+
+// FP32 PCG (vectors, SpMV and V-cycle in FP32; dot products accumulated in FP64).
+static int pcg32(pfem_ctx *c, const float *b, float *x, double rtol, int itmax, int *iters) {
+    hipStream_t s = c->stream;
+    const int64_t n = c->ndof;
+    float *r = c->f_r.p, *z = c->f_z.p, *p = c->f_p.p, *q = c->f_q.p;
+    double *sc = c->scal.p;
+    fill<float>(n, 0.f, x, s);
+    PF_HIP(hipMemcpyAsync(r, b, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    amg_apply32(c, c->amg.get(), r, z);
+    dot_dev<float, float>(n, r, z, c->partial.p, sc + 0, s);
+    double rz = c->sync_read(sc + 0);
+    *iters = 0;
+    if (rz < 0) return 2;
+    const double rn0 = std::sqrt(rz);
+    if (rn0 == 0.0) return 0;
+    PF_HIP(hipMemcpyAsync(p, z, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    for (int it = 1; it <= itmax; ++it) {
+        spmv_fine32_dot(c->K, p, q, c->partial.p, sc + 1, s);
+        const double pq = c->sync_read(sc + 1);
+        if (!(pq > 0)) { *iters = it; return 2; }
+        cg_update<float>(n, sc + 0, sc + 1, p, q, x, r, s);
+        amg_apply32(c, c->amg.get(), r, z);
+        dot_dev<float, float>(n, r, z, c->partial.p, sc + 2, s);
+        const double rzn = c->sync_read(sc + 2);
+        *iters = it;
+        if (rzn < 0) return 2;
+        if (std::sqrt(rzn) <= rtol * rn0) return 0;
+        xpby_ratio<float>(n, sc + 2, sc + 0, z, p, s);
+        PF_HIP(hipMemcpyAsync(sc + 0, sc + 2, sizeof(double), hipMemcpyDeviceToDevice, s));
+        rz = rzn;
+    }
+    return 1;
+}
+
+// FP64 iterative refinement around FP32 PCG: r = b - K x in FP64, inner solve in
+// FP32 to a fixed relative accuracy, x += d in FP64, until ||r|| <= rtol ||b||.
+static int ir_solve(pfem_ctx *c, const double *b, double *x, double rtol, int itmax, int *iters) {
+    hipStream_t s = c->stream;
+    const int64_t n = c->ndof;
+    double *r = c->w_s.p, *sc = c->scal.p;
+    fill<double>(n, 0.0, x, s);
+    dot_dev<double, double>(n, b, b, c->partial.p, sc + 4, s);
+    const double bn = std::sqrt(c->sync_read(sc + 4));
+    *iters = 0;
+    if (bn == 0.0) return 0;
+    PF_HIP(hipMemcpyAsync(r, b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    double rn = bn;
+    const double inner_tol = 1e-4;  // each refinement gains ~4 digits
+    for (int outer = 0; outer < 30; ++outer) {
+        convert<double, float>(n, r, c->f_b.p, s);
+        int it = 0;
+        int rc = pcg32(c, c->f_b.p, c->f_x.p, std::max(inner_tol, 0.5 * rtol * bn / rn), itmax - *iters, &it);
+        *iters += it;
+        if (rc == 2) return 2;
+        convert<float, double>(n, c->f_x.p, c->w_t.p, s);
+        axpby<double>(n, 1.0, c->w_t.p, 1.0, x, s);
+        residual_fine64(c->K, x, b, r, s);  // FP64 refinement residual
+        dot_dev<double, double>(n, r, r, c->partial.p, sc + 4, s);
+        rn = std::sqrt(c->sync_read(sc + 4));
+        if (c->opt.verbose > 1) fprintf(stderr, "      IR %2d  inner %3d  |r|/|b| = %.3e\n", outer, it, rn / bn);
+        if (rn <= rtol * bn) return 0;
+        if (*iters >= itmax) return 1;
+    }
+    return 1;
+}
+
+int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *used, double *relres) {
+    const double t0 = now();
+    if (rebuild || !c->pc_ready) pc_setup(c);
+    const double t1 = now();
+    int it = 0, total = 0, which = 0, rc;
+    const int itmax = c->opt.cg_itmax > 0 ? c->opt.cg_itmax : 200;
+    if (c->opt.precision == PFEM_PREC_FP32_IR) {
+        hipStream_t s = c->stream;
+        if (!c->f_r.p)
+            for (auto *b : {&c->f_r, &c->f_z, &c->f_p, &c->f_q, &c->f_x, &c->f_b}) b->alloc(c->ndof);
+        if (!c->K.v32.p) c->K.v32.alloc(9 * c->K.nslots);
+        convert<double, float>(9 * c->K.nslots, c->K.v64.p, c->K.v32.p, s);  // FP32 copy of the imposed K
+        which = 3;
+        rc = ir_solve(c, c->rhs.p, c->dU.p, rtol, 4 * itmax, &it);
+        total += it;
+    } else {
+        rc = pcg64(c, c->rhs.p, c->dU.p, rtol, itmax, &it);
+        total += it;
+        if (rc == 1) {  // Solver.jl:270-273: refresh the preconditioner, retry once
+            pc_setup(c);
+            which = 1;
+            rc = pcg64(c, c->rhs.p, c->dU.p, rtol, itmax, &it);
+            total += it;
+        }
+        if (rc != 0) {  // in place of the direct fallback (Solver.jl:262, :284)
+            which = 2;
+            rc = bicgstab64(c, c->rhs.p, c->dU.p, rtol, 2 * itmax, &it);
+            total += it;
+        }
+    }
+    // true residual ||b - K dU|| / ||b||
+    hipStream_t s = c->stream;
+    residual_fine64(c->K, c->dU.p, c->rhs.p, c->w_s.p, s);
+    dot_dev<double, double>(c->ndof, c->w_s.p, c->w_s.p, c->partial.p, c->scal.p + 5, s);
+    dot_dev<double, double>(c->ndof, c->rhs.p, c->rhs.p, c->partial.p, c->scal.p + 6, s);
+    PF_HIP(hipMemcpyAsync(c->h_scal, c->scal.p, 8 * sizeof(double), hipMemcpyDeviceToHost, s));
+    c->sync();
+    const double rr = std::sqrt(c->h_scal[5]), bb = std::sqrt(c->h_scal[6]);
+    if (relres) *relres = bb > 0 ? rr / bb : 0.0;
+    if (niter) *niter = total;
+    if (used) *used = which;
+    c->st.t_solve += now() - t1;
+    c->st.n_solve++;
+    c->st.n_krylov += total;
+    (void)t0;
+    if (rc == 1) return PFEM_ERR_NOT_CONVERGED;
+    if (rc == 2) return PFEM_ERR_NON_SPD;
+    return PFEM_OK;
+}
+
+}  // namespace pfem
+
+extern "C" {
+
+// _forcing_term (Solver.jl:159-171); it is 1-based
+static double forcing_term(const pfem_options &o, int it, double rk, double rkm1, double eta_prev) {
+    if (it == 1 || rkm1 <= 0) return o.eta_max;
+    double eta = o.ew_gamma * std::pow(rk / rkm1, o.ew_alpha);
+    const double safe = o.ew_gamma * std::pow(eta_prev, o.ew_alpha);
+    if (safe > 0.1) eta = std::max(eta, safe);
+    return std::min(std::max(eta, o.eta_min), o.eta_max);
+}
+
+int pfem_newton(pfem_ctx *c, double lambda, double tol, int maxiter, pfem_newton_result *res, pfem_fail *fail) {
+    if (!c || !res || maxiter < 1 || maxiter > 64) return PFEM_ERR_BAD_ARG;
+    std::memset(res, 0, sizeof *res);
+    try {
+        hipStream_t s = c->stream;
+        const double T0 = now();
+        const double fext_norm = std::fabs(lambda) * c->fext0_norm;  // Solver.jl:336
+        double ref = 1.0, rkm1 = 0.0, eta_prev = c->opt.eta_max;
+        bool first = true;
+        for (int it = 1; it <= maxiter; ++it) {
+            double t = now();
+            int rc = assemble_impl(c, 0, fail);  // Solver.jl:347
+            if (rc) return rc;
+            res->t_assemble += now() - t;
+            t = now();
+            sub_external(c->ndof, lambda, c->Fext0.p, c->R.p, s);  // :349
+            impose_dirichlet_dev(c->K, c->nbc, c->bcdofs.p, c->bcflag.p, c->bcval.p, lambda, c->U.p, c->dg.p,
+                                 c->nbcrows, c->bcrows.p, c->R.p, s);  // :354-355
+            dot_dev<double, double>(c->ndof, c->R.p, c->R.p, c->partial.p, c->scal.p + 7, s);
+            const double rnorm = std::sqrt(c->sync_read(c->scal.p + 7));  // :358
+            res->t_bc += now() - t;
+            res->residuals[it - 1] = rnorm;
+            res->iters = it;
+            if (it == 1) ref = std::max(std::max(rnorm, fext_norm), 1.0);  // :360-362
+            if (c->opt.verbose) fprintf(stderr, "    iter %d  |R| = %.6e\n", it, rnorm);
+            if (rnorm <= tol * ref) {  // :364
+                res->converged = 1;
+                break;
+            }
+            if (it == maxiter) break;
+            const double eta = forcing_term(c->opt, it, rnorm, rkm1, eta_prev);  // :371
+            const bool rebuild = first;  // :374
+            first = false;
+            axpby<double>(c->ndof, -1.0, c->R.p, 0.0, c->rhs.p, s);  // :378
+            t = now();
+            const double setup0 = c->st.t_setup;
+            int niter = 0, used = 0;
+            double rel = 0;
+            rc = linear_solve_impl(c, eta, rebuild, &niter, &used, &rel);  // :379
+            res->t_setup += c->st.t_setup - setup0;
+            res->t_solve += now() - t - (c->st.t_setup - setup0);
+            res->krylov_iters[it - 1] = niter;
+            res->solver_used[it - 1] = used;
+            if (c->opt.verbose)
+                fprintf(stderr, "    iter %d  eta = %.2e  krylov = %d (solver %d, true relres %.2e)\n", it, eta, niter,
+                        used, rel);
+            if (rc == PFEM_ERR_NOT_CONVERGED || rc == PFEM_ERR_NON_SPD) {
+                // keep the (inexact) correction, as the reference keeps the direct fallback's
+            } else if (rc) {
+                return rc;
+            }
+            axpby<double>(c->ndof, 1.0, c->dU.p, 1.0, c->U.p, s);  // :383
+            rkm1 = rnorm;
+            eta_prev = eta;
+        }
+        c->sync();
+        res->t_total = now() - T0;
+        return PFEM_OK;
+    } catch (const hip_failure &e) {
+        if (fail) fail->hip = (int32_t)e.code;
+        return PFEM_ERR_HIP;
+    } catch (const status_failure &e) {
+        return e.status;
+    } catch (const std::exception &) {
+        return PFEM_ERR_BAD_ARG;
+    }
+}
+
+int pfem_commit(pfem_ctx *c, pfem_fail *fail) {
+    if (!c) return PFEM_ERR_BAD_ARG;
+    try {
+        return assemble_impl(c, 1, fail);  // Solver.jl:470-471
+    } catch (const hip_failure &e) {
+        if (fail) fail->hip = (int32_t)e.code;
+        return PFEM_ERR_HIP;
+    }
+}
+
+int pfem_element_batch(int32_t kind, const pfem_material *mat, int64_t nelem, const double *Xe, const double *ue,
+                       double *epsp, double *abar, double *cpinv, int commit, double *Fe, double *Ke, double *sig,
+                       pfem_fail *fail) {
+    if (!mat || nelem <= 0 || !Xe || !ue || !epsp || !abar || !cpinv || !Fe || !Ke || !sig) return PFEM_ERR_BAD_ARG;
+    if (kind != PFEM_HEX8_FINITE && kind != PFEM_HEX8_FINITE_FBAR) return PFEM_ERR_BAD_ARG;
+    DevMat dm;
+    int rc = make_devmat(*mat, dm);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PFEM_ERR_NO_DEVICE;
+    try {
+        hipStream_t s;
+        PF_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        struct SG { hipStream_t s; ~SG() { (void)hipStreamDestroy(s); } } sg{s};
+        const int64_t ngp = 8 * nelem;
+        dbuf<double> X(24 * nelem), U(24 * nelem), ep(6 * ngp), ab(ngp), cp(6 * ngp), sg6(6 * ngp),
+            pk((size_t)ngp * PK), g0(24 * nelem), fe(24 * nelem), ke(576 * nelem);
+        dbuf<int64_t> cn(8 * nelem);
+        dbuf<DevMat> md(1);
+        dbuf<unsigned long long> fk(1);
+        std::vector<int64_t> hc(8 * nelem);
+        for (int64_t k = 0; k < 8 * nelem; ++k) hc[k] = k;
+        X.upload(Xe, 24 * nelem, s); U.upload(ue, 24 * nelem, s); cn.upload(hc.data(), 8 * nelem, s);
+        ep.upload(epsp, 6 * ngp, s); ab.upload(abar, ngp, s); cp.upload(cpinv, 6 * ngp, s);
+        md.upload(&dm, 1, s);
+        sg6.zero(s);
+        PF_HIP(hipMemsetAsync(fk.p, 0xff, sizeof(unsigned long long), s));
+        ElementArgs a{};
+        a.nelem = nelem; a.X = X.p; a.conn = cn.p; a.U = U.p; a.matid = nullptr; a.mats = md.p;
+        a.fbar = kind == PFEM_HEX8_FINITE_FBAR; a.commit = commit; a.epsp = ep.p; a.abar = ab.p; a.cpinv = cp.p;
+        a.sig = sg6.p; a.packet = pk.p; a.g0 = g0.p; a.fail_key = fk.p;
+        launch_gp_kernel(a, 0, nelem, s);
+        unsigned long long key = 0;
+        PF_HIP(hipMemcpyAsync(&key, fk.p, sizeof key, hipMemcpyDeviceToHost, s));
+        PF_HIP(hipStreamSynchronize(s));
+        if (key != ~0ull) {
+            dbuf<double> J(1);
+            launch_detF_kernel(a, (int64_t)key, J.p, s);
+            double hJ = 0;
+            J.download(&hJ, 1, s);
+            PF_HIP(hipStreamSynchronize(s));
+            if (fail) { fail->e = (int64_t)(key / 8); fail->g = (int32_t)(key % 8); fail->J = hJ; }
+            return PFEM_ERR_INVERTED;
+        }
+        launch_dense_ke_kernel(a, nelem, fe.p, ke.p, s);
+        fe.download(Fe, 24 * nelem, s);
+        ke.download(Ke, 576 * nelem, s);
+        sg6.download(sig, 6 * ngp, s);
+        if (commit) {
+            ep.download(epsp, 6 * ngp, s);
+            ab.download(abar, ngp, s);
+            cp.download(cpinv, 6 * ngp, s);
+        }
+        PF_HIP(hipStreamSynchronize(s));
+        return PFEM_OK;
+    } catch (const hip_failure &e) {
+        if (fail) fail->hip = (int32_t)e.code;
+        return PFEM_ERR_HIP;
+    }
+}
+
+int pfem_bench_spmv(pfem_ctx *c, int prec, int reps, double *sec) {
+    if (!c || !sec || reps < 1) return PFEM_ERR_BAD_ARG;
+    try {
+        hipStream_t s = c->stream;
+        const int64_t n = c->ndof;
+        if (prec == 1) {
+            if (!c->K.v32.p) {
+                c->K.v32.alloc(9 * c->K.nslots);
+                convert<double, float>(9 * c->K.nslots, c->K.v64.p, c->K.v32.p, s);
+            }
+            if (!c->f_p.p) { c->f_p.alloc(n); c->f_q.alloc(n); }
+            fill<float>(n, 1.0f, c->f_p.p, s);
+            spmv_fine32(c->K, c->f_p.p, c->f_q.p, s);
+            PF_HIP(hipEventRecord(c->ev0, s));
+            for (int i = 0; i < reps; ++i) spmv_fine32(c->K, c->f_p.p, c->f_q.p, s);
+            PF_HIP(hipEventRecord(c->ev1, s));
+        } else {
+            fill<double>(n, 1.0, c->w_p.p, s);
+            spmv_fine64(c->K, c->w_p.p, c->w_q.p, s);
+            PF_HIP(hipEventRecord(c->ev0, s));
+            for (int i = 0; i < reps; ++i) spmv_fine64(c->K, c->w_p.p, c->w_q.p, s);
+            PF_HIP(hipEventRecord(c->ev1, s));
+        }
+        c->sync();
+        float ms = 0;
+        PF_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        *sec = 1e-3 * ms / reps;
+        c->st.t_spmv = *sec;
+        return PFEM_OK;
+    } catch (const hip_failure &e) {
+        return PFEM_ERR_HIP;
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,614 @@
+// element.hip -- Hex8 finite-strain (F-bar) element integration for gfx950.
+//
+// Two kernels per assembly:
+//   gp_kernel      one thread per Gauss point over all elements.  Nodal X/U of a
+//                  32-element tile are staged in LDS.  Per GP: reference gradients,
+//                  centroid F0/J0 (F-bar), F, Fbar = (J0/J)^(1/3) F, trial be and its
+//                  eigen-decomposition, J2 radial return, P = tau Fbar^-T and the
+//                  9x9 two-point tangent A = dP/dFbar, written (scaled by the
+//                  quadrature weight) to a per-GP packet.  Restates
+//                  element_force_tangent_finite! Elements.jl:405-494 with
+//                  finite_kinematics FiniteStrain.jl:148-170, finite_stress_update
+//                  :292-346, dPdF :503-547 and return_map Materials.jl:159-250.
+//   scatter_kernel 16 threads per element of one colour: thread (a, half) contracts
+//                  the 8 packets into the 3x3 blocks Ke(a, b) for four b and adds them
+//                  into the fine BSR K through a precomputed slot map (no search, no
+//                  atomics: same-colour elements share no node, Assembly.jl:370-385).
+//                  Ke = sum_g w G^T A Geff with Geff = s G + (s/3) vec(F)(g0 - gJ)^T
+//                  (Elements.jl:470-478, :524-531).
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+
+namespace pfem {
+namespace {
+
+constexpr double EIG_TOL = 1e-9;  // FiniteStrain.jl:69
+
+// 3x3 helpers, column-major m[i + 3j]
+__device__ __forceinline__ double det3(const double *m) {
+    return m[0] * (m[4] * m[8] - m[7] * m[5]) - m[3] * (m[1] * m[8] - m[7] * m[2]) +
+           m[6] * (m[1] * m[5] - m[4] * m[2]);
+}
+__device__ __forceinline__ void inv3(const double *m, double *o) {
+    double d = det3(m);
+    o[0] = (m[4] * m[8] - m[7] * m[5]) / d;
+    o[3] = -(m[3] * m[8] - m[6] * m[5]) / d;
+    o[6] = (m[3] * m[7] - m[6] * m[4]) / d;
+    o[1] = -(m[1] * m[8] - m[7] * m[2]) / d;
+    o[4] = (m[0] * m[8] - m[6] * m[2]) / d;
+    o[7] = -(m[0] * m[7] - m[6] * m[1]) / d;
+    o[2] = (m[1] * m[5] - m[4] * m[2]) / d;
+    o[5] = -(m[0] * m[5] - m[3] * m[2]) / d;
+    o[8] = (m[0] * m[4] - m[3] * m[1]) / d;
+}
+
+// Symmetric 3x3 eigen-decomposition by cyclic Jacobi rotations (Golub & Van
+// Loan, Matrix Computations, Alg. 8.5.2/8.5.3).  Input: the 6 distinct
+// entries; output ascending eigenvalues w and eigenvectors as columns of Q.
+__device__ void eig_sym3(double a00, double a11, double a22, double a01, double a12,
+                         double a02, double *w, double *Q) {
+    double A[3][3] = {{a00, a01, a02}, {a01, a11, a12}, {a02, a12, a22}};
+    double V[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int sweep = 0; sweep < 50; ++sweep) {
+        double off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[1][2] * A[1][2];
+        double dia = A[0][0] * A[0][0] + A[1][1] * A[1][1] + A[2][2] * A[2][2];
+        if (off <= 1e-36 * dia || off == 0.0) break;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const int p = (r == 2) ? 1 : 0;
+            const int q = (r == 0) ? 1 : 2;
+            double apq = A[p][q];
+            if (apq == 0.0) continue;
+            double theta = (A[q][q] - A[p][p]) / (2 * apq);
+            double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
+            double c = 1 / sqrt(t * t + 1), s = t * c;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                double akp = A[k][p], akq = A[k][q];
+                A[k][p] = c * akp - s * akq;
+                A[k][q] = s * akp + c * akq;
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                double apk = A[p][k], aqk = A[q][k];
+                A[p][k] = c * apk - s * aqk;
+                A[q][k] = s * apk + c * aqk;
+            }
+            A[p][q] = A[q][p] = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                double vkp = V[k][p], vkq = V[k][q];
+                V[k][p] = c * vkp - s * vkq;
+                V[k][q] = s * vkp + c * vkq;
+            }
+        }
+    }
+    double ev[3] = {A[0][0], A[1][1], A[2][2]};
+    int o0 = 0, o1 = 1, o2 = 2, tmp;
+    if (ev[o0] > ev[o1]) { tmp = o0; o0 = o1; o1 = tmp; }
+    if (ev[o1] > ev[o2]) { tmp = o1; o1 = o2; o2 = tmp; }
+    if (ev[o0] > ev[o1]) { tmp = o0; o0 = o1; o1 = tmp; }
+    const int ord[3] = {o0, o1, o2};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        w[c] = ev[ord[c]];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) Q[k + 3 * c] = V[k][ord[c]];
+    }
+}
+
+// dN/dxi of the trilinear hex at natural point xi (Elements.jl:62-75), node order
+// of NODE_NAT (Elements.jl:26-30).
+__device__ __forceinline__ void hex8_dshape(double x0, double x1, double x2, double *dN) {
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        const double sa = ((a + 1) & 2) ? 1.0 : -1.0;  // xi_a: -,+,+,-,-,+,+,-
+        const double ta = (a & 2) ? 1.0 : -1.0;         // eta_a: -,-,+,+,-,-,+,+
+        const double ua = (a & 4) ? 1.0 : -1.0;         // zeta_a
+        dN[3 * a + 0] = 0.125 * sa * (1 + ta * x1) * (1 + ua * x2);
+        dN[3 * a + 1] = 0.125 * ta * (1 + sa * x0) * (1 + ua * x2);
+        dN[3 * a + 2] = 0.125 * ua * (1 + sa * x0) * (1 + ta * x1);
+    }
+}
+
+// Reference gradients dN/dX = dN J^-1 with J = Xe^T dN (Elements.jl:89, :155-161).
+__device__ __forceinline__ void ref_grads(const double *Xe, double x0, double x1, double x2,
+                                          double *dNdX, double &detJ) {
+    double dN[24];
+    hex8_dshape(x0, x1, x2, dN);
+    double J[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            double s = 0.0;
+#pragma unroll
+            for (int a = 0; a < 8; ++a) s += Xe[3 * a + i] * dN[3 * a + j];
+            J[i + 3 * j] = s;
+        }
+    detJ = det3(J);
+    double Ji[9];
+    inv3(J, Ji);
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            dNdX[3 * a + j] = dN[3 * a + 0] * Ji[0 + 3 * j] + dN[3 * a + 1] * Ji[1 + 3 * j] +
+                              dN[3 * a + 2] * Ji[2 + 3 * j];
+}
+
+__device__ __forceinline__ void defgrad(const double *ue, const double *dNdX, double *F) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) F[k] = (k % 4 == 0) ? 1.0 : 0.0;
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) F[i + 3 * j] += ue[3 * a + i] * dNdX[3 * a + j];
+}
+
+__device__ __forceinline__ double yield_stress(const DevMat &m, double a) {
+    return m.sy0 + (m.ssat - m.sy0) * (1 - exp(-m.delta * a)) + m.Hiso * a;
+}
+__device__ __forceinline__ double yield_slope(const DevMat &m, double a) {
+    return m.delta * (m.ssat - m.sy0) * exp(-m.delta * a) + m.Hiso;
+}
+
+// Divided difference of (1/2) ln at (bA, bB) with the reference's degenerate
+// switch (FiniteStrain.jl:467-476).  Off the switch, 0.5*log1p(d/bB)/d is the
+// same quantity as (0.5 ln bA - 0.5 ln bB)/(bA - bB) without cancellation.
+__device__ __forceinline__ double halflog_dd(double bA, double bB) {
+    if (fabs(bA - bB) < EIG_TOL * (fabs(bA) + fabs(bB) + 1.0)) return 1.0 / (bA + bB);
+    double d = bA - bB;
+    return 0.5 * log1p(d / bB) / d;
+}
+
+__global__ __launch_bounds__(256) void gp_kernel(ElementArgs a, int64_t e_begin, int64_t e_end) {
+    __shared__ double sX[32][24];
+    __shared__ double sU[32][24];
+    const int le = threadIdx.x >> 3, g = threadIdx.x & 7;
+    const int64_t e = e_begin + (int64_t)blockIdx.x * 32 + le;
+    const bool valid = e < e_end;
+    if (valid) {
+        const int64_t node = a.conn[8 * e + g];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            sX[le][3 * g + d] = a.X[3 * node + d];
+            sU[le][3 * g + d] = a.U[3 * node + d];
+        }
+    }
+    __syncthreads();
+    if (!valid) return;
+    const double *Xe = sX[le];
+    const double *ue = sU[le];
+    const DevMat m = a.mats[a.matid ? a.matid[e] : 0];
+    const int64_t gp = 8 * e + g;
+
+    // --- geometry at this GP (Elements.jl:130-161): GAUSS_PTS order :34-42
+    const double gq = 0.57735026918962576451;  // 1/sqrt(3)
+    const double x0 = (((g + 1) & 2) ? gq : -gq), x1 = ((g & 2) ? gq : -gq), x2 = ((g & 4) ? gq : -gq);
+    double dNdX[24], detJ;
+    ref_grads(Xe, x0, x1, x2, dNdX, detJ);
+    const double w = detJ;  // GAUSS_WTS == 1
+
+    // --- F-bar centroid quantities (Elements.jl:421-424, :517-519)
+    double J0 = 1.0, F0i[9];
+    if (a.fbar) {
+        double dNdX0[24], dj0, F0[9];
+        ref_grads(Xe, 0.0, 0.0, 0.0, dNdX0, dj0);
+        defgrad(ue, dNdX0, F0);
+        J0 = det3(F0);
+        inv3(F0, F0i);
+        if (g == 0) {  // g0 = G0^T vec(F0^-T) (Elements.jl:528)
+#pragma unroll
+            for (int n = 0; n < 8; ++n)
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    a.g0[24 * e + 3 * n + k] = dNdX0[3 * n + 0] * F0i[0 + 3 * k] +
+                                               dNdX0[3 * n + 1] * F0i[1 + 3 * k] +
+                                               dNdX0[3 * n + 2] * F0i[2 + 3 * k];
+        }
+    }
+    double F[9];
+    defgrad(ue, dNdX, F);
+    const double Jg = det3(F);
+    const double s = a.fbar ? cbrt(J0 / Jg) : 1.0;  // Elements.jl:437
+    double Fb[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Fb[k] = s * F[k];
+    const double Jb = det3(Fb);
+    if (Jb <= 0) {  // ElementInversionError (FiniteStrain.jl:151, Elements.jl:447)
+        atomicMin(a.fail_key, (unsigned long long)gp);
+        return;
+    }
+
+    // --- trial elastic left Cauchy-Green be = Fb Cpi Fb^T (FiniteStrain.jl:155-156)
+    const double *cp = a.cpinv + 6 * gp;
+    const double c0 = cp[0], c1 = cp[1], c2 = cp[2], c3 = cp[3], c4 = cp[4], c5 = cp[5];
+    double M[9];  // M = Fb * Cpi (column-major)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        M[i + 0] = Fb[i] * c0 + Fb[i + 3] * c3 + Fb[i + 6] * c5;
+        M[i + 3] = Fb[i] * c3 + Fb[i + 3] * c1 + Fb[i + 6] * c4;
+        M[i + 6] = Fb[i] * c5 + Fb[i + 3] * c4 + Fb[i + 6] * c2;
+    }
+    double be[6];  // (00,11,22,01,12,02)
+    {
+        const int I[6] = {0, 1, 2, 0, 1, 0}, Jx[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+        for (int t = 0; t < 6; ++t)
+            be[t] = M[I[t]] * Fb[Jx[t]] + M[I[t] + 3] * Fb[Jx[t] + 3] + M[I[t] + 6] * Fb[Jx[t] + 6];
+    }
+    double b[3], Q[9];
+    eig_sym3(be[0], be[1], be[2], be[3], be[4], be[5], b, Q);
+    const double l0 = 0.5 * log(b[0]), l1 = 0.5 * log(b[1]), l2 = 0.5 * log(b[2]);
+    double ee[6];  // trial Hencky strain, engineering shear (FiniteStrain.jl:162-168)
+    {
+        const int I[6] = {0, 1, 2, 0, 1, 0}, Jx[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+        for (int t = 0; t < 6; ++t) {
+            double v = l0 * Q[I[t]] * Q[Jx[t]] + l1 * Q[I[t] + 3] * Q[Jx[t] + 3] +
+                       l2 * Q[I[t] + 6] * Q[Jx[t] + 6];
+            ee[t] = (t < 3) ? v : 2 * v;
+        }
+    }
+
+    // --- J2 radial return with zero additive plastic strain and zero back stress
+    //     (Materials.jl:159-250 as called at FiniteStrain.jl:318)
+    const double G = m.G, lam = m.lam;
+    const double tr = ee[0] + ee[1] + ee[2];
+    double tau[6];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) tau[k] = lam * tr + 2 * G * ee[k];
+#pragma unroll
+    for (int k = 3; k < 6; ++k) tau[k] = G * ee[k];
+    const double p = (tau[0] + tau[1] + tau[2]) / 3;
+    double xi[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) xi[k] = tau[k] - (k < 3 ? p : 0.0);
+    const double nrm = sqrt(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2] +
+                            2 * (xi[3] * xi[3] + xi[4] * xi[4] + xi[5] * xi[5]));
+    const double s32 = 1.2247448713915890491;  // sqrt(3/2)
+    const double qtr = s32 * nrm;
+    const double an = a.abar[gp];
+    const double ftr = qtr - yield_stress(m, an);
+    double cdev = 0.0, cnn = 0.0, dg = 0.0;  // D = C - cdev*Idev + cnn*nh nh^T
+    double nh[6] = {0, 0, 0, 0, 0, 0};
+    const bool plastic = ftr > 0.0;
+    if (plastic) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) nh[k] = xi[k] / nrm;
+        if (m.delta > 0 && m.ssat > m.sy0) {  // Voce local Newton (Materials.jl:201-213)
+            dg = ftr / (3 * G + yield_slope(m, an));
+            for (int it = 0; it < 50; ++it) {
+                double Hp = yield_slope(m, an + dg);
+                double r = qtr - 3 * G * dg - yield_stress(m, an + dg);
+                double ddg = r / (3 * G + Hp);
+                dg += ddg;
+                if (fabs(ddg) <= 1e-13 * (dg + 1e-14)) break;
+            }
+        } else {
+            dg = ftr / (3 * G + m.Hiso);  // Materials.jl:218
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) tau[k] -= (2 * G * dg * s32) * nh[k];
+        const double Hp = yield_slope(m, an + dg);
+        const double b0 = 2 * G * dg / qtr, g0c = 2 * G / (3 * G + Hp);
+        cdev = 3 * G * b0;
+        cnn = 3 * G * (b0 - g0c);
+    }
+    // --- P = tau Fb^-T (FiniteStrain.jl:560-561)
+    double Fbi[9];
+    inv3(Fb, Fbi);
+    double T[9] = {tau[0], tau[3], tau[5], tau[3], tau[1], tau[4], tau[5], tau[4], tau[2]};
+    double Fit[9];  // Fb^-T
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Fit[r + 3 * c] = Fbi[c + 3 * r];
+    double P[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            P[i + 3 * j] = T[i] * Fit[0 + 3 * j] + T[i + 3] * Fit[1 + 3 * j] + T[i + 6] * Fit[2 + 3 * j];
+
+    double *pk = a.packet + (size_t)gp * PK;
+    const double sw = s * w;
+
+    // --- A = dP/dFbar, one column per unit perturbation dF = e_p e_q^T
+    //     (FiniteStrain.jl:503-547).  dbe = e_p m^T + m e_p^T with m = (Fb Cpi)[:,q].
+    double gam[3][3];
+#pragma unroll
+    for (int A_ = 0; A_ < 3; ++A_)
+#pragma unroll
+        for (int B_ = 0; B_ < 3; ++B_)
+            gam[A_][B_] = (A_ == B_) ? 1.0 / (2 * b[A_]) : halflog_dd(b[A_], b[B_]);
+    double qv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll 1
+    for (int col = 0; col < 9; ++col) {
+        const int pp = col % 3, qq = col / 3;
+        double u[3], v[3];
+#pragma unroll
+        for (int A_ = 0; A_ < 3; ++A_) {
+            u[A_] = Q[pp + 3 * A_];
+            v[A_] = Q[0 + 3 * A_] * M[0 + 3 * qq] + Q[1 + 3 * A_] * M[1 + 3 * qq] +
+                    Q[2 + 3 * A_] * M[2 + 3 * qq];
+        }
+        double Y[3][3];  // gamma o (Q^T dbe Q)
+#pragma unroll
+        for (int A_ = 0; A_ < 3; ++A_)
+#pragma unroll
+            for (int B_ = 0; B_ < 3; ++B_) Y[A_][B_] = gam[A_][B_] * (u[A_] * v[B_] + v[A_] * u[B_]);
+        double QY[9];  // Q * Y
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int B_ = 0; B_ < 3; ++B_)
+                QY[i + 3 * B_] = Q[i] * Y[0][B_] + Q[i + 3] * Y[1][B_] + Q[i + 6] * Y[2][B_];
+        double de[6];  // d eps_e, engineering shear (FiniteStrain.jl:485)
+        {
+            const int I[6] = {0, 1, 2, 0, 1, 0}, Jx[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+            for (int t = 0; t < 6; ++t) {
+                double vv = QY[I[t]] * Q[Jx[t]] + QY[I[t] + 3] * Q[Jx[t] + 3] + QY[I[t] + 6] * Q[Jx[t] + 6];
+                de[t] = (t < 3) ? vv : 2 * vv;
+            }
+        }
+        // d tau = D de with D = C - cdev*Idev + cnn*nh nh^T (Materials.jl:247)
+        const double trd = de[0] + de[1] + de[2];
+        double nde = 0.0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) nde += nh[k] * de[k];
+        double dt[6];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            dt[k] = lam * trd + 2 * G * de[k] - cdev * (de[k] - trd / 3) + cnn * nde * nh[k];
+#pragma unroll
+        for (int k = 3; k < 6; ++k) dt[k] = G * de[k] - cdev * 0.5 * de[k] + cnn * nde * nh[k];
+        const double dT[9] = {dt[0], dt[3], dt[5], dt[3], dt[1], dt[4], dt[5], dt[4], dt[2]};
+        // dP = dtau Fb^-T - (tau Fb^-T[:,q]) (Fb^-T[p,:])   (FiniteStrain.jl:539-540)
+        double wq[3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+            wq[r] = T[r] * Fit[0 + 3 * qq] + T[r + 3] * Fit[1 + 3 * qq] + T[r + 6] * Fit[2 + 3 * qq];
+        const double fcol = F[col];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                double dP = dT[r] * Fit[0 + 3 * c] + dT[r + 3] * Fit[1 + 3 * c] + dT[r + 6] * Fit[2 + 3 * c] -
+                            wq[r] * Fit[pp + 3 * c];
+                pk[(r + 3 * c) + 9 * col] = sw * dP;
+                qv[r + 3 * c] += dP * fcol;
+            }
+    }
+    if (a.fbar) {
+        double Fi[9];
+        inv3(F, Fi);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            pk[81 + k] = (sw / 3) * qv[k];
+            pk[90 + k] = Fi[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) pk[99 + k] = w * P[k];
+#pragma unroll
+    for (int k = 0; k < 24; ++k) pk[108 + k] = dNdX[k];
+
+    // --- Kirchhoff stress out (Elements.jl:481-482)
+    double *sg = a.sig + 6 * gp;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) sg[k] = tau[k];
+
+    if (a.commit) {  // exp-map plastic update (FiniteStrain.jl:329-343), Elements.jl:483-491
+        double dep[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) dep[k] = (dg * s32) * (k < 3 ? nh[k] : 2 * nh[k]);
+        double ev[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) ev[k] = ee[k] - dep[k];
+        double wv[3], V[9];
+        eig_sym3(2 * ev[0], 2 * ev[1], 2 * ev[2], ev[3], ev[4], ev[5], wv, V);
+        const double x0e = exp(wv[0]), x1e = exp(wv[1]), x2e = exp(wv[2]);
+        double Bn[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                Bn[i + 3 * j] = x0e * V[i] * V[j] + x1e * V[i + 3] * V[j + 3] + x2e * V[i + 6] * V[j + 6];
+        double Tm[9];  // Fbi * Bn
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                Tm[i + 3 * j] = Fbi[i] * Bn[0 + 3 * j] + Fbi[i + 3] * Bn[1 + 3 * j] + Fbi[i + 6] * Bn[2 + 3 * j];
+        double cn[6];
+        {
+            const int I[6] = {0, 1, 2, 0, 1, 0}, Jx[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+            for (int t = 0; t < 6; ++t)
+                cn[t] = Tm[I[t]] * Fbi[Jx[t]] + Tm[I[t] + 3] * Fbi[Jx[t] + 3] + Tm[I[t] + 6] * Fbi[Jx[t] + 6];
+        }
+        double *ep = a.epsp + 6 * gp;
+        double *cw = a.cpinv + 6 * gp;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            ep[k] += dep[k];
+            cw[k] = cn[k];
+        }
+        a.abar[gp] = an + dg;
+    }
+}
+
+// Contraction of the 8 GP packets of an element into 3x3 blocks Ke(a, b),
+// b in [4*half, 4*half+4), plus the residual part of node a (half == 0).
+__device__ __forceinline__ void contract_blocks(const double *__restrict__ pe,
+                                                const double *__restrict__ g0e, int a, int half,
+                                                int fbar, double acc[4][9], double fe[3]) {
+#pragma unroll
+    for (int bb = 0; bb < 4; ++bb)
+#pragma unroll
+        for (int t = 0; t < 9; ++t) acc[bb][t] = 0.0;
+    fe[0] = fe[1] = fe[2] = 0.0;
+    double g0b[4][3];
+#pragma unroll
+    for (int bb = 0; bb < 4; ++bb)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) g0b[bb][k] = fbar ? g0e[3 * (4 * half + bb) + k] : 0.0;
+#pragma unroll 1
+    for (int g = 0; g < 8; ++g) {
+        const double *pk = pe + g * PK;
+        const double dNa0 = pk[108 + 3 * a], dNa1 = pk[109 + 3 * a], dNa2 = pk[110 + 3 * a];
+        double Ua[3][9];  // Ua[i][(k,l)] = sum_j dNa_j SA[(i,j),(k,l)]
+#pragma unroll
+        for (int c = 0; c < 9; ++c)
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                Ua[i][c] = dNa0 * pk[i + 9 * c] + dNa1 * pk[i + 3 + 9 * c] + dNa2 * pk[i + 6 + 9 * c];
+        double h[3] = {0, 0, 0}, Fi[9];
+        if (fbar) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) h[i] = dNa0 * pk[81 + i] + dNa1 * pk[84 + i] + dNa2 * pk[87 + i];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) Fi[k] = pk[90 + k];
+        }
+#pragma unroll
+        for (int bb = 0; bb < 4; ++bb) {
+            const int bn = 4 * half + bb;
+            const double d0 = pk[108 + 3 * bn], d1 = pk[109 + 3 * bn], d2 = pk[110 + 3 * bn];
+            double gd[3] = {0, 0, 0};
+            if (fbar) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    gd[k] = g0b[bb][k] - (d0 * Fi[0 + 3 * k] + d1 * Fi[1 + 3 * k] + d2 * Fi[2 + 3 * k]);
+            }
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    acc[bb][3 * i + k] += Ua[i][k] * d0 + Ua[i][k + 3] * d1 + Ua[i][k + 6] * d2 + h[i] * gd[k];
+        }
+        if (half == 0) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                fe[i] += dNa0 * pk[99 + i] + dNa1 * pk[102 + i] + dNa2 * pk[105 + i];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void scatter_kernel(const double *__restrict__ packet,
+                                                      const double *__restrict__ g0,
+                                                      const int64_t *__restrict__ elist, int64_t count,
+                                                      const int64_t *__restrict__ conn,
+                                                      const int32_t *__restrict__ slots,
+                                                      double *__restrict__ Kv, double *__restrict__ R,
+                                                      int fbar) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t li = t >> 4;
+    if (li >= count) return;
+    const int sub = (int)(t & 15), a = sub >> 1, half = sub & 1;
+    const int64_t e = elist[li];
+    double acc[4][9], fe[3];
+    contract_blocks(packet + (size_t)e * 8 * PK, g0 + 24 * e, a, half, fbar, acc, fe);
+#pragma unroll
+    for (int bb = 0; bb < 4; ++bb) {
+        const int64_t pos = slots[64 * e + 8 * a + 4 * half + bb];
+        double *kb = Kv + 9 * pos;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) kb[k] += acc[bb][k];
+    }
+    if (half == 0) {
+        const int64_t n = conn[8 * e + a];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) R[3 * n + i] += fe[i];
+    }
+}
+
+__global__ __launch_bounds__(256) void dense_ke_kernel(const double *__restrict__ packet,
+                                                       const double *__restrict__ g0, int64_t nelem,
+                                                       double *__restrict__ Fe, double *__restrict__ Ke,
+                                                       int fbar) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t e = t >> 4;
+    if (e >= nelem) return;
+    const int sub = (int)(t & 15), a = sub >> 1, half = sub & 1;
+    double acc[4][9], fe[3];
+    contract_blocks(packet + (size_t)e * 8 * PK, g0 + 24 * e, a, half, fbar, acc, fe);
+#pragma unroll
+    for (int bb = 0; bb < 4; ++bb) {
+        const int bn = 4 * half + bb;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                Ke[576 * e + (3 * a + i) + 24 * (3 * bn + k)] = acc[bb][3 * i + k];
+    }
+    if (half == 0) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) Fe[24 * e + 3 * a + i] = fe[i];
+    }
+}
+
+// det(Fbar) of one Gauss point, for the ElementInversionError report.
+__global__ void detF_kernel(ElementArgs a, int64_t gp, double *out) {
+    const int64_t e = gp / 8;
+    const int g = (int)(gp % 8);
+    double Xe[24], ue[24];
+    for (int n = 0; n < 8; ++n)
+        for (int d = 0; d < 3; ++d) {
+            const int64_t node = a.conn[8 * e + n];
+            Xe[3 * n + d] = a.X[3 * node + d];
+            ue[3 * n + d] = a.U[3 * node + d];
+        }
+    const double gq = 0.57735026918962576451;
+    double dNdX[24], dj, F[9];
+    ref_grads(Xe, (((g + 1) & 2) ? gq : -gq), ((g & 2) ? gq : -gq), ((g & 4) ? gq : -gq), dNdX, dj);
+    defgrad(ue, dNdX, F);
+    double s = 1.0;
+    if (a.fbar) {
+        double dNdX0[24], F0[9];
+        ref_grads(Xe, 0.0, 0.0, 0.0, dNdX0, dj);
+        defgrad(ue, dNdX0, F0);
+        s = cbrt(det3(F0) / det3(F));
+    }
+    for (int k = 0; k < 9; ++k) F[k] *= s;
+    *out = det3(F);
+}
+
+}  // namespace
+
+void launch_gp_kernel(const ElementArgs &a, int64_t e_begin, int64_t e_count, hipStream_t s) {
+    if (e_count <= 0) return;
+    const int64_t blocks = (e_count + 31) / 32;
+    hipLaunchKernelGGL(gp_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a, e_begin, e_begin + e_count);
+    PF_HIP(hipGetLastError());
+}
+
+void launch_scatter_kernel(const ElementArgs &a, const int64_t *elem_list, int64_t count,
+                           const int32_t *slots, double *Kv, double *R, hipStream_t s) {
+    if (count <= 0) return;
+    const int64_t threads = 16 * count;
+    const int64_t blocks = (threads + 255) / 256;
+    hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a.packet, a.g0, elem_list,
+                       count, a.conn, slots, Kv, R, a.fbar);
+    PF_HIP(hipGetLastError());
+}
+
+void launch_dense_ke_kernel(const ElementArgs &a, int64_t nelem, double *Fe, double *Ke, hipStream_t s) {
+    if (nelem <= 0) return;
+    const int64_t blocks = (16 * nelem + 255) / 256;
+    hipLaunchKernelGGL(dense_ke_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a.packet, a.g0, nelem, Fe,
+                       Ke, a.fbar);
+    PF_HIP(hipGetLastError());
+}
+
+void launch_detF_kernel(const ElementArgs &a, int64_t gp, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(detF_kernel, dim3(1), dim3(1), 0, s, a, gp, out);
+    PF_HIP(hipGetLastError());
+}
+
+}  // namespace pfem

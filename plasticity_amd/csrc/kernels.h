@@ -1,0 +1,39 @@
+// kernels.h -- host-side launchers of the device kernels (internal).
+#pragma once
+#include "common.h"
+
+namespace pfem {
+
+// sparse.hip
+void spmv_fine64(const FineMatrix &K, const double *x, double *y, hipStream_t s);
+void spmv_fine32(const FineMatrix &K, const float *x, float *y, hipStream_t s);
+void spmv_fine64_dot(const FineMatrix &K, const double *x, double *y, double *partial, double *out,
+                     hipStream_t s);
+void spmv_fine32_dot(const FineMatrix &K, const float *x, float *y, double *partial, double *out,
+                     hipStream_t s);
+void residual_fine64(const FineMatrix &K, const double *x, const double *b, double *r, hipStream_t s);
+size_t reduce_partials_needed(int64_t nb);
+template <class A, class B>
+void dot_dev(int64_t n, const A *a, const B *b, double *partial, double *out, hipStream_t s);
+template <class T>
+void axpy_ratio(int64_t n, const double *num, const double *den, double sign, const T *x, T *y, hipStream_t s);
+template <class T>
+void xpby_ratio(int64_t n, const double *num, const double *den, const T *z, T *p, hipStream_t s);
+template <class T>
+void cg_update(int64_t n, const double *num, const double *den, const T *p, const T *q, T *x, T *r,
+               hipStream_t s);
+template <class T>
+void axpby(int64_t n, double a, const T *x, double b, T *y, hipStream_t s);
+template <class S, class D>
+void convert(int64_t n, const S *src, D *dst, hipStream_t s);
+template <class T>
+void fill(int64_t n, T v, T *d, hipStream_t s);
+void sub_external(int64_t n, double lambda, const double *fext, double *R, hipStream_t s);
+void impose_dirichlet_dev(FineMatrix &K, int64_t nbc, const int64_t *bcdofs, const uint8_t *flag,
+                          const double *val, double lambda, const double *U, double *dg, int64_t nrows,
+                          const int32_t *rows, double *R, hipStream_t s);
+void block_diag_inverse3(const FineMatrix &K, double *dinv, hipStream_t s);
+template <class T>
+void block_jacobi3(int64_t nb, const double *dinv, const T *r, T *z, hipStream_t s);
+
+}  // namespace pfem

@@ -1,0 +1,112 @@
+"""GPU element kernels (gp_kernel + tangent contraction, through pfem_element_batch)
+against the oracle: Fe, Ke, Kirchhoff stress and committed history within 1e-10
+relative; FD gate on the GPU tangent; inversion error; degenerate-eigenvalue branch."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import plasticity_amd as pa
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+MAT = dict(E=200e9, nu=0.3, sy0=250e6, Hiso=1e9)
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
+
+
+def random_states(n, seed, fbar, plastic=True, distort=0.06):
+    rng = np.random.default_rng(seed)
+    mat = O.Material(**MAT)
+    nodes, conn = O.box_mesh(1, 1, 1, 1, 1, 1)
+    X0 = nodes[conn[0]]
+    Xs, ues, eps, abs_, cps = [], [], [], [], []
+    for _ in range(n):
+        X = X0 * (0.5 + rng.random(3)) + distort * rng.normal(size=(8, 3))
+        F1 = np.eye(3) + 0.03 * rng.normal(size=(3, 3))
+        F2 = F1 + 0.02 * rng.normal(size=(3, 3))
+        e, a, c = np.zeros((8, 6)), np.zeros(8), np.tile([1.0, 1, 1, 0, 0, 0], (8, 1))
+        if plastic:
+            O.element_finite(fbar, mat, X.ravel(), (X @ F1.T - X).ravel(), e, a, c, commit=True)
+        Xs.append(X.ravel())
+        ues.append((X @ F2.T - X).ravel() + 1e-3 * rng.normal(size=24))
+        eps.append(e); abs_.append(a); cps.append(c)
+    return np.array(Xs), np.array(ues), np.array(eps), np.array(abs_), np.array(cps)
+
+
+@pytest.mark.parametrize("fbar", [0, 1])
+def test_element_batch_matches_oracle(fbar, need_gpu):
+    print(f"[element batch] fbar={fbar} n=64", flush=True)
+    X, ue, ep, ab, cp = random_states(64, 11 + fbar, fbar)
+    mat = O.Material(**MAT)
+    Fe, Ke, sig = pa.element_batch("finite_fbar" if fbar else "finite", pa.J2Material(**MAT), X, ue,
+                                   ep.copy(), ab.copy(), cp.copy())
+    worst = 0.0
+    for e in range(X.shape[0]):
+        f, K, s = O.element_finite(fbar, mat, X[e], ue[e], ep[e].copy(), ab[e].copy(), cp[e].copy())
+        worst = max(worst, rel(Fe[e], f), rel(Ke[e], K), rel(sig[e], s))
+    print(f"  worst relative difference {worst:.2e}")
+    assert worst < 1e-10
+
+
+def test_element_batch_commit_matches_oracle(need_gpu):
+    X, ue, ep, ab, cp = random_states(32, 5, 1)
+    mat = O.Material(**MAT)
+    e2, a2, c2 = ep.copy(), ab.copy(), cp.copy()
+    pa.element_batch("finite_fbar", pa.J2Material(**MAT), X, ue, e2, a2, c2, commit=True)
+    for e in range(X.shape[0]):
+        eo, ao, co = ep[e].copy(), ab[e].copy(), cp[e].copy()
+        O.element_finite(1, mat, X[e], ue[e], eo, ao, co, commit=True)
+        assert rel(a2[e], ao) < 1e-10 and rel(c2[e], co) < 1e-12 and rel(e2[e], eo) < 1e-10
+
+
+def test_golden_elements(need_gpu):
+    g = np.load(os.path.join(HERE, "golden", "elements.npz"))
+    Fe, Ke, sig = pa.element_batch("finite_fbar", pa.J2Material(**MAT), g["X"], g["ue"], g["epsp"].copy(),
+                                   g["abar"].copy(), g["cpinv"].copy())
+    assert rel(Fe, g["Fe"]) < 1e-10 and rel(Ke, g["Ke"]) < 1e-10 and rel(sig, g["tau"]) < 1e-10
+
+
+def test_gpu_tangent_vs_fd_of_gpu_force(need_gpu):
+    """F2 master gate on the device kernels: Ke vs central differences of Fe (<1e-6)."""
+    X, ue, ep, ab, cp = random_states(1, 3, 1)
+    mat = pa.J2Material(**MAT)
+    _, Ke, _ = pa.element_batch("finite_fbar", mat, X, ue, ep.copy(), ab.copy(), cp.copy())
+    h = 1e-9 * np.abs(ue).max() * 100
+    Up = np.repeat(ue, 24, axis=0) + h * np.eye(24)
+    Um = np.repeat(ue, 24, axis=0) - h * np.eye(24)
+    rep = lambda a: np.ascontiguousarray(np.repeat(a, 24, axis=0))  # noqa: E731
+    Fp, _, _ = pa.element_batch("finite_fbar", mat, rep(X), Up, rep(ep), rep(ab), rep(cp))
+    Fm, _, _ = pa.element_batch("finite_fbar", mat, rep(X), Um, rep(ep), rep(ab), rep(cp))
+    Kfd = ((Fp - Fm) / (2 * h)).T
+    assert rel(Ke[0], Kfd) < 1e-6
+
+
+def test_gpu_inversion_error(need_gpu):
+    nodes, conn = O.box_mesh(1, 1, 1, 1, 1, 1)
+    X = nodes[conn[0]]
+    ue = (X @ np.diag([-1.1, 1.0, 1.0]).T - X).ravel()
+    for kind in ("finite", "finite_fbar"):
+        with pytest.raises(pa.ElementInversionError) as ei:
+            pa.element_batch(kind, pa.J2Material(**MAT), X.ravel()[None], ue[None])
+        assert ei.value.J < 0 and ei.value.e == 0
+
+
+def test_degenerate_eigenvalue_branch(need_gpu):
+    """Homogeneous uniaxial states have exactly repeated lateral eigenvalues: the
+    EIG_TOL switch of _dhalflog_contract (FiniteStrain.jl:471-472) is taken."""
+    mat = O.Material(**MAT)
+    nodes, conn = O.box_mesh(1, 1, 1, 1, 1, 1)
+    X = nodes[conn[0]]
+    Xs, ues = [], []
+    for s in (1.001, 1.01, 1.05, 0.97):
+        F = np.diag([s, 1 - 0.4 * (s - 1), 1 - 0.4 * (s - 1)])
+        Xs.append(X.ravel())
+        ues.append((X @ F.T - X).ravel())
+    Fe, Ke, sig = pa.element_batch("finite_fbar", pa.J2Material(**MAT), np.array(Xs), np.array(ues))
+    for e in range(len(Xs)):
+        f, K, s = O.element_finite(1, mat, Xs[e], ues[e])
+        assert rel(Ke[e], K) < 1e-10 and rel(Fe[e], f) < 1e-10
