@@ -91,6 +91,7 @@ def main():
     ap.add_argument("--n", type=int, default=100, help="elements per edge")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_ir"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--amg-rebuild", type=int, default=1, help="0: once per load step, 1: every Newton iteration")
     ap.add_argument("--verbose", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -104,7 +105,7 @@ def main():
     wl = workload(1, n=args.n)
     log(f"[bench] workload: {wl.name}, nodes {wl.mesh.nnodes}, dof {3 * wl.mesh.nnodes}")
     opts = pa.default_options(eta_min=wl.krylov_rtol, eta_max=wl.krylov_rtol, precision=args.precision,
-                              verbose=args.verbose)
+                              verbose=args.verbose, amg_rebuild=args.amg_rebuild)
     t0 = time.perf_counter()
     m = pa.Model(wl.mesh, pa.J2Material(**wl.materials[0]), options=opts)
     m.constrain(wl.dir_dofs, wl.dir_vals, wl.dir_ramp)
@@ -165,7 +166,8 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f32+f64-IR",
         "data": "synthetic (generated jittered 100^3 Hex8 mesh)",
         "config": {"workload": f"{wl.name}: {wl.mesh.nelem} Hex8 F-bar elements, {ndof} DOF, "
-                               f"PCG+SA-AMG rtol {wl.krylov_rtol:g}, precision {args.precision}",
+                               f"PCG+SA-AMG rtol {wl.krylov_rtol:g}, precision {args.precision}, "
+                               f"AMG rebuilt {'every Newton iteration' if args.amg_rebuild else 'once per load step'}",
                    "newton_steps_timed": newton, "krylov_its_timed": krylov,
                    "t_assemble_s": t_asm, "t_amg_setup_s": t_setup, "t_solve_s": t_solve,
                    "amg_levels": list(s1.amg_ndof[:s1.amg_levels])},

@@ -79,7 +79,8 @@ typedef struct pfem_options {
     int32_t coarse_max;         /* coarsest-level size target in DOF (default 1500) */
     int32_t max_levels;         /* AMG level cap (default 10) */
     int32_t verbose;            /* print per-iteration lines to stderr */
-    int32_t reserved[7];
+    int32_t reserved[7];        /* reserved[0]: AMG rebuild policy, 0 = once per load step
+                                   (Solver.jl:374), 1 = every Newton iteration */
 } pfem_options;
 
 typedef struct pfem_fail {
@@ -137,7 +138,7 @@ int pfem_residual_norm(pfem_ctx *ctx, double *out);
 /* _linear_solve!(dU, K, b = -R, ls, rtol, rebuild_pc) (Solver.jl:235-290).
  * dU stays on the device; niter = Krylov iterations (all attempts);
  * solver_used: 0 PCG, 1 PCG after AMG refresh, 2 BiCGStab fallback,
- * 3 FP32-IR.  true_relres = ||b - K dU|| / ||b||. */
+ * 3 FP32-IR with PCG, 4 FP32-IR with BiCGStab.  true_relres = ||b - K dU|| / ||b||. */
 int pfem_linear_solve(pfem_ctx *ctx, double rtol, int rebuild_pc, int *niter,
                       int *solver_used, double *true_relres);
 

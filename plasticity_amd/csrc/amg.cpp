@@ -328,69 +328,21 @@ double spectral_radius(pfem_ctx *c, Amg *a, int l, int iters) {
     return lam > 0 ? lam : 1.0;
 }
 
-// Dense inverse of the coarsest operator on the host (Gauss-Jordan, partial
-// pivoting); zero rows (dropped rigid-body columns) are decoupled with a unit
-// diagonal.
+// Dense inverse of the coarsest operator, computed on the device.
 void coarse_inverse(pfem_ctx *c, Amg *a) {
-    hipStream_t s = c->stream;
     AmgLevel &L = a->L.back();
     const int64_t n = L.n;
-    std::vector<double> M((size_t)n * n, 0.0);
-    if (a->L.size() == 1) {  // the fine level itself is the coarsest
-        std::vector<double> hv(9 * c->K.nslots);
-        c->K.v64.download(hv.data(), hv.size(), s);
-        c->sync();
-        const FineMatrix &K = c->K;
-        for (int64_t i = 0; i < K.nb; ++i) {
-            const int64_t base = K.h_slice_ptr[i / SELL_C] + i % SELL_C;
-            for (int k = 0; k < K.h_rowlen[i]; ++k) {
-                const int64_t pos = base + (int64_t)k * SELL_C;
-                const int64_t j = K.h_colind[pos];
-                for (int r = 0; r < 3; ++r)
-                    for (int q = 0; q < 3; ++q) M[(3 * i + r) * n + 3 * j + q] = hv[9 * pos + 3 * r + q];
-            }
-        }
-    } else {
-        std::vector<double> hv(36 * L.nnzb);
-        L.A.download(hv.data(), hv.size(), s);
-        c->sync();
-        for (int64_t i = 0; i < L.nb; ++i)
-            for (int32_t k = L.h_rowptr[i]; k < L.h_rowptr[i + 1]; ++k) {
-                const int64_t j = L.h_colind[k];
-                for (int r = 0; r < 6; ++r)
-                    for (int q = 0; q < 6; ++q) M[(6 * i + r) * n + 6 * j + q] = hv[36 * k + 6 * r + q];
-            }
+    if (!a->gj_work.p) {
+        a->gj_work.alloc((size_t)2 * n * n);
+        a->gj_prow.alloc(2 * n);
+        a->gj_fcol.alloc(n);
     }
-    for (int64_t r = 0; r < n; ++r) {
-        bool zero = true;
-        for (int64_t q = 0; q < n && zero; ++q) zero = M[r * n + q] == 0.0;
-        if (zero) M[r * n + r] = 1.0;
-    }
-    std::vector<double> I((size_t)n * n, 0.0);
-    for (int64_t r = 0; r < n; ++r) I[r * n + r] = 1.0;
-    for (int64_t col = 0; col < n; ++col) {
-        int64_t piv = col;
-        double best = std::fabs(M[col * n + col]);
-        for (int64_t r = col + 1; r < n; ++r)
-            if (std::fabs(M[r * n + col]) > best) { best = std::fabs(M[r * n + col]); piv = r; }
-        if (piv != col) {
-            std::swap_ranges(M.begin() + col * n, M.begin() + (col + 1) * n, M.begin() + piv * n);
-            std::swap_ranges(I.begin() + col * n, I.begin() + (col + 1) * n, I.begin() + piv * n);
-        }
-        const double d = M[col * n + col];
-        const double id = d != 0.0 ? 1.0 / d : 0.0;
-        double *mc = &M[col * n], *ic = &I[col * n];
-        for (int64_t q = 0; q < n; ++q) { mc[q] *= id; ic[q] *= id; }
-        for (int64_t r = 0; r < n; ++r) {
-            if (r == col) continue;
-            const double f = M[r * n + col];
-            if (f == 0.0) continue;
-            double *mr = &M[r * n], *ir = &I[r * n];
-            for (int64_t q = 0; q < n; ++q) { mr[q] -= f * mc[q]; ir[q] -= f * ic[q]; }
-        }
-    }
-    a->coarse_inv.upload(I.data(), (size_t)n * n, s);
-    c->sync();
+    if (a->L.size() == 1)
+        dense_coarse_inverse(L.nb, 3, true, c->K.slice_ptr.p, c->K.rowlen.p, nullptr, c->K.colind.p, c->K.v64.p,
+                             a->gj_work.p, a->gj_prow.p, a->gj_fcol.p, a->coarse_inv.p, c->stream);
+    else
+        dense_coarse_inverse(L.nb, 6, false, nullptr, nullptr, L.rowptr.p, L.colind.p, L.A.p, a->gj_work.p,
+                             a->gj_prow.p, a->gj_fcol.p, a->coarse_inv.p, c->stream);
 }
 
 }  // namespace

@@ -55,7 +55,7 @@ struct Amg {
     // coarsest level: dense inverse (FP64)
     int64_t nc = 0;
     dbuf<double> coarse_inv;
-    std::vector<double> h_dense;
+    dbuf<double> gj_work, gj_prow, gj_fcol;  // Gauss-Jordan scratch
     int smoother = PFEM_SMOOTH_CHEBYSHEV;
     int degree = 2;
     bool have32 = false;
@@ -103,5 +103,10 @@ void cheb_step(int64_t nb, const DT *dinv, const XT *Ad, XT *r, XT *d, XT *x, do
                hipStream_t s);  // r -= Ad; d = c1 d + c2 Dinv r; x += d
 template <class XT>
 void dense_gemv(int64_t n, const double *Ainv, const XT *b, XT *x, hipStream_t s);
+// dense inverse of a block-sparse operator (SELL fine level or CSR of blocks) by
+// Gauss-Jordan with partial pivoting; W is 2 n^2, prow 2n, fcol n scratch
+void dense_coarse_inverse(int64_t nb, int bs, bool sell, const int64_t *slice_ptr, const int32_t *rowlen,
+                          const int32_t *rowptr, const int32_t *colind, const double *A, double *W, double *prow,
+                          double *fcol, double *inv, hipStream_t s);
 
 }  // namespace pfem

@@ -337,6 +337,106 @@ __global__ void dense_gemv_kernel(int64_t n, const double *__restrict__ Ainv, co
     if (lane == 0) x[row] = (XT)s;
 }
 
+
+// ---- coarsest level: dense [M | I] Gauss-Jordan with partial pivoting on the device
+template <bool SELL>
+__global__ void dense_from_blocks_kernel(int64_t nb, int bs, const int64_t *__restrict__ slice_ptr,
+                                         const int32_t *__restrict__ rowlen, const int32_t *__restrict__ rowptr,
+                                         const int32_t *__restrict__ colind, const double *__restrict__ A,
+                                         int64_t n, double *__restrict__ W) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    RowIt<SELL> row(i, slice_ptr, rowlen, rowptr);
+    for (int k = 0; k < row.len; ++k) {
+        const int64_t pos = row.pos(k);
+        const int64_t j = colind[pos];
+        const double *a = A + (int64_t)bs * bs * pos;
+        for (int r = 0; r < bs; ++r)
+            for (int q = 0; q < bs; ++q) W[(bs * i + r) * 2 * n + bs * j + q] = a[bs * r + q];
+    }
+}
+__global__ void dense_prepare_kernel(int64_t n, double *__restrict__ W) {
+    // unit right half; zero rows (dropped coarse dofs) decoupled with a unit diagonal
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    double *row = W + r * 2 * n;
+    bool zero = true;
+    for (int64_t c = 0; c < n && zero; ++c) zero = row[c] == 0.0;
+    if (zero) row[r] = 1.0;
+    for (int64_t c = 0; c < n; ++c) row[n + c] = (c == r) ? 1.0 : 0.0;
+}
+__global__ __launch_bounds__(1024) void gj_pivot_kernel(int64_t n, int64_t k, double *__restrict__ W,
+                                                        double *__restrict__ prow, double *__restrict__ fcol) {
+    __shared__ double bv[1024];
+    __shared__ int64_t bi[1024];
+    const int t = threadIdx.x;
+    double best = -1.0;
+    int64_t arg = k;
+    for (int64_t r = k + t; r < n; r += 1024) {
+        const double v = fabs(W[r * 2 * n + k]);
+        if (v > best) { best = v; arg = r; }
+    }
+    bv[t] = best;
+    bi[t] = arg;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if (t < off && (bv[t + off] > bv[t] || (bv[t + off] == bv[t] && bi[t + off] < bi[t]))) {
+            bv[t] = bv[t + off];
+            bi[t] = bi[t + off];
+        }
+        __syncthreads();
+    }
+    const int64_t p = bi[0];
+    if (p != k)
+        for (int64_t c = t; c < 2 * n; c += 1024) {
+            const double a = W[k * 2 * n + c];
+            W[k * 2 * n + c] = W[p * 2 * n + c];
+            W[p * 2 * n + c] = a;
+        }
+    __syncthreads();
+    const double piv = W[k * 2 * n + k];
+    const double ip = piv != 0.0 ? 1.0 / piv : 0.0;
+    for (int64_t c = t; c < 2 * n; c += 1024) prow[c] = W[k * 2 * n + c] * ip;
+    for (int64_t r = t; r < n; r += 1024) fcol[r] = W[r * 2 * n + k];
+}
+__global__ void gj_update_kernel(int64_t n, int64_t k, double *__restrict__ W, const double *__restrict__ prow,
+                                 const double *__restrict__ fcol) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t r = t / (2 * n), c = t % (2 * n);
+    if (r >= n) return;
+    if (r == k) W[r * 2 * n + c] = prow[c];
+    else W[r * 2 * n + c] -= fcol[r] * prow[c];
+}
+__global__ void gj_extract_kernel(int64_t n, const double *__restrict__ W, double *__restrict__ inv) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * n) return;
+    const int64_t r = t / n, c = t % n;
+    inv[t] = W[r * 2 * n + n + c];
+}
+
+}  // namespace
+
+void dense_coarse_inverse(int64_t nb, int bs, bool sell, const int64_t *slice_ptr, const int32_t *rowlen,
+                          const int32_t *rowptr, const int32_t *colind, const double *A, double *W, double *prow,
+                          double *fcol, double *inv, hipStream_t s) {
+    const int64_t n = nb * bs;
+    PF_HIP(hipMemsetAsync(W, 0, sizeof(double) * 2 * n * n, s));
+    if (sell)
+        hipLaunchKernelGGL((dense_from_blocks_kernel<true>), dim3(grid_for(nb, 64)), dim3(64), 0, s, nb, bs,
+                           slice_ptr, rowlen, rowptr, colind, A, n, W);
+    else
+        hipLaunchKernelGGL((dense_from_blocks_kernel<false>), dim3(grid_for(nb, 64)), dim3(64), 0, s, nb, bs,
+                           slice_ptr, rowlen, rowptr, colind, A, n, W);
+    hipLaunchKernelGGL(dense_prepare_kernel, dim3(grid_for(n, 64)), dim3(64), 0, s, n, W);
+    for (int64_t k = 0; k < n; ++k) {
+        hipLaunchKernelGGL(gj_pivot_kernel, dim3(1), dim3(1024), 0, s, n, k, W, prow, fcol);
+        hipLaunchKernelGGL(gj_update_kernel, dim3(grid_for(2 * n * n, 256)), dim3(256), 0, s, n, k, W, prow, fcol);
+    }
+    hipLaunchKernelGGL(gj_extract_kernel, dim3(grid_for(n * n, 256)), dim3(256), 0, s, n, W, inv);
+    PF_HIP(hipGetLastError());
+}
+
+namespace {
 }  // namespace
 
 template <int BS>

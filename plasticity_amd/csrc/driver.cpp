@@ -88,68 +88,105 @@ static void pc_setup(pfem_ctx *c) {
     c->st.n_setup++;
 }
 
-// PCG on the fine operator, FP64 (Krylov.jl cg! semantics as used at Solver.jl:258:
-// x0 = 0, stop when sqrt(r'z) <= rtol * sqrt(r0'z0)).  Returns 0 solved,
-// 1 not converged (itmax), 2 breakdown.
-static int pcg64(pfem_ctx *c, const double *b, double *x, double rtol, int itmax, int *iters) {
+// Precision traits: fine operator, V-cycle and workspace for FP64 / FP32 vectors.
+template <class T> struct Prec;
+template <> struct Prec<double> {
+    static void spmv(pfem_ctx *c, const double *x, double *y) { spmv_fine64(c->K, x, y, c->stream); }
+    static void spmv_dot(pfem_ctx *c, const double *x, double *y, double *out) {
+        spmv_fine64_dot(c->K, x, y, c->partial.p, out, c->stream);
+    }
+    static void pc(pfem_ctx *c, const double *r, double *z) { amg_apply64(c, c->amg.get(), r, z); }
+    static double *w(pfem_ctx *c, int k) {
+        pfem::dbuf<double> *b[] = {&c->w_r, &c->w_z, &c->w_p, &c->w_q, &c->w_rh, &c->w_v, &c->w_s, &c->w_t};
+        return b[k]->p;
+    }
+};
+template <> struct Prec<float> {
+    static void spmv(pfem_ctx *c, const float *x, float *y) { spmv_fine32(c->K, x, y, c->stream); }
+    static void spmv_dot(pfem_ctx *c, const float *x, float *y, double *out) {
+        spmv_fine32_dot(c->K, x, y, c->partial.p, out, c->stream);
+    }
+    static void pc(pfem_ctx *c, const float *r, float *z) { amg_apply32(c, c->amg.get(), r, z); }
+    static float *w(pfem_ctx *c, int k) {
+        pfem::dbuf<float> *b[] = {&c->f_r, &c->f_z, &c->f_p, &c->f_q, &c->f_rh, &c->f_v, &c->f_s, &c->f_t};
+        return b[k]->p;
+    }
+};
+
+// PCG (Krylov.jl cg! semantics as used at Solver.jl:258: x0 = 0, stop when
+// sqrt(r'z) <= rtol * sqrt(r0'z0)).  Dot products accumulate in FP64.  Returns
+// 0 solved, 1 not converged (itmax), 2 breakdown (p'Kp <= 0 or r'z < 0: the
+// operator or the preconditioner is not positive definite).
+template <class T>
+static int pcg(pfem_ctx *c, const T *b, T *x, double rtol, int itmax, int *iters) {
     hipStream_t s = c->stream;
     const int64_t n = c->ndof;
-    double *r = c->w_r.p, *z = c->w_z.p, *p = c->w_p.p, *q = c->w_q.p;
+    T *r = Prec<T>::w(c, 0), *z = Prec<T>::w(c, 1), *p = Prec<T>::w(c, 2), *q = Prec<T>::w(c, 3);
     double *sc = c->scal.p;
-    fill<double>(n, 0.0, x, s);
-    PF_HIP(hipMemcpyAsync(r, b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    amg_apply64(c, c->amg.get(), r, z);
-    dot_dev<double, double>(n, r, z, c->partial.p, sc + 0, s);
+    fill<T>(n, T(0), x, s);
+    PF_HIP(hipMemcpyAsync(r, b, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    Prec<T>::pc(c, r, z);
+    dot_dev<T, T>(n, r, z, c->partial.p, sc + 0, s);
     double rz = c->sync_read(sc + 0);
     *iters = 0;
     if (rz < 0) return 2;
     const double rn0 = std::sqrt(rz);
     if (rn0 == 0.0) return 0;
     const double eps = rtol * rn0;
-    PF_HIP(hipMemcpyAsync(p, z, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    PF_HIP(hipMemcpyAsync(p, z, n * sizeof(T), hipMemcpyDeviceToDevice, s));
     for (int it = 1; it <= itmax; ++it) {
         PF_HIP(hipEventRecord(c->ev0, s));
-        spmv_fine64_dot(c->K, p, q, c->partial.p, sc + 1, s);  // q = K p, sc[1] = p'q
+        Prec<T>::spmv_dot(c, p, q, sc + 1);  // q = K p, sc[1] = p'q
         PF_HIP(hipEventRecord(c->ev1, s));
         const double pq = c->sync_read(sc + 1);
-        {
+        if (sizeof(T) == 8) {
             float ms = 0.f;
             PF_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
             c->st.t_pcg_spmv += 1e-3 * ms;
             c->st.n_pcg_spmv++;
         }
-        if (!(pq > 0)) { *iters = it; return 2; }
-        cg_update<double>(n, sc + 0, sc + 1, p, q, x, r, s);  // x += a p, r -= a q
-        amg_apply64(c, c->amg.get(), r, z);
-        dot_dev<double, double>(n, r, z, c->partial.p, sc + 2, s);
+        if (!(pq > 0)) {
+            *iters = it;
+            if (c->opt.verbose > 1) fprintf(stderr, "      pcg breakdown p'Kp = %.3e at it %d\n", pq, it);
+            return 2;
+        }
+        cg_update<T>(n, sc + 0, sc + 1, p, q, x, r, s);  // x += a p, r -= a q
+        Prec<T>::pc(c, r, z);
+        dot_dev<T, T>(n, r, z, c->partial.p, sc + 2, s);
         const double rzn = c->sync_read(sc + 2);
         *iters = it;
-        if (rzn < 0) return 2;
-        if (c->opt.verbose > 1) fprintf(stderr, "      pcg %3d  sqrt(rz) = %.3e  (target %.3e)\n", it, std::sqrt(rzn), eps);
+        if (!(rzn >= 0)) {
+            if (c->opt.verbose > 1) fprintf(stderr, "      pcg breakdown r'z = %.3e at it %d\n", rzn, it);
+            return 2;
+        }
+        if (c->opt.verbose > 2) fprintf(stderr, "      pcg %3d  sqrt(rz) = %.3e  (target %.3e)\n", it, std::sqrt(rzn), eps);
         if (std::sqrt(rzn) <= eps) return 0;
-        xpby_ratio<double>(n, sc + 2, sc + 0, z, p, s);  // p = z + (rzn/rz) p
+        xpby_ratio<T>(n, sc + 2, sc + 0, z, p, s);  // p = z + (rzn/rz) p
         PF_HIP(hipMemcpyAsync(sc + 0, sc + 2, sizeof(double), hipMemcpyDeviceToDevice, s));
         rz = rzn;
     }
     return 1;
 }
 
-// Right-preconditioned BiCGStab (van der Vorst 1992) with the same V-cycle;
-// stops on the true residual ||b - Kx|| <= rtol ||b||.
-static int bicgstab64(pfem_ctx *c, const double *b, double *x, double rtol, int itmax, int *iters) {
+// Right-preconditioned BiCGStab (van der Vorst, SIAM J. Sci. Stat. Comput. 13, 1992)
+// with the same V-cycle; stops on ||b - Kx|| <= rtol ||b||.  Handles the indefinite
+// tangents on which PCG breaks down.
+template <class T>
+static int bicgstab(pfem_ctx *c, const T *b, T *x, double rtol, int itmax, int *iters) {
     hipStream_t s = c->stream;
     const int64_t n = c->ndof;
-    double *r = c->w_r.p, *rh = c->w_rh.p, *p = c->w_p.p, *v = c->w_v.p, *ph = c->w_z.p, *sv = c->w_s.p,
-           *sh = c->w_q.p, *t = c->w_t.p, *sc = c->scal.p;
-    auto dot = [&](const double *a, const double *bb) {
-        dot_dev<double, double>(n, a, bb, c->partial.p, sc + 8, s);
+    T *r = Prec<T>::w(c, 0), *ph = Prec<T>::w(c, 1), *p = Prec<T>::w(c, 2), *sh = Prec<T>::w(c, 3),
+      *rh = Prec<T>::w(c, 4), *v = Prec<T>::w(c, 5), *sv = Prec<T>::w(c, 6), *t = Prec<T>::w(c, 7);
+    double *sc = c->scal.p;
+    auto dot = [&](const T *a, const T *bb) {
+        dot_dev<T, T>(n, a, bb, c->partial.p, sc + 8, s);
         return c->sync_read(sc + 8);
     };
-    fill<double>(n, 0.0, x, s);
-    PF_HIP(hipMemcpyAsync(r, b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    PF_HIP(hipMemcpyAsync(rh, b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    fill<double>(n, 0.0, p, s);
-    fill<double>(n, 0.0, v, s);
+    fill<T>(n, T(0), x, s);
+    PF_HIP(hipMemcpyAsync(r, b, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    PF_HIP(hipMemcpyAsync(rh, b, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    fill<T>(n, T(0), p, s);
+    fill<T>(n, T(0), v, s);
     const double bn = std::sqrt(dot(b, b));
     *iters = 0;
     if (bn == 0.0) return 0;
@@ -159,28 +196,28 @@ static int bicgstab64(pfem_ctx *c, const double *b, double *x, double rtol, int 
         const double rho_n = dot(rh, r);
         if (rho_n == 0.0) return 2;
         const double beta = (rho_n / rho) * (alpha / omega);
-        axpby<double>(n, -omega, v, 1.0, p, s);  // p = p - w v
-        axpby<double>(n, 1.0, r, beta, p, s);    // p = r + beta p
-        amg_apply64(c, c->amg.get(), p, ph);
-        spmv_fine64(c->K, ph, v, s);
+        axpby<T>(n, -omega, v, 1.0, p, s);  // p = p - w v
+        axpby<T>(n, 1.0, r, beta, p, s);    // p = r + beta p
+        Prec<T>::pc(c, p, ph);
+        Prec<T>::spmv(c, ph, v);
         const double rv = dot(rh, v);
         if (rv == 0.0) return 2;
         alpha = rho_n / rv;
-        PF_HIP(hipMemcpyAsync(sv, r, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-        axpby<double>(n, -alpha, v, 1.0, sv, s);  // s = r - a v
-        axpby<double>(n, alpha, ph, 1.0, x, s);   // x += a ph
+        PF_HIP(hipMemcpyAsync(sv, r, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+        axpby<T>(n, -alpha, v, 1.0, sv, s);  // s = r - a v
+        axpby<T>(n, alpha, ph, 1.0, x, s);   // x += a ph
         const double sn = std::sqrt(dot(sv, sv));
         if (sn <= rtol * bn) return 0;
-        amg_apply64(c, c->amg.get(), sv, sh);
-        spmv_fine64(c->K, sh, t, s);
+        Prec<T>::pc(c, sv, sh);
+        Prec<T>::spmv(c, sh, t);
         const double tt = dot(t, t);
         if (tt == 0.0) return 2;
         omega = dot(t, sv) / tt;
-        axpby<double>(n, omega, sh, 1.0, x, s);  // x += w sh
-        PF_HIP(hipMemcpyAsync(r, sv, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-        axpby<double>(n, -omega, t, 1.0, r, s);  // r = s - w t
+        axpby<T>(n, omega, sh, 1.0, x, s);  // x += w sh
+        PF_HIP(hipMemcpyAsync(r, sv, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+        axpby<T>(n, -omega, t, 1.0, r, s);  // r = s - w t
         const double rn = std::sqrt(dot(r, r));
-        if (c->opt.verbose > 1) fprintf(stderr, "      bicgstab %3d  |r|/|b| = %.3e\n", it, rn / bn);
+        if (c->opt.verbose > 2) fprintf(stderr, "      bicgstab %3d  |r|/|b| = %.3e\n", it, rn / bn);
         if (rn <= rtol * bn) return 0;
         if (omega == 0.0) return 2;
         rho = rho_n;
@@ -188,43 +225,10 @@ static int bicgstab64(pfem_ctx *c, const double *b, double *x, double rtol, int 
     return 1;
 }
 
-// FP32 PCG (vectors, SpMV and V-cycle in FP32; dot products accumulated in FP64).
-static int pcg32(pfem_ctx *c, const float *b, float *x, double rtol, int itmax, int *iters) {
-    hipStream_t s = c->stream;
-    const int64_t n = c->ndof;
-    float *r = c->f_r.p, *z = c->f_z.p, *p = c->f_p.p, *q = c->f_q.p;
-    double *sc = c->scal.p;
-    fill<float>(n, 0.f, x, s);
-    PF_HIP(hipMemcpyAsync(r, b, n * sizeof(float), hipMemcpyDeviceToDevice, s));
-    amg_apply32(c, c->amg.get(), r, z);
-    dot_dev<float, float>(n, r, z, c->partial.p, sc + 0, s);
-    double rz = c->sync_read(sc + 0);
-    *iters = 0;
-    if (rz < 0) return 2;
-    const double rn0 = std::sqrt(rz);
-    if (rn0 == 0.0) return 0;
-    PF_HIP(hipMemcpyAsync(p, z, n * sizeof(float), hipMemcpyDeviceToDevice, s));
-    for (int it = 1; it <= itmax; ++it) {
-        spmv_fine32_dot(c->K, p, q, c->partial.p, sc + 1, s);
-        const double pq = c->sync_read(sc + 1);
-        if (!(pq > 0)) { *iters = it; return 2; }
-        cg_update<float>(n, sc + 0, sc + 1, p, q, x, r, s);
-        amg_apply32(c, c->amg.get(), r, z);
-        dot_dev<float, float>(n, r, z, c->partial.p, sc + 2, s);
-        const double rzn = c->sync_read(sc + 2);
-        *iters = it;
-        if (rzn < 0) return 2;
-        if (std::sqrt(rzn) <= rtol * rn0) return 0;
-        xpby_ratio<float>(n, sc + 2, sc + 0, z, p, s);
-        PF_HIP(hipMemcpyAsync(sc + 0, sc + 2, sizeof(double), hipMemcpyDeviceToDevice, s));
-        rz = rzn;
-    }
-    return 1;
-}
-
-// FP64 iterative refinement around FP32 PCG: r = b - K x in FP64, inner solve in
-// FP32 to a fixed relative accuracy, x += d in FP64, until ||r|| <= rtol ||b||.
-static int ir_solve(pfem_ctx *c, const double *b, double *x, double rtol, int itmax, int *iters) {
+// FP64 iterative refinement around FP32 Krylov: r = b - K x in FP64, inner FP32
+// solve (PCG, or BiCGStab once PCG has broken down on this system) of K d = r to a
+// modest relative accuracy, x += d in FP64, until ||b - K x|| <= rtol ||b||.
+static int ir_solve(pfem_ctx *c, const double *b, double *x, double rtol, int itmax, int *iters, int *used) {
     hipStream_t s = c->stream;
     const int64_t n = c->ndof;
     double *r = c->w_s.p, *sc = c->scal.p;
@@ -235,57 +239,68 @@ static int ir_solve(pfem_ctx *c, const double *b, double *x, double rtol, int it
     if (bn == 0.0) return 0;
     PF_HIP(hipMemcpyAsync(r, b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
     double rn = bn;
-    const double inner_tol = 1e-4;  // each refinement gains ~4 digits
-    for (int outer = 0; outer < 30; ++outer) {
+    bool indefinite = false;
+    const double inner_tol = 1e-4;
+    for (int outer = 0; outer < 40; ++outer) {
         convert<double, float>(n, r, c->f_b.p, s);
-        int it = 0;
-        int rc = pcg32(c, c->f_b.p, c->f_x.p, std::max(inner_tol, 0.5 * rtol * bn / rn), itmax - *iters, &it);
-        *iters += it;
-        if (rc == 2) return 2;
+        const double tol = std::max(inner_tol, 0.5 * rtol * bn / rn);
+        int it = 0, rc = 2;
+        if (!indefinite) {
+            rc = pcg<float>(c, c->f_b.p, c->f_x.p, tol, std::max(1, itmax - *iters), &it);
+            *iters += it;
+            if (rc == 2) indefinite = true;
+        }
+        if (indefinite) {
+            rc = bicgstab<float>(c, c->f_b.p, c->f_x.p, tol, std::max(1, itmax - *iters), &it);
+            *iters += it;
+            if (rc == 2) return 2;
+        }
         convert<float, double>(n, c->f_x.p, c->w_t.p, s);
         axpby<double>(n, 1.0, c->w_t.p, 1.0, x, s);
         residual_fine64(c->K, x, b, r, s);  // FP64 refinement residual
         dot_dev<double, double>(n, r, r, c->partial.p, sc + 4, s);
         rn = std::sqrt(c->sync_read(sc + 4));
-        if (c->opt.verbose > 1) fprintf(stderr, "      IR %2d  inner %3d  |r|/|b| = %.3e\n", outer, it, rn / bn);
-        if (rn <= rtol * bn) return 0;
+        if (c->opt.verbose > 1)
+            fprintf(stderr, "      IR %2d  inner %3d (%s)  |r|/|b| = %.3e\n", outer, it, indefinite ? "bicgstab" : "pcg",
+                    rn / bn);
+        if (rn <= rtol * bn) break;
         if (*iters >= itmax) return 1;
     }
-    return 1;
+    *used = indefinite ? 4 : 3;
+    return rn <= rtol * bn ? 0 : 1;
 }
 
 int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *used, double *relres) {
-    const double t0 = now();
     if (rebuild || !c->pc_ready) pc_setup(c);
     const double t1 = now();
     int it = 0, total = 0, which = 0, rc;
     const int itmax = c->opt.cg_itmax > 0 ? c->opt.cg_itmax : 200;
+    hipStream_t s = c->stream;
     if (c->opt.precision == PFEM_PREC_FP32_IR) {
-        hipStream_t s = c->stream;
         if (!c->f_r.p)
-            for (auto *b : {&c->f_r, &c->f_z, &c->f_p, &c->f_q, &c->f_x, &c->f_b}) b->alloc(c->ndof);
+            for (auto *b : {&c->f_r, &c->f_z, &c->f_p, &c->f_q, &c->f_x, &c->f_b, &c->f_rh, &c->f_v, &c->f_s, &c->f_t})
+                b->alloc(c->ndof);
         if (!c->K.v32.p) c->K.v32.alloc(9 * c->K.nslots);
         convert<double, float>(9 * c->K.nslots, c->K.v64.p, c->K.v32.p, s);  // FP32 copy of the imposed K
         which = 3;
-        rc = ir_solve(c, c->rhs.p, c->dU.p, rtol, 4 * itmax, &it);
+        rc = ir_solve(c, c->rhs.p, c->dU.p, rtol, 4 * itmax, &it, &which);
         total += it;
     } else {
-        rc = pcg64(c, c->rhs.p, c->dU.p, rtol, itmax, &it);
+        rc = pcg<double>(c, c->rhs.p, c->dU.p, rtol, itmax, &it);
         total += it;
         if (rc == 1) {  // Solver.jl:270-273: refresh the preconditioner, retry once
             pc_setup(c);
             which = 1;
-            rc = pcg64(c, c->rhs.p, c->dU.p, rtol, itmax, &it);
+            rc = pcg<double>(c, c->rhs.p, c->dU.p, rtol, itmax, &it);
             total += it;
         }
         if (rc != 0) {  // in place of the direct fallback (Solver.jl:262, :284)
             which = 2;
-            rc = bicgstab64(c, c->rhs.p, c->dU.p, rtol, 2 * itmax, &it);
+            rc = bicgstab<double>(c, c->rhs.p, c->dU.p, rtol, 2 * itmax, &it);
             total += it;
         }
     }
     // true residual ||b - K dU|| / ||b||
-    hipStream_t s = c->stream;
     residual_fine64(c->K, c->dU.p, c->rhs.p, c->w_s.p, s);
     dot_dev<double, double>(c->ndof, c->w_s.p, c->w_s.p, c->partial.p, c->scal.p + 5, s);
     dot_dev<double, double>(c->ndof, c->rhs.p, c->rhs.p, c->partial.p, c->scal.p + 6, s);
@@ -298,7 +313,6 @@ int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *us
     c->st.t_solve += now() - t1;
     c->st.n_solve++;
     c->st.n_krylov += total;
-    (void)t0;
     if (rc == 1) return PFEM_ERR_NOT_CONVERGED;
     if (rc == 2) return PFEM_ERR_NON_SPD;
     return PFEM_OK;
@@ -348,7 +362,10 @@ int pfem_newton(pfem_ctx *c, double lambda, double tol, int maxiter, pfem_newton
             }
             if (it == maxiter) break;
             const double eta = forcing_term(c->opt, it, rnorm, rkm1, eta_prev);  // :371
-            const bool rebuild = first;  // :374
+            // :374 rebuilds once per load step; amg_rebuild = 1 rebuilds from every
+            // Newton iteration's tangent (the plastic iterations then need about half the
+            // Krylov iterations of a hierarchy built from the elastic predictor tangent)
+            const bool rebuild = first || c->opt.reserved[0] == 1;
             first = false;
             axpby<double>(c->ndof, -1.0, c->R.p, 0.0, c->rhs.p, s);  // :378
             t = now();

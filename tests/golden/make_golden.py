@@ -5,7 +5,7 @@ pinned by the closed-form / finite-difference / invariant tests in tests/.
 
   config0.npz   workload 0 (8^3 F-bar tension to 5%, 20 steps, tol 1e-10):
                 U and abar after steps 1, 10 and 20, Newton residual histories
-  jitter6.npz   6^3 jittered block (seed 0, +-10%), compression 20% in 80 steps
+  jitter6.npz   6^3 jittered block (seed 0, +-10%), compression 20% in 80 steps, tol 1e-10
   elements.npz  per-element Fe/Ke/tau for distorted elements at plastic states
 Run:  python tests/golden/make_golden.py
 """
@@ -65,7 +65,9 @@ def element_states():
 if __name__ == "__main__":
     np.savez_compressed(os.path.join(HERE, "elements.npz"), **element_states())
     np.savez_compressed(os.path.join(HERE, "config0.npz"), **run_workload(workload(0), {1, 10, 20}))
-    np.savez_compressed(os.path.join(HERE, "jitter6.npz"), **run_workload(workload(1, n=6, nsteps=80), {20, 40, 80}))
+    wl = workload(1, n=6, nsteps=80)
+    wl.newton_tol = wl.krylov_rtol = 1e-10
+    np.savez_compressed(os.path.join(HERE, "jitter6.npz"), **run_workload(wl, {20, 40, 80}))
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")

@@ -40,6 +40,13 @@ def run_gpu(wl, keep, **opts):
     return m, res, out
 
 
+def jitter6():
+    """6^3 jittered compression with Newton and Krylov tolerances 1e-10 (as the fixture)."""
+    wl = workload(1, n=6, nsteps=80)
+    wl.newton_tol = wl.krylov_rtol = 1e-10
+    return wl
+
+
 def test_linear_solve_matches_direct(need_gpu):
     print("[linear solve] 6^3 jittered, plastic state", flush=True)
     mesh = pa.box_mesh(1, 1, 1, 6, 6, 6, jitter=0.1, seed=0)
@@ -81,7 +88,7 @@ def test_config0_matches_oracle_golden(need_gpu):
 def test_jittered_compression_matches_oracle_golden(need_gpu):
     g = np.load(os.path.join(HERE, "golden", "jitter6.npz"))
     print("[newton] 6^3 jittered F-bar compression 20% in 80 steps", flush=True)
-    m, res, out = run_gpu(workload(1, n=6, nsteps=80), {20, 40, 80})
+    m, res, out = run_gpu(jitter6(), {20, 40, 80})
     assert res.converged
     print("  newton iters total gpu", sum(res.iters), "oracle", int(g["newton_iters"].sum()), flush=True)
     for n in (20, 40, 80):
@@ -94,7 +101,7 @@ def test_jittered_compression_matches_oracle_golden(need_gpu):
 def test_fp32_ir_matches_fp64_oracle(need_gpu):
     g = np.load(os.path.join(HERE, "golden", "jitter6.npz"))
     print("[newton fp32-IR] 6^3 jittered compression", flush=True)
-    m, res, out = run_gpu(workload(1, n=6, nsteps=80), {20, 80}, precision="fp32_ir")
+    m, res, out = run_gpu(jitter6(), {20, 80}, precision="fp32_ir")
     assert res.converged
     for n in (20, 80):
         eu = rel(out[f"U_{n}"], g[f"U_{n}"])
