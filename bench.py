@@ -149,6 +149,10 @@ def main():
     n_asm = s1.n_assemble - s0.n_assemble
     gp_ms = (s1.t_gp_kernel - s0.t_gp_kernel) / max(n_asm, 1) * 1e3
     sc_ms = (s1.t_scatter_kernel - s0.t_scatter_kernel) / max(n_asm, 1) * 1e3
+    nset = max(s1.n_setup - s0.n_setup, 1)
+    log(f"[bench] AMG set-up per rebuild: rho {(s1.t_setup_rho - s0.t_setup_rho) / nset * 1e3:.1f} ms, "
+        f"P/RAP {(s1.t_setup_galerkin - s0.t_setup_galerkin) / nset * 1e3:.1f} ms, "
+        f"coarse inverse {(s1.t_setup_coarse - s0.t_setup_coarse) / nset * 1e3:.1f} ms ({nset} rebuilds)")
     log(f"[bench] timed {args.steps} load steps: {elapsed:.3f}s, {newton} Newton steps, {krylov} Krylov its; "
         f"assemble {t_asm:.3f}s (gp kernel {gp_ms:.2f} ms + scatter {sc_ms:.2f} ms per assembly), "
         f"AMG set-up {t_setup:.3f}s, solve {t_solve:.3f}s; fine SpMV {t_sp * 1e3:.3f} ms = {achieved} GB/s; "

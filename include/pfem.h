@@ -192,6 +192,7 @@ typedef struct pfem_stats {
     double t_spmv;                           /* last pfem_bench_spmv result (s per SpMV) */
     double t_pcg_spmv;                       /* accumulated device time of the fine FP64 SpMV in PCG (s) */
     int64_t n_pcg_spmv;                      /* number of those launches */
+    double t_setup_rho, t_setup_galerkin, t_setup_coarse;  /* AMG numeric set-up phases (s) */
     int64_t n_assemble, n_setup, n_solve, n_krylov;
     int64_t device_bytes;                    /* device memory held by the context */
 } pfem_stats;

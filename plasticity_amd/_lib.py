@@ -65,6 +65,7 @@ class Stats(C.Structure):
                 ("t_assemble", C.c_double), ("t_setup", C.c_double), ("t_solve", C.c_double),
                 ("t_gp_kernel", C.c_double), ("t_scatter_kernel", C.c_double), ("t_spmv", C.c_double),
                 ("t_pcg_spmv", C.c_double), ("n_pcg_spmv", C.c_int64),
+                ("t_setup_rho", C.c_double), ("t_setup_galerkin", C.c_double), ("t_setup_coarse", C.c_double),
                 ("n_assemble", C.c_int64), ("n_setup", C.c_int64), ("n_solve", C.c_int64),
                 ("n_krylov", C.c_int64), ("device_bytes", C.c_int64)]
 

@@ -350,6 +350,8 @@ void coarse_inverse(pfem_ctx *c, Amg *a) {
 void amg_numeric_setup(pfem_ctx *c, Amg *a) {
     hipStream_t s = c->stream;
     const bool want32 = c->opt.precision == PFEM_PREC_FP32_IR;
+    double t_rho = 0.0;
+    const double t_begin = now();
     for (size_t l = 0; l < a->L.size(); ++l) {
         AmgLevel &L = a->L[l];
         const bool fine = (l == 0);
@@ -357,7 +359,9 @@ void amg_numeric_setup(pfem_ctx *c, Amg *a) {
         if (L.bs == 3) amg_block_inverse<3>(L.nb, fine ? c->K.diagpos.p : L.diagpos.p, Av, L.dinv.p, s);
         else amg_block_inverse<6>(L.nb, L.diagpos.p, Av, L.dinv.p, s);
         if (l + 1 == a->L.size()) break;  // coarsest: dense solve
+        const double tr = now();
         L.rho = spectral_radius(c, a, (int)l, 12);
+        t_rho += now() - tr;
         const double omega = (4.0 / 3.0) / L.rho;
         AmgLevel &N = a->L[l + 1];
         if (L.bs == 3) {
@@ -378,7 +382,13 @@ void amg_numeric_setup(pfem_ctx *c, Amg *a) {
                        L.Q.p, N.A.p, s);
         }
     }
+    c->sync();
+    const double t_coarse0 = now();
     coarse_inverse(c, a);
+    c->sync();
+    c->st.t_setup_coarse += now() - t_coarse0;
+    c->st.t_setup_rho += t_rho;
+    c->st.t_setup_galerkin += t_coarse0 - t_begin - t_rho;
     if (want32) {
         for (size_t l = 0; l < a->L.size(); ++l) {
             AmgLevel &L = a->L[l];

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "amg.h"
+#include "sell.h"
 
 namespace pfem {
 namespace {
@@ -249,39 +250,6 @@ __global__ void bsr_residual_kernel(int64_t nb, const int32_t *__restrict__ rowp
     r[o] = (XT)((VT)b[o] - s);
 }
 
-// fine SELL residual / product with a caller-chosen value array (FP64 or FP32 copy)
-template <class VT, class XT, bool RES>
-__global__ __launch_bounds__(256) void sell3_kernel(int64_t nb, const int64_t *__restrict__ slice_ptr,
-                                                    const int32_t *__restrict__ rowlen,
-                                                    const int32_t *__restrict__ colind, const VT *__restrict__ vals,
-                                                    const XT *__restrict__ x, const XT *__restrict__ b,
-                                                    XT *__restrict__ y) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= nb) return;
-    const int64_t base = slice_ptr[i >> 6] + (i & 63);
-    const int len = rowlen[i];
-    VT y0 = 0, y1 = 0, y2 = 0;
-#pragma unroll 3
-    for (int k = 0; k < len; ++k) {
-        const int64_t pos = base + (int64_t)k * SELL_C;
-        const int64_t j = colind[pos];
-        const VT *v = vals + 9 * pos;
-        const VT x0 = (VT)x[3 * j], x1 = (VT)x[3 * j + 1], x2 = (VT)x[3 * j + 2];
-        y0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
-        y1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
-        y2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
-    }
-    if (RES) {
-        y[3 * i] = (XT)((VT)b[3 * i] - y0);
-        y[3 * i + 1] = (XT)((VT)b[3 * i + 1] - y1);
-        y[3 * i + 2] = (XT)((VT)b[3 * i + 2] - y2);
-    } else {
-        y[3 * i] = (XT)y0;
-        y[3 * i + 1] = (XT)y1;
-        y[3 * i + 2] = (XT)y2;
-    }
-}
-
 template <int BS, class DT, class XT>
 __global__ void block_diag_apply_kernel(int64_t nb, const DT *__restrict__ dinv, const XT *__restrict__ r,
                                         XT *__restrict__ z, double scale) {
@@ -491,15 +459,15 @@ void bsr_residual(int64_t nb, const int32_t *rowptr, const int32_t *colind, cons
 template <int BS, class VT, class XT>
 void sell_residual(const FineMatrix &K, const VT *vals, const XT *x, const XT *b, XT *r, hipStream_t s) {
     static_assert(BS == 3, "fine level is 3x3");
-    hipLaunchKernelGGL((sell3_kernel<VT, XT, true>), dim3(grid_for(K.nb, 256)), dim3(256), 0, s, K.nb,
-                       K.slice_ptr.p, K.rowlen.p, K.colind.p, vals, x, b, r);
+    hipLaunchKernelGGL((sell_lds_kernel<VT, XT, SELL_RES>), dim3((unsigned)K.nslices), dim3(SELL_C), 0, s, K.nb,
+                       K.slice_ptr.p, K.colind.p, vals, x, b, r, nullptr);
     PF_HIP(hipGetLastError());
 }
 template <int BS, class VT, class XT>
 void sell_spmv(const FineMatrix &K, const VT *vals, const XT *x, XT *y, hipStream_t s) {
     static_assert(BS == 3, "fine level is 3x3");
-    hipLaunchKernelGGL((sell3_kernel<VT, XT, false>), dim3(grid_for(K.nb, 256)), dim3(256), 0, s, K.nb,
-                       K.slice_ptr.p, K.rowlen.p, K.colind.p, vals, x, (const XT *)nullptr, y);
+    hipLaunchKernelGGL((sell_lds_kernel<VT, XT, SELL_Y>), dim3((unsigned)K.nslices), dim3(SELL_C), 0, s, K.nb,
+                       K.slice_ptr.p, K.colind.p, vals, x, (const XT *)nullptr, y, nullptr);
     PF_HIP(hipGetLastError());
 }
 template <int BS, class DT, class XT>

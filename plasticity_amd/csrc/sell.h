@@ -1,0 +1,68 @@
+// sell.h -- the fine-level SELL-64 BSR(3x3) product kernel, shared by sparse.hip and
+// amg.hip (internal).
+//
+// One 64-lane workgroup per slice of 64 block rows.  For each slice column k the
+// workgroup first copies the 64 contiguous 3x3 blocks (64 x 72 B in FP64, 64 x 36 B in
+// FP32) into LDS with 16-byte loads -- every global load instruction covers 1 KB of
+// consecutive bytes -- then each lane takes its own block from LDS and gathers x.
+// Padding slots carry zero values and a valid column, so lanes never branch on the
+// row length.
+#pragma once
+#include "common.h"
+
+namespace pfem {
+
+enum SellMode { SELL_Y = 0, SELL_RES = 1, SELL_DOT = 2 };
+
+template <class VT, class XT, int MODE>
+__global__ __launch_bounds__(64) void sell_lds_kernel(int64_t nb, const int64_t *__restrict__ slice_ptr,
+                                                      const int32_t *__restrict__ colind,
+                                                      const VT *__restrict__ vals, const XT *__restrict__ x,
+                                                      const XT *__restrict__ b, XT *__restrict__ y,
+                                                      double *__restrict__ partial) {
+    constexpr int NV = SELL_C * 9 * (int)sizeof(VT) / 16;  // 16-byte chunks per slice column
+    __shared__ __attribute__((aligned(16))) VT tile[SELL_C * 9];
+    const int lane = threadIdx.x;
+    const int64_t s = blockIdx.x;
+    const int64_t i = s * SELL_C + lane;
+    const int64_t base = slice_ptr[s];
+    const int w = (int)((slice_ptr[s + 1] - base) / SELL_C);
+    VT y0 = 0, y1 = 0, y2 = 0;
+    for (int k = 0; k < w; ++k) {
+        const int64_t col0 = base + (int64_t)k * SELL_C;
+        const uint4 *src = reinterpret_cast<const uint4 *>(vals + 9 * col0);
+        uint4 *dst = reinterpret_cast<uint4 *>(tile);
+#pragma unroll
+        for (int t = lane; t < NV; t += SELL_C) dst[t] = src[t];
+        const int64_t j = colind[col0 + lane];
+        const XT x0 = x[3 * j], x1 = x[3 * j + 1], x2 = x[3 * j + 2];
+        __syncthreads();
+        const VT *v = tile + 9 * lane;
+        y0 += v[0] * (VT)x0 + v[1] * (VT)x1 + v[2] * (VT)x2;
+        y1 += v[3] * (VT)x0 + v[4] * (VT)x1 + v[5] * (VT)x2;
+        y2 += v[6] * (VT)x0 + v[7] * (VT)x1 + v[8] * (VT)x2;
+        __syncthreads();
+    }
+    double d = 0.0;
+    if (i < nb) {
+        if (MODE == SELL_RES) {
+            y[3 * i] = (XT)((VT)b[3 * i] - y0);
+            y[3 * i + 1] = (XT)((VT)b[3 * i + 1] - y1);
+            y[3 * i + 2] = (XT)((VT)b[3 * i + 2] - y2);
+        } else {
+            y[3 * i] = (XT)y0;
+            y[3 * i + 1] = (XT)y1;
+            y[3 * i + 2] = (XT)y2;
+            if (MODE == SELL_DOT)
+                d = (double)x[3 * i] * (double)y0 + (double)x[3 * i + 1] * (double)y1 +
+                    (double)x[3 * i + 2] * (double)y2;
+        }
+    }
+    if (MODE == SELL_DOT) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) d += __shfl_down(d, off, 64);
+        if (lane == 0) partial[s] = d;
+    }
+}
+
+}  // namespace pfem

@@ -12,6 +12,7 @@
 
 #include "common.h"
 #include "kernels.h"
+#include "sell.h"
 
 namespace pfem {
 namespace {
@@ -37,69 +38,6 @@ __device__ __forceinline__ double block_sum(double v) {
     }
     __syncthreads();
     return s;
-}
-
-template <class VT, class XT, bool DOT>
-__global__ __launch_bounds__(256) void sell_spmv_kernel(int64_t nb, const int64_t *__restrict__ slice_ptr,
-                                                        const int32_t *__restrict__ rowlen,
-                                                        const int32_t *__restrict__ colind,
-                                                        const VT *__restrict__ vals,
-                                                        const XT *__restrict__ x, XT *__restrict__ y,
-                                                        double *__restrict__ partial) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    double dotv = 0.0;
-    if (i < nb) {
-        const int64_t base = slice_ptr[i >> 6] + (i & 63);
-        const int len = rowlen[i];
-        VT y0 = 0, y1 = 0, y2 = 0;
-#pragma unroll 3
-        for (int k = 0; k < len; ++k) {
-            const int64_t pos = base + (int64_t)k * SELL_C;
-            const int64_t j = colind[pos];
-            const VT *v = vals + 9 * pos;
-            const VT x0 = (VT)x[3 * j], x1 = (VT)x[3 * j + 1], x2 = (VT)x[3 * j + 2];
-            y0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
-            y1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
-            y2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
-        }
-        y[3 * i] = (XT)y0;
-        y[3 * i + 1] = (XT)y1;
-        y[3 * i + 2] = (XT)y2;
-        if (DOT) dotv = (double)x[3 * i] * (double)y0 + (double)x[3 * i + 1] * (double)y1 +
-                        (double)x[3 * i + 2] * (double)y2;
-    }
-    if (DOT) {
-        double s = block_sum(dotv);
-        if (threadIdx.x == 0) partial[blockIdx.x] = s;
-    }
-}
-
-// r = b - K x  (FP64)
-__global__ __launch_bounds__(256) void sell_residual_kernel(int64_t nb, const int64_t *__restrict__ slice_ptr,
-                                                            const int32_t *__restrict__ rowlen,
-                                                            const int32_t *__restrict__ colind,
-                                                            const double *__restrict__ vals,
-                                                            const double *__restrict__ x,
-                                                            const double *__restrict__ b,
-                                                            double *__restrict__ r) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= nb) return;
-    const int64_t base = slice_ptr[i >> 6] + (i & 63);
-    const int len = rowlen[i];
-    double y0 = 0, y1 = 0, y2 = 0;
-#pragma unroll 3
-    for (int k = 0; k < len; ++k) {
-        const int64_t pos = base + (int64_t)k * SELL_C;
-        const int64_t j = colind[pos];
-        const double *v = vals + 9 * pos;
-        const double x0 = x[3 * j], x1 = x[3 * j + 1], x2 = x[3 * j + 2];
-        y0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
-        y1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
-        y2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
-    }
-    r[3 * i] = b[3 * i] - y0;
-    r[3 * i + 1] = b[3 * i + 1] - y1;
-    r[3 * i + 2] = b[3 * i + 2] - y2;
 }
 
 // ---------------------------------------------------------------- reductions
@@ -283,39 +221,37 @@ inline unsigned grid_strided(int64_t n) {
 
 // ---------------------------------------------------------------- launchers
 void spmv_fine64(const FineMatrix &K, const double *x, double *y, hipStream_t s) {
-    hipLaunchKernelGGL((sell_spmv_kernel<double, double, false>), dim3(grid_for(K.nb, 256)), dim3(256), 0, s,
-                       K.nb, K.slice_ptr.p, K.rowlen.p, K.colind.p, K.v64.p, x, y, nullptr);
+    hipLaunchKernelGGL((sell_lds_kernel<double, double, SELL_Y>), dim3((unsigned)K.nslices), dim3(SELL_C), 0, s,
+                       K.nb, K.slice_ptr.p, K.colind.p, K.v64.p, x, (const double *)nullptr, y, nullptr);
     PF_HIP(hipGetLastError());
 }
 void spmv_fine32(const FineMatrix &K, const float *x, float *y, hipStream_t s) {
-    hipLaunchKernelGGL((sell_spmv_kernel<float, float, false>), dim3(grid_for(K.nb, 256)), dim3(256), 0, s,
-                       K.nb, K.slice_ptr.p, K.rowlen.p, K.colind.p, K.v32.p, x, y, nullptr);
+    hipLaunchKernelGGL((sell_lds_kernel<float, float, SELL_Y>), dim3((unsigned)K.nslices), dim3(SELL_C), 0, s,
+                       K.nb, K.slice_ptr.p, K.colind.p, K.v32.p, x, (const float *)nullptr, y, nullptr);
     PF_HIP(hipGetLastError());
 }
 void spmv_fine64_dot(const FineMatrix &K, const double *x, double *y, double *partial, double *out,
                      hipStream_t s) {
-    const unsigned g = grid_for(K.nb, 256);
-    hipLaunchKernelGGL((sell_spmv_kernel<double, double, true>), dim3(g), dim3(256), 0, s, K.nb, K.slice_ptr.p,
-                       K.rowlen.p, K.colind.p, K.v64.p, x, y, partial);
-    hipLaunchKernelGGL(finish_sum_kernel, dim3(1), dim3(1024), 0, s, (int)g, partial, out);
+    hipLaunchKernelGGL((sell_lds_kernel<double, double, SELL_DOT>), dim3((unsigned)K.nslices), dim3(SELL_C), 0, s,
+                       K.nb, K.slice_ptr.p, K.colind.p, K.v64.p, x, (const double *)nullptr, y, partial);
+    hipLaunchKernelGGL(finish_sum_kernel, dim3(1), dim3(1024), 0, s, (int)K.nslices, partial, out);
     PF_HIP(hipGetLastError());
 }
 void spmv_fine32_dot(const FineMatrix &K, const float *x, float *y, double *partial, double *out,
                      hipStream_t s) {
-    const unsigned g = grid_for(K.nb, 256);
-    hipLaunchKernelGGL((sell_spmv_kernel<float, float, true>), dim3(g), dim3(256), 0, s, K.nb, K.slice_ptr.p,
-                       K.rowlen.p, K.colind.p, K.v32.p, x, y, partial);
-    hipLaunchKernelGGL(finish_sum_kernel, dim3(1), dim3(1024), 0, s, (int)g, partial, out);
+    hipLaunchKernelGGL((sell_lds_kernel<float, float, SELL_DOT>), dim3((unsigned)K.nslices), dim3(SELL_C), 0, s,
+                       K.nb, K.slice_ptr.p, K.colind.p, K.v32.p, x, (const float *)nullptr, y, partial);
+    hipLaunchKernelGGL(finish_sum_kernel, dim3(1), dim3(1024), 0, s, (int)K.nslices, partial, out);
     PF_HIP(hipGetLastError());
 }
 void residual_fine64(const FineMatrix &K, const double *x, const double *b, double *r, hipStream_t s) {
-    hipLaunchKernelGGL(sell_residual_kernel, dim3(grid_for(K.nb, 256)), dim3(256), 0, s, K.nb, K.slice_ptr.p,
-                       K.rowlen.p, K.colind.p, K.v64.p, x, b, r);
+    hipLaunchKernelGGL((sell_lds_kernel<double, double, SELL_RES>), dim3((unsigned)K.nslices), dim3(SELL_C), 0, s,
+                       K.nb, K.slice_ptr.p, K.colind.p, K.v64.p, x, b, r, nullptr);
     PF_HIP(hipGetLastError());
 }
 
 size_t reduce_partials_needed(int64_t nb) {
-    size_t a = (size_t)grid_for(nb, 256);
+    size_t a = (size_t)((nb + SELL_C - 1) / SELL_C);
     return a > (size_t)RED_BLOCKS ? a : (size_t)RED_BLOCKS;
 }
 
