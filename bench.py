@@ -16,6 +16,8 @@ from __future__ import annotations
 
 import argparse
 import json
+
+import numpy as np
 import os
 import sys
 import time
@@ -38,49 +40,73 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(wl, newton_iters_per_step, krylov_per_newton, budget_s=30.0):
-    """Oracle (C/OpenMP restatement of the reference algorithm) on the host cores: one
-    full-size assembly + Dirichlet imposition + norm of the same workload, timed.  The
-    linear-solve share is estimated from the oracle's CSC SpMV time times the GPU's
-    Krylov count per Newton step times the SpMV-equivalents of one PCG iteration."""
+def cpu_baseline(newton_per_step, krylov_per_newton, n_sample=50):
+    """Oracle (C/OpenMP restatement of the reference algorithm: colour-ordered FP64
+    assembly, impose_dirichlet!, PCG preconditioned by one SA-AMG V-cycle with rigid-body
+    near-null space and Jacobi(2/3) smoothing) timed on the host cores on a bounded
+    sample: the same workload on an n_sample^3 mesh.  Every piece is O(N), so the
+    per-Newton-step time is scaled by the DOF ratio to the 100^3 workload.  Krylov
+    iterations per Newton step are taken from the GPU run of the same algorithm and the
+    AMG set-up is amortised once per load step (the reference's rebuild policy)."""
     import numpy as np
 
     import oracle as O
+    from plasticity_amd.workloads import workload
     nthreads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     nthreads = max(1, min(nthreads, 16))
-    t0 = time.perf_counter()
+    wl = workload(1, n=n_sample)
     M = O.OracleModel(wl.mesh.nodes, wl.mesh.elements, O.Material(**wl.materials[0]), nthreads=nthreads)
     M.constrain(wl.dir_dofs, wl.dir_vals, wl.dir_ramp)
-    bcs = M.freeze_bcs()
-    t_build = time.perf_counter() - t0
+    (rd, rv), (fd, fv) = M.freeze_bcs()
+    isbc = np.zeros(M.ndof, np.uint8)
+    isbc[rd] = 1
+    isbc[fd] = 1
     U = np.zeros(M.ndof)
-    st = M._new_state(8 * M.ne)
     t1 = time.perf_counter()
-    K, R = M.assemble(U, st)
+    K, R = M.assemble(U, M._new_state(8 * M.ne))
     t_asm = time.perf_counter() - t1
     t2 = time.perf_counter()
-    (rd, rv), (fd, fv) = bcs
-    O.impose_dirichlet(K, R, rd, rv, True, 1 / 40, U)
-    O.impose_dirichlet(K, R, fd, fv, False, 1 / 40, U)
+    O.impose_dirichlet(K, R, rd, rv, True, 1 / wl.nsteps, U)
+    O.impose_dirichlet(K, R, fd, fv, False, 1 / wl.nsteps, U)
     float(np.sqrt((R ** 2).sum()))
     t_bc = time.perf_counter() - t2
-    x = np.ones(M.ndof)
-    y = np.empty(M.ndof)
-    reps = 3
     t3 = time.perf_counter()
+    amg = O.OracleAMG(K, M.nodes, isbc, coarse_max=1000, nthreads=nthreads)
+    t_setup = time.perf_counter() - t3
+    b = -R
+    reps = 5
+    t4 = time.perf_counter()
     for _ in range(reps):
-        O.lib.or_csc_spmv_omp(M.ndof, K.indptr, K.indices, K.data, x, y, nthreads)
-    t_spmv = (time.perf_counter() - t3) / reps
-    spmv_equiv = 6.0  # PCG SpMV + Chebyshev(2) pre/post + two residuals per V-cycle (fine level)
-    t_solve = krylov_per_newton * spmv_equiv * t_spmv
-    per_newton = t_asm + t_bc + t_solve
+        z = amg.vcycle(b)
+        amg.apply_A(z)
+    t_iter = (time.perf_counter() - t4) / reps
+    per_newton_sample = t_asm + t_bc + t_setup / newton_per_step + krylov_per_newton * t_iter
+    scale = 3090903 / M.ndof
     return {
-        "value": per_newton, "unit": "s/Newton step", "cores": nthreads, "kind": "port",
-        "sample": (f"oracle (C/OpenMP) on the full 100^3 workload: 1 assembly {t_asm:.2f}s + BC/norm "
-                   f"{t_bc:.2f}s, CSC SpMV {t_spmv * 1e3:.1f} ms x {krylov_per_newton:.1f} PCG its x "
-                   f"{spmv_equiv:.0f} SpMV-equivalents (solve share estimated, AMG set-up excluded); "
-                   f"model build {t_build:.1f}s untimed"),
+        "value": per_newton_sample * scale, "unit": "s/Newton step", "cores": nthreads, "kind": "port",
+        "sample": (f"oracle (C/OpenMP, {nthreads} threads) on the same workload at {n_sample}^3 ({M.ndof} DOF), "
+                   f"scaled x{scale:.2f} by DOF to 100^3: assembly {t_asm:.2f}s + BC/norm {t_bc:.2f}s + "
+                   f"AMG set-up {t_setup:.2f}s / {newton_per_step:.1f} Newton steps per load step + "
+                   f"{krylov_per_newton:.1f} PCG its x {t_iter * 1e3:.1f} ms (V-cycle + SpMV)"),
     }
+
+
+def copy_calibration(nbytes=1 << 30, reps=10):
+    """Plain device-to-device copy of a known size through the HIP runtime (its blit
+    kernel reads nbytes and writes nbytes per call): the reference point for the
+    FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"[bench] copy calibration: {reps} x {nbytes} B device-to-device")
 
 
 def main():
@@ -93,16 +119,30 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--amg-rebuild", type=int, default=1, help="0: once per load step, 1: every Newton iteration")
     ap.add_argument("--verbose", type=int, default=0)
+    ap.add_argument("--mode", default="newton", choices=["newton", "spmv", "copy"],
+                    help="newton: the metric; spmv: only repeated fine SpMVs (for counter runs); "
+                         "copy: a plain 1 GiB device-to-device copy, the calibration of byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1 "
               "(see DESIGN.md for how the path would shard)", file=sys.stderr)
         sys.exit(2)
 
+    if args.mode == "copy":
+        return copy_calibration()
+
     import plasticity_amd as pa
     from plasticity_amd.workloads import workload
 
     wl = workload(1, n=args.n)
+    if args.mode == "spmv":
+        m = pa.Model(wl.mesh, pa.J2Material(**wl.materials[0]), options=pa.default_options())
+        m.assemble(np.zeros(m.ndof))
+        sec = m.bench_spmv(prec=0 if args.precision == "fp64" else 1, reps=20)
+        st = m.stats()
+        b = spmv_bytes(st.nnzb, st.nnodes, st.ndof, 8 if args.precision == "fp64" else 4)
+        log(f"[bench] fine SpMV x20: {sec * 1e3:.3f} ms each, {b / sec / 1e9:.0f} GB/s algorithmic ({b} B)")
+        return
     log(f"[bench] workload: {wl.name}, nodes {wl.mesh.nnodes}, dof {3 * wl.mesh.nnodes}")
     opts = pa.default_options(eta_min=wl.krylov_rtol, eta_max=wl.krylov_rtol, precision=args.precision,
                               verbose=args.verbose, amg_rebuild=args.amg_rebuild)
@@ -160,7 +200,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(wl, newton / args.steps, krylov / max(newton - args.steps, 1))
+            cpu = cpu_baseline(newton / args.steps, krylov / max(newton, 1))
             log(f"[bench] cpu baseline {cpu}")
         except Exception as e:  # the baseline never blocks the GPU line
             log(f"[bench] cpu baseline failed: {e!r}")

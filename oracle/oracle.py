@@ -23,7 +23,7 @@ __all__ = [
     "finite_stress_update", "dPdF", "element_finite", "ref_grads", "box_mesh",
     "jitter_u01", "build_csc", "greedy_colors", "colour_lists", "on_face",
     "impose_dirichlet", "forcing_term", "ElementInversionError", "OracleModel",
-    "rigid_body_modes",
+    "rigid_body_modes", "OracleAMG",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -79,6 +79,16 @@ lib.or_assemble.argtypes = [C.c_int, C.POINTER(_Mat), C.c_int64, C.c_int64, _dp,
                             C.POINTER(C.c_double)]
 lib.or_assemble.restype = C.c_int
 lib.or_csc_spmv_omp.argtypes = [C.c_int64, _ip, _ip, _dp, _dp, _dp, C.c_int]
+lib.or_amg_setup.argtypes = [C.c_int64, _ip, _ip, _dp, _dp, np.ctypeslib.ndpointer(dtype=np.uint8),
+                             C.c_int64, C.c_int]
+lib.or_amg_setup.restype = C.c_void_p
+lib.or_amg_free.argtypes = [C.c_void_p]
+lib.or_amg_levels.argtypes = [C.c_void_p, _ip]
+lib.or_amg_levels.restype = C.c_int
+lib.or_amg_pcg.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_int]
+lib.or_amg_pcg.restype = C.c_int
+lib.or_amg_vcycle.argtypes = [C.c_void_p, _dp, _dp]
+lib.or_amg_apply_A.argtypes = [C.c_void_p, _dp, _dp]
 lib.or_impose_dirichlet.argtypes = [C.c_int64, _ip, _ip, _dp, _dp, C.c_int64, _ip, _dp, C.c_int,
                                     C.c_double, _dp]
 
@@ -251,6 +261,44 @@ def forcing_term(it, rk, rkm1, eta_prev, eta_min=1e-8, eta_max=0.1, gamma=0.9, a
     if safe > 0.1:
         eta = max(eta, safe)
     return min(max(eta, eta_min), eta_max)
+
+
+class OracleAMG:
+    """PCG + smoothed-aggregation AMG (rigid-body near-null space, Jacobi(2/3) smoother)
+    on the oracle's CSC K -- the reference's CG+AMG linear solve (Solver.jl:235-290)."""
+
+    def __init__(self, K, nodes, isbc, coarse_max=1000, nthreads=0):
+        self.K = K
+        self.n = K.shape[0]
+        self._h = lib.or_amg_setup(self.n, K.indptr, K.indices, K.data,
+                                   np.ascontiguousarray(nodes, dtype=np.float64),
+                                   np.ascontiguousarray(isbc, dtype=np.uint8), coarse_max, nthreads)
+
+    def levels(self):
+        sz = np.zeros(16, dtype=np.int64)
+        n = lib.or_amg_levels(self._h, sz)
+        return list(sz[:n])
+
+    def solve(self, b, rtol, itmax=200):
+        """-> (x, iterations); iterations < 0 flags a breakdown or no convergence."""
+        x = np.zeros(self.n)
+        it = lib.or_amg_pcg(self._h, _v(b), x, rtol, itmax)
+        return x, it
+
+    def apply_A(self, x):
+        y = np.zeros(self.n)
+        lib.or_amg_apply_A(self._h, _v(x), y)
+        return y
+
+    def vcycle(self, b):
+        x = np.zeros(self.n)
+        lib.or_amg_vcycle(self._h, _v(b), x)
+        return x
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib.or_amg_free(self._h)
+            self._h = None
 
 
 def rigid_body_modes(nodes):
