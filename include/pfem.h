@@ -80,7 +80,8 @@ typedef struct pfem_options {
     int32_t max_levels;         /* AMG level cap (default 10) */
     int32_t verbose;            /* print per-iteration lines to stderr */
     int32_t reserved[7];        /* reserved[0]: AMG rebuild policy, 0 = once per load step
-                                   (Solver.jl:374), 1 = every Newton iteration */
+                                   (Solver.jl:374), 1 = every Newton iteration, 2 = at the
+                                   first two Newton iterations of a load step */
 } pfem_options;
 
 typedef struct pfem_fail {

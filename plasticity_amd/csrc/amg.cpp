@@ -255,9 +255,16 @@ Amg *amg_create(pfem_ctx *c) {
             for (int32_t q = Qptr[i]; q < Qptr[i + 1]; ++q) Qrow[q] = (int32_t)i;
         for (int64_t I = 0; I < nagg; ++I)
             for (int32_t q = Cptr[I]; q < Cptr[I + 1]; ++q) Crow[q] = (int32_t)I;
+        L.nP = (int64_t)Pidx.size();
+        L.nQ = (int64_t)Qidx.size();
+        L.nR = (int64_t)Ridx.size();
         up(L.agg, agg, s);
         up(L.Pt, Pt, s);
-        up(L.P_rowptr, Pptr, s); up(L.P_colind, Pidx, s); L.P.alloc(std::max<size_t>(1, Pidx.size() * bs * 6));
+        std::vector<int32_t> Prow(Pidx.size());
+        for (int64_t i = 0; i < L.nb; ++i)
+            for (int32_t m = Pptr[i]; m < Pptr[i + 1]; ++m) Prow[m] = (int32_t)i;
+        up(L.P_rowptr, Pptr, s); up(L.P_colind, Pidx, s); up(L.P_row, Prow, s);
+        L.P.alloc(std::max<size_t>(1, Pidx.size() * bs * 6));
         up(L.R_rowptr, Rptr, s); up(L.R_colind, Ridx, s); up(L.R_src, Rsrc, s);
         L.R.alloc(std::max<size_t>(1, Ridx.size() * bs * 6));
         up(L.Q_rowptr, Qptr, s); up(L.Q_colind, Qidx, s); up(L.Q_row, Qrow, s);
@@ -303,28 +310,26 @@ void amg_destroy(Amg *a) { delete a; }
 // ---------------------------------------------------------------- numeric set-up
 namespace {
 
-// rho(D^-1 A) by power iteration from a fixed start vector.
+// rho(D^-1 A) by power iteration.  The iterate is kept between set-ups, so later
+// rebuilds (a slowly changing tangent) start from the previous dominant direction and
+// need few iterations; normalisation happens on the device, one host read at the end.
 double spectral_radius(pfem_ctx *c, Amg *a, int l, int iters) {
     AmgLevel &L = a->L[l];
     hipStream_t s = c->stream;
-    double *x = L.d.p, *y = L.r.p;
-    PF_HIP(hipMemcpyAsync(x, L.pw.p, L.n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    dot_dev<double, double>(L.n, x, x, c->partial.p, c->scal.p + 10, s);
-    double nx = std::sqrt(c->sync_read(c->scal.p + 10));
-    double lam = 0.0;
+    double *x = L.pw.p, *y = L.r.p;
+    double *nrm = c->scal.p + 10;
+    dot_dev<double, double>(L.n, x, x, c->partial.p, nrm, s);
     for (int it = 0; it < iters; ++it) {
-        axpby<double>(L.n, 1.0 / nx, x, 0.0, x, s);  // normalise
+        scale_by_rnorm(L.n, nrm, x, s);  // ||x|| = 1
         if (l == 0) sell_spmv<3, double, double>(c->K, c->K.v64.p, x, L.t.p, s);
         else bsr_spmv<6, 6, double, double>(L.nb, L.rowptr.p, L.colind.p, L.A.p, x, L.t.p, 0.0, s);
         if (L.bs == 3) block_diag_apply<3, double, double>(L.nb, L.dinv.p, L.t.p, y, 1.0, s);
         else block_diag_apply<6, double, double>(L.nb, L.dinv.p, L.t.p, y, 1.0, s);
-        dot_dev<double, double>(L.n, y, y, c->partial.p, c->scal.p + 10, s);
-        const double ny = std::sqrt(c->sync_read(c->scal.p + 10));
-        lam = ny;  // ||D^-1 A x|| with ||x|| = 1
-        std::swap(x, y);
-        nx = ny;
-        if (nx == 0.0) break;
+        dot_dev<double, double>(L.n, y, y, c->partial.p, nrm, s);  // ||D^-1 A x||^2
+        PF_HIP(hipMemcpyAsync(x, y, L.n * sizeof(double), hipMemcpyDeviceToDevice, s));
     }
+    const double lam = std::sqrt(c->sync_read(nrm));
+    L.pw_warm = true;
     return lam > 0 ? lam : 1.0;
 }
 
@@ -360,23 +365,23 @@ void amg_numeric_setup(pfem_ctx *c, Amg *a) {
         else amg_block_inverse<6>(L.nb, L.diagpos.p, Av, L.dinv.p, s);
         if (l + 1 == a->L.size()) break;  // coarsest: dense solve
         const double tr = now();
-        L.rho = spectral_radius(c, a, (int)l, 12);
+        L.rho = spectral_radius(c, a, (int)l, L.pw_warm ? 4 : 12);
         t_rho += now() - tr;
         const double omega = (4.0 / 3.0) / L.rho;
         AmgLevel &N = a->L[l + 1];
         if (L.bs == 3) {
-            amg_smooth_P<3, true>(L.nb, c->K.slice_ptr.p, c->K.rowlen.p, nullptr, c->K.colind.p, Av, L.agg.p, L.Pt.p,
-                                  L.dinv.p, omega, L.P_rowptr.p, L.P_colind.p, L.P.p, s);
-            amg_transpose_P<3>((int64_t)L.R_colind.n, L.R_src.p, L.P.p, L.R.p, s);
-            amg_AP<3, true>((int64_t)L.Q_colind.n, L.Q_row.p, L.Q_colind.p, c->K.slice_ptr.p, c->K.rowlen.p, nullptr,
+            amg_smooth_P<3, true>(L.nP, L.P_row.p, L.P_colind.p, c->K.slice_ptr.p, c->K.rowlen.p,
+                                  nullptr, c->K.colind.p, Av, L.agg.p, L.Pt.p, L.dinv.p, omega, L.P.p, s);
+            amg_transpose_P<3>(L.nR, L.R_src.p, L.P.p, L.R.p, s);
+            amg_AP<3, true>(L.nQ, L.Q_row.p, L.Q_colind.p, c->K.slice_ptr.p, c->K.rowlen.p, nullptr,
                             c->K.colind.p, Av, L.P_rowptr.p, L.P_colind.p, L.P.p, L.Q.p, s);
             amg_RAP<3>(N.nnzb, L.C_row.p, N.colind.p, L.R_rowptr.p, L.R_colind.p, L.R.p, L.Q_rowptr.p, L.Q_colind.p,
                        L.Q.p, N.A.p, s);
         } else {
-            amg_smooth_P<6, false>(L.nb, nullptr, nullptr, L.rowptr.p, L.colind.p, Av, L.agg.p, L.Pt.p, L.dinv.p,
-                                   omega, L.P_rowptr.p, L.P_colind.p, L.P.p, s);
-            amg_transpose_P<6>((int64_t)L.R_colind.n, L.R_src.p, L.P.p, L.R.p, s);
-            amg_AP<6, false>((int64_t)L.Q_colind.n, L.Q_row.p, L.Q_colind.p, nullptr, nullptr, L.rowptr.p,
+            amg_smooth_P<6, false>(L.nP, L.P_row.p, L.P_colind.p, nullptr, nullptr, L.rowptr.p,
+                                   L.colind.p, Av, L.agg.p, L.Pt.p, L.dinv.p, omega, L.P.p, s);
+            amg_transpose_P<6>(L.nR, L.R_src.p, L.P.p, L.R.p, s);
+            amg_AP<6, false>(L.nQ, L.Q_row.p, L.Q_colind.p, nullptr, nullptr, L.rowptr.p,
                              L.colind.p, Av, L.P_rowptr.p, L.P_colind.p, L.P.p, L.Q.p, s);
             amg_RAP<6>(N.nnzb, L.C_row.p, N.colind.p, L.R_rowptr.p, L.R_colind.p, L.R.p, L.Q_rowptr.p, L.Q_colind.p,
                        L.Q.p, N.A.p, s);

@@ -31,9 +31,10 @@ struct AmgLevel {
     double rho = 2.0;
     // prolongator to this level from the next: P (nb x nagg, bs x 6 blocks)
     int64_t nagg = 0;
+    int64_t nP = 0, nQ = 0, nR = 0;  // true block counts (buffers hold at least one entry)
     dbuf<int32_t> agg;           // node -> aggregate (-1 none)
     dbuf<double> Pt;             // tentative block per node (bs x 6), zero if agg < 0
-    dbuf<int32_t> P_rowptr, P_colind;
+    dbuf<int32_t> P_rowptr, P_colind, P_row;
     dbuf<double> P;
     dbuf<float> P32;
     // restriction R = P^T (nagg x nb, 6 x bs blocks)
@@ -46,7 +47,8 @@ struct AmgLevel {
     dbuf<int32_t> C_row;         // row of every block of the next level's operator
     // V-cycle vectors
     dbuf<double> x, b, r, d, t;
-    dbuf<double> pw;             // power-iteration start vector
+    dbuf<double> pw;             // power-iteration vector, kept between set-ups (warm start)
+    bool pw_warm = false;
     dbuf<float> x32, b32, r32, d32, t32;
 };
 
@@ -72,10 +74,9 @@ void amg_apply32(pfem_ctx *c, Amg *a, const float *r, float *z);
 template <int BS>
 void amg_block_inverse(int64_t nb, const int32_t *diagpos, const double *A, double *dinv, hipStream_t s);
 template <int BS, bool SELL>
-void amg_smooth_P(int64_t nb, const int64_t *slice_ptr, const int32_t *rowlen, const int32_t *rowptr,
-                  const int32_t *colind, const double *A, const int32_t *agg, const double *Pt,
-                  const double *dinv, double omega, const int32_t *P_rowptr, const int32_t *P_colind,
-                  double *P, hipStream_t s);
+void amg_smooth_P(int64_t nP, const int32_t *P_row, const int32_t *P_colind, const int64_t *slice_ptr,
+                  const int32_t *rowlen, const int32_t *rowptr, const int32_t *colind, const double *A,
+                  const int32_t *agg, const double *Pt, const double *dinv, double omega, double *P, hipStream_t s);
 template <int BS, bool SELL>
 void amg_AP(int64_t nQ, const int32_t *Q_row, const int32_t *Q_colind, const int64_t *slice_ptr,
             const int32_t *rowlen, const int32_t *rowptr, const int32_t *colind, const double *A,
@@ -103,6 +104,7 @@ void cheb_step(int64_t nb, const DT *dinv, const XT *Ad, XT *r, XT *d, XT *x, do
                hipStream_t s);  // r -= Ad; d = c1 d + c2 Dinv r; x += d
 template <class XT>
 void dense_gemv(int64_t n, const double *Ainv, const XT *b, XT *x, hipStream_t s);
+void scale_by_rnorm(int64_t n, const double *nrm2, double *x, hipStream_t s);
 // dense inverse of a block-sparse operator (SELL fine level or CSR of blocks) by
 // Gauss-Jordan with partial pivoting; W is 2 n^2, prow 2n, fcol n scratch
 void dense_coarse_inverse(int64_t nb, int bs, bool sell, const int64_t *slice_ptr, const int32_t *rowlen,

@@ -88,53 +88,52 @@ __global__ void block_inverse_kernel(int64_t nb, const int32_t *__restrict__ dia
         for (int c = 0; c < BS; ++c) o[BS * r + c] = v[r][c];
 }
 
-// P row i = Pt_i [at agg(i)] - omega * Dinv_i * sum_k A_ik Pt_{col k} [at agg(col k)]
+// P block (i, J) = Pt_i [if J = agg(i)] - omega * Dinv_i * sum_{k: agg(col k) = J} A_ik Pt_{col k};
+// one thread per P block, accumulated in registers.
 template <int BS, bool SELL>
-__global__ void smooth_P_kernel(int64_t nb, const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ rowlen,
+__global__ void smooth_P_kernel(int64_t nP, const int32_t *__restrict__ P_row, const int32_t *__restrict__ P_colind,
+                                const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ rowlen,
                                 const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colind,
                                 const double *__restrict__ A, const int32_t *__restrict__ agg,
                                 const double *__restrict__ Pt, const double *__restrict__ dinv, double omega,
-                                const int32_t *__restrict__ P_rowptr, const int32_t *__restrict__ P_colind,
                                 double *__restrict__ P) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nb) return;
-    const int p0 = P_rowptr[i], p1 = P_rowptr[i + 1];
-    if (p0 == p1) return;
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nP) return;
+    const int64_t i = P_row[m];
+    const int32_t J = P_colind[m];
     constexpr int B6 = BS * 6;
-    for (int m = p0; m < p1; ++m)
-        for (int t = 0; t < B6; ++t) P[(int64_t)B6 * m + t] = 0.0;
+    double acc[B6];
+#pragma unroll
+    for (int t = 0; t < B6; ++t) acc[t] = 0.0;
     RowIt<SELL> row(i, slice_ptr, rowlen, rowptr);
     for (int k = 0; k < row.len; ++k) {
         const int64_t pos = row.pos(k);
         const int32_t j = colind[pos];
-        const int32_t J = agg[j];
-        if (J < 0) continue;
-        const int m = find_sorted(P_colind, p0, p1, J);
-        if (m < 0) continue;
+        if (agg[j] != J) continue;
         const double *a = A + (int64_t)BS * BS * pos;
         const double *pt = Pt + (int64_t)B6 * j;
-        double *o = P + (int64_t)B6 * m;
+#pragma unroll
         for (int r = 0; r < BS; ++r)
+#pragma unroll
             for (int c = 0; c < 6; ++c) {
                 double s = 0.0;
+#pragma unroll
                 for (int q = 0; q < BS; ++q) s += a[BS * r + q] * pt[6 * q + c];
-                o[6 * r + c] += s;
+                acc[6 * r + c] += s;
             }
     }
     const double *di = dinv + (int64_t)BS * BS * i;
-    const int32_t Ji = agg[i];
-    for (int m = p0; m < p1; ++m) {
-        double *o = P + (int64_t)B6 * m;
-        double tmp[B6];
-        for (int r = 0; r < BS; ++r)
-            for (int c = 0; c < 6; ++c) {
-                double s = 0.0;
-                for (int q = 0; q < BS; ++q) s += di[BS * r + q] * o[6 * q + c];
-                tmp[6 * r + c] = s;
-            }
-        const bool own = (P_colind[m] == Ji);
-        for (int t = 0; t < B6; ++t) o[t] = (own ? Pt[(int64_t)B6 * i + t] : 0.0) - omega * tmp[t];
-    }
+    const bool own = (agg[i] == J);
+    double *o = P + (int64_t)B6 * m;
+#pragma unroll
+    for (int r = 0; r < BS; ++r)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            double s = 0.0;
+#pragma unroll
+            for (int q = 0; q < BS; ++q) s += di[BS * r + q] * acc[6 * q + c];
+            o[6 * r + c] = (own ? Pt[(int64_t)B6 * i + 6 * r + c] : 0.0) - omega * s;
+        }
 }
 
 // Q block (i, J) = sum_k A_ik P_{col k, J}
@@ -291,6 +290,13 @@ __global__ void cheb_step_kernel(int64_t nb, const DT *__restrict__ dinv, const 
     }
 }
 
+// x *= 1/sqrt(*nrm2) (power-iteration normalisation without a host round trip)
+__global__ void scale_by_rnorm_kernel(int64_t n, const double *__restrict__ nrm2, double *__restrict__ x) {
+    const double s = *nrm2 > 0.0 ? 1.0 / sqrt(*nrm2) : 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] *= s;
+}
+
 template <class XT>
 __global__ void dense_gemv_kernel(int64_t n, const double *__restrict__ Ainv, const XT *__restrict__ b,
                                   XT *__restrict__ x) {
@@ -413,11 +419,11 @@ void amg_block_inverse(int64_t nb, const int32_t *diagpos, const double *A, doub
     PF_HIP(hipGetLastError());
 }
 template <int BS, bool SELL>
-void amg_smooth_P(int64_t nb, const int64_t *slice_ptr, const int32_t *rowlen, const int32_t *rowptr,
-                  const int32_t *colind, const double *A, const int32_t *agg, const double *Pt, const double *dinv,
-                  double omega, const int32_t *P_rowptr, const int32_t *P_colind, double *P, hipStream_t s) {
-    hipLaunchKernelGGL((smooth_P_kernel<BS, SELL>), dim3(grid_for(nb, 128)), dim3(128), 0, s, nb, slice_ptr, rowlen,
-                       rowptr, colind, A, agg, Pt, dinv, omega, P_rowptr, P_colind, P);
+void amg_smooth_P(int64_t nP, const int32_t *P_row, const int32_t *P_colind, const int64_t *slice_ptr,
+                  const int32_t *rowlen, const int32_t *rowptr, const int32_t *colind, const double *A,
+                  const int32_t *agg, const double *Pt, const double *dinv, double omega, double *P, hipStream_t s) {
+    hipLaunchKernelGGL((smooth_P_kernel<BS, SELL>), dim3(grid_for(nP, 128)), dim3(128), 0, s, nP, P_row, P_colind,
+                       slice_ptr, rowlen, rowptr, colind, A, agg, Pt, dinv, omega, P);
     PF_HIP(hipGetLastError());
 }
 template <int BS, bool SELL>
@@ -482,6 +488,13 @@ void cheb_step(int64_t nb, const DT *dinv, const XT *Ad, XT *r, XT *d, XT *x, do
                        x, c1, c2);
     PF_HIP(hipGetLastError());
 }
+void scale_by_rnorm(int64_t n, const double *nrm2, double *x, hipStream_t s) {
+    int64_t g = (n + 255) / 256;
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(scale_by_rnorm_kernel, dim3((unsigned)(g < 1 ? 1 : g)), dim3(256), 0, s, n, nrm2, x);
+    PF_HIP(hipGetLastError());
+}
+
 template <class XT>
 void dense_gemv(int64_t n, const double *Ainv, const XT *b, XT *x, hipStream_t s) {
     hipLaunchKernelGGL((dense_gemv_kernel<XT>), dim3(grid_for(n, 4)), dim3(256), 0, s, n, Ainv, b, x);
@@ -492,9 +505,9 @@ void dense_gemv(int64_t n, const double *Ainv, const XT *b, XT *x, hipStream_t s
 template void amg_block_inverse<3>(int64_t, const int32_t *, const double *, double *, hipStream_t);
 template void amg_block_inverse<6>(int64_t, const int32_t *, const double *, double *, hipStream_t);
 #define PF_SM(BS, SELL)                                                                                            \
-    template void amg_smooth_P<BS, SELL>(int64_t, const int64_t *, const int32_t *, const int32_t *,                \
-                                         const int32_t *, const double *, const int32_t *, const double *,          \
-                                         const double *, double, const int32_t *, const int32_t *, double *,        \
+    template void amg_smooth_P<BS, SELL>(int64_t, const int32_t *, const int32_t *, const int64_t *,                \
+                                         const int32_t *, const int32_t *, const int32_t *, const double *,         \
+                                         const int32_t *, const double *, const double *, double, double *,         \
                                          hipStream_t);                                                              \
     template void amg_AP<BS, SELL>(int64_t, const int32_t *, const int32_t *, const int64_t *, const int32_t *,     \
                                    const int32_t *, const int32_t *, const double *, const int32_t *,               \

@@ -364,8 +364,10 @@ int pfem_newton(pfem_ctx *c, double lambda, double tol, int maxiter, pfem_newton
             const double eta = forcing_term(c->opt, it, rnorm, rkm1, eta_prev);  // :371
             // :374 rebuilds once per load step; amg_rebuild = 1 rebuilds from every
             // Newton iteration's tangent (the plastic iterations then need about half the
-            // Krylov iterations of a hierarchy built from the elastic predictor tangent)
-            const bool rebuild = first || c->opt.reserved[0] == 1;
+            // Krylov iterations of a hierarchy built from the elastic predictor tangent);
+            // amg_rebuild = 2 rebuilds at the first and second iteration (the second is the
+            // first plastic tangent) and reuses that hierarchy afterwards
+            const bool rebuild = first || c->opt.reserved[0] == 1 || (c->opt.reserved[0] == 2 && it == 2);
             first = false;
             axpby<double>(c->ndof, -1.0, c->R.p, 0.0, c->rhs.p, s);  // :378
             t = now();
