@@ -364,6 +364,7 @@ void amg_numeric_setup(pfem_ctx *c, Amg *a) {
         if (L.bs == 3) amg_block_inverse<3>(L.nb, fine ? c->K.diagpos.p : L.diagpos.p, Av, L.dinv.p, s);
         else amg_block_inverse<6>(L.nb, L.diagpos.p, Av, L.dinv.p, s);
         if (l + 1 == a->L.size()) break;  // coarsest: dense solve
+        c->sync();
         const double tr = now();
         L.rho = spectral_radius(c, a, (int)l, L.pw_warm ? 4 : 12);
         t_rho += now() - tr;

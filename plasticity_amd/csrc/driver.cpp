@@ -285,6 +285,15 @@ int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *us
         which = 3;
         rc = ir_solve(c, c->rhs.p, c->dU.p, rtol, 4 * itmax, &it, &which);
         total += it;
+        if (rc != 0) {  // FP32 refinement failed: the FP64 ladder takes over
+            which = 2;
+            rc = pcg<double>(c, c->rhs.p, c->dU.p, rtol, itmax, &it);
+            total += it;
+            if (rc != 0) {
+                rc = bicgstab<double>(c, c->rhs.p, c->dU.p, rtol, 2 * itmax, &it);
+                total += it;
+            }
+        }
     } else {
         rc = pcg<double>(c, c->rhs.p, c->dU.p, rtol, itmax, &it);
         total += it;
@@ -308,6 +317,10 @@ int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *us
     c->sync();
     const double rr = std::sqrt(c->h_scal[5]), bb = std::sqrt(c->h_scal[6]);
     if (relres) *relres = bb > 0 ? rr / bb : 0.0;
+    if (!(rr <= bb)) {  // a failed solve that made things worse is not applied
+        fill<double>(c->ndof, 0.0, c->dU.p, s);
+        c->sync();
+    }
     if (niter) *niter = total;
     if (used) *used = which;
     c->st.t_solve += now() - t1;
