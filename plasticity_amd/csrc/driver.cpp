@@ -225,10 +225,69 @@ static int bicgstab(pfem_ctx *c, const T *b, T *x, double rtol, int itmax, int *
     return 1;
 }
 
+// FP32 PCG with FP64 iterative refinement folded into the iteration ("reliable
+// updates", van der Vorst & Ye, SIAM J. Sci. Comput. 22, 2000): the search vectors,
+// the SpMV with the FP32 copy of K and the FP32 V-cycle drive the iteration; the
+// solution accumulates in FP64, and whenever the recursive FP32 residual has dropped
+// by 1e-3 (or every 25 iterations) the true residual r = b - K x is recomputed with the
+// FP64 K and replaces it -- refinement without discarding the Krylov directions.
+// Stops on ||b - K x|| <= rtol ||b|| measured in FP64.  0 solved, 1 itmax, 2 breakdown.
+static int pcg_refined(pfem_ctx *c, const double *b, double *x, double rtol, int itmax, int *iters) {
+    hipStream_t s = c->stream;
+    const int64_t n = c->ndof;
+    float *r = c->f_r.p, *z = c->f_z.p, *p = c->f_p.p, *q = c->f_q.p;
+    double *r64 = c->w_s.p, *sc = c->scal.p;
+    fill<double>(n, 0.0, x, s);
+    dot_dev<double, double>(n, b, b, c->partial.p, sc + 4, s);
+    const double bn = std::sqrt(c->sync_read(sc + 4));
+    *iters = 0;
+    if (bn == 0.0) return 0;
+    convert<double, float>(n, b, r, s);
+    Prec<float>::pc(c, r, z);
+    dot_dev<float, float>(n, r, z, c->partial.p, sc + 0, s);
+    double rz = c->sync_read(sc + 0);
+    if (!(rz > 0)) return 2;
+    double rz_ref = rz;
+    int since = 0;
+    PF_HIP(hipMemcpyAsync(p, z, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    for (int it = 1; it <= itmax; ++it) {
+        *iters = it;
+        Prec<float>::spmv_dot(c, p, q, sc + 1);
+        const double pq = c->sync_read(sc + 1);
+        if (!(pq > 0)) return 2;
+        cg_update_mixed(n, sc + 0, sc + 1, p, q, x, r, s);  // x64 += a p ; r32 -= a q
+        Prec<float>::pc(c, r, z);
+        dot_dev<float, float>(n, r, z, c->partial.p, sc + 2, s);
+        double rzn = c->sync_read(sc + 2);
+        if (!(rzn >= 0)) return 2;
+        ++since;
+        if (rzn <= 1e-6 * rz_ref || since >= 25 || rzn <= rtol * rtol * rz_ref) {
+            residual_fine64(c->K, x, b, r64, s);  // FP64 refinement residual
+            dot_dev<double, double>(n, r64, r64, c->partial.p, sc + 4, s);
+            const double rn = std::sqrt(c->sync_read(sc + 4));
+            if (c->opt.verbose > 1) fprintf(stderr, "      refine at it %3d  |b-Kx|/|b| = %.3e\n", it, rn / bn);
+            if (rn <= rtol * bn) return 0;
+            convert<double, float>(n, r64, r, s);  // residual replacement
+            Prec<float>::pc(c, r, z);
+            dot_dev<float, float>(n, r, z, c->partial.p, sc + 2, s);
+            rzn = c->sync_read(sc + 2);
+            if (!(rzn > 0)) return 2;
+            rz_ref = rzn;
+            since = 0;
+        }
+        xpby_ratio<float>(n, sc + 2, sc + 0, z, p, s);  // p = z + (rzn/rz) p
+        PF_HIP(hipMemcpyAsync(sc + 0, sc + 2, sizeof(double), hipMemcpyDeviceToDevice, s));
+        rz = rzn;
+    }
+    return 1;
+}
+
 // FP64 iterative refinement around FP32 Krylov: r = b - K x in FP64, inner FP32
-// solve (PCG, or BiCGStab once PCG has broken down on this system) of K d = r to a
-// modest relative accuracy, x += d in FP64, until ||b - K x|| <= rtol ||b||.
-static int ir_solve(pfem_ctx *c, const double *b, double *x, double rtol, int itmax, int *iters, int *used) {
+// solve (PCG, or BiCGStab once PCG has broken down on this system or when
+// `indefinite` is passed in) of K d = r to a modest relative accuracy, x += d in FP64,
+// until ||b - K x|| <= rtol ||b||.
+static int ir_solve(pfem_ctx *c, const double *b, double *x, double rtol, int itmax, bool indefinite, int *iters,
+                    int *used) {
     hipStream_t s = c->stream;
     const int64_t n = c->ndof;
     double *r = c->w_s.p, *sc = c->scal.p;
@@ -239,7 +298,6 @@ static int ir_solve(pfem_ctx *c, const double *b, double *x, double rtol, int it
     if (bn == 0.0) return 0;
     PF_HIP(hipMemcpyAsync(r, b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
     double rn = bn;
-    bool indefinite = false;
     const double inner_tol = 1e-4;
     for (int outer = 0; outer < 40; ++outer) {
         convert<double, float>(n, r, c->f_b.p, s);
@@ -283,8 +341,12 @@ int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *us
         if (!c->K.v32.p) c->K.v32.alloc(9 * c->K.nslots);
         convert<double, float>(9 * c->K.nslots, c->K.v64.p, c->K.v32.p, s);  // FP32 copy of the imposed K
         which = 3;
-        rc = ir_solve(c, c->rhs.p, c->dU.p, rtol, 4 * itmax, &it, &which);
+        rc = pcg_refined(c, c->rhs.p, c->dU.p, rtol, 2 * itmax, &it);
         total += it;
+        if (rc != 0) {  // indefinite (2) or stalled (1): refinement sweeps, BiCGStab after a breakdown
+            rc = ir_solve(c, c->rhs.p, c->dU.p, rtol, 4 * itmax, rc == 2, &it, &which);
+            total += it;
+        }
         if (rc != 0) {  // FP32 refinement failed: the FP64 ladder takes over
             which = 2;
             rc = pcg<double>(c, c->rhs.p, c->dU.p, rtol, itmax, &it);

@@ -14,6 +14,15 @@ namespace pfem {
 
 enum SellMode { SELL_Y = 0, SELL_RES = 1, SELL_DOT = 2 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2): group b
+// runs on XCD b % 8.  Remap so that XCD x works through one contiguous eighth of the
+// slices; the x-vector neighbours of a slice (+-1, +-row, +-plane) then mostly live in
+// the same L2.  Bijective on [0, G).
+__device__ __forceinline__ int64_t xcd_contiguous(int64_t b, int64_t G) {
+    const int64_t q = G >> 3, r = G & 7, x = b & 7, k = b >> 3;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
+
 template <class VT, class XT, int MODE>
 __global__ __launch_bounds__(64) void sell_lds_kernel(int64_t nb, const int64_t *__restrict__ slice_ptr,
                                                       const int32_t *__restrict__ colind,
@@ -23,7 +32,7 @@ __global__ __launch_bounds__(64) void sell_lds_kernel(int64_t nb, const int64_t 
     constexpr int NV = SELL_C * 9 * (int)sizeof(VT) / 16;  // 16-byte chunks per slice column
     __shared__ __attribute__((aligned(16))) VT tile[SELL_C * 9];
     const int lane = threadIdx.x;
-    const int64_t s = blockIdx.x;
+    const int64_t s = xcd_contiguous(blockIdx.x, gridDim.x);
     const int64_t i = s * SELL_C + lane;
     const int64_t base = slice_ptr[s];
     const int w = (int)((slice_ptr[s + 1] - base) / SELL_C);

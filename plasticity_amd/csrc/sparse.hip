@@ -88,6 +88,15 @@ __global__ void cg_update_kernel(int64_t n, const double *num, const double *den
         r[i] = (T)((double)r[i] - al * (double)q[i]);
     }
 }
+// x64 += a p32 ; r32 -= a q32  (a = num/den): FP32 Krylov vectors, FP64 solution
+__global__ void cg_update_mixed_kernel(int64_t n, const double *num, const double *den, const float *__restrict__ p,
+                                       const float *__restrict__ q, double *__restrict__ x, float *__restrict__ r) {
+    const double al = *num / *den;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        x[i] += al * (double)p[i];
+        r[i] = (float)((double)r[i] - al * (double)q[i]);
+    }
+}
 template <class T>
 __global__ void axpby_kernel(int64_t n, double a, const T *__restrict__ x, double b, T *__restrict__ y) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -278,6 +287,11 @@ template <class T>
 void cg_update(int64_t n, const double *num, const double *den, const T *p, const T *q, T *x, T *r,
                hipStream_t s) {
     hipLaunchKernelGGL((cg_update_kernel<T>), dim3(grid_strided(n)), dim3(256), 0, s, n, num, den, p, q, x, r);
+    PF_HIP(hipGetLastError());
+}
+void cg_update_mixed(int64_t n, const double *num, const double *den, const float *p, const float *q, double *x,
+                     float *r, hipStream_t s) {
+    hipLaunchKernelGGL(cg_update_mixed_kernel, dim3(grid_strided(n)), dim3(256), 0, s, n, num, den, p, q, x, r);
     PF_HIP(hipGetLastError());
 }
 template <class T>
