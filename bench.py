@@ -27,6 +27,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "s/Newton step @3M DOF, rtol 1e-8: FP64 CG+AMG vs FP32-IR+learned PC"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per fine-SpMV launch at 100^3 (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors
+# from a 1 GiB device copy), measured with rocprofv3 --pmc on `--mode spmv --n 100`:
+# 1,291,914.9 KiB fetched + 24,148.5 KiB written per dispatch.
+SPMV_TRAFFIC_100 = (1291914.9 * 2.0 + 24148.5) * 1024
 FP64_PEAK_TFLOPS = 78.6
 
 
@@ -184,7 +188,7 @@ def main():
     nnzb, nb, ndof = s1.nnzb, s1.nnodes, s1.ndof
     n_sp = s1.n_pcg_spmv - s0.n_pcg_spmv
     t_sp = (s1.t_pcg_spmv - s0.t_pcg_spmv) / max(n_sp, 1)
-    bytes_sp = spmv_bytes(nnzb, nb, ndof, 8 if args.precision == "fp64" else 4)
+    bytes_sp = spmv_bytes(nnzb, nb, ndof, 8)  # the Krylov operator is the FP64 K in both modes
     achieved = bytes_sp / t_sp / 1e9 if n_sp else None
     n_asm = s1.n_assemble - s0.n_assemble
     gp_ms = (s1.t_gp_kernel - s0.t_gp_kernel) / max(n_asm, 1) * 1e3
@@ -215,9 +219,10 @@ def main():
                    "newton_steps_timed": newton, "krylov_its_timed": krylov,
                    "t_assemble_s": t_asm, "t_amg_setup_s": t_setup, "t_solve_s": t_solve,
                    "amg_levels": list(s1.amg_ndof[:s1.amg_levels])},
-        "roofline": {"kernel": "fine BSR(3x3) SpMV (sell_spmv_kernel, FP64) in PCG", "bound": "hbm",
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+        "roofline": {"kernel": "fine BSR(3x3) SpMV (sell_lds_kernel<double,double,2>, FP64) in PCG",
+                     "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                     "traffic": SPMV_TRAFFIC_100 if args.n == 100 else None,
                      "bytes_per_launch": bytes_sp, "avg_launch_ms": t_sp * 1e3},
         "cpu_baseline": cpu,
     }

@@ -8,7 +8,10 @@
 //                     its initial value, atol = 0), refresh-and-retry, then a
 //                     BiCGStab fallback with the same preconditioner in place of the
 //                     reference's direct solve (a 3M-DOF LU is not an option on the
-//                     device); or FP32 PCG inside FP64 iterative refinement.
+//                     device); or, in FP32-IR mode, FP64 flexible PCG whose
+//                     preconditioner is the FP32 V-cycle (each iteration refines the
+//                     FP64 solution with an FP32 correction), then FP32 Krylov inside
+//                     FP64 iterative refinement, then the FP64 ladder.
 //  pfem_newton        newton! (Solver.jl:317-389) with the Eisenstat-Walker forcing
 //                     term (:159-171) and the reference-scaled convergence test.
 #include <algorithm>
@@ -225,62 +228,79 @@ static int bicgstab(pfem_ctx *c, const T *b, T *x, double rtol, int itmax, int *
     return 1;
 }
 
-// FP32 PCG with FP64 iterative refinement folded into the iteration ("reliable
-// updates", van der Vorst & Ye, SIAM J. Sci. Comput. 22, 2000): the search vectors,
-// the SpMV with the FP32 copy of K and the FP32 V-cycle drive the iteration; the
-// solution accumulates in FP64, and whenever the recursive FP32 residual has dropped
-// by 1e-3 (or every 25 iterations) the true residual r = b - K x is recomputed with the
-// FP64 K and replaces it -- refinement without discarding the Krylov directions.
-// Stops on ||b - K x|| <= rtol ||b|| measured in FP64.  0 solved, 1 itmax, 2 breakdown.
-static int pcg_refined(pfem_ctx *c, const double *b, double *x, double rtol, int itmax, int *iters) {
+// Flexible PCG (FCG(1), Notay, SIAM J. Sci. Comput. 22, 2000) in FP64 with any
+// preconditioner z = M(r): beta = -(z_new' q)/(p' q), so a preconditioner that varies
+// from one application to the next (an FP32 V-cycle, a network) keeps the iteration
+// convergent.  Stop test as pcg() (sqrt(r'z) relative to its initial value) when
+// `pc_norm`, else ||r|| <= rtol ||b|| (preconditioners that need not be SPD).
+// 0 solved, 1 itmax, 2 breakdown; stalls (no residual decrease over `stall` iterations)
+// return 3 so that the caller can fall back.
+template <class PC>
+static int fpcg(pfem_ctx *c, const double *b, double *x, double rtol, int itmax, bool pc_norm, int stall,
+                int *iters, PC &&pcfun) {
     hipStream_t s = c->stream;
     const int64_t n = c->ndof;
-    float *r = c->f_r.p, *z = c->f_z.p, *p = c->f_p.p, *q = c->f_q.p;
-    double *r64 = c->w_s.p, *sc = c->scal.p;
+    double *r = c->w_r.p, *z = c->w_z.p, *p = c->w_p.p, *q = c->w_q.p, *sc = c->scal.p;
     fill<double>(n, 0.0, x, s);
-    dot_dev<double, double>(n, b, b, c->partial.p, sc + 4, s);
-    const double bn = std::sqrt(c->sync_read(sc + 4));
+    PF_HIP(hipMemcpyAsync(r, b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    pcfun(r, z);
+    dot_dev<double, double>(n, r, z, c->partial.p, sc + 0, s);
+    dot_dev<double, double>(n, r, r, c->partial.p, sc + 3, s);
+    PF_HIP(hipMemcpyAsync(c->h_scal, sc, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+    c->sync();
+    double rz = c->h_scal[0], rr = c->h_scal[3];
     *iters = 0;
-    if (bn == 0.0) return 0;
-    convert<double, float>(n, b, r, s);
-    Prec<float>::pc(c, r, z);
-    dot_dev<float, float>(n, r, z, c->partial.p, sc + 0, s);
-    double rz = c->sync_read(sc + 0);
-    if (!(rz > 0)) return 2;
-    double rz_ref = rz;
-    int since = 0;
-    PF_HIP(hipMemcpyAsync(p, z, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    if (rr == 0.0) return 0;
+    if (pc_norm && !(rz > 0)) return 2;
+    const double eps = pc_norm ? rtol * std::sqrt(rz) : rtol * std::sqrt(rr);
+    double best = std::sqrt(rr);
+    int best_it = 0;
+    PF_HIP(hipMemcpyAsync(p, z, n * sizeof(double), hipMemcpyDeviceToDevice, s));
     for (int it = 1; it <= itmax; ++it) {
         *iters = it;
-        Prec<float>::spmv_dot(c, p, q, sc + 1);
-        const double pq = c->sync_read(sc + 1);
-        if (!(pq > 0)) return 2;
-        cg_update_mixed(n, sc + 0, sc + 1, p, q, x, r, s);  // x64 += a p ; r32 -= a q
-        Prec<float>::pc(c, r, z);
-        dot_dev<float, float>(n, r, z, c->partial.p, sc + 2, s);
-        double rzn = c->sync_read(sc + 2);
-        if (!(rzn >= 0)) return 2;
-        ++since;
-        if (rzn <= 1e-6 * rz_ref || since >= 25 || rzn <= rtol * rtol * rz_ref) {
-            residual_fine64(c->K, x, b, r64, s);  // FP64 refinement residual
-            dot_dev<double, double>(n, r64, r64, c->partial.p, sc + 4, s);
-            const double rn = std::sqrt(c->sync_read(sc + 4));
-            if (c->opt.verbose > 1) fprintf(stderr, "      refine at it %3d  |b-Kx|/|b| = %.3e\n", it, rn / bn);
-            if (rn <= rtol * bn) return 0;
-            convert<double, float>(n, r64, r, s);  // residual replacement
-            Prec<float>::pc(c, r, z);
-            dot_dev<float, float>(n, r, z, c->partial.p, sc + 2, s);
-            rzn = c->sync_read(sc + 2);
-            if (!(rzn > 0)) return 2;
-            rz_ref = rzn;
-            since = 0;
+        PF_HIP(hipEventRecord(c->ev0, s));
+        Prec<double>::spmv_dot(c, p, q, sc + 1);  // q = K p, sc[1] = p'q
+        PF_HIP(hipEventRecord(c->ev1, s));
+        cg_update<double>(n, sc + 0, sc + 1, p, q, x, r, s);  // x += a p, r -= a q
+        pcfun(r, z);
+        dot_dev<double, double>(n, r, z, c->partial.p, sc + 2, s);  // r'z
+        dot_dev<double, double>(n, z, q, c->partial.p, sc + 4, s);  // z'q
+        dot_dev<double, double>(n, r, r, c->partial.p, sc + 3, s);  // r'r
+        PF_HIP(hipMemcpyAsync(c->h_scal, sc, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
+        c->sync();
+        const double pq = c->h_scal[1], rzn = c->h_scal[2], rrn = c->h_scal[3];
+        float ms = 0.f;
+        PF_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->st.t_pcg_spmv += 1e-3 * ms;
+        c->st.n_pcg_spmv++;
+        if (!(pq > 0) || (pc_norm && !(rzn >= 0))) {
+            if (c->opt.verbose > 1) fprintf(stderr, "      fpcg breakdown p'Kp = %.3e r'z = %.3e at it %d\n", pq, rzn, it);
+            return 2;
         }
-        xpby_ratio<float>(n, sc + 2, sc + 0, z, p, s);  // p = z + (rzn/rz) p
+        const double nrm = pc_norm ? std::sqrt(rzn) : std::sqrt(rrn);
+        if (c->opt.verbose > 2) fprintf(stderr, "      fpcg %3d  %.3e  (target %.3e)\n", it, nrm, eps);
+        if (nrm <= eps) return 0;
+        if (std::sqrt(rrn) < 0.999 * best) { best = std::sqrt(rrn); best_it = it; }
+        else if (stall > 0 && it - best_it >= stall) return 3;
+        // p = z + beta p, beta = -(z'q)/(p'q)
+        const double beta = -c->h_scal[4] / pq;
+        axpby<double>(n, 1.0, z, beta, p, s);
         PF_HIP(hipMemcpyAsync(sc + 0, sc + 2, sizeof(double), hipMemcpyDeviceToDevice, s));
         rz = rzn;
     }
     return 1;
 }
+
+// The FP32 V-cycle as a preconditioner of FP64 vectors: r -> FP32, V-cycle on the FP32
+// copies of every level, z -> FP64.
+struct Amg32Pc {
+    pfem_ctx *c;
+    void operator()(const double *r, double *z) const {
+        convert<double, float>(c->ndof, r, c->f_r.p, c->stream);
+        amg_apply32(c, c->amg.get(), c->f_r.p, c->f_z.p);
+        convert<float, double>(c->ndof, c->f_z.p, z, c->stream);
+    }
+};
 
 // FP64 iterative refinement around FP32 Krylov: r = b - K x in FP64, inner FP32
 // solve (PCG, or BiCGStab once PCG has broken down on this system or when
@@ -341,7 +361,7 @@ int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *us
         if (!c->K.v32.p) c->K.v32.alloc(9 * c->K.nslots);
         convert<double, float>(9 * c->K.nslots, c->K.v64.p, c->K.v32.p, s);  // FP32 copy of the imposed K
         which = 3;
-        rc = pcg_refined(c, c->rhs.p, c->dU.p, rtol, 2 * itmax, &it);
+        rc = fpcg(c, c->rhs.p, c->dU.p, rtol, itmax, true, 0, &it, Amg32Pc{c});
         total += it;
         if (rc != 0) {  // indefinite (2) or stalled (1): refinement sweeps, BiCGStab after a breakdown
             rc = ir_solve(c, c->rhs.p, c->dU.p, rtol, 4 * itmax, rc == 2, &it, &which);
