@@ -128,8 +128,9 @@ def main():
                          "copy: a plain 1 GiB device-to-device copy, the calibration of byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only; --gpus must be 1 "
-              "(see DESIGN.md for how the path would shard)", file=sys.stderr)
+        print("bench.py: this implementation runs the Newton-step path on one GPU only; --gpus must be 1 "
+              "(multi-GPU sharding by element slabs with node halos is designed but not built)",
+              file=sys.stderr)
         sys.exit(2)
 
     if args.mode == "copy":

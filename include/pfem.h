@@ -60,7 +60,9 @@ typedef struct pfem_mesh_desc {
 
 typedef enum pfem_precision {
     PFEM_PREC_FP64 = 0,         /* plain FP64 PCG + SA-AMG (the drop-in) */
-    PFEM_PREC_FP32_IR = 1       /* FP32 PCG + FP32 V-cycle inside FP64 iterative refinement */
+    PFEM_PREC_FP32_IR = 1       /* FP64 flexible PCG refined by FP32 V-cycle corrections (FP32 copies of
+                                   every AMG level and of the fine K); FP32 Krylov inside FP64 iterative
+                                   refinement after a breakdown; the FP64 ladder last */
 } pfem_precision;
 
 typedef enum pfem_smoother {
@@ -138,8 +140,8 @@ int pfem_residual_norm(pfem_ctx *ctx, double *out);
 
 /* _linear_solve!(dU, K, b = -R, ls, rtol, rebuild_pc) (Solver.jl:235-290).
  * dU stays on the device; niter = Krylov iterations (all attempts);
- * solver_used: 0 PCG, 1 PCG after AMG refresh, 2 BiCGStab fallback,
- * 3 FP32-IR with PCG, 4 FP32-IR with BiCGStab.  true_relres = ||b - K dU|| / ||b||. */
+ * solver_used: 0 PCG, 1 PCG after AMG refresh, 2 BiCGStab fallback (or the FP64
+ * ladder after FP32-IR failed), 3 FP32-IR flexible PCG, 4 FP32-IR with BiCGStab.  true_relres = ||b - K dU|| / ||b||. */
 int pfem_linear_solve(pfem_ctx *ctx, double rtol, int rebuild_pc, int *niter,
                       int *solver_used, double *true_relres);
 

@@ -5,7 +5,7 @@
  * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg only.
  *
  * Every function cites the reference file:line (lagrangian/src/...) it restates.
- * Deliberate numerical deviation (documented in DESIGN.md):
+ * Deliberate numerical deviation:
  *   _dhalflog_contract's off-diagonal divided difference (ln bA - ln bB)/(2(bA-bB))
  *   is evaluated as 0.5*log1p((bA-bB)/bB)/(bA-bB), which is the same quantity
  *   without the cancellation of two nearly equal logarithms; the reference's
