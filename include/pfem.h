@@ -141,7 +141,9 @@ int pfem_residual_norm(pfem_ctx *ctx, double *out);
 /* _linear_solve!(dU, K, b = -R, ls, rtol, rebuild_pc) (Solver.jl:235-290).
  * dU stays on the device; niter = Krylov iterations (all attempts);
  * solver_used: 0 PCG, 1 PCG after AMG refresh, 2 BiCGStab fallback (or the FP64
- * ladder after FP32-IR failed), 3 FP32-IR flexible PCG, 4 FP32-IR with BiCGStab.  true_relres = ||b - K dU|| / ||b||. */
+ * ladder after FP32-IR failed), 3 FP32-IR flexible PCG, 4 FP32-IR with BiCGStab,
+ * 5 flexible CG with the learned preconditioner, 6 the same finished by AMG.
+ * true_relres = ||b - K dU|| / ||b||. */
 int pfem_linear_solve(pfem_ctx *ctx, double rtol, int rebuild_pc, int *niter,
                       int *solver_used, double *true_relres);
 
@@ -205,6 +207,51 @@ int pfem_get_stats(pfem_ctx *ctx, pfem_stats *st);
 /* Time `reps` fine-level SpMVs y = K x (FP64 if prec == 0, FP32 if 1) on the
  * context's stream with events; returns mean seconds per SpMV. */
 int pfem_bench_spmv(pfem_ctx *ctx, int prec, int reps, double *sec_per_spmv);
+
+/* ---------------------------------------------------------------------------
+ * Learned preconditioner.  No counterpart in the reference: the network and
+ * its training are additions beside the drop-in path.  A bias-free 3D
+ * convolutional U-net (4 levels, channels[0..3] per level, ReLU, additive skip
+ * connections) on the structured node grid of a box mesh (node index
+ * x + nx*(y + ny*z)); BF16 weights and activations on MFMA with FP32
+ * accumulation; hand-written forward, backward and Adam kernels.  It maps the
+ * 3-component nodal residual to a correction; inside FP64 flexible CG it
+ * replaces the V-cycle, and AMG takes over from the current iterate when the
+ * iteration stalls.  Trained on (residual, V-cycle correction) samples that
+ * the solver logs from its own AMG-preconditioned solves, with the
+ * scale-invariant loss 1 - cos(N(r), z).
+ * ------------------------------------------------------------------------- */
+typedef struct pfem_net pfem_net;
+
+typedef struct pfem_net_desc {
+    int32_t nx, ny, nz;         /* node grid (x fastest) */
+    int32_t channels[4];        /* per level, each 16, 32 or 64 (default 16, 32, 64, 64) */
+    int64_t capacity;           /* sample store: (residual, correction) pairs kept on the device */
+    uint64_t seed;              /* weight initialisation (He normal) */
+} pfem_net_desc;
+
+int pfem_net_create(const pfem_net_desc *desc, pfem_net **out);
+void pfem_net_destroy(pfem_net *net);
+int64_t pfem_net_num_params(const pfem_net *net);
+/* FP32 master weights, layer by layer, each [tap][c_in][c_out]. */
+int pfem_net_get_params(pfem_net *net, float *w);
+int pfem_net_set_params(pfem_net *net, const float *w);   /* also resets Adam */
+/* out = N(in) for host 3-channel fields (3 * nx*ny*nz floats, node-major). */
+int pfem_net_forward(pfem_net *net, const float *in, float *out);
+/* loss = 1 - cos(N(in), target) and its gradient w.r.t. the weights (nparams
+ * floats, may be NULL); inputs are rounded to BF16 like logged samples. */
+int pfem_net_loss_grad(pfem_net *net, const float *in, const float *target, double *loss, float *grad);
+/* Append one sample (host data) to the store (ring buffer). */
+int pfem_net_add_pair(pfem_net *net, const float *r, const float *z);
+int pfem_net_num_pairs(const pfem_net *net, int64_t *count);
+/* `steps` Adam steps (beta 0.9/0.999, eps 1e-8), each over `batch` samples drawn
+ * from the store; mean_loss = mean loss over the samples seen. */
+int pfem_net_train(pfem_net *net, int steps, int batch, double lr, uint64_t seed, double *mean_loss);
+/* mode 0: detach; 1: log (r, V-cycle(r)) samples from every `stride`-th
+ * preconditioner application of pfem_linear_solve / pfem_newton into the
+ * net's store; 2: precondition pfem_linear_solve with the network (flexible
+ * CG, AMG fallback).  The net's grid must hold exactly the context's nodes. */
+int pfem_attach_net(pfem_ctx *ctx, pfem_net *net, int mode, int stride);
 
 /* Human-readable text for a status. */
 const char *pfem_status_string(int status);

@@ -73,7 +73,7 @@ lib.or_jitter_u01.restype = C.c_double
 lib.or_build_csc.argtypes = [C.c_int64, C.c_int64, _ip, _ip, C.c_void_p, C.POINTER(C.c_int64)]
 lib.or_greedy_colors.argtypes = [C.c_int64, C.c_int64, _ip, _i32p]
 lib.or_greedy_colors.restype = C.c_int
-lib.or_assemble.argtypes = [C.c_int, C.POINTER(_Mat), C.c_int64, C.c_int64, _dp, _ip, _ip, _ip,
+lib.or_assemble.argtypes = [C.c_int, C.POINTER(_Mat), C.c_void_p, C.c_int64, C.c_int64, _dp, _ip, _ip, _ip,
                             C.c_int, _ip, _ip, _dp, _dp, _dp, _dp, _dp, C.c_int, _dp, _dp,
                             C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int),
                             C.POINTER(C.c_double)]
@@ -325,11 +325,17 @@ class OracleModel:
     default (Solver.jl:237-240).
     """
 
-    def __init__(self, nodes, conn, mat, element="finite_fbar", nthreads=0):
+    def __init__(self, nodes, conn, mat, element="finite_fbar", nthreads=0, material_id=None):
         import scipy.sparse as sp
         self.nodes = np.ascontiguousarray(nodes, dtype=np.float64)
         self.conn = np.ascontiguousarray(conn, dtype=np.int64)
-        self.mat = mat
+        mats = list(mat) if isinstance(mat, (list, tuple)) else [mat]
+        self.mat = mats[0]
+        self._mats = (_Mat * len(mats))(*[m._m for m in mats])
+        self.matid = None if material_id is None else np.ascontiguousarray(material_id, dtype=np.int32)
+        if self.matid is not None:
+            assert self.matid.shape == (self.conn.shape[0],) and 0 <= self.matid.min() and \
+                self.matid.max() < len(mats)
         self.fbar = {"finite_fbar": 1, "finite": 0}[element]
         self.nn, self.ne = self.nodes.shape[0], self.conn.shape[0]
         self.ndof = 3 * self.nn
@@ -397,7 +403,8 @@ class OracleModel:
     def assemble(self, U, state, commit=False, nthreads=None):
         """assemble_threaded! (Assembly.jl:340-416): K.data and R overwritten."""
         fe, fg, fJ = C.c_int64(), C.c_int(), C.c_double()
-        rc = lib.or_assemble(self.fbar, self.mat.ref, self.nn, self.ne, self.nodes, self.conn,
+        rc = lib.or_assemble(self.fbar, self._mats, None if self.matid is None else self.matid.ctypes.data,
+                             self.nn, self.ne, self.nodes, self.conn,
                              self.colptr, self.rowval, self.ncolors, self.cptr, self.corder,
                              _v(U), state["epsp"], state["abar"], state["cpinv"], state["sig"],
                              int(commit), self.K.data, self.R,

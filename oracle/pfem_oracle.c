@@ -637,7 +637,7 @@ static int64_t find_nz(const int64_t *rowval, int64_t lo, int64_t hi, int64_t gr
     return -1;
 }
 
-int or_assemble(int fbar, const or_mat *m, int64_t nnodes, int64_t nelem,
+int or_assemble(int fbar, const or_mat *mats, const int32_t *matid, int64_t nnodes, int64_t nelem,
                 const double *nodes, const int64_t *conn, const int64_t *colptr,
                 const int64_t *rowval, int ncolors, const int64_t *colour_ptr,
                 const int64_t *colour_order, const double *U, double *epsp,
@@ -670,6 +670,7 @@ int or_assemble(int fbar, const or_mat *m, int64_t nnodes, int64_t nelem,
                     ue[3 * a + d] = U[ed[3 * a + d]];
                 }
             int g; double J;
+            const or_mat *m = matid ? mats + matid[e] : mats; /* per-element phase (two-phase blocks) */
             int st = or_element_finite(fbar, m, Xe, ue, epsp + 48 * e, abar + 8 * e,
                                        cpinv + 48 * e, sig + 48 * e, commit, Fe, Ke, &g, &J);
             if (st) {

@@ -98,8 +98,11 @@ int or_greedy_colors(int64_t nnodes, int64_t nelem, const int64_t *conn, int *co
 
 /* Colour-ordered assembly (Assembly.jl:370-416): K.nzval and R overwritten.
  * colour_order lists elements colour by colour; colour_ptr has ncolors+1
- * offsets.  nthreads <= 0 => OpenMP default.  Returns 0 or 1 (inversion). */
-int or_assemble(int fbar, const or_mat *m, int64_t nnodes, int64_t nelem,
+ * offsets.  nthreads <= 0 => OpenMP default.  Returns 0 or 1 (inversion).
+ * mats: one material, or with matid (per element, may be NULL) an array of
+ * them -- the return map is local to each Gauss point, so a two-phase block is
+ * the same loop with the element's own material. */
+int or_assemble(int fbar, const or_mat *mats, const int32_t *matid, int64_t nnodes, int64_t nelem,
                 const double *nodes, const int64_t *conn, const int64_t *colptr,
                 const int64_t *rowval, int ncolors, const int64_t *colour_ptr,
                 const int64_t *colour_order, const double *U, double *epsp,

@@ -70,6 +70,11 @@ class Stats(C.Structure):
                 ("n_krylov", C.c_int64), ("device_bytes", C.c_int64)]
 
 
+class NetDesc(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("channels", C.c_int32 * 4),
+                ("capacity", C.c_int64), ("seed", C.c_uint64)]
+
+
 # symbol -> (restype, argtypes); every declaration of include/pfem.h
 _vp = C.c_void_p
 _i64p = C.POINTER(C.c_int64)
@@ -104,6 +109,17 @@ SIGNATURES = {
     "pfem_get_stats": (C.c_int, [_vp, C.POINTER(Stats)]),
     "pfem_bench_spmv": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
     "pfem_status_string": (C.c_char_p, [C.c_int]),
+    "pfem_net_create": (C.c_int, [C.POINTER(NetDesc), C.POINTER(_vp)]),
+    "pfem_net_destroy": (None, [_vp]),
+    "pfem_net_num_params": (C.c_int64, [_vp]),
+    "pfem_net_get_params": (C.c_int, [_vp, _vp]),
+    "pfem_net_set_params": (C.c_int, [_vp, _vp]),
+    "pfem_net_forward": (C.c_int, [_vp, _vp, _vp]),
+    "pfem_net_loss_grad": (C.c_int, [_vp, _vp, _vp, _dp, _vp]),
+    "pfem_net_add_pair": (C.c_int, [_vp, _vp, _vp]),
+    "pfem_net_num_pairs": (C.c_int, [_vp, _i64p]),
+    "pfem_net_train": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double, C.c_uint64, _dp]),
+    "pfem_attach_net": (C.c_int, [_vp, _vp, C.c_int, C.c_int]),
 }
 
 _lib = None

@@ -61,6 +61,12 @@ struct pfem_ctx {
     std::unique_ptr<pfem::Amg, void (*)(pfem::Amg *)> amg{nullptr, pfem::amg_destroy};
     bool pc_ready = false;
 
+    // learned preconditioner (pfem_attach_net)
+    pfem_net *net = nullptr;
+    int net_mode = 0, net_stride = 1;
+    int64_t net_calls = 0;
+    bool pc_stale = false;  // AMG rebuild requested while the network preconditions
+
     // statistics
     pfem_stats st{};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -20,6 +20,7 @@
 
 #include "amg.h"
 #include "context.h"
+#include "unet.h"
 
 using namespace pfem;
 
@@ -302,6 +303,27 @@ struct Amg32Pc {
     }
 };
 
+struct Amg64Pc {
+    pfem_ctx *c;
+    void operator()(const double *r, double *z) const { amg_apply64(c, c->amg.get(), r, z); }
+};
+// The learned preconditioner: z = N(r) on free DOFs, z = r on constrained ones.
+struct NetPc {
+    pfem_ctx *c;
+    void operator()(const double *r, double *z) const { net_apply(c->net, r, c->bcflag.p, z, c->stream); }
+};
+// Any preconditioner, with every `net_stride`-th application logged as a training
+// sample (r, z) into the attached network's store.
+template <class Inner>
+struct LoggingPc {
+    pfem_ctx *c;
+    Inner inner;
+    void operator()(const double *r, double *z) {
+        inner(r, z);
+        if (c->net_calls++ % c->net_stride == 0) net_log_pair(c->net, r, z, c->bcflag.p, c->stream);
+    }
+};
+
 // FP64 iterative refinement around FP32 Krylov: r = b - K x in FP64, inner FP32
 // solve (PCG, or BiCGStab once PCG has broken down on this system or when
 // `indefinite` is passed in) of K d = r to a modest relative accuracy, x += d in FP64,
@@ -348,20 +370,83 @@ static int ir_solve(pfem_ctx *c, const double *b, double *x, double rtol, int it
     return rn <= rtol * bn ? 0 : 1;
 }
 
+// FP32 workspaces and the FP32 copy of the imposed fine K.
+static void prepare_fp32(pfem_ctx *c) {
+    if (!c->f_r.p)
+        for (auto *b : {&c->f_r, &c->f_z, &c->f_p, &c->f_q, &c->f_x, &c->f_b, &c->f_rh, &c->f_v, &c->f_s, &c->f_t})
+            b->alloc(c->ndof);
+    if (!c->K.v32.p) c->K.v32.alloc(9 * c->K.nslots);
+    convert<double, float>(9 * c->K.nslots, c->K.v64.p, c->K.v32.p, c->stream);
+}
+
+// Flexible CG with the network; when it stalls (no 0.1% gain in the best residual over
+// 25 iterations), breaks down or hits itmax, AMG solves for the remaining residual
+// b - K x and the correction is added (the network's progress is kept).
+static int learned_solve(pfem_ctx *c, double rtol, int itmax, int *total, int *which) {
+    hipStream_t s = c->stream;
+    int it = 0;
+    *which = 5;
+    int rc = fpcg(c, c->rhs.p, c->dU.p, rtol, itmax, false, 25, &it, NetPc{c});
+    *total += it;
+    if (c->opt.verbose > 1) fprintf(stderr, "      learned fcg: rc %d after %d its\n", rc, it);
+    if (rc == 0) return 0;
+    *which = 6;
+    if (c->pc_stale || !c->pc_ready) {
+        pc_setup(c);
+        c->pc_stale = false;
+    }
+    residual_fine64(c->K, c->dU.p, c->rhs.p, c->w_s.p, s);  // remaining residual
+    dot_dev<double, double>(c->ndof, c->w_s.p, c->w_s.p, c->partial.p, c->scal.p + 6, s);
+    dot_dev<double, double>(c->ndof, c->rhs.p, c->rhs.p, c->partial.p, c->scal.p + 7, s);
+    PF_HIP(hipMemcpyAsync(c->h_scal, c->scal.p, 8 * sizeof(double), hipMemcpyDeviceToHost, s));
+    c->sync();
+    const double rn = std::sqrt(c->h_scal[6]), bn = std::sqrt(c->h_scal[7]);
+    if (!(rn > rtol * bn)) return 0;
+    const double rt = std::min(0.5, rtol * bn / rn);
+    if (c->opt.precision == PFEM_PREC_FP32_IR) {
+        rc = fpcg(c, c->w_s.p, c->w_x.p, rt, itmax, true, 0, &it, Amg32Pc{c});
+    } else {
+        rc = pcg<double>(c, c->w_s.p, c->w_x.p, rt, itmax, &it);
+    }
+    *total += it;
+    if (rc != 0) {
+        int it2 = 0;
+        rc = bicgstab<double>(c, c->w_s.p, c->w_x.p, rt, 2 * itmax, &it2);
+        *total += it2;
+    }
+    axpby<double>(c->ndof, 1.0, c->w_x.p, 1.0, c->dU.p, s);
+    return rc;
+}
+
 int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *used, double *relres) {
-    if (rebuild || !c->pc_ready) pc_setup(c);
+    const bool learned = c->net && c->net_mode == 2;
+    if (learned) {
+        if (c->opt.precision == PFEM_PREC_FP32_IR) prepare_fp32(c);
+        if (rebuild) c->pc_stale = true;  // the AMG hierarchy is rebuilt only if it is needed
+    } else if (rebuild || !c->pc_ready) {
+        pc_setup(c);
+    }
     const double t1 = now();
     int it = 0, total = 0, which = 0, rc;
     const int itmax = c->opt.cg_itmax > 0 ? c->opt.cg_itmax : 200;
     hipStream_t s = c->stream;
-    if (c->opt.precision == PFEM_PREC_FP32_IR) {
-        if (!c->f_r.p)
-            for (auto *b : {&c->f_r, &c->f_z, &c->f_p, &c->f_q, &c->f_x, &c->f_b, &c->f_rh, &c->f_v, &c->f_s, &c->f_t})
-                b->alloc(c->ndof);
-        if (!c->K.v32.p) c->K.v32.alloc(9 * c->K.nslots);
-        convert<double, float>(9 * c->K.nslots, c->K.v64.p, c->K.v32.p, s);  // FP32 copy of the imposed K
+    if (learned) {
+        rc = learned_solve(c, rtol, itmax, &total, &which);
+    } else if (c->net && c->net_mode == 1 && c->opt.precision != PFEM_PREC_FP32_IR) {
+        rc = fpcg(c, c->rhs.p, c->dU.p, rtol, itmax, true, 0, &it, LoggingPc<Amg64Pc>{c, Amg64Pc{c}});
+        total += it;
+        if (rc != 0) {
+            which = 2;
+            rc = bicgstab<double>(c, c->rhs.p, c->dU.p, rtol, 2 * itmax, &it);
+            total += it;
+        }
+    } else if (c->opt.precision == PFEM_PREC_FP32_IR) {
+        prepare_fp32(c);
         which = 3;
-        rc = fpcg(c, c->rhs.p, c->dU.p, rtol, itmax, true, 0, &it, Amg32Pc{c});
+        if (c->net && c->net_mode == 1)
+            rc = fpcg(c, c->rhs.p, c->dU.p, rtol, itmax, true, 0, &it, LoggingPc<Amg32Pc>{c, Amg32Pc{c}});
+        else
+            rc = fpcg(c, c->rhs.p, c->dU.p, rtol, itmax, true, 0, &it, Amg32Pc{c});
         total += it;
         if (rc != 0) {  // indefinite (2) or stalled (1): refinement sweeps, BiCGStab after a breakdown
             rc = ir_solve(c, c->rhs.p, c->dU.p, rtol, 4 * itmax, rc == 2, &it, &which);

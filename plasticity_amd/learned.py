@@ -1,0 +1,166 @@
+"""Learned preconditioner: host side of the U-net (pfem_net_* in include/pfem.h).
+
+The network is a bias-free 3D convolutional U-net over the structured node grid of a
+box mesh (4 levels, channels 16/32/64/64, BF16 MFMA kernels with FP32 accumulation).
+It is trained on (residual, V-cycle correction) samples that the solver logs from its
+own AMG-preconditioned solves, with Adam and the scale-invariant loss
+1 - cos(N(r), z), and then preconditions flexible CG with an AMG fallback on stall.
+
+    net = UNet((n + 1,) * 3, capacity=2000)
+    collect_samples(net, seeds=range(2, 21), n=63)      # Newton solves, samples logged
+    net.train(steps=4000, batch=4, lr=1e-3)
+    model.attach_net(net, mode="precondition")          # FCG + network, AMG fallback
+
+The network and its training are additions with no counterpart in the reference
+solver (lagrangian/src/Solver.jl:235-290 uses AMG-preconditioned CG only).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import time
+
+import numpy as np
+
+from . import _lib as L
+
+CHANNELS = (16, 32, 64, 64)
+MODES = {"off": 0, "log": 1, "precondition": 2}
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "unet_16_32_64_64.npz")
+
+
+def _f32(a, n):
+    a = np.ascontiguousarray(a, dtype=np.float32).reshape(-1)
+    if a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+class UNet:
+    """Device-resident network for a node grid (nx, ny, nz), x fastest."""
+
+    def __init__(self, grid, channels=CHANNELS, capacity=0, seed=0):
+        self.grid = tuple(int(g) for g in grid)
+        self.channels = tuple(int(c) for c in channels)
+        self.nvox = int(np.prod(self.grid))
+        d = L.NetDesc(self.grid[0], self.grid[1], self.grid[2], (C.c_int32 * 4)(*self.channels),
+                      int(capacity), int(seed))
+        h = C.c_void_p()
+        L.check(L.lib().pfem_net_create(C.byref(d), C.byref(h)), "pfem_net_create")
+        self._h = h
+        self.nparams = int(L.lib().pfem_net_num_params(h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib().pfem_net_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        w = np.empty(self.nparams, dtype=np.float32)
+        L.check(L.lib().pfem_net_get_params(self._h, L.ptr(w)), "net_get_params")
+        return w
+
+    def set_params(self, w):
+        w = _f32(w, self.nparams)
+        L.check(L.lib().pfem_net_set_params(self._h, L.ptr(w)), "net_set_params")
+
+    def forward(self, x):
+        x = _f32(x, 3 * self.nvox)
+        out = np.empty(3 * self.nvox, dtype=np.float32)
+        L.check(L.lib().pfem_net_forward(self._h, L.ptr(x), L.ptr(out)), "net_forward")
+        return out.reshape(-1, 3)
+
+    def loss_grad(self, x, target):
+        x = _f32(x, 3 * self.nvox)
+        t = _f32(target, 3 * self.nvox)
+        g = np.empty(self.nparams, dtype=np.float32)
+        loss = C.c_double()
+        L.check(L.lib().pfem_net_loss_grad(self._h, L.ptr(x), L.ptr(t), C.byref(loss), L.ptr(g)), "net_loss_grad")
+        return loss.value, g
+
+    def add_pair(self, r, z):
+        L.check(L.lib().pfem_net_add_pair(self._h, L.ptr(_f32(r, 3 * self.nvox)), L.ptr(_f32(z, 3 * self.nvox))),
+                "net_add_pair")
+
+    @property
+    def num_pairs(self):
+        n = C.c_int64()
+        L.check(L.lib().pfem_net_num_pairs(self._h, C.byref(n)), "net_num_pairs")
+        return n.value
+
+    def train(self, steps, batch=4, lr=1e-3, seed=0):
+        loss = C.c_double()
+        L.check(L.lib().pfem_net_train(self._h, int(steps), int(batch), float(lr), int(seed), C.byref(loss)),
+                "net_train")
+        return loss.value
+
+    def save(self, path):
+        os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+        np.savez(path, w=self.get_params(), channels=np.asarray(self.channels))
+
+    def load(self, path):
+        with np.load(path, allow_pickle=False) as f:
+            if tuple(int(c) for c in f["channels"]) != self.channels:
+                raise ValueError("channel configuration of the weight file differs")
+            self.set_params(f["w"])
+        return self
+
+
+def attach(model, net, mode="precondition", stride=1):
+    """Attach `net` to a Model: 'log' samples from AMG solves, 'precondition' uses the
+    network in flexible CG with AMG fallback, 'off' detaches."""
+    m = MODES[mode] if isinstance(mode, str) else int(mode)
+    L.check(L.lib().pfem_attach_net(model._h, None if net is None else net._h, m, int(stride)), "attach_net")
+    model._net = net if m else None
+
+
+def grid_of(mesh):
+    """(nx+1, ny+1, nz+1) node counts of a box mesh (node = x + nx*(y + ny*z))."""
+    if mesh.dims is None:
+        raise ValueError("the learned preconditioner needs a structured box mesh")
+    nx, ny, nz = mesh.dims[:3]
+    return (nx + 1, ny + 1, nz + 1)
+
+
+def sample_model(seed, n, inclusion, options=None):
+    """A workload-1-style Model for sample collection / evaluation."""
+    from .model import J2Material, Model
+    from .workloads import workload
+    wl = workload(3, n=n, seed=seed, inclusion=inclusion)
+    mats = [J2Material(**m) for m in wl.materials]
+    m = Model(wl.mesh, mats if len(mats) > 1 else mats[0], options=options, material_id=wl.material_id)
+    m.constrain(wl.dir_dofs, wl.dir_vals, wl.dir_ramp)
+    m.reset()
+    m._freeze()
+    return m, wl
+
+
+def inclusion_fraction(seed):
+    """Training meshes span inclusion fractions 0-30% (a fixed function of the seed)."""
+    return 0.3 * ((seed * 7) % 11) / 10.0
+
+
+def collect_samples(net, seeds, n, load_steps=(1, 2, 3), stride=2, log=print):
+    """Run Newton load steps on the sample meshes with the AMG solver, logging every
+    `stride`-th (residual, V-cycle correction) pair into `net`'s store."""
+    from .model import default_options
+    t0 = time.perf_counter()
+    for s in seeds:
+        m, wl = sample_model(s, n, inclusion_fraction(s), options=default_options())
+        attach(m, net, "log", stride)
+        for k in load_steps:
+            conv, hist, kry, _ = m.newton(k / wl.nsteps, tol=wl.newton_tol, maxiter=25)
+            if not conv:
+                break
+            m.commit()
+        attach(m, None, "off")
+        log(f"[learned] seed {s} inclusions {inclusion_fraction(s):.2f}: samples {net.num_pairs} "
+            f"({time.perf_counter() - t0:.1f}s)")
+        m.close()
+    return net.num_pairs

@@ -360,6 +360,12 @@ class Model:
                 callback(n, self)
         return out
 
+    def attach_net(self, net, mode="precondition", stride=1):
+        """Learned preconditioner (pfem_attach_net): 'log' samples, 'precondition', 'off'."""
+        from .learned import attach
+        attach(self, net, mode, stride)
+        return self
+
     def stats(self):
         st = L.Stats()
         L.check(L.lib().pfem_get_stats(self._h, C.byref(st)), "get_stats")

@@ -12,6 +12,10 @@ The x = Lx face gets the ramped axial displacement.
               40 steps, Krylov rtol 1e-8
   workload 2  200^3 block, jitter seed 1, two-phase (20% ellipsoidal inclusions,
               sy 750 MPa vs 250 MPa), tension to 10% in 20 steps
+  workload 3  samples for the learned preconditioner: workload-1-style blocks with
+              jitter seed `seed` and ellipsoidal inclusions (sy 750 MPa) covering
+              `inclusion` of the volume (seeds 2-20 train, 100-104 evaluate),
+              compression in 160 steps
   workload 4  150^3 notched bar (20% section reduction), +-5% jitter (seed 7),
               tension to 30% in 60 steps
 Sizes can be overridden (n=...) for parity runs on small meshes.
@@ -70,7 +74,8 @@ def uniaxial_constraints(mesh: Mesh, disp: float):
             np.asarray(ramp, dtype=np.uint8))
 
 
-def workload(k: int, n: int | None = None, nsteps: int | None = None) -> Workload:
+def workload(k: int, n: int | None = None, nsteps: int | None = None, seed: int | None = None,
+             inclusion: float = 0.0) -> Workload:
     if k == 0:
         n = 8 if n is None else n
         mesh = box_mesh(1.0, 1.0, 1.0, n, n, n)
@@ -90,6 +95,18 @@ def workload(k: int, n: int | None = None, nsteps: int | None = None) -> Workloa
         hard = dict(STEEL, sy0=750e6)
         return Workload(f"two-phase block {n}^3 tension 10%", mesh, [STEEL, hard], mid, d, v, r, nsteps or 20,
                         1e-8, 1e-8)
+    if k == 3:
+        n = 64 if n is None else n
+        seed = 2 if seed is None else seed
+        mesh = box_mesh(1.0, 1.0, 1.0, n, n, n, jitter=0.1, seed=seed)
+        mid = ellipsoid_inclusions(mesh, inclusion, seed=seed) if inclusion > 0 else None
+        d, v, r = uniaxial_constraints(mesh, -0.2)
+        hard = dict(STEEL, sy0=750e6)
+        mats = [STEEL, hard] if mid is not None else [STEEL]
+        # 160 load steps: Newton without line search inverts elements of two-phase
+        # blocks in compression at 0.5% strain increments on meshes this coarse
+        return Workload(f"sample block {n}^3 seed {seed} inclusions {inclusion:.2f}", mesh, mats, mid, d, v, r,
+                        nsteps or 160, 1e-8, 1e-8)
     if k == 4:
         n = 150 if n is None else n
         mesh = notched_bar_mesh(n, length=1.0, notch_depth=0.2, jitter=0.05, seed=7)

@@ -1,0 +1,119 @@
+"""Learned preconditioner on the GPU: the BF16-MFMA U-net kernels against the PyTorch
+FP32 reference (oracle/unet_ref.py), Adam training, and flexible CG with the network
+and AMG fallback inside the Newton solver.
+
+Tolerances: the forward pass rounds weights, input and every activation to BF16 exactly
+where the reference does, so the outputs differ only by FP32 accumulation order and
+the rare rounding flip it causes (relative L2 error <= 2e-2, typically ~1e-3).  The
+backward pass also feeds BF16-rounded gradient tensors to the MFMA units, which the
+FP32 reference does not, so each layer's weight gradient is compared with relative L2
+error <= 8e-2 and cosine similarity >= 0.99 -- a transposed, flipped or misplaced tap
+gives errors of order 1."""
+import numpy as np
+import pytest
+import torch
+
+import oracle.unet_ref as U
+import plasticity_amd as pa
+from plasticity_amd.learned import UNet, attach, sample_model
+
+pytestmark = pytest.mark.gpu
+CH = (16, 32, 64, 64)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-30))
+
+
+def _field(grid, seed):
+    nx, ny, nz = grid
+    r = np.random.default_rng(seed)
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    smooth = np.stack([np.sin(0.4 * x + 0.3 * y) * np.cos(0.2 * z), np.cos(0.3 * x - 0.5 * z),
+                       np.sin(0.25 * (x + y + z))], axis=-1).reshape(-1, 3)
+    return (smooth + 0.3 * r.standard_normal(smooth.shape)).astype(np.float32)
+
+
+@pytest.mark.parametrize("grid", [(13, 11, 9), (16, 17, 18)])
+def test_forward_matches_torch_reference(need_gpu, grid):
+    print(f"[learned] forward vs torch, grid {grid}", flush=True)
+    net = UNet(grid, CH, seed=11)
+    w = net.get_params()
+    x = _field(grid, 1)
+    y = net.forward(x)
+    ps = [torch.tensor(p) for p in U.split_params(w, CH)]
+    ref = U.forward(ps, x, grid[::-1], CH, bf16=True).numpy()
+    e = rel(y, ref)
+    print(f"  relative L2 error {e:.2e}", flush=True)
+    assert np.isfinite(y).all() and np.linalg.norm(ref) > 0
+    assert e <= 2e-2
+
+
+def test_loss_gradient_matches_torch_reference(need_gpu):
+    grid = (12, 10, 11)
+    print(f"[learned] loss gradient vs torch, grid {grid}", flush=True)
+    net = UNet(grid, CH, seed=5)
+    w = net.get_params()
+    x = _field(grid, 2)
+    t = _field(grid, 3)
+    loss, g = net.loss_grad(x, t)
+    lref, gref = U.loss_and_grad(w, x, t, grid[::-1], CH, bf16=True)
+    print(f"  loss {loss:.6f} ref {lref:.6f}", flush=True)
+    assert abs(loss - lref) <= 2e-3
+    off = 0
+    for k, p in enumerate(U.split_params(w, CH)):
+        n = p.size
+        a, b = g[off:off + n], gref[off:off + n]
+        off += n
+        if np.linalg.norm(b) == 0:
+            assert np.linalg.norm(a) == 0
+            continue
+        e = rel(a, b)
+        cs = float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b)))
+        print(f"  layer {k:2d}: rel err {e:.2e} cos {cs:.5f}", flush=True)
+        assert e <= 8e-2 and cs >= 0.99
+
+
+def test_padding_weights_stay_zero_and_training_lowers_the_loss(need_gpu):
+    grid = (9, 9, 9)
+    net = UNet(grid, CH, capacity=8, seed=2)
+    w0 = net.get_params()
+    first = U.split_params(w0, CH)[0]
+    assert np.all(first[:, 3:, :] == 0)
+    # samples: target = a smoothed copy of the input
+    for s in range(8):
+        x = _field(grid, 10 + s)
+        xs = x.reshape(9, 9, 9, 3)
+        t = (xs + np.roll(xs, 1, 0) + np.roll(xs, -1, 0) + np.roll(xs, 1, 2) + np.roll(xs, -1, 2)) / 5
+        net.add_pair(x, t.reshape(-1, 3))
+    assert net.num_pairs == 8
+    l0 = net.train(1, batch=8, lr=1e-3, seed=0)
+    l1 = net.train(150, batch=4, lr=1e-3, seed=1)
+    l2 = net.train(20, batch=8, lr=1e-4, seed=2)
+    print(f"[learned] training loss {l0:.4f} -> {l1:.4f} -> {l2:.4f}", flush=True)
+    assert l2 < 0.5 * l0
+    w1 = U.split_params(net.get_params(), CH)
+    assert np.all(w1[0][:, 3:, :] == 0) and np.all(w1[-1][:, :, 3:] == 0)
+
+
+def test_learned_fcg_converges_with_amg_fallback(need_gpu):
+    n = 12
+    print(f"[learned] FCG with the network on a {n}^3 sample block", flush=True)
+    o = pa.default_options(eta_min=1e-8, eta_max=1e-8)
+    m, wl = sample_model(100, n, 0.1, options=o)
+    net = UNet((n + 1,) * 3, CH, capacity=64, seed=3)
+    # samples from the AMG solves of the first load step
+    attach(m, net, "log", 1)
+    for k in (1, 2):
+        conv, hist, kry, res = m.newton(k / wl.nsteps, tol=wl.newton_tol, maxiter=25)
+        assert conv
+        m.commit()
+    assert net.num_pairs > 0
+    net.train(100, batch=4, lr=1e-3)
+    attach(m, net, "precondition")
+    conv, hist, kry, res = m.newton(3 / wl.nsteps, tol=wl.newton_tol, maxiter=25)
+    used = [res.solver_used[i] for i in range(len(kry))]
+    print(f"  newton {len(hist)} krylov {kry} solver {used}", flush=True)
+    assert conv
+    assert set(used) <= {5, 6}
+    attach(m, None, "off")
