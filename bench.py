@@ -27,6 +27,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "s/Newton step @3M DOF, rtol 1e-8: FP64 CG+AMG vs FP32-IR+learned PC"
 HBM_PEAK_GBS = 8000.0
+# FP64 operations per Hex8 F-bar element with all 8 Gauss points plastic: instrumented
+# count of the oracle's element routine (oracle/flopcount.cpp, tests/test_flopcount.py)
+ELEMENT_FLOPS = 160073
 # HBM bytes per fine-SpMV launch at 100^3 (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors
 # from a 1 GiB device copy), measured with rocprofv3 --pmc on `--mode spmv --n 100`:
 # 1,291,914.9 KiB fetched + 24,148.5 KiB written per dispatch.
@@ -118,7 +121,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3, help="timed load steps")
     ap.add_argument("--warmup", type=int, default=1, help="untimed load steps")
-    ap.add_argument("--n", type=int, default=100, help="elements per edge")
+    ap.add_argument("--n", type=int, default=None, help="elements per edge (default: the workload's own)")
+    ap.add_argument("--workload", type=int, default=1, choices=[1, 2, 4],
+                    help="1: 100^3 jittered compression (the metric), 2: 200^3 two-phase tension, "
+                         "4: 150^3 notched bar tension")
+    ap.add_argument("--learned", default=None, metavar="WEIGHTS",
+                    help="precondition with the learned U-net (weights .npz) in flexible GCR, AMG fallback")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_ir"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--amg-rebuild", type=int, default=1, help="0: once per load step, 1: every Newton iteration")
@@ -139,7 +147,7 @@ def main():
     import plasticity_amd as pa
     from plasticity_amd.workloads import workload
 
-    wl = workload(1, n=args.n)
+    wl = workload(args.workload, n=args.n)
     if args.mode == "spmv":
         m = pa.Model(wl.mesh, pa.J2Material(**wl.materials[0]), options=pa.default_options())
         m.assemble(np.zeros(m.ndof))
@@ -152,10 +160,16 @@ def main():
     opts = pa.default_options(eta_min=wl.krylov_rtol, eta_max=wl.krylov_rtol, precision=args.precision,
                               verbose=args.verbose, amg_rebuild=args.amg_rebuild)
     t0 = time.perf_counter()
-    m = pa.Model(wl.mesh, pa.J2Material(**wl.materials[0]), options=opts)
+    mats = [pa.J2Material(**d) for d in wl.materials]
+    m = pa.Model(wl.mesh, mats if len(mats) > 1 else mats[0], options=opts, material_id=wl.material_id)
     m.constrain(wl.dir_dofs, wl.dir_vals, wl.dir_ramp)
     m.reset()
     m._freeze()
+    net = None
+    if args.learned:
+        from plasticity_amd.learned import UNet, grid_of
+        net = UNet(grid_of(wl.mesh)).load(args.learned)
+        m.attach_net(net, "precondition")
     log(f"[bench] model set-up {time.perf_counter() - t0:.2f}s")
 
     def load_step(k):
@@ -203,7 +217,7 @@ def main():
         f"AMG set-up {t_setup:.3f}s, solve {t_solve:.3f}s; fine SpMV {t_sp * 1e3:.3f} ms = {achieved} GB/s; "
         f"AMG levels {list(s1.amg_ndof[:s1.amg_levels])} OC {s1.amg_operator_complexity:.3f}")
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and args.workload == 1:
         try:
             cpu = cpu_baseline(newton / args.steps, krylov / max(newton, 1))
             log(f"[bench] cpu baseline {cpu}")
@@ -213,9 +227,10 @@ def main():
         "metric": METRIC, "value": per_newton, "unit": "s/Newton step", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": False,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f32+f64-IR",
-        "data": "synthetic (generated jittered 100^3 Hex8 mesh)",
+        "data": f"synthetic (generated {wl.name} Hex8 mesh)",
         "config": {"workload": f"{wl.name}: {wl.mesh.nelem} Hex8 F-bar elements, {ndof} DOF, "
-                               f"PCG+SA-AMG rtol {wl.krylov_rtol:g}, precision {args.precision}, "
+                               f"{'GCR + learned U-net/block-Jacobi, AMG fallback' if net else 'PCG+SA-AMG'} "
+                               f"rtol {wl.krylov_rtol:g}, precision {args.precision}, "
                                f"AMG rebuilt {'every Newton iteration' if args.amg_rebuild else 'once per load step'}",
                    "newton_steps_timed": newton, "krylov_its_timed": krylov,
                    "t_assemble_s": t_asm, "t_amg_setup_s": t_setup, "t_solve_s": t_solve,
@@ -223,9 +238,17 @@ def main():
         "roofline": {"kernel": "fine BSR(3x3) SpMV (sell_lds_kernel<double,double,2>, FP64) in PCG",
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-                     "traffic": SPMV_TRAFFIC_100 if args.n == 100 else None,
+                     "traffic": SPMV_TRAFFIC_100 if (args.workload == 1 and ndof == 3090903) else None,
                      "bytes_per_launch": bytes_sp, "avg_launch_ms": t_sp * 1e3},
         "cpu_baseline": cpu,
+        "element_roofline": {"kernel": "gp_kernel + scatter_kernel (Hex8 F-bar integration and tangent "
+                                       "contraction, FP64)", "bound": "fp64",
+                             "achieved": ELEMENT_FLOPS * wl.mesh.nelem / ((gp_ms + sc_ms) * 1e-3) / 1e12,
+                             "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                             "frac": ELEMENT_FLOPS * wl.mesh.nelem / ((gp_ms + sc_ms) * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                             "flops_per_element": ELEMENT_FLOPS,
+                             "elements_per_s": wl.mesh.nelem / ((gp_ms + sc_ms) * 1e-3),
+                             "gp_kernel_ms": gp_ms, "scatter_ms": sc_ms},
     }
     print(json.dumps(out), flush=True)
 

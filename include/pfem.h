@@ -83,7 +83,9 @@ typedef struct pfem_options {
     int32_t verbose;            /* print per-iteration lines to stderr */
     int32_t reserved[7];        /* reserved[0]: AMG rebuild policy, 0 = once per load step
                                    (Solver.jl:374), 1 = every Newton iteration, 2 = at the
-                                   first two Newton iterations of a load step */
+                                   first two Newton iterations of a load step;
+                                   reserved[1]: directions kept by the flexible CG of the
+                                   learned preconditioner (<= 0: 16) */
 } pfem_options;
 
 typedef struct pfem_fail {
@@ -215,9 +217,10 @@ int pfem_bench_spmv(pfem_ctx *ctx, int prec, int reps, double *sec_per_spmv);
  * connections) on the structured node grid of a box mesh (node index
  * x + nx*(y + ny*z)); BF16 weights and activations on MFMA with FP32
  * accumulation; hand-written forward, backward and Adam kernels.  It maps the
- * 3-component nodal residual to a correction; inside FP64 flexible CG it
- * replaces the V-cycle, and AMG takes over from the current iterate when the
- * iteration stalls.  Trained on (residual, V-cycle correction) samples that
+ * 3-component nodal residual to a smooth correction n; the preconditioner
+ * z = g n + D^-1 (r - g K n) (g = r'n / n'Kn, D the 3x3 block diagonal of K)
+ * replaces the V-cycle inside FP64 flexible CG, and AMG takes over from the
+ * current iterate when the iteration stalls.  Trained on (residual, V-cycle correction) samples that
  * the solver logs from its own AMG-preconditioned solves, with the
  * scale-invariant loss 1 - cos(N(r), z).
  * ------------------------------------------------------------------------- */
@@ -250,7 +253,10 @@ int pfem_net_train(pfem_net *net, int steps, int batch, double lr, uint64_t seed
 /* mode 0: detach; 1: log (r, V-cycle(r)) samples from every `stride`-th
  * preconditioner application of pfem_linear_solve / pfem_newton into the
  * net's store; 2: precondition pfem_linear_solve with the network (flexible
- * CG, AMG fallback).  The net's grid must hold exactly the context's nodes. */
+ * CG, AMG fallback); 3: the same preconditioner without its network
+ * (block-Jacobi flexible CG, AMG fallback; net may be NULL), the baseline that
+ * shows what the network adds.  The net's grid must hold exactly the
+ * context's nodes. */
 int pfem_attach_net(pfem_ctx *ctx, pfem_net *net, int mode, int stride);
 
 /* Human-readable text for a status. */

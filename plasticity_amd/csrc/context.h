@@ -66,6 +66,8 @@ struct pfem_ctx {
     int net_mode = 0, net_stride = 1;
     int64_t net_calls = 0;
     bool pc_stale = false;  // AMG rebuild requested while the network preconditions
+    pfem::dbuf<double> net_dinv;  // block-diagonal inverse of K for the learned preconditioner
+    pfem::dbuf<double> fcg_P, fcg_Q; // truncated flexible CG: search directions and K times them
 
     // statistics
     pfem_stats st{};

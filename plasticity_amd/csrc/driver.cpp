@@ -292,6 +292,82 @@ static int fpcg(pfem_ctx *c, const double *b, double *x, double rtol, int itmax,
     return 1;
 }
 
+// Flexible truncated GCR(m) (Eisenstat, Elman & Schultz, SIAM J. Numer. Anal. 20,
+// 1983; flexible form: Vuik, J. Comput. Appl. Math. 61, 1995): with z = M(r) and
+// q = K z, the pair (p, q) is orthogonalised against the last m pairs in the q'q inner
+// product, then alpha = r'q / q'q minimises ||r|| along p.  It needs neither a
+// symmetric K (the F-bar tangent is not symmetric) nor a linear preconditioner (a
+// network), and costs one SpMV per iteration.  Stops on ||r|| <= rtol ||b||;
+// 0 solved, 1 itmax, 2 breakdown (q = 0), 3 stall (no 0.1% gain of the best residual
+// over `stall` iterations).
+template <class PC>
+static int gcr_m(pfem_ctx *c, const double *b, double *x, double rtol, int itmax, int m, int stall, int *iters,
+                 PC &&pcfun) {
+    hipStream_t s = c->stream;
+    const int64_t n = c->ndof;
+    const int slots = m + 1;
+    if (c->fcg_P.n < (size_t)slots * n) {
+        c->fcg_P.alloc((size_t)slots * n);
+        c->fcg_Q.alloc((size_t)slots * n);
+    }
+    double *r = c->w_r.p, *sc = c->scal.p;
+    fill<double>(n, 0.0, x, s);
+    PF_HIP(hipMemcpyAsync(r, b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    dot_dev<double, double>(n, r, r, c->partial.p, sc + 3, s);
+    const double bn = std::sqrt(c->sync_read(sc + 3));
+    *iters = 0;
+    if (bn == 0.0) return 0;
+    const double eps = rtol * bn;
+    std::vector<double> qq(slots, 0.0);
+    double best = bn;
+    int best_it = 0;
+    for (int it = 1; it <= itmax; ++it) {
+        *iters = it;
+        const int k = (it - 1) % slots;
+        const int nold = std::min(it - 1, m);
+        double *pk = c->fcg_P.p + (size_t)k * n, *qk = c->fcg_Q.p + (size_t)k * n;
+        pcfun(r, pk);
+        PF_HIP(hipEventRecord(c->ev0, s));
+        spmv_fine64(c->K, pk, qk, s);  // q = K z
+        PF_HIP(hipEventRecord(c->ev1, s));
+        for (int j = 1; j <= nold; ++j) {
+            const int sj = (k - j + slots) % slots;
+            dot_dev<double, double>(n, qk, c->fcg_Q.p + (size_t)sj * n, c->partial.p, sc + 16 + j, s);
+        }
+        if (nold > 0) {
+            PF_HIP(hipMemcpyAsync(c->h_scal + 16, sc + 16, (nold + 1) * sizeof(double), hipMemcpyDeviceToHost, s));
+            c->sync();
+            for (int j = 1; j <= nold; ++j) {
+                const int sj = (k - j + slots) % slots;
+                const double beta = c->h_scal[16 + j] / qq[sj];
+                axpby<double>(n, -beta, c->fcg_P.p + (size_t)sj * n, 1.0, pk, s);
+                axpby<double>(n, -beta, c->fcg_Q.p + (size_t)sj * n, 1.0, qk, s);
+            }
+        }
+        dot_dev<double, double>(n, qk, qk, c->partial.p, sc + 1, s);
+        dot_dev<double, double>(n, r, qk, c->partial.p, sc + 2, s);
+        PF_HIP(hipMemcpyAsync(c->h_scal, sc, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+        c->sync();
+        float ms = 0.f;
+        PF_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->st.t_pcg_spmv += 1e-3 * ms;
+        c->st.n_pcg_spmv++;
+        const double qqk = c->h_scal[1], rq = c->h_scal[2];
+        if (!(qqk > 0)) return 2;
+        qq[k] = qqk;
+        const double alpha = rq / qqk;
+        axpby<double>(n, alpha, pk, 1.0, x, s);
+        axpby<double>(n, -alpha, qk, 1.0, r, s);
+        dot_dev<double, double>(n, r, r, c->partial.p, sc + 3, s);
+        const double rn = std::sqrt(c->sync_read(sc + 3));
+        if (c->opt.verbose > 2) fprintf(stderr, "      gcr(%d) %3d  %.3e  (target %.3e)\n", m, it, rn, eps);
+        if (rn <= eps) return 0;
+        if (rn < 0.999 * best) { best = rn; best_it = it; }
+        else if (stall > 0 && it - best_it >= stall) return 3;
+    }
+    return 1;
+}
+
 // The FP32 V-cycle as a preconditioner of FP64 vectors: r -> FP32, V-cycle on the FP32
 // copies of every level, z -> FP64.
 struct Amg32Pc {
@@ -307,10 +383,31 @@ struct Amg64Pc {
     pfem_ctx *c;
     void operator()(const double *r, double *z) const { amg_apply64(c, c->amg.get(), r, z); }
 };
-// The learned preconditioner: z = N(r) on free DOFs, z = r on constrained ones.
-struct NetPc {
+// The learned preconditioner, a two-level method whose coarse correction is the
+// network: n = N(r) (constrained DOFs 0), scaled by the exact line search
+// g = r'n / n'Kn (0 if n'Kn <= 0), then one block-Jacobi sweep on what is left:
+//   z = g n + D^-1 (r - g K n).
+// Without a network (attach mode 3) it is block-Jacobi alone, z = D^-1 r: the same
+// method without its learned part.
+struct LearnedPc {
     pfem_ctx *c;
-    void operator()(const double *r, double *z) const { net_apply(c->net, r, c->bcflag.p, z, c->stream); }
+    void operator()(const double *r, double *z) const {
+        hipStream_t s = c->stream;
+        const int64_t n = c->ndof;
+        double *sc = c->scal.p;
+        if (!c->net) {
+            block_jacobi3<double>(c->nnodes, c->net_dinv.p, r, z, s);
+            return;
+        }
+        double *nv = c->w_t.p, *q = c->w_v.p, *r1 = c->w_rh.p;
+        net_apply(c->net, r, c->bcflag.p, nv, s);
+        spmv_fine64_dot(c->K, nv, q, c->partial.p, sc + 10, s);  // q = K n, sc[10] = n'Kn
+        dot_dev<double, double>(n, r, nv, c->partial.p, sc + 11, s);  // sc[11] = r'n
+        PF_HIP(hipMemcpyAsync(r1, r, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        axpy_safe_ratio(n, sc + 11, sc + 10, -1.0, q, r1, s);  // r1 = r - g K n
+        block_jacobi3<double>(c->nnodes, c->net_dinv.p, r1, z, s);
+        axpy_safe_ratio(n, sc + 11, sc + 10, 1.0, nv, z, s);  // z += g n
+    }
 };
 // Any preconditioner, with every `net_stride`-th application logged as a training
 // sample (r, z) into the attached network's store.
@@ -379,24 +476,30 @@ static void prepare_fp32(pfem_ctx *c) {
     convert<double, float>(9 * c->K.nslots, c->K.v64.p, c->K.v32.p, c->stream);
 }
 
-// Flexible CG with the network; when it stalls (no 0.1% gain in the best residual over
-// 25 iterations), breaks down or hits itmax, AMG solves for the remaining residual
-// b - K x and the correction is added (the network's progress is kept).
+// GCR(m) with the learned preconditioner; when it stalls (no 0.1% gain in the best
+// residual over 40 iterations), breaks down or hits itmax, AMG solves for the
+// remaining residual b - K x and the correction is added (the progress is kept).
 static int learned_solve(pfem_ctx *c, double rtol, int itmax, int *total, int *which) {
     hipStream_t s = c->stream;
     int it = 0;
     *which = 5;
-    int rc = fpcg(c, c->rhs.p, c->dU.p, rtol, itmax, false, 25, &it, NetPc{c});
+    if (!c->net_dinv.p) c->net_dinv.alloc(9 * c->nnodes);
+    block_diag_inverse3(c->K, c->net_dinv.p, s);
+    const int m = c->opt.reserved[1] > 0 ? std::min(c->opt.reserved[1], 40) : 16;
+    int rc = gcr_m(c, c->rhs.p, c->dU.p, rtol, itmax, m, 40, &it, LearnedPc{c});
     *total += it;
-    if (c->opt.verbose > 1) fprintf(stderr, "      learned fcg: rc %d after %d its\n", rc, it);
+    if (c->opt.verbose > 1) fprintf(stderr, "      learned gcr: rc %d after %d its\n", rc, it);
     if (rc == 0) return 0;
     *which = 6;
     if (c->pc_stale || !c->pc_ready) {
         pc_setup(c);
         c->pc_stale = false;
     }
-    residual_fine64(c->K, c->dU.p, c->rhs.p, c->w_s.p, s);  // remaining residual
-    dot_dev<double, double>(c->ndof, c->w_s.p, c->w_s.p, c->partial.p, c->scal.p + 6, s);
+    // remaining residual b2 = b - K x and the AMG correction x2 live in the GCR direction
+    // store (free now): PCG and BiCGStab use every w_* workspace themselves
+    double *b2 = c->fcg_P.p, *x2 = c->fcg_P.p + c->ndof;
+    residual_fine64(c->K, c->dU.p, c->rhs.p, b2, s);
+    dot_dev<double, double>(c->ndof, b2, b2, c->partial.p, c->scal.p + 6, s);
     dot_dev<double, double>(c->ndof, c->rhs.p, c->rhs.p, c->partial.p, c->scal.p + 7, s);
     PF_HIP(hipMemcpyAsync(c->h_scal, c->scal.p, 8 * sizeof(double), hipMemcpyDeviceToHost, s));
     c->sync();
@@ -404,22 +507,22 @@ static int learned_solve(pfem_ctx *c, double rtol, int itmax, int *total, int *w
     if (!(rn > rtol * bn)) return 0;
     const double rt = std::min(0.5, rtol * bn / rn);
     if (c->opt.precision == PFEM_PREC_FP32_IR) {
-        rc = fpcg(c, c->w_s.p, c->w_x.p, rt, itmax, true, 0, &it, Amg32Pc{c});
+        rc = fpcg(c, b2, x2, rt, itmax, true, 0, &it, Amg32Pc{c});
     } else {
-        rc = pcg<double>(c, c->w_s.p, c->w_x.p, rt, itmax, &it);
+        rc = pcg<double>(c, b2, x2, rt, itmax, &it);
     }
     *total += it;
     if (rc != 0) {
         int it2 = 0;
-        rc = bicgstab<double>(c, c->w_s.p, c->w_x.p, rt, 2 * itmax, &it2);
+        rc = bicgstab<double>(c, b2, x2, rt, 2 * itmax, &it2);
         *total += it2;
     }
-    axpby<double>(c->ndof, 1.0, c->w_x.p, 1.0, c->dU.p, s);
+    axpby<double>(c->ndof, 1.0, x2, 1.0, c->dU.p, s);
     return rc;
 }
 
 int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *used, double *relres) {
-    const bool learned = c->net && c->net_mode == 2;
+    const bool learned = c->net_mode >= 2;
     if (learned) {
         if (c->opt.precision == PFEM_PREC_FP32_IR) prepare_fp32(c);
         if (rebuild) c->pc_stale = true;  // the AMG hierarchy is rebuilt only if it is needed

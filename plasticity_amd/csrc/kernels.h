@@ -24,6 +24,8 @@ void cg_update(int64_t n, const double *num, const double *den, const T *p, cons
                hipStream_t s);
 template <class T>
 void axpby(int64_t n, double a, const T *x, double b, T *y, hipStream_t s);
+void axpy_safe_ratio(int64_t n, const double *num, const double *den, double sign, const double *x, double *y,
+                     hipStream_t s);
 void cg_update_mixed(int64_t n, const double *num, const double *den, const float *p, const float *q, double *x,
                      float *r, hipStream_t s);
 template <class S, class D>

@@ -70,6 +70,14 @@ __global__ void axpy_ratio_kernel(int64_t n, const double *num, const double *de
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         y[i] = (T)((double)y[i] + al * (double)x[i]);
 }
+// y += sign * a * x with a = num/den, or a = 0 when den <= 0 (a search direction of
+// non-positive curvature contributes nothing)
+__global__ void axpy_safe_ratio_kernel(int64_t n, const double *num, const double *den, double sign,
+                                       const double *__restrict__ x, double *__restrict__ y) {
+    const double a = *den > 0 ? sign * (*num / *den) : 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] += a * x[i];
+}
 template <class T>
 __global__ void xpby_ratio_kernel(int64_t n, const double *num, const double *den, const T *__restrict__ z,
                                   T *__restrict__ p) {
@@ -287,6 +295,11 @@ template <class T>
 void cg_update(int64_t n, const double *num, const double *den, const T *p, const T *q, T *x, T *r,
                hipStream_t s) {
     hipLaunchKernelGGL((cg_update_kernel<T>), dim3(grid_strided(n)), dim3(256), 0, s, n, num, den, p, q, x, r);
+    PF_HIP(hipGetLastError());
+}
+void axpy_safe_ratio(int64_t n, const double *num, const double *den, double sign, const double *x, double *y,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(axpy_safe_ratio_kernel, dim3(grid_strided(n)), dim3(256), 0, s, n, num, den, sign, x, y);
     PF_HIP(hipGetLastError());
 }
 void cg_update_mixed(int64_t n, const double *num, const double *den, const float *p, const float *q, double *x,

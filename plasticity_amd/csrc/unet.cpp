@@ -165,7 +165,7 @@ void net_apply(pfem_net *n, const double *r, const uint8_t *mask, double *z, hip
     launch_absmax(3 * n->nvox[0], r, mask, n->red.p, n->scal.p + 4, s);
     launch_input_from_dofs(n->nvox[0], r, mask, n->scal.p + 4, n->x16.p, s);
     net_forward(n, s);
-    launch_output_to_dofs(n->nvox[0], n->out16.p, mask, r, z, s);
+    launch_output_to_dofs(n->nvox[0], n->out16.p, mask, nullptr, z, s);
 }
 
 void net_log_pair(pfem_net *n, const double *r, const double *z, const uint8_t *mask, hipStream_t s) {
@@ -402,10 +402,12 @@ int pfem_net_train(pfem_net *n, int steps, int batch, double lr, uint64_t seed, 
 }
 
 int pfem_attach_net(pfem_ctx *c, pfem_net *n, int mode, int stride) {
-    if (!c || mode < 0 || mode > 2 || stride < 1) return PFEM_ERR_BAD_ARG;
-    if (mode > 0 && (!n || n->nvox[0] != c->nnodes || n->device != c->device)) return PFEM_ERR_BAD_ARG;
+    if (!c || mode < 0 || mode > 3 || stride < 1) return PFEM_ERR_BAD_ARG;
+    if ((mode == 1 || mode == 2) && (!n || n->nvox[0] != c->nnodes || n->device != c->device))
+        return PFEM_ERR_BAD_ARG;
+    if (mode == 3) n = nullptr;
     if (hipDeviceSynchronize() != hipSuccess) return PFEM_ERR_HIP;
-    c->net = mode > 0 ? n : nullptr;
+    c->net = (mode == 1 || mode == 2) ? n : nullptr;
     c->net_mode = mode;
     c->net_stride = stride;
     c->net_calls = 0;

@@ -93,7 +93,7 @@ void net_forward(pfem_net *n, hipStream_t s);                      // x16 -> out
 void net_backward(pfem_net *n, bool accumulate, float scale, hipStream_t s);  // gr[11] = dL/dout16 -> G
 void net_pack(pfem_net *n, hipStream_t s);
 // z = N(r) for a nodal vector on the net's grid: constrained DOFs (mask) are dropped
-// on input and copied from r on output.
+// on input and set to 0 on output.
 void net_apply(pfem_net *n, const double *r, const uint8_t *mask, double *z, hipStream_t s);
 // Append the sample (r, z) (constrained DOFs zeroed, each scaled by its max |.|).
 void net_log_pair(pfem_net *n, const double *r, const double *z, const uint8_t *mask, hipStream_t s);

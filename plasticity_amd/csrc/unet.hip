@@ -272,14 +272,15 @@ __global__ void input_from_pairs_kernel(int64_t nvox, const uint16_t *__restrict
         x16[q] = c < 3 ? p3[3 * v + c] : (uint16_t)0;
     }
 }
-// 16-channel FP32 output -> nodal FP64 vector; constrained DOFs get r (identity rows).
+// 16-channel FP32 output -> nodal FP64 vector; constrained DOFs get r (identity rows),
+// or 0 when r is null.
 __global__ void output_to_dofs_kernel(int64_t nvox, const float *__restrict__ o16, const uint8_t *__restrict__ mask,
                                       const double *__restrict__ r, double *__restrict__ z) {
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nvox * 3;
          q += (int64_t)gridDim.x * blockDim.x) {
         const int64_t v = q / 3;
         const int c = (int)(q % 3);
-        z[q] = (mask && mask[q]) ? r[q] : (double)o16[v * 16 + c];
+        z[q] = (mask && mask[q]) ? (r ? r[q] : 0.0) : (double)o16[v * 16 + c];
     }
 }
 // Nodal FP64 vector (constrained DOFs zeroed) -> BF16 3-channel sample scaled by 1/scale.

@@ -25,7 +25,7 @@ import numpy as np
 from . import _lib as L
 
 CHANNELS = (16, 32, 64, 64)
-MODES = {"off": 0, "log": 1, "precondition": 2}
+MODES = {"off": 0, "log": 1, "precondition": 2, "jacobi": 3}
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "unet_16_32_64_64.npz")
 
 
@@ -114,7 +114,8 @@ class UNet:
 
 def attach(model, net, mode="precondition", stride=1):
     """Attach `net` to a Model: 'log' samples from AMG solves, 'precondition' uses the
-    network in flexible CG with AMG fallback, 'off' detaches."""
+    network (coarse correction + block-Jacobi) in flexible CG with AMG fallback,
+    'jacobi' the same flexible CG without the network, 'off' detaches."""
     m = MODES[mode] if isinstance(mode, str) else int(mode)
     L.check(L.lib().pfem_attach_net(model._h, None if net is None else net._h, m, int(stride)), "attach_net")
     model._net = net if m else None
@@ -149,14 +150,19 @@ def inclusion_fraction(seed):
 def collect_samples(net, seeds, n, load_steps=(1, 2, 3), stride=2, log=print):
     """Run Newton load steps on the sample meshes with the AMG solver, logging every
     `stride`-th (residual, V-cycle correction) pair into `net`'s store."""
-    from .model import default_options
+    from .model import ElementInversionError, default_options
     t0 = time.perf_counter()
     for s in seeds:
         m, wl = sample_model(s, n, inclusion_fraction(s), options=default_options())
         attach(m, net, "log", stride)
         for k in load_steps:
-            conv, hist, kry, _ = m.newton(k / wl.nsteps, tol=wl.newton_tol, maxiter=25)
+            try:
+                conv, hist, kry, _ = m.newton(k / wl.nsteps, tol=wl.newton_tol, maxiter=25)
+            except ElementInversionError as e:  # samples logged so far are kept
+                log(f"[learned] seed {s} load step {k}: {e}")
+                break
             if not conv:
+                log(f"[learned] seed {s} load step {k}: Newton did not converge")
                 break
             m.commit()
         attach(m, None, "off")
@@ -164,3 +170,96 @@ def collect_samples(net, seeds, n, load_steps=(1, 2, 3), stride=2, log=print):
             f"({time.perf_counter() - t0:.1f}s)")
         m.close()
     return net.num_pairs
+
+
+def evaluate(net_weights, seeds, n, load_steps=(1, 2, 3), precision="fp64", log=print):
+    """Newton load steps on held-out sample meshes at linear rtol 1e-8, once with the AMG
+    solver and once with the learned preconditioner in flexible GCR (AMG fallback), and
+    once with the same GCR without the network (block-Jacobi).  Returns per-seed totals."""
+    from .model import default_options
+    out = []
+    for s in seeds:
+        row = {"seed": s, "inclusion": inclusion_fraction(s)}
+        for label in ("amg", "learned", "jacobi"):
+            m, wl = sample_model(s, n, inclusion_fraction(s),
+                                 options=default_options(precision=precision, eta_min=1e-8, eta_max=1e-8))
+            net = None
+            if label == "learned":
+                net = UNet(grid_of(wl.mesh), CHANNELS)
+                net.set_params(net_weights)
+                attach(m, net, "precondition")
+            elif label == "jacobi":
+                attach(m, None, "jacobi")
+            kry = newton = 0
+            t_solve = 0.0
+            used = []
+            for k in load_steps:
+                conv, hist, kr, res = m.newton(k / wl.nsteps, tol=wl.newton_tol, maxiter=25)
+                if not conv:
+                    raise RuntimeError(f"seed {s} load step {k} ({label}) did not converge")
+                m.commit()
+                kry += sum(kr)
+                newton += len(hist)
+                t_solve += res.t_solve
+                used += [res.solver_used[i] for i in range(len(kr))]
+            row[label] = {"krylov": kry, "newton": newton, "t_solve": t_solve,
+                          "fallbacks": sum(1 for u in used if u == 6), "solves": len(used)}
+            attach(m, None, "off")
+            if net is not None:
+                net.close()
+            m.close()
+        log(f"[learned] eval seed {s}: AMG {row['amg']['krylov']} its {row['amg']['t_solve']:.3f}s | "
+            f"learned {row['learned']['krylov']} its {row['learned']['t_solve']:.3f}s, "
+            f"fallbacks {row['learned']['fallbacks']}/{row['learned']['solves']} | "
+            f"jacobi {row['jacobi']['krylov']} its {row['jacobi']['t_solve']:.3f}s, "
+            f"fallbacks {row['jacobi']['fallbacks']}/{row['jacobi']['solves']}")
+        out.append(row)
+    return out
+
+
+def _seeds(spec):
+    a, b = spec.split("-") if "-" in spec else (spec, spec)
+    return list(range(int(a), int(b) + 1))
+
+
+def main(argv=None):
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="train / evaluate the learned preconditioner")
+    ap.add_argument("cmd", choices=["train", "eval"])
+    ap.add_argument("--n", type=int, default=63, help="elements per side of the sample blocks")
+    ap.add_argument("--seeds", default="2-20")
+    ap.add_argument("--load-steps", default="1-5")
+    ap.add_argument("--stride", type=int, default=1)
+    ap.add_argument("--capacity", type=int, default=20000)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--weights", default=WEIGHTS)
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_ir"])
+    a = ap.parse_args(argv)
+    log = lambda s: print(s, flush=True)  # noqa: E731
+    if a.cmd == "train":
+        net = UNet((a.n + 1,) * 3, CHANNELS, capacity=a.capacity, seed=0)
+        collect_samples(net, _seeds(a.seeds), a.n, load_steps=_seeds(a.load_steps), stride=a.stride, log=log)
+        t0 = time.perf_counter()
+        chunk = max(1, a.steps // 20)
+        done = 0
+        while done < a.steps:
+            k = min(chunk, a.steps - done)
+            lr = a.lr if done < 0.8 * a.steps else 0.1 * a.lr
+            loss = net.train(k, batch=a.batch, lr=lr, seed=done)
+            done += k
+            log(f"[learned] adam step {done}/{a.steps}: mean loss {loss:.4f} ({time.perf_counter() - t0:.1f}s)")
+        net.save(a.out or a.weights)
+        log(f"[learned] saved {a.out or a.weights}")
+    else:
+        with np.load(a.weights, allow_pickle=False) as f:
+            w = f["w"].copy()
+        rows = evaluate(w, _seeds(a.seeds), a.n, load_steps=_seeds(a.load_steps), precision=a.precision, log=log)
+        print(json.dumps(rows))
+
+
+if __name__ == "__main__":
+    main()

@@ -85,6 +85,9 @@ def default_options(**kw):
         if k == "amg_rebuild":  # 0: once per load step (Solver.jl:374), 1: every Newton iteration
             o.reserved[0] = int(v)
             continue
+        if k == "fcg_directions":  # directions kept by the learned preconditioner's flexible CG
+            o.reserved[1] = int(v)
+            continue
         if k == "precision" and isinstance(v, str):
             v = {"fp64": L.PREC_FP64, "fp32_ir": L.PREC_FP32_IR}[v]
         if k == "smoother" and isinstance(v, str):

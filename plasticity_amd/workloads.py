@@ -15,7 +15,7 @@ The x = Lx face gets the ramped axial displacement.
   workload 3  samples for the learned preconditioner: workload-1-style blocks with
               jitter seed `seed` and ellipsoidal inclusions (sy 750 MPa) covering
               `inclusion` of the volume (seeds 2-20 train, 100-104 evaluate),
-              compression in 160 steps
+              compression in 320 steps
   workload 4  150^3 notched bar (20% section reduction), +-5% jitter (seed 7),
               tension to 30% in 60 steps
 Sizes can be overridden (n=...) for parity runs on small meshes.
@@ -103,10 +103,10 @@ def workload(k: int, n: int | None = None, nsteps: int | None = None, seed: int 
         d, v, r = uniaxial_constraints(mesh, -0.2)
         hard = dict(STEEL, sy0=750e6)
         mats = [STEEL, hard] if mid is not None else [STEEL]
-        # 160 load steps: Newton without line search inverts elements of two-phase
-        # blocks in compression at 0.5% strain increments on meshes this coarse
+        # 320 load steps: Newton without line search diverges or inverts elements of
+        # two-phase blocks in compression at 0.125% strain increments (32^3 blocks)
         return Workload(f"sample block {n}^3 seed {seed} inclusions {inclusion:.2f}", mesh, mats, mid, d, v, r,
-                        nsteps or 160, 1e-8, 1e-8)
+                        nsteps or 320, 1e-8, 1e-8)
     if k == 4:
         n = 150 if n is None else n
         mesh = notched_bar_mesh(n, length=1.0, notch_depth=0.2, jitter=0.05, seed=7)

@@ -117,3 +117,48 @@ def test_learned_fcg_converges_with_amg_fallback(need_gpu):
     assert conv
     assert set(used) <= {5, 6}
     attach(m, None, "off")
+
+
+def _solve_its(m, lam, rtol, mode, net=None):
+    attach(m, net, mode) if mode != "amg" else attach(m, None, "off")
+    m.assemble()
+    m.subtract_external(lam)
+    m.impose_dirichlet(lam)
+    dU, it, used, relres = m.linear_solve(rtol)
+    attach(m, None, "off")
+    return it, used, relres
+
+
+def test_trained_network_gains_over_the_method_without_it(need_gpu):
+    """Train on samples logged from two sample blocks, then on a held-out block count the
+    GCR iterations that reduce the residual of the first Newton system 100-fold: the
+    trained network's preconditioner needs fewer than the same GCR with block-Jacobi
+    alone (no network) and fewer than with the untrained network."""
+    n = 16
+    grid = (n + 1,) * 3
+    net = UNet(grid, CH, capacity=2000, seed=1)
+    for seed in (2, 3):
+        m, wl = sample_model(seed, n, 0.15, options=pa.default_options())
+        attach(m, net, "log", 1)
+        for k in (1, 2, 3):
+            conv, *_ = m.newton(k / wl.nsteps, tol=wl.newton_tol, maxiter=25)
+            assert conv
+            m.commit()
+        attach(m, None, "off")
+        m.close()
+    print(f"[learned] gain test: {net.num_pairs} samples", flush=True)
+    loss = net.train(400, batch=4, lr=1e-3, seed=0)
+    print(f"  mean training loss {loss:.3f}", flush=True)
+    fresh = UNet(grid, CH, seed=1)
+    m, wl = sample_model(100, n, 0.15, options=pa.default_options(cg_itmax=400))
+    lam = 1 / wl.nsteps
+    its = {}
+    for label, mode, nn in [("trained", "precondition", net), ("untrained", "precondition", fresh),
+                            ("jacobi", "jacobi", None), ("amg", "amg", None)]:
+        it, used, relres = _solve_its(m, lam, 1e-2, mode, nn)
+        its[label] = it
+        print(f"  {label:9s}: {it:3d} its to rtol 1e-2 (solver {used}, relres {relres:.2e})", flush=True)
+        if label != "amg":  # GCR stops on ||r|| <= rtol ||b||; PCG on the preconditioned norm
+            assert used == 5 and relres <= 1e-2 * 1.0001
+    assert its["trained"] < its["jacobi"]
+    assert its["trained"] < its["untrained"]
