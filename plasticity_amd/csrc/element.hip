@@ -10,10 +10,10 @@
 //                  element_force_tangent_finite! Elements.jl:405-494 with
 //                  finite_kinematics FiniteStrain.jl:148-170, finite_stress_update
 //                  :292-346, dPdF :503-547 and return_map Materials.jl:159-250.
-//   scatter_kernel 16 threads per element of one colour: thread (a, half) contracts
-//                  the 8 packets into the 3x3 blocks Ke(a, b) for four b and adds them
-//                  into the fine BSR K through a precomputed slot map (no search, no
-//                  atomics: same-colour elements share no node, Assembly.jl:370-385).
+//   scatter_kernel one 64-lane wave per element of one colour: lane (a, b) contracts
+//                  the 8 packets into the 3x3 block Ke(a, b) and adds it into the fine
+//                  BSR K through a precomputed slot map (no search, no atomics:
+//                  same-colour elements share no node, Assembly.jl:370-385).
 //                  Ke = sum_g w G^T A Geff with Geff = s G + (s/3) vec(F)(g0 - gJ)^T
 //                  (Elements.jl:470-478, :524-531).
 #include <hip/hip_runtime.h>
@@ -500,28 +500,84 @@ __device__ __forceinline__ void contract_blocks(const double *__restrict__ pe,
     }
 }
 
-__global__ __launch_bounds__(256) void scatter_kernel(const double *__restrict__ packet,
-                                                      const double *__restrict__ g0,
-                                                      const int64_t *__restrict__ elist, int64_t count,
-                                                      const int64_t *__restrict__ conn,
-                                                      const int32_t *__restrict__ slots,
-                                                      double *__restrict__ Kv, double *__restrict__ R,
-                                                      int fbar) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t li = t >> 4;
+// One 64-lane wave per element of one colour; lane (a, b) = (lane / 8, lane % 8)
+// accumulates the 3x3 block Ke(a, b) over the 8 GP packets and adds it into the fine
+// BSR K through the precomputed slot map (same-colour elements share no node, so no
+// atomics; colours run serially, Assembly.jl:370-385).  The wave first copies the
+// element's 8 packets into LDS with coalesced 16-byte loads; the lanes then read them
+// as LDS broadcasts.  Each lane keeps 9 accumulators.
+//   Ke(a,b)_ik = sum_g [ sum_jl dNa_j dNb_l SA_(i+3j),(k+3l) + h_i (g0_b - gJ_b)_k ]
+//   h_i = sum_j dNa_j q_(i+3j),  gJ_b,k = sum_j dNb_j (F^-1)_jk        (Elements.jl:470-478)
+__global__ __launch_bounds__(64) void scatter_kernel(const double *__restrict__ packet,
+                                                     const double *__restrict__ g0,
+                                                     const int64_t *__restrict__ elist, int64_t count,
+                                                     const int64_t *__restrict__ conn,
+                                                     const int32_t *__restrict__ slots,
+                                                     double *__restrict__ Kv, double *__restrict__ R, int fbar) {
+    __shared__ __attribute__((aligned(16))) double sp[8 * PK];
+    const int64_t li = blockIdx.x;
     if (li >= count) return;
-    const int sub = (int)(t & 15), a = sub >> 1, half = sub & 1;
     const int64_t e = elist[li];
-    double acc[4][9], fe[3];
-    contract_blocks(packet + (size_t)e * 8 * PK, g0 + 24 * e, a, half, fbar, acc, fe);
-#pragma unroll
-    for (int bb = 0; bb < 4; ++bb) {
-        const int64_t pos = slots[64 * e + 8 * a + 4 * half + bb];
-        double *kb = Kv + 9 * pos;
-#pragma unroll
-        for (int k = 0; k < 9; ++k) kb[k] += acc[bb][k];
+    const int lane = threadIdx.x, a = lane >> 3, b = lane & 7;
+    {  // the element's 8 packets are contiguous (8448 B): 16-byte loads, all in flight at once
+        const double2 *src = reinterpret_cast<const double2 *>(packet + (size_t)e * 8 * PK);
+        double2 *dst = reinterpret_cast<double2 *>(sp);
+        for (int q = lane; q < 4 * PK; q += 64) dst[q] = src[q];
     }
-    if (half == 0) {
+    __syncthreads();
+    double acc[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) acc[t] = 0.0;
+    double fe[3] = {0.0, 0.0, 0.0};
+    double g0b[3] = {0.0, 0.0, 0.0};
+    if (fbar)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) g0b[k] = g0[24 * e + 3 * b + k];
+#pragma unroll 1
+    for (int g = 0; g < 8; ++g) {
+        const double *pk = sp + g * PK;
+        double dNa[3], dNb[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            dNa[j] = pk[108 + 3 * a + j];
+            dNb[j] = pk[108 + 3 * b + j];
+        }
+        double w[3][3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int l = 0; l < 3; ++l) w[j][l] = dNa[j] * dNb[l];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                double sum = 0.0;
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+#pragma unroll
+                    for (int l = 0; l < 3; ++l) sum += w[j][l] * pk[(i + 3 * j) + 9 * (k + 3 * l)];
+                acc[3 * i + k] += sum;
+            }
+        if (fbar) {
+            double h[3], gd[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) h[i] = dNa[0] * pk[81 + i] + dNa[1] * pk[84 + i] + dNa[2] * pk[87 + i];
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                gd[k] = g0b[k] - (dNb[0] * pk[90 + 3 * k] + dNb[1] * pk[91 + 3 * k] + dNb[2] * pk[92 + 3 * k]);
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) acc[3 * i + k] += h[i] * gd[k];
+        }
+        if (b == 0)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) fe[i] += dNa[0] * pk[99 + i] + dNa[1] * pk[102 + i] + dNa[2] * pk[105 + i];
+    }
+    double *kb = Kv + 9 * (int64_t)slots[64 * e + 8 * a + b];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) kb[t] += acc[t];
+    if (b == 0) {
         const int64_t n = conn[8 * e + a];
 #pragma unroll
         for (int i = 0; i < 3; ++i) R[3 * n + i] += fe[i];
@@ -591,10 +647,8 @@ void launch_gp_kernel(const ElementArgs &a, int64_t e_begin, int64_t e_count, hi
 void launch_scatter_kernel(const ElementArgs &a, const int64_t *elem_list, int64_t count,
                            const int32_t *slots, double *Kv, double *R, hipStream_t s) {
     if (count <= 0) return;
-    const int64_t threads = 16 * count;
-    const int64_t blocks = (threads + 255) / 256;
-    hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a.packet, a.g0, elem_list,
-                       count, a.conn, slots, Kv, R, a.fbar);
+    hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)count), dim3(64), 0, s, a.packet, a.g0, elem_list, count,
+                       a.conn, slots, Kv, R, a.fbar);
     PF_HIP(hipGetLastError());
 }
 
