@@ -131,6 +131,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--amg-rebuild", type=int, default=1, help="0: once per load step, 1: every Newton iteration")
     ap.add_argument("--verbose", type=int, default=0)
+    ap.add_argument("--fp16-fine", action="store_true",
+                    help="FP32-IR: fine-level smoother products with an FP16 copy of the scaled K")
     ap.add_argument("--mode", default="newton", choices=["newton", "spmv", "copy"],
                     help="newton: the metric; spmv: only repeated fine SpMVs (for counter runs); "
                          "copy: a plain 1 GiB device-to-device copy, the calibration of byte counters")
@@ -158,7 +160,7 @@ def main():
         return
     log(f"[bench] workload: {wl.name}, nodes {wl.mesh.nnodes}, dof {3 * wl.mesh.nnodes}")
     opts = pa.default_options(eta_min=wl.krylov_rtol, eta_max=wl.krylov_rtol, precision=args.precision,
-                              verbose=args.verbose, amg_rebuild=args.amg_rebuild)
+                              verbose=args.verbose, amg_rebuild=args.amg_rebuild, fp16_fine=args.fp16_fine)
     t0 = time.perf_counter()
     mats = [pa.J2Material(**d) for d in wl.materials]
     m = pa.Model(wl.mesh, mats if len(mats) > 1 else mats[0], options=opts, material_id=wl.material_id)

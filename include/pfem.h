@@ -84,8 +84,11 @@ typedef struct pfem_options {
     int32_t reserved[7];        /* reserved[0]: AMG rebuild policy, 0 = once per load step
                                    (Solver.jl:374), 1 = every Newton iteration, 2 = at the
                                    first two Newton iterations of a load step;
-                                   reserved[1]: directions kept by the flexible CG of the
-                                   learned preconditioner (<= 0: 16) */
+                                   reserved[1]: directions kept by the flexible GCR of the
+                                   learned preconditioner (<= 0: 16);
+                                   reserved[2]: FP32-IR only, 1 = the fine-level products
+                                   inside the V-cycle's smoother use an FP16 copy of S K S
+                                   (S = diag|K_ii|^-1/2), 0 = the FP32 copy of K (default) */
 } pfem_options;
 
 typedef struct pfem_fail {

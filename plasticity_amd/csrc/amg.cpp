@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <type_traits>
 
 #include "context.h"
 
@@ -447,15 +448,27 @@ template <> struct Vec<float> {
     static const float *K(pfem_ctx *c) { return c->K.v32.p; }
 };
 
+// The fine-level products inside the FP32 V-cycle's smoother use the FP16 scaled copy
+// when it is present; the residual restricted to the coarse level (vcycle) stays FP32.
 template <class T>
 void level_matvec(pfem_ctx *c, AmgLevel &L, int l, const T *x, T *y) {
-    if (l == 0) sell_spmv<3, T, T>(c->K, Vec<T>::K(c), x, y, c->stream);
-    else bsr_spmv<6, 6, T, T>(L.nb, L.rowptr.p, L.colind.p, Vec<T>::A(L), x, y, 0.0, c->stream);
+    if (l == 0) {
+        if constexpr (std::is_same<T, float>::value)
+            if (c->half_fine) return spmv_fine_half(c->K, x, y, c->stream);
+        sell_spmv<3, T, T>(c->K, Vec<T>::K(c), x, y, c->stream);
+    } else {
+        bsr_spmv<6, 6, T, T>(L.nb, L.rowptr.p, L.colind.p, Vec<T>::A(L), x, y, 0.0, c->stream);
+    }
 }
 template <class T>
-void level_residual(pfem_ctx *c, AmgLevel &L, int l, const T *x, const T *b, T *r) {
-    if (l == 0) sell_residual<3, T, T>(c->K, Vec<T>::K(c), x, b, r, c->stream);
-    else bsr_residual<6, T, T>(L.nb, L.rowptr.p, L.colind.p, Vec<T>::A(L), x, b, r, c->stream);
+void level_residual(pfem_ctx *c, AmgLevel &L, int l, const T *x, const T *b, T *r, bool in_smoother = true) {
+    if (l == 0) {
+        if constexpr (std::is_same<T, float>::value)
+            if (c->half_fine && in_smoother) return residual_fine_half(c->K, x, b, r, c->stream);
+        sell_residual<3, T, T>(c->K, Vec<T>::K(c), x, b, r, c->stream);
+    } else {
+        bsr_residual<6, T, T>(L.nb, L.rowptr.p, L.colind.p, Vec<T>::A(L), x, b, r, c->stream);
+    }
 }
 template <class T>
 void diag_apply(AmgLevel &L, const T *r, T *z, double scale, hipStream_t s) {
@@ -513,7 +526,7 @@ void vcycle(pfem_ctx *c, Amg *a, int l, const T *b, T *x) {
     }
     AmgLevel &N = a->L[l + 1];
     smooth<T>(c, a, l, b, x, true);
-    level_residual<T>(c, L, l, x, b, Vec<T>::r(L));
+    level_residual<T>(c, L, l, x, b, Vec<T>::r(L), false);
     if (L.bs == 3) bsr_spmv<6, 3, T, T>(L.nagg, L.R_rowptr.p, L.R_colind.p, Vec<T>::R(L), Vec<T>::r(L), Vec<T>::b(N), 0.0, s);
     else bsr_spmv<6, 6, T, T>(L.nagg, L.R_rowptr.p, L.R_colind.p, Vec<T>::R(L), Vec<T>::r(L), Vec<T>::b(N), 0.0, s);
     vcycle<T>(c, a, l + 1, Vec<T>::b(N), Vec<T>::x(N));

@@ -67,6 +67,7 @@ struct pfem_ctx {
     int64_t net_calls = 0;
     bool pc_stale = false;  // AMG rebuild requested while the network preconditions
     pfem::dbuf<double> net_dinv;  // block-diagonal inverse of K for the learned preconditioner
+    bool half_fine = false;       // FP32 V-cycle: fine-level products with the FP16 scaled K
     pfem::dbuf<double> fcg_P, fcg_Q; // truncated flexible CG: search directions and K times them
 
     // statistics

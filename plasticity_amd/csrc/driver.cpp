@@ -474,6 +474,8 @@ static void prepare_fp32(pfem_ctx *c) {
             b->alloc(c->ndof);
     if (!c->K.v32.p) c->K.v32.alloc(9 * c->K.nslots);
     convert<double, float>(9 * c->K.nslots, c->K.v64.p, c->K.v32.p, c->stream);
+    c->half_fine = c->opt.reserved[2] == 1;  // FP16 scaled fine level inside the V-cycle's smoother
+    if (c->half_fine) half_scaled_copy(c->K, c->stream);
 }
 
 // GCR(m) with the learned preconditioner; when it stalls (no 0.1% gain in the best

@@ -13,6 +13,10 @@ void spmv_fine32_dot(const FineMatrix &K, const float *x, float *y, double *part
                      hipStream_t s);
 void residual_fine64(const FineMatrix &K, const double *x, const double *b, double *r, hipStream_t s);
 size_t reduce_partials_needed(int64_t nb);
+// FP16 copy of the diagonally scaled fine K (K.v16, K.sinv) and its products (FP32 vectors)
+void half_scaled_copy(FineMatrix &K, hipStream_t s);
+void spmv_fine_half(const FineMatrix &K, const float *x, float *y, hipStream_t s);
+void residual_fine_half(const FineMatrix &K, const float *x, const float *b, float *r, hipStream_t s);
 template <class A, class B>
 void dot_dev(int64_t n, const A *a, const B *b, double *partial, double *out, hipStream_t s);
 template <class T>

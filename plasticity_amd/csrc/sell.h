@@ -74,4 +74,54 @@ __global__ __launch_bounds__(64) void sell_lds_kernel(int64_t nb, const int64_t 
     }
 }
 
+// The same product with an FP16 copy of the diagonally scaled matrix
+// Khat = S K S, S = diag(1/sqrt|K_ii|) (unit diagonal, |Khat_ij| <~ 1, so FP16 holds it
+// without overflow): y = S^-1 Khat S^-1 x, evaluated as sinv_i * sum_j Khat_ij (sinv_j x_j)
+// with sinv = sqrt|K_ii| and FP32 accumulation.  Used for the fine-level products of
+// the FP32 V-cycle; 22 B per block instead of 40 B (FP32) or 76 B (FP64).
+template <int MODE>
+__global__ __launch_bounds__(64) void sell_half_kernel(int64_t nb, const int64_t *__restrict__ slice_ptr,
+                                                       const int32_t *__restrict__ colind,
+                                                       const _Float16 *__restrict__ vals,
+                                                       const float *__restrict__ sinv, const float *__restrict__ x,
+                                                       const float *__restrict__ b, float *__restrict__ y) {
+    constexpr int NV = SELL_C * 9 * 2 / 16;  // 72 chunks of 16 B per slice column
+    __shared__ __attribute__((aligned(16))) _Float16 tile[SELL_C * 9];
+    const int lane = threadIdx.x;
+    const int64_t s = xcd_contiguous(blockIdx.x, gridDim.x);
+    const int64_t i = s * SELL_C + lane;
+    const int64_t base = slice_ptr[s];
+    const int w = (int)((slice_ptr[s + 1] - base) / SELL_C);
+    float y0 = 0.f, y1 = 0.f, y2 = 0.f;
+    for (int k = 0; k < w; ++k) {
+        const int64_t col0 = base + (int64_t)k * SELL_C;
+        const uint4 *src = reinterpret_cast<const uint4 *>(vals + 9 * col0);
+        uint4 *dst = reinterpret_cast<uint4 *>(tile);
+        for (int t = lane; t < NV; t += SELL_C) dst[t] = src[t];
+        const int64_t j = colind[col0 + lane];
+        const float x0 = x[3 * j] * sinv[3 * j], x1 = x[3 * j + 1] * sinv[3 * j + 1],
+                    x2 = x[3 * j + 2] * sinv[3 * j + 2];
+        __syncthreads();
+        const _Float16 *v = tile + 9 * lane;
+        y0 += (float)v[0] * x0 + (float)v[1] * x1 + (float)v[2] * x2;
+        y1 += (float)v[3] * x0 + (float)v[4] * x1 + (float)v[5] * x2;
+        y2 += (float)v[6] * x0 + (float)v[7] * x1 + (float)v[8] * x2;
+        __syncthreads();
+    }
+    if (i < nb) {
+        y0 *= sinv[3 * i];
+        y1 *= sinv[3 * i + 1];
+        y2 *= sinv[3 * i + 2];
+        if (MODE == SELL_RES) {
+            y[3 * i] = b[3 * i] - y0;
+            y[3 * i + 1] = b[3 * i + 1] - y1;
+            y[3 * i + 2] = b[3 * i + 2] - y2;
+        } else {
+            y[3 * i] = y0;
+            y[3 * i + 1] = y1;
+            y[3 * i + 2] = y2;
+        }
+    }
+}
+
 }  // namespace pfem

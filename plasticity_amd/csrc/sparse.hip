@@ -297,6 +297,57 @@ void cg_update(int64_t n, const double *num, const double *den, const T *p, cons
     hipLaunchKernelGGL((cg_update_kernel<T>), dim3(grid_strided(n)), dim3(256), 0, s, n, num, den, p, q, x, r);
     PF_HIP(hipGetLastError());
 }
+// sinv = sqrt|K_ii| (1 where the diagonal is 0), then the FP16 bits of Khat = S K S.
+__global__ void half_scale_diag_kernel(int64_t nb, const int32_t *__restrict__ diagpos, const double *__restrict__ v,
+                                       float *__restrict__ sinv) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= 3 * nb) return;
+    const int64_t i = q / 3;
+    const int c = (int)(q % 3);
+    const double d = fabs(v[9 * (int64_t)diagpos[i] + 4 * c]);
+    sinv[q] = d > 0 ? (float)sqrt(d) : 1.f;
+}
+__global__ __launch_bounds__(64) void half_scale_values_kernel(int64_t nb, const int64_t *__restrict__ slice_ptr,
+                                                               const int32_t *__restrict__ colind,
+                                                               const double *__restrict__ v,
+                                                               const float *__restrict__ sinv,
+                                                               _Float16 *__restrict__ h) {
+    const int lane = threadIdx.x;
+    const int64_t s = blockIdx.x, i = s * SELL_C + lane;
+    const int64_t base = slice_ptr[s];
+    const int w = (int)((slice_ptr[s + 1] - base) / SELL_C);
+    for (int k = 0; k < w; ++k) {
+        const int64_t pos = base + (int64_t)k * SELL_C + lane;
+        const int64_t j = colind[pos];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double val = i < nb ? v[9 * pos + 3 * r + c] / ((double)sinv[3 * i + r] * sinv[3 * j + c]) : 0.0;
+                h[9 * pos + 3 * r + c] = (_Float16)val;
+            }
+    }
+}
+void half_scaled_copy(FineMatrix &K, hipStream_t s) {
+    if (!K.sinv.p) K.sinv.alloc(3 * K.nb);
+    if (!K.v16.p) K.v16.alloc(9 * K.nslots);
+    hipLaunchKernelGGL(half_scale_diag_kernel, dim3(grid_for(3 * K.nb, 256)), dim3(256), 0, s, K.nb, K.diagpos.p,
+                       K.v64.p, K.sinv.p);
+    hipLaunchKernelGGL(half_scale_values_kernel, dim3((unsigned)K.nslices), dim3(SELL_C), 0, s, K.nb,
+                       K.slice_ptr.p, K.colind.p, K.v64.p, K.sinv.p, reinterpret_cast<_Float16 *>(K.v16.p));
+    PF_HIP(hipGetLastError());
+}
+void spmv_fine_half(const FineMatrix &K, const float *x, float *y, hipStream_t s) {
+    hipLaunchKernelGGL((sell_half_kernel<SELL_Y>), dim3((unsigned)K.nslices), dim3(SELL_C), 0, s, K.nb,
+                       K.slice_ptr.p, K.colind.p, reinterpret_cast<const _Float16 *>(K.v16.p), K.sinv.p, x,
+                       (const float *)nullptr, y);
+    PF_HIP(hipGetLastError());
+}
+void residual_fine_half(const FineMatrix &K, const float *x, const float *b, float *r, hipStream_t s) {
+    hipLaunchKernelGGL((sell_half_kernel<SELL_RES>), dim3((unsigned)K.nslices), dim3(SELL_C), 0, s, K.nb,
+                       K.slice_ptr.p, K.colind.p, reinterpret_cast<const _Float16 *>(K.v16.p), K.sinv.p, x, b, r);
+    PF_HIP(hipGetLastError());
+}
 void axpy_safe_ratio(int64_t n, const double *num, const double *den, double sign, const double *x, double *y,
                      hipStream_t s) {
     hipLaunchKernelGGL(axpy_safe_ratio_kernel, dim3(grid_strided(n)), dim3(256), 0, s, n, num, den, sign, x, y);

@@ -85,8 +85,11 @@ def default_options(**kw):
         if k == "amg_rebuild":  # 0: once per load step (Solver.jl:374), 1: every Newton iteration
             o.reserved[0] = int(v)
             continue
-        if k == "fcg_directions":  # directions kept by the learned preconditioner's flexible CG
+        if k == "fcg_directions":  # directions kept by the learned preconditioner's flexible GCR
             o.reserved[1] = int(v)
+            continue
+        if k == "fp16_fine":  # FP32-IR: smoother products on the fine level with the FP16 scaled K
+            o.reserved[2] = 1 if v else 0
             continue
         if k == "precision" and isinstance(v, str):
             v = {"fp64": L.PREC_FP64, "fp32_ir": L.PREC_FP32_IR}[v]
