@@ -125,6 +125,13 @@ def main():
     ap.add_argument("--workload", type=int, default=1, choices=[1, 2, 4],
                     help="1: 100^3 jittered compression (the metric), 2: 200^3 two-phase tension, "
                          "4: 150^3 notched bar tension")
+    ap.add_argument("--nsteps", type=int, default=None,
+                    help="load steps of the full load path (default: the workload's own)")
+    ap.add_argument("--line-search", type=int, default=0, metavar="HALVINGS",
+                    help="Newton backtracking on ||R|| (not in the reference; 0 = full Newton steps)")
+    ap.add_argument("--compare", action="store_true",
+                    help="after the timed run, repeat the same load steps untimed with the plain FP64 path "
+                         "and report the relative displacement and equivalent-plastic-strain differences")
     ap.add_argument("--learned", default=None, metavar="WEIGHTS",
                     help="precondition with the learned U-net (weights .npz) in flexible GCR, AMG fallback")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_ir"])
@@ -149,7 +156,7 @@ def main():
     import plasticity_amd as pa
     from plasticity_amd.workloads import workload
 
-    wl = workload(args.workload, n=args.n)
+    wl = workload(args.workload, n=args.n, nsteps=args.nsteps)
     if args.mode == "spmv":
         m = pa.Model(wl.mesh, pa.J2Material(**wl.materials[0]), options=pa.default_options())
         m.assemble(np.zeros(m.ndof))
@@ -160,7 +167,8 @@ def main():
         return
     log(f"[bench] workload: {wl.name}, nodes {wl.mesh.nnodes}, dof {3 * wl.mesh.nnodes}")
     opts = pa.default_options(eta_min=wl.krylov_rtol, eta_max=wl.krylov_rtol, precision=args.precision,
-                              verbose=args.verbose, amg_rebuild=args.amg_rebuild, fp16_fine=args.fp16_fine)
+                              verbose=args.verbose, amg_rebuild=args.amg_rebuild, fp16_fine=args.fp16_fine,
+                              line_search=args.line_search)
     t0 = time.perf_counter()
     mats = [pa.J2Material(**d) for d in wl.materials]
     m = pa.Model(wl.mesh, mats if len(mats) > 1 else mats[0], options=opts, material_id=wl.material_id)
@@ -218,6 +226,25 @@ def main():
         f"assemble {t_asm:.3f}s (gp kernel {gp_ms:.2f} ms + scatter {sc_ms:.2f} ms per assembly), "
         f"AMG set-up {t_setup:.3f}s, solve {t_solve:.3f}s; fine SpMV {t_sp * 1e3:.3f} ms = {achieved} GB/s; "
         f"AMG levels {list(s1.amg_ndof[:s1.amg_levels])} OC {s1.amg_operator_complexity:.3f}")
+    compare = None
+    if args.compare:
+        U_run, a_run = m.U, m.get_state()["abar"]
+        m.attach_net(None, "off") if net is not None else None
+        m.close()
+        o64 = pa.default_options(eta_min=wl.krylov_rtol, eta_max=wl.krylov_rtol, amg_rebuild=args.amg_rebuild,
+                                 line_search=args.line_search)
+        m64 = pa.Model(wl.mesh, mats if len(mats) > 1 else mats[0], options=o64, material_id=wl.material_id)
+        m64.constrain(wl.dir_dofs, wl.dir_vals, wl.dir_ramp)
+        r64 = m64.solve(nsteps=wl.nsteps, tol=wl.newton_tol, steps=range(1, args.warmup + args.steps + 1))
+        if not r64.converged:
+            raise RuntimeError("FP64 comparison run did not converge")
+        U64, a64 = m64.U, m64.get_state()["abar"]
+        compare = {"reference": "plain FP64 GPU path (PCG + SA-AMG), same load steps",
+                   "u_rel_err": float(np.linalg.norm(U_run - U64) / np.linalg.norm(U64)),
+                   "abar_rel_err": float(np.linalg.norm(a_run - a64) / max(np.linalg.norm(a64), 1e-300))}
+        log(f"[bench] vs FP64 path: displacement rel err {compare['u_rel_err']:.2e}, "
+            f"abar rel err {compare['abar_rel_err']:.2e}")
+        m64.close()
     cpu = None
     if not args.no_cpu and args.workload == 1:
         try:
@@ -236,7 +263,8 @@ def main():
                                f"AMG rebuilt {'every Newton iteration' if args.amg_rebuild else 'once per load step'}",
                    "newton_steps_timed": newton, "krylov_its_timed": krylov,
                    "t_assemble_s": t_asm, "t_amg_setup_s": t_setup, "t_solve_s": t_solve,
-                   "amg_levels": list(s1.amg_ndof[:s1.amg_levels])},
+                   "amg_levels": list(s1.amg_ndof[:s1.amg_levels]),
+                   **({"vs_fp64_path": compare} if compare else {})},
         "roofline": {"kernel": "fine BSR(3x3) SpMV (sell_lds_kernel<double,double,2>, FP64) in PCG",
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,

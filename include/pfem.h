@@ -88,7 +88,10 @@ typedef struct pfem_options {
                                    learned preconditioner (<= 0: 16);
                                    reserved[2]: FP32-IR only, 1 = the fine-level products
                                    inside the V-cycle's smoother use an FP16 copy of S K S
-                                   (S = diag|K_ii|^-1/2), 0 = the FP32 copy of K (default) */
+                                   (S = diag|K_ii|^-1/2), 0 = the FP32 copy of K (default);
+                                   reserved[3]: pfem_newton backtracking on ||R|| with at
+                                   most this many step halvings (0: the reference's full
+                                   Newton step, Solver.jl:383) */
 } pfem_options;
 
 typedef struct pfem_fail {

@@ -625,7 +625,8 @@ int pfem_newton(pfem_ctx *c, double lambda, double tol, int maxiter, pfem_newton
         const double fext_norm = std::fabs(lambda) * c->fext0_norm;  // Solver.jl:336
         double ref = 1.0, rkm1 = 0.0, eta_prev = c->opt.eta_max;
         bool first = true;
-        for (int it = 1; it <= maxiter; ++it) {
+        // K, R and ||R|| at the current U (assemble, R -= lambda Fext, both Dirichlet passes)
+        auto evaluate = [&](double *rn) -> int {
             double t = now();
             int rc = assemble_impl(c, 0, fail);  // Solver.jl:347
             if (rc) return rc;
@@ -635,8 +636,22 @@ int pfem_newton(pfem_ctx *c, double lambda, double tol, int maxiter, pfem_newton
             impose_dirichlet_dev(c->K, c->nbc, c->bcdofs.p, c->bcflag.p, c->bcval.p, lambda, c->U.p, c->dg.p,
                                  c->nbcrows, c->bcrows.p, c->R.p, s);  // :354-355
             dot_dev<double, double>(c->ndof, c->R.p, c->R.p, c->partial.p, c->scal.p + 7, s);
-            const double rnorm = std::sqrt(c->sync_read(c->scal.p + 7));  // :358
+            *rn = std::sqrt(c->sync_read(c->scal.p + 7));  // :358
             res->t_bc += now() - t;
+            return PFEM_OK;
+        };
+        // Optional backtracking on ||R|| (reserved[3] = most halvings; 0 = the reference's
+        // full Newton step).  The residual at the accepted point is the next iteration's.
+        const int ls_max = std::max(0, std::min(c->opt.reserved[3], 20));
+        double rnorm = 0.0;
+        bool have_residual = false;
+        for (int it = 1; it <= maxiter; ++it) {
+            if (!have_residual) {
+                const int rc0 = evaluate(&rnorm);
+                if (rc0) return rc0;
+            }
+            have_residual = false;
+            int rc = PFEM_OK;
             res->residuals[it - 1] = rnorm;
             res->iters = it;
             if (it == 1) ref = std::max(std::max(rnorm, fext_norm), 1.0);  // :360-362
@@ -655,7 +670,7 @@ int pfem_newton(pfem_ctx *c, double lambda, double tol, int maxiter, pfem_newton
             const bool rebuild = first || c->opt.reserved[0] == 1 || (c->opt.reserved[0] == 2 && it == 2);
             first = false;
             axpby<double>(c->ndof, -1.0, c->R.p, 0.0, c->rhs.p, s);  // :378
-            t = now();
+            const double t = now();
             const double setup0 = c->st.t_setup;
             int niter = 0, used = 0;
             double rel = 0;
@@ -673,6 +688,26 @@ int pfem_newton(pfem_ctx *c, double lambda, double tol, int maxiter, pfem_newton
                 return rc;
             }
             axpby<double>(c->ndof, 1.0, c->dU.p, 1.0, c->U.p, s);  // :383
+            if (ls_max > 0 && it < maxiter) {
+                double alpha = 1.0, rn_new = 0.0;
+                for (int h = 0;; ++h) {
+                    const int rc1 = evaluate(&rn_new);
+                    if (rc1 != PFEM_OK && rc1 != PFEM_ERR_INVERTED) return rc1;
+                    if (rc1 == PFEM_OK && rn_new <= (1.0 - 1e-4 * alpha) * rnorm) break;
+                    if (h == ls_max) {
+                        if (rc1 != PFEM_OK) return rc1;
+                        break;  // no decrease found: keep the shortest step
+                    }
+                    axpby<double>(c->ndof, -0.5 * alpha, c->dU.p, 1.0, c->U.p, s);  // U = U0 + alpha/2 dU
+                    alpha *= 0.5;
+                    if (c->opt.verbose) fprintf(stderr, "    line search: step %.4f\n", alpha);
+                }
+                rkm1 = rnorm;
+                eta_prev = eta;
+                rnorm = rn_new;
+                have_residual = true;
+                continue;
+            }
             rkm1 = rnorm;
             eta_prev = eta;
         }

@@ -88,6 +88,9 @@ def default_options(**kw):
         if k == "fcg_directions":  # directions kept by the learned preconditioner's flexible GCR
             o.reserved[1] = int(v)
             continue
+        if k == "line_search":  # Newton backtracking on ||R||: most step halvings (0 = full steps)
+            o.reserved[3] = int(v)
+            continue
         if k == "fp16_fine":  # FP32-IR: smoother products on the fine level with the FP16 scaled K
             o.reserved[2] = 1 if v else 0
             continue
