@@ -129,6 +129,8 @@ def main():
                     help="load steps of the full load path (default: the workload's own)")
     ap.add_argument("--line-search", type=int, default=0, metavar="HALVINGS",
                     help="Newton backtracking on ||R|| (not in the reference; 0 = full Newton steps)")
+    ap.add_argument("--rho-iters", type=int, nargs=2, default=(0, 0), metavar=("COLD", "WARM"),
+                    help="power-iteration steps of the smoother's rho estimate (0 0: library defaults)")
     ap.add_argument("--compare", action="store_true",
                     help="after the timed run, repeat the same load steps untimed with the plain FP64 path "
                          "and report the relative displacement and equivalent-plastic-strain differences")
@@ -168,7 +170,7 @@ def main():
     log(f"[bench] workload: {wl.name}, nodes {wl.mesh.nnodes}, dof {3 * wl.mesh.nnodes}")
     opts = pa.default_options(eta_min=wl.krylov_rtol, eta_max=wl.krylov_rtol, precision=args.precision,
                               verbose=args.verbose, amg_rebuild=args.amg_rebuild, fp16_fine=args.fp16_fine,
-                              line_search=args.line_search)
+                              line_search=args.line_search, rho_iters=args.rho_iters)
     t0 = time.perf_counter()
     mats = [pa.J2Material(**d) for d in wl.materials]
     m = pa.Model(wl.mesh, mats if len(mats) > 1 else mats[0], options=opts, material_id=wl.material_id)

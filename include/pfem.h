@@ -91,7 +91,10 @@ typedef struct pfem_options {
                                    (S = diag|K_ii|^-1/2), 0 = the FP32 copy of K (default);
                                    reserved[3]: pfem_newton backtracking on ||R|| with at
                                    most this many step halvings (0: the reference's full
-                                   Newton step, Solver.jl:383) */
+                                   Newton step, Solver.jl:383);
+                                   reserved[4], reserved[5]: power-iteration steps of the
+                                   rho(D^-1 A) estimate behind the Chebyshev bounds, first
+                                   set-up and warm-started rebuilds (<= 0: 25 and 6) */
 } pfem_options;
 
 typedef struct pfem_fail {
@@ -181,7 +184,9 @@ typedef struct pfem_newton_result {
 int pfem_newton(pfem_ctx *ctx, double lambda, double tol, int maxiter,
                 pfem_newton_result *res, pfem_fail *fail);
 
-/* Commit pass of solve! (Solver.jl:470-471). */
+/* Commit pass of solve! (Solver.jl:470-471): the history (eps_p, abar, Cp^-1) and
+ * the stress are updated from the current U.  solve! discards the K and R this pass
+ * re-assembles, so the device K and R are left as they were. */
 int pfem_commit(pfem_ctx *ctx, pfem_fail *fail);
 
 /* Element-level entry for kernel checks: for nelem independent elements with

@@ -369,7 +369,9 @@ void amg_numeric_setup(pfem_ctx *c, Amg *a) {
         const double tr = now();
         // power iteration converges from below, slowly where the top of the spectrum is
         // crowded (12 steps gave 0.91 rho on a notched bar): 25 steps cold, 6 warm
-        L.rho = spectral_radius(c, a, (int)l, L.pw_warm ? 6 : 25);
+        const int cold = c->opt.reserved[4] > 0 ? c->opt.reserved[4] : 25;
+        const int warm = c->opt.reserved[5] > 0 ? c->opt.reserved[5] : 6;
+        L.rho = spectral_radius(c, a, (int)l, L.pw_warm ? warm : cold);
         t_rho += now() - tr;
         const double omega = (4.0 / 3.0) / L.rho;
         AmgLevel &N = a->L[l + 1];

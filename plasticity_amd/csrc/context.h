@@ -82,6 +82,6 @@ namespace pfem {
 inline double now() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
-int assemble_impl(pfem_ctx *c, int commit, pfem_fail *fail);
+int assemble_impl(pfem_ctx *c, int commit, pfem_fail *fail, bool state_only = false);
 int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *used, double *relres);
 }  // namespace pfem

@@ -46,13 +46,16 @@ static ElementArgs element_args(pfem_ctx *c, int commit) {
     return a;
 }
 
-int assemble_impl(pfem_ctx *c, int commit, pfem_fail *fail) {
+int assemble_impl(pfem_ctx *c, int commit, pfem_fail *fail, bool state_only) {
     hipStream_t s = c->stream;
     const double t0 = now();
     ElementArgs a = element_args(c, commit);
+    if (state_only) a.packet = nullptr;  // stress and history only: no tangent, no K or R
     PF_HIP(hipMemsetAsync(c->fail_key.p, 0xff, sizeof(unsigned long long), s));
-    c->K.v64.zero(s);  // Assembly.jl:361-362
-    c->R.zero(s);
+    if (!state_only) {
+        c->K.v64.zero(s);  // Assembly.jl:361-362
+        c->R.zero(s);
+    }
     PF_HIP(hipEventRecord(c->ev0, s));
     launch_gp_kernel(a, 0, c->nelem, s);
     PF_HIP(hipEventRecord(c->ev1, s));
@@ -61,13 +64,14 @@ int assemble_impl(pfem_ctx *c, int commit, pfem_fail *fail) {
     c->sync();
     float ms_gp = 0.f;
     PF_HIP(hipEventElapsedTime(&ms_gp, c->ev0, c->ev1));
-    c->st.t_gp_kernel += 1e-3 * ms_gp;
+    if (!state_only) c->st.t_gp_kernel += 1e-3 * ms_gp;  // per full assembly (n_assemble)
     if (key != ~0ull) {  // ElementInversionError(e, g, J)
         launch_detF_kernel(a, (int64_t)key, c->scal.p, s);
         double J = c->sync_read(c->scal.p);
         if (fail) { fail->e = (int64_t)(key / 8); fail->g = (int32_t)(key % 8); fail->J = J; }
         return PFEM_ERR_INVERTED;
     }
+    if (state_only) return PFEM_OK;
     PF_HIP(hipEventRecord(c->ev0, s));
     for (int col = 0; col < c->ncolors; ++col)
         launch_scatter_kernel(a, c->color_order.p + c->color_ptr[col], c->color_ptr[col + 1] - c->color_ptr[col],
@@ -727,7 +731,10 @@ int pfem_newton(pfem_ctx *c, double lambda, double tol, int maxiter, pfem_newton
 int pfem_commit(pfem_ctx *c, pfem_fail *fail) {
     if (!c) return PFEM_ERR_BAD_ARG;
     try {
-        return assemble_impl(c, 1, fail);  // Solver.jl:470-471
+        // Solver.jl:470-471 re-assembles with commit = true and discards K and R; only the
+        // history update is needed, so the tangent, K and R are skipped (they keep the
+        // last Newton iteration's values)
+        return assemble_impl(c, 1, fail, true);
     } catch (const hip_failure &e) {
         if (fail) fail->hip = (int32_t)e.code;
         return PFEM_ERR_HIP;

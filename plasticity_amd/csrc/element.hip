@@ -315,6 +315,51 @@ __global__ __launch_bounds__(256) void gp_kernel(ElementArgs a, int64_t e_begin,
         for (int j = 0; j < 3; ++j)
             P[i + 3 * j] = T[i] * Fit[0 + 3 * j] + T[i + 3] * Fit[1 + 3 * j] + T[i + 6] * Fit[2 + 3 * j];
 
+    // --- Kirchhoff stress out (Elements.jl:481-482)
+    double *sg = a.sig + 6 * gp;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) sg[k] = tau[k];
+
+    if (a.commit) {  // exp-map plastic update (FiniteStrain.jl:329-343), Elements.jl:483-491
+        double dep[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) dep[k] = (dg * s32) * (k < 3 ? nh[k] : 2 * nh[k]);
+        double ev[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) ev[k] = ee[k] - dep[k];
+        double wv[3], V[9];
+        eig_sym3(2 * ev[0], 2 * ev[1], 2 * ev[2], ev[3], ev[4], ev[5], wv, V);
+        const double x0e = exp(wv[0]), x1e = exp(wv[1]), x2e = exp(wv[2]);
+        double Bn[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                Bn[i + 3 * j] = x0e * V[i] * V[j] + x1e * V[i + 3] * V[j + 3] + x2e * V[i + 6] * V[j + 6];
+        double Tm[9];  // Fbi * Bn
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                Tm[i + 3 * j] = Fbi[i] * Bn[0 + 3 * j] + Fbi[i + 3] * Bn[1 + 3 * j] + Fbi[i + 6] * Bn[2 + 3 * j];
+        double cn[6];
+        {
+            const int I[6] = {0, 1, 2, 0, 1, 0}, Jx[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+            for (int t = 0; t < 6; ++t)
+                cn[t] = Tm[I[t]] * Fbi[Jx[t]] + Tm[I[t] + 3] * Fbi[Jx[t] + 3] + Tm[I[t] + 6] * Fbi[Jx[t] + 6];
+        }
+        double *ep = a.epsp + 6 * gp;
+        double *cw = a.cpinv + 6 * gp;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            ep[k] += dep[k];
+            cw[k] = cn[k];
+        }
+        a.abar[gp] = an + dg;
+    }
+    if (!a.packet) return;  // state-only pass (the commit of a converged load step)
+
     double *pk = a.packet + (size_t)gp * PK;
     const double sw = s * w;
 
@@ -399,49 +444,6 @@ __global__ __launch_bounds__(256) void gp_kernel(ElementArgs a, int64_t e_begin,
 #pragma unroll
     for (int k = 0; k < 24; ++k) pk[108 + k] = dNdX[k];
 
-    // --- Kirchhoff stress out (Elements.jl:481-482)
-    double *sg = a.sig + 6 * gp;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) sg[k] = tau[k];
-
-    if (a.commit) {  // exp-map plastic update (FiniteStrain.jl:329-343), Elements.jl:483-491
-        double dep[6];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) dep[k] = (dg * s32) * (k < 3 ? nh[k] : 2 * nh[k]);
-        double ev[6];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) ev[k] = ee[k] - dep[k];
-        double wv[3], V[9];
-        eig_sym3(2 * ev[0], 2 * ev[1], 2 * ev[2], ev[3], ev[4], ev[5], wv, V);
-        const double x0e = exp(wv[0]), x1e = exp(wv[1]), x2e = exp(wv[2]);
-        double Bn[9];
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-                Bn[i + 3 * j] = x0e * V[i] * V[j] + x1e * V[i + 3] * V[j + 3] + x2e * V[i + 6] * V[j + 6];
-        double Tm[9];  // Fbi * Bn
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-                Tm[i + 3 * j] = Fbi[i] * Bn[0 + 3 * j] + Fbi[i + 3] * Bn[1 + 3 * j] + Fbi[i + 6] * Bn[2 + 3 * j];
-        double cn[6];
-        {
-            const int I[6] = {0, 1, 2, 0, 1, 0}, Jx[6] = {0, 1, 2, 1, 2, 2};
-#pragma unroll
-            for (int t = 0; t < 6; ++t)
-                cn[t] = Tm[I[t]] * Fbi[Jx[t]] + Tm[I[t] + 3] * Fbi[Jx[t] + 3] + Tm[I[t] + 6] * Fbi[Jx[t] + 6];
-        }
-        double *ep = a.epsp + 6 * gp;
-        double *cw = a.cpinv + 6 * gp;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            ep[k] += dep[k];
-            cw[k] = cn[k];
-        }
-        a.abar[gp] = an + dg;
-    }
 }
 
 // Contraction of the 8 GP packets of an element into 3x3 blocks Ke(a, b),

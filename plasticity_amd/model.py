@@ -88,6 +88,9 @@ def default_options(**kw):
         if k == "fcg_directions":  # directions kept by the learned preconditioner's flexible GCR
             o.reserved[1] = int(v)
             continue
+        if k == "rho_iters":  # (cold, warm) power-iteration steps of the Chebyshev bound estimate
+            o.reserved[4], o.reserved[5] = int(v[0]), int(v[1])
+            continue
         if k == "line_search":  # Newton backtracking on ||R||: most step halvings (0 = full steps)
             o.reserved[3] = int(v)
             continue
