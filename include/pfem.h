@@ -94,7 +94,7 @@ typedef struct pfem_options {
                                    Newton step, Solver.jl:383);
                                    reserved[4], reserved[5]: power-iteration steps of the
                                    rho(D^-1 A) estimate behind the Chebyshev bounds, first
-                                   set-up and warm-started rebuilds (<= 0: 25 and 6) */
+                                   set-up and warm-started rebuilds (<= 0: 25 and 4) */
 } pfem_options;
 
 typedef struct pfem_fail {

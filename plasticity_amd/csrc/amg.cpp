@@ -368,9 +368,9 @@ void amg_numeric_setup(pfem_ctx *c, Amg *a) {
         c->sync();
         const double tr = now();
         // power iteration converges from below, slowly where the top of the spectrum is
-        // crowded (12 steps gave 0.91 rho on a notched bar): 25 steps cold, 6 warm
+        // crowded (12 steps gave 0.91 rho on a notched bar): 25 steps cold, 4 warm
         const int cold = c->opt.reserved[4] > 0 ? c->opt.reserved[4] : 25;
-        const int warm = c->opt.reserved[5] > 0 ? c->opt.reserved[5] : 6;
+        const int warm = c->opt.reserved[5] > 0 ? c->opt.reserved[5] : 4;
         L.rho = spectral_radius(c, a, (int)l, L.pw_warm ? warm : cold);
         t_rho += now() - tr;
         const double omega = (4.0 / 3.0) / L.rho;
@@ -406,7 +406,7 @@ void amg_numeric_setup(pfem_ctx *c, Amg *a) {
             if (!L.x32.p) {
                 L.x32.alloc(L.n); L.b32.alloc(L.n); L.r32.alloc(L.n); L.d32.alloc(L.n); L.t32.alloc(L.n);
             }
-            L.dinv32.alloc(L.dinv.n);
+            if (!L.dinv32.p) L.dinv32.alloc(L.dinv.n);
             convert<double, float>((int64_t)L.dinv.n, L.dinv.p, L.dinv32.p, s);
             if (l > 0) {
                 if (!L.A32.p) L.A32.alloc(L.A.n);
@@ -420,6 +420,24 @@ void amg_numeric_setup(pfem_ctx *c, Amg *a) {
         }
         a->have32 = true;
     }
+    c->sync();
+}
+
+// Fine-level refresh: the level-0 smoother follows the current K (block-diagonal inverse,
+// warm-started rho), the interpolation and the coarse operators stay as last built.
+void amg_refresh_fine(pfem_ctx *c, Amg *a) {
+    hipStream_t s = c->stream;
+    AmgLevel &L = a->L[0];
+    amg_block_inverse<3>(L.nb, c->K.diagpos.p, c->K.v64.p, L.dinv.p, s);
+    if (a->L.size() > 1) {
+        const double tr = now();
+        const int warm = c->opt.reserved[5] > 0 ? c->opt.reserved[5] : 4;
+        L.rho = spectral_radius(c, a, 0, warm);
+        c->st.t_setup_rho += now() - tr;
+    } else {
+        coarse_inverse(c, a);
+    }
+    if (a->have32 && L.dinv32.p) convert<double, float>((int64_t)L.dinv.n, L.dinv.p, L.dinv32.p, s);
     c->sync();
 }
 

@@ -67,6 +67,7 @@ struct Amg {
 Amg *amg_create(pfem_ctx *c);                 // symbolic set-up (host)
 void amg_destroy(Amg *a);
 void amg_numeric_setup(pfem_ctx *c, Amg *a);  // numeric set-up (device)
+void amg_refresh_fine(pfem_ctx *c, Amg *a);  // level-0 smoother from the current K only
 void amg_apply64(pfem_ctx *c, Amg *a, const double *r, double *z);  // z = V-cycle(r)
 void amg_apply32(pfem_ctx *c, Amg *a, const float *r, float *z);
 

@@ -451,7 +451,7 @@ int pfem_linear_solve(pfem_ctx *c, double rtol, int rebuild_pc, int *niter, int 
     return guarded(nullptr, [&]() -> int {
         // b = -R (Solver.jl:378)
         axpby<double>(c->ndof, -1.0, c->R.p, 0.0, c->rhs.p, c->stream);
-        return linear_solve_impl(c, rtol, rebuild_pc, niter, used, relres);
+        return linear_solve_impl(c, rtol, rebuild_pc ? 1 : 0, niter, used, relres);
     });
 }
 

@@ -527,13 +527,19 @@ static int learned_solve(pfem_ctx *c, double rtol, int itmax, int *total, int *w
     return rc;
 }
 
+// rebuild: 0 reuse the preconditioner, 1 rebuild it, 2 refresh only the fine-level
+// smoother from the current K (interpolation and coarse operators kept)
 int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *used, double *relres) {
     const bool learned = c->net_mode >= 2;
     if (learned) {
         if (c->opt.precision == PFEM_PREC_FP32_IR) prepare_fp32(c);
         if (rebuild) c->pc_stale = true;  // the AMG hierarchy is rebuilt only if it is needed
-    } else if (rebuild || !c->pc_ready) {
+    } else if (rebuild == 1 || !c->pc_ready) {
         pc_setup(c);
+    } else if (rebuild == 2) {
+        const double t0 = now();
+        amg_refresh_fine(c, c->amg.get());
+        c->st.t_setup += now() - t0;
     }
     const double t1 = now();
     int it = 0, total = 0, which = 0, rc;
@@ -671,7 +677,10 @@ int pfem_newton(pfem_ctx *c, double lambda, double tol, int maxiter, pfem_newton
             // Krylov iterations of a hierarchy built from the elastic predictor tangent);
             // amg_rebuild = 2 rebuilds at the first and second iteration (the second is the
             // first plastic tangent) and reuses that hierarchy afterwards
-            const bool rebuild = first || c->opt.reserved[0] == 1 || (c->opt.reserved[0] == 2 && it == 2);
+            // amg_rebuild = 3 builds at the first iteration and afterwards refreshes only
+            // the fine-level smoother from the current tangent
+            const int policy = c->opt.reserved[0];
+            const int rebuild = (first || policy == 1 || (policy == 2 && it == 2)) ? 1 : (policy == 3 ? 2 : 0);
             first = false;
             axpby<double>(c->ndof, -1.0, c->R.p, 0.0, c->rhs.p, s);  // :378
             const double t = now();
