@@ -7,7 +7,9 @@ from .mesh import Mesh, box_mesh, dof, ellipsoid_inclusions, greedy_colors, jitt
     on_face, select_nodes  # noqa: F401
 from .model import ElementInversionError, J2Material, Model, SolveResult, default_options, \
     element_batch  # noqa: F401
+from .vtu import gauss_strain, von_mises, write_vtu  # noqa: F401
 
 __all__ = ["Mesh", "box_mesh", "dof", "on_face", "select_nodes", "jitter_u01", "greedy_colors",
            "notched_bar_mesh", "ellipsoid_inclusions", "J2Material", "Model", "SolveResult",
-           "ElementInversionError", "element_batch", "default_options"]
+           "ElementInversionError", "element_batch", "default_options", "write_vtu", "gauss_strain",
+           "von_mises"]

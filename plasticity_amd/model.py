@@ -372,6 +372,11 @@ class Model:
                 callback(n, self)
         return out
 
+    def write_vtu(self, filename):
+        """write_vtu (Visualization.jl:129-203): ASCII .vtu of the committed state."""
+        from .vtu import write_vtu
+        return write_vtu(filename, self)
+
     def attach_net(self, net, mode="precondition", stride=1):
         """Learned preconditioner (pfem_attach_net): 'log' samples, 'precondition', 'off'."""
         from .learned import attach
