@@ -29,7 +29,7 @@ typedef enum pfem_status {
     PFEM_ERR_INVERTED = 1,      /* ElementInversionError(e, g, J): pfem_fail filled (FiniteStrain.jl:42-52) */
     PFEM_ERR_NOT_CONVERGED = 2, /* Krylov not converged after the fallback ladder (Solver.jl:270-286) */
     PFEM_ERR_NON_SPD = 3,       /* breakdown: p'Ap <= 0 or r'z <= 0 with no fallback left */
-    PFEM_ERR_UNSUPPORTED = 4,   /* material / element branch not implemented (Hkin > 0) */
+    PFEM_ERR_UNSUPPORTED = 4,   /* material / element branch not implemented */
     PFEM_ERR_HIP = 5,           /* HIP runtime error: code in pfem_fail.hip */
     PFEM_ERR_BAD_ARG = 6,       /* invalid argument (guards of Materials.jl:90-103 included) */
     PFEM_ERR_NO_DEVICE = 7      /* no GPU visible */
@@ -37,11 +37,16 @@ typedef enum pfem_status {
 
 typedef enum pfem_element_kind {
     PFEM_HEX8_FINITE = 0,       /* Hex8Finite      (FiniteStrain.jl:64) */
-    PFEM_HEX8_FINITE_FBAR = 1   /* Hex8FiniteFbar  (FiniteStrain.jl:65) */
+    PFEM_HEX8_FINITE_FBAR = 1,  /* Hex8FiniteFbar  (FiniteStrain.jl:65) */
+    PFEM_HEX8_SMALL = 2         /* Hex8Small       (FiniteStrain.jl:63, Elements.jl:323-360) */
 } pfem_element_kind;
 
 /* J2Material inputs (Materials.jl:82-108).  ssat = sy0 and delta = 0 select the
- * linear isotropic law. */
+ * linear isotropic law; Hkin > 0 adds linear kinematic hardening (with finite
+ * strain the back stress is stored in the reference configuration and pushed
+ * forward by the polar rotation of F, FiniteStrain.jl:172-346, which makes the
+ * tangent non-symmetric).  Saturation together with Hkin > 0 is rejected
+ * (Materials.jl:98-101). */
 typedef struct pfem_material {
     double E, nu, sy0, Hiso, Hkin, ssat, delta;
 } pfem_material;
@@ -128,10 +133,15 @@ int pfem_reset(pfem_ctx *ctx);
 
 int pfem_set_displacement(pfem_ctx *ctx, const double *U);
 int pfem_get_displacement(pfem_ctx *ctx, double *U);
-/* Committed Gauss state; any pointer may be NULL.  epsp, cpinv, sig: 6 x ngp;
- * abar: ngp.  sig holds the last assembled Kirchhoff stress. */
-int pfem_set_state(pfem_ctx *ctx, const double *epsp, const double *abar, const double *cpinv);
-int pfem_get_state(pfem_ctx *ctx, double *epsp, double *abar, double *cpinv, double *sig);
+/* Committed Gauss state (GaussState, Mesh.jl:124-163); any pointer may be NULL.
+ * epsp, beta, cpinv, sig: 6 x ngp; abar: ngp.  beta is the back stress (for
+ * finite strain in the reference configuration); it is stored only for the
+ * small-strain element or when a material has Hkin > 0 -- otherwise it is
+ * identically zero (set ignores it, get returns zeros).  sig holds the last
+ * assembled stress: Kirchhoff for finite strain, Cauchy for small strain. */
+int pfem_set_state(pfem_ctx *ctx, const double *epsp, const double *beta, const double *abar,
+                   const double *cpinv);
+int pfem_get_state(pfem_ctx *ctx, double *epsp, double *beta, double *abar, double *cpinv, double *sig);
 
 /* assemble! (Assembly.jl:238-257): K (device) and R = F_int overwritten from U
  * (host array, or NULL to use the device-resident U) and the committed
@@ -189,14 +199,17 @@ int pfem_newton(pfem_ctx *ctx, double lambda, double tol, int maxiter,
  * re-assembles, so the device K and R are left as they were. */
 int pfem_commit(pfem_ctx *ctx, pfem_fail *fail);
 
-/* Element-level entry for kernel checks: for nelem independent elements with
- * node coordinates Xe (24 per element, node-major) and displacements ue (24),
- * and per-GP history (6x8, 8, 6x8 per element), compute Fe (24), Ke (576,
- * column-major 24x24) and sig (6x8) on the device with the production
- * per-GP and tangent-contraction kernels.  commit != 0 updates the history
- * arrays.  material applies to all elements. */
+/* Element-level entry for kernel checks (element_force_tangent_finite!
+ * Elements.jl:405-494, element_force_tangent! :323-360): for nelem independent
+ * elements with node coordinates Xe (24 per element, node-major) and
+ * displacements ue (24), and per-GP history (epsp, beta 6x8, abar 8, cpinv 6x8
+ * per element; beta NULL = zero back stress, cpinv may be NULL for the
+ * small-strain kind), compute Fe (24), Ke (576, column-major 24x24) and sig
+ * (6x8) on the device with the production per-GP and tangent-contraction
+ * kernels.  commit != 0 updates the history arrays given.  material applies to
+ * all elements. */
 int pfem_element_batch(int32_t element_kind, const pfem_material *mat, int64_t nelem,
-                       const double *Xe, const double *ue, double *epsp, double *abar,
+                       const double *Xe, const double *ue, double *epsp, double *beta, double *abar,
                        double *cpinv, int commit, double *Fe, double *Ke, double *sig,
                        pfem_fail *fail);
 

@@ -64,7 +64,7 @@ int main() {
     or_mat m;
     if (or_material_init(&m, 200e9, 0.3, 250e6, 1e9, 0.0, 250e6, 0.0)) return 1;
     // a slightly distorted unit-cube element (h = 0.01)
-    fd Xe[24], ue[24], ep[48], ab[8], cp[48], sg[48], Fe[24], Ke[576];
+    fd Xe[24], ue[24], ep[48], bt[48], ab[8], cp[48], sg[48], Fe[24], Ke[576];
     const int nat[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
     for (int a = 0; a < 8; ++a)
         for (int d = 0; d < 3; ++d) Xe[3 * a + d] = fd(0.01 * nat[a][d] + 0.0007 * ((a * 3 + d * 5) % 7 - 3) / 3.0);
@@ -74,18 +74,18 @@ int main() {
         for (int a = 0; a < 8; ++a)
             for (int d = 0; d < 3; ++d)
                 ue[3 * a + d] = fd(amp[k] * ((a * 7 + d * 3) % 5 - 2) + (d == 0 ? -amp[k] * 4 * nat[a][0] : 0.0));
-        for (int i = 0; i < 48; ++i) { ep[i] = fd(0.0); sg[i] = fd(0.0); cp[i] = fd(i % 6 < 3 ? 1.0 : 0.0); }
+        for (int i = 0; i < 48; ++i) { ep[i] = fd(0.0); bt[i] = fd(0.0); sg[i] = fd(0.0); cp[i] = fd(i % 6 < 3 ? 1.0 : 0.0); }
         for (int i = 0; i < 8; ++i) ab[i] = fd(0.0);
         int g = -1;
         fd J(0.0);
         g_cnt = Counts{};
-        int st = or_element_finite(1, &m, Xe, ue, ep, ab, cp, sg, 0, Fe, Ke, &g, &J);
+        int st = or_element_finite(1, &m, Xe, ue, ep, bt, ab, cp, sg, 0, Fe, Ke, &g, &J);
         const Counts c = g_cnt;
         int nplastic = 0;
         // a second pass with commit to read the plastic multiplier increments
         fd ab2[8];
         for (int i = 0; i < 8; ++i) ab2[i] = ab[i];
-        or_element_finite(1, &m, Xe, ue, ep, ab2, cp, sg, 1, Fe, Ke, &g, &J);
+        or_element_finite(1, &m, Xe, ue, ep, bt, ab2, cp, sg, 1, Fe, Ke, &g, &J);
         for (int i = 0; i < 8; ++i) nplastic += ab2[i].v > 0;
         printf("%s: status %d, plastic Gauss points %d/8: flops per element %lld "
                "(add/sub %lld, mul %lld, div %lld, sqrt/cbrt/log/exp %lld)\n",

@@ -21,6 +21,7 @@ import numpy as np
 __all__ = [
     "lib", "Material", "return_map", "eig_sym3", "finite_kinematics",
     "finite_stress_update", "dPdF", "element_finite", "ref_grads", "box_mesh",
+    "polar_RU", "dR_polar", "dtau_dbeta", "element_small",
     "jitter_u01", "build_csc", "greedy_colors", "colour_lists", "on_face",
     "impose_dirichlet", "forcing_term", "ElementInversionError", "OracleModel",
     "rigid_body_modes", "OracleAMG",
@@ -60,12 +61,18 @@ lib.or_return_map.argtypes = [C.POINTER(_Mat), _dp, _dp, _dp, C.c_double, _dp, _
                               C.POINTER(C.c_double), _dp]
 lib.or_eig_sym3.argtypes = [_dp, _dp, _dp]
 lib.or_finite_kinematics.argtypes = [_dp, _dp, C.POINTER(_Kin)]
-lib.or_finite_stress_update.argtypes = [C.POINTER(_Mat), C.POINTER(_Kin), _dp, C.c_double,
-                                        _dp, _dp, C.POINTER(C.c_double), _dp, C.c_void_p]
-lib.or_dPdF.argtypes = [C.POINTER(_Kin), _dp, _dp, _dp, _dp, _dp]
+lib.or_finite_stress_update.argtypes = [C.POINTER(_Mat), C.POINTER(_Kin), _dp, _dp, _dp, C.c_double,
+                                        _dp, _dp, _dp, C.POINTER(C.c_double), _dp, C.c_void_p,
+                                        _dp, _dp, _dp]
+lib.or_polar_RU.argtypes = [_dp, _dp, _dp]
+lib.or_dR_polar.argtypes = [_dp, _dp, _dp, _dp]
+lib.or_dtau_dbeta.argtypes = [C.POINTER(_Mat), _dp, _dp, C.c_double, _dp]
+lib.or_dPdF.argtypes = [C.POINTER(_Kin), _dp, _dp, _dp, _dp, C.c_void_p, C.c_void_p, C.c_void_p,
+                        C.c_void_p, _dp]
 lib.or_ref_grads.argtypes = [_dp, _dp, _dp, C.POINTER(C.c_double)]
-lib.or_element_finite.argtypes = [C.c_int, C.POINTER(_Mat), _dp, _dp, _dp, _dp, _dp, _dp,
+lib.or_element_finite.argtypes = [C.c_int, C.POINTER(_Mat), _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                   C.c_int, _dp, _dp, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+lib.or_element_small.argtypes = [C.POINTER(_Mat), _dp, _dp, _dp, _dp, _dp, _dp, C.c_int, _dp, _dp]
 lib.or_element_finite.restype = C.c_int
 lib.or_box_mesh.argtypes = [C.c_double] * 3 + [C.c_int] * 3 + [C.c_double, C.c_uint64, _dp, _ip]
 lib.or_jitter_u01.argtypes = [C.c_uint64, C.c_int64, C.c_int]
@@ -74,7 +81,7 @@ lib.or_build_csc.argtypes = [C.c_int64, C.c_int64, _ip, _ip, C.c_void_p, C.POINT
 lib.or_greedy_colors.argtypes = [C.c_int64, C.c_int64, _ip, _i32p]
 lib.or_greedy_colors.restype = C.c_int
 lib.or_assemble.argtypes = [C.c_int, C.POINTER(_Mat), C.c_void_p, C.c_int64, C.c_int64, _dp, _ip, _ip, _ip,
-                            C.c_int, _ip, _ip, _dp, _dp, _dp, _dp, _dp, C.c_int, _dp, _dp,
+                            C.c_int, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int, _dp, _dp,
                             C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int),
                             C.POINTER(C.c_double)]
 lib.or_assemble.restype = C.c_int
@@ -152,17 +159,50 @@ def finite_kinematics(F, cpi=(1, 1, 1, 0, 0, 0)):
     return k
 
 
-def finite_stress_update(mat, kin, epn, an, want_cpi=True):
+def finite_stress_update(mat, kin, epn, an, want_cpi=True, F=None, bref=None, full=False):
+    """finite_stress_update (FiniteStrain.jl:292-346).  F (3x3) is needed when mat.Hkin > 0
+    (polar rotation of the back stress); bref is the reference-configuration back stress.
+    Returns (tau, ep, abar, D, cpi_new), or with full=True also (bref_new, beta_sp, R, U)."""
     tau, ep, D, cn = np.zeros(6), np.zeros(6), np.zeros(36), np.zeros(6)
+    bnew, bsp, R, U = np.zeros(6), np.zeros(6), np.zeros(9), np.zeros(9)
+    if F is None:
+        assert mat.Hkin == 0, "F is required with kinematic hardening"
+        F = np.eye(3)
     a = C.c_double()
-    lib.or_finite_stress_update(mat.ref, C.byref(kin), _v(epn, 6), an, tau, ep, C.byref(a), D,
-                                cn.ctypes.data if want_cpi else None)
-    return tau, ep, a.value, D.reshape(6, 6), cn
+    lib.or_finite_stress_update(mat.ref, C.byref(kin), _F9(F), _v(epn, 6),
+                                _v(np.zeros(6) if bref is None else bref, 6), an, tau, ep, bnew,
+                                C.byref(a), D, cn.ctypes.data if want_cpi else None, bsp, R, U)
+    out = (tau, ep, a.value, D.reshape(6, 6), cn)
+    if full:
+        out += (bnew, bsp, R.reshape(3, 3, order="F"), U.reshape(3, 3, order="F"))
+    return out
 
 
-def dPdF(kin, cpi, D, tau, F):
+def polar_RU(F):
+    R, U = np.zeros(9), np.zeros(9)
+    lib.or_polar_RU(_F9(F), R, U)
+    return R.reshape(3, 3, order="F"), U.reshape(3, 3, order="F")
+
+
+def dR_polar(R, U, dF):
+    dR = np.zeros(9)
+    lib.or_dR_polar(_F9(R), _F9(U), _F9(dF), dR)
+    return dR.reshape(3, 3, order="F")
+
+
+def dtau_dbeta(mat, ee, bsp, an):
+    out = np.zeros(36)
+    lib.or_dtau_dbeta(mat.ref, _v(ee, 6), _v(bsp, 6), an, out)
+    return out.reshape(6, 6)
+
+
+def dPdF(kin, cpi, D, tau, F, bref=None, R=None, U=None, dtdb=None):
     A = np.zeros(81)
-    lib.or_dPdF(C.byref(kin), _v(cpi, 6), _v(np.asarray(D).reshape(6, 6)), _v(tau, 6), _F9(F), A)
+    kin_args = [None] * 4
+    if bref is not None:
+        kin_args = [_v(bref, 6), _F9(R), _F9(U), _v(np.asarray(dtdb).reshape(6, 6))]
+    lib.or_dPdF(C.byref(kin), _v(cpi, 6), _v(np.asarray(D).reshape(6, 6)), _v(tau, 6), _F9(F),
+                *[None if x is None else x.ctypes.data for x in kin_args], A)
     return A.reshape(9, 9, order="F")
 
 
@@ -173,23 +213,39 @@ def ref_grads(Xe, xi):
     return dNdX.reshape(8, 3), d.value
 
 
-def element_finite(fbar, mat, Xe, ue, epsp=None, abar=None, cpinv=None, commit=False):
-    """Fe (24), Ke (24x24), sig (8x6).  State arrays (8x6 / 8) updated in place on commit."""
+def element_finite(fbar, mat, Xe, ue, epsp=None, abar=None, cpinv=None, commit=False, beta=None):
+    """Fe (24), Ke (24x24), sig (8x6).  State arrays (8x6 / 8) updated in place on commit;
+    beta is the reference-configuration back stress (zero when omitted)."""
     if epsp is None:
         epsp = np.zeros((8, 6))
     if abar is None:
         abar = np.zeros(8)
     if cpinv is None:
         cpinv = np.tile([1.0, 1, 1, 0, 0, 0], (8, 1))
-    for arr in (epsp, abar, cpinv):
+    if beta is None:
+        beta = np.zeros((8, 6))
+    for arr in (epsp, beta, abar, cpinv):
         assert arr.dtype == np.float64 and arr.flags.c_contiguous
     sig = np.zeros((8, 6))
     Fe, Ke = np.zeros(24), np.zeros(576)
     g, J = C.c_int(), C.c_double()
-    rc = lib.or_element_finite(int(fbar), mat.ref, _v(Xe, 24), _v(ue, 24), epsp, abar, cpinv, sig,
+    rc = lib.or_element_finite(int(fbar), mat.ref, _v(Xe, 24), _v(ue, 24), epsp, beta, abar, cpinv, sig,
                                int(commit), Fe, Ke, C.byref(g), C.byref(J))
     if rc:
         raise ElementInversionError(0, g.value, J.value)
+    return Fe, Ke.reshape(24, 24, order="F"), sig
+
+
+def element_small(mat, Xe, ue, epsp=None, beta=None, abar=None, commit=False):
+    """Small-strain Hex8 (Elements.jl:323-360): Fe (24), Ke (24x24), sig (8x6)."""
+    epsp = np.zeros((8, 6)) if epsp is None else epsp
+    beta = np.zeros((8, 6)) if beta is None else beta
+    abar = np.zeros(8) if abar is None else abar
+    for arr in (epsp, beta, abar):
+        assert arr.dtype == np.float64 and arr.flags.c_contiguous
+    sig = np.zeros((8, 6))
+    Fe, Ke = np.zeros(24), np.zeros(576)
+    lib.or_element_small(mat.ref, _v(Xe, 24), _v(ue, 24), epsp, beta, abar, sig, int(commit), Fe, Ke)
     return Fe, Ke.reshape(24, 24, order="F"), sig
 
 
@@ -336,7 +392,7 @@ class OracleModel:
         if self.matid is not None:
             assert self.matid.shape == (self.conn.shape[0],) and 0 <= self.matid.min() and \
                 self.matid.max() < len(mats)
-        self.fbar = {"finite_fbar": 1, "finite": 0}[element]
+        self.kind = {"finite": 0, "finite_fbar": 1, "small": 2}[element]
         self.nn, self.ne = self.nodes.shape[0], self.conn.shape[0]
         self.ndof = 3 * self.nn
         self.colptr, self.rowval = build_csc(self.nn, self.conn)
@@ -359,8 +415,8 @@ class OracleModel:
     def _new_state(ngp):
         cp = np.zeros((ngp, 6))
         cp[:, :3] = 1.0
-        return {"epsp": np.zeros((ngp, 6)), "abar": np.zeros(ngp), "cpinv": cp,
-                "sig": np.zeros((ngp, 6))}
+        return {"epsp": np.zeros((ngp, 6)), "beta": np.zeros((ngp, 6)), "abar": np.zeros(ngp),
+                "cpinv": cp, "sig": np.zeros((ngp, 6))}
 
     # BC builders (Model.jl:102-145)
     def fix(self, nodes, comp="all"):
@@ -403,10 +459,10 @@ class OracleModel:
     def assemble(self, U, state, commit=False, nthreads=None):
         """assemble_threaded! (Assembly.jl:340-416): K.data and R overwritten."""
         fe, fg, fJ = C.c_int64(), C.c_int(), C.c_double()
-        rc = lib.or_assemble(self.fbar, self._mats, None if self.matid is None else self.matid.ctypes.data,
+        rc = lib.or_assemble(self.kind, self._mats, None if self.matid is None else self.matid.ctypes.data,
                              self.nn, self.ne, self.nodes, self.conn,
                              self.colptr, self.rowval, self.ncolors, self.cptr, self.corder,
-                             _v(U), state["epsp"], state["abar"], state["cpinv"], state["sig"],
+                             _v(U), state["epsp"], state["beta"], state["abar"], state["cpinv"], state["sig"],
                              int(commit), self.K.data, self.R,
                              self.nthreads if nthreads is None else nthreads,
                              C.byref(fe), C.byref(fg), C.byref(fJ))

@@ -261,12 +261,108 @@ void or_finite_kinematics(const double F[9], const double cpi[6], or_kin *k) {
     k->ok = 1;
 }
 
-void or_finite_stress_update(const or_mat *m, const or_kin *k, const double epn[6],
-                             double an, double tau[6], double ep[6], double *aout,
-                             double D[36], double cpi_new[6]) {
-    /* FiniteStrain.jl:292-346 with Hkin == 0 (beta_sp = 0, R = U = I) */
-    double Z6[6] = {0}, dep[6], bt[6];
-    or_return_map(m, k->ee, Z6, Z6, an, tau, dep, bt, aout, D); /* :318 */
+/* polar_RU (FiniteStrain.jl:190-205): F = R U, U = (F^T F)^(1/2), R = F U^-1. */
+void or_polar_RU(const double F[9], double R[9], double U[9]) {
+    double Ft[9], C[9], w[3], V[9], sq[3], isq[3], Ui[9] = {0};
+    tr3(F, Ft);
+    mm3(Ft, F, C);
+    M3(C, 1, 0) = M3(C, 0, 1); M3(C, 2, 0) = M3(C, 0, 2); M3(C, 2, 1) = M3(C, 1, 2);
+    or_eig_sym3(C, w, V);
+    for (int a = 0; a < 3; ++a) {
+        sq[a] = sqrt(w[a] > 0.0 ? w[a] : 0.0); /* floor at 0 (:195-197) */
+        isq[a] = sq[a] > 0 ? 1.0 / sq[a] : 0.0;
+    }
+    memset(U, 0, 9 * sizeof(double));
+    for (int a = 0; a < 3; ++a)
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                M3(U, i, j) += sq[a] * M3(V, i, a) * M3(V, j, a);
+                M3(Ui, i, j) += isq[a] * M3(V, i, a) * M3(V, j, a);
+            }
+    mm3(F, Ui, R);
+}
+
+/* dR_polar (FiniteStrain.jl:215-228): Omega U + U Omega = R^T dF - dF^T R solved in
+ * U's eigenbasis, dR = R Omega. */
+void or_dR_polar(const double R[9], const double U[9], const double dF[9], double dR[9]) {
+    double Rt[9], A[9], At[9], rhs[9], w[3], V[9], Vt[9], rb[9], Ob[9], t[9];
+    tr3(R, Rt);
+    mm3(Rt, dF, A);
+    tr3(A, At);
+    for (int i = 0; i < 9; ++i) rhs[i] = A[i] - At[i];
+    double Us[9];
+    memcpy(Us, U, sizeof Us);
+    M3(Us, 1, 0) = M3(Us, 0, 1); M3(Us, 2, 0) = M3(Us, 0, 2); M3(Us, 2, 1) = M3(Us, 1, 2);
+    or_eig_sym3(Us, w, V);
+    tr3(V, Vt);
+    mm3(Vt, rhs, t);
+    mm3(t, V, rb);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) M3(Ob, i, j) = M3(rb, i, j) / (w[i] + w[j]);
+    mm3(V, Ob, t);
+    mm3(t, Vt, t);
+    mm3(R, t, dR);
+}
+
+/* dtau_dbeta (FiniteStrain.jl:245-262), row-major 6x6; zero in the elastic branch. */
+void or_dtau_dbeta(const or_mat *m, const double ee[6], const double bsp[6], double an,
+                   double out[36]) {
+    const double G = m->G, H = m->Hiso + m->Hkin, s32 = sqrt(1.5);
+    double st[6], xi[6];
+    for (int i = 0; i < 6; ++i) {
+        double s = 0.0;
+        for (int j = 0; j < 6; ++j) s += m->C[6 * i + j] * ee[j];
+        st[i] = s;
+    }
+    double p = (st[0] + st[1] + st[2]) / 3;
+    for (int k = 0; k < 6; ++k) xi[k] = st[k] - (k < 3 ? p : 0.0) - bsp[k];
+    double nrm = sqrt(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2] +
+                      2 * (xi[3] * xi[3] + xi[4] * xi[4] + xi[5] * xi[5]));
+    double Y = m->sy0 + m->Hiso * an;
+    double f = s32 * nrm - Y;
+    memset(out, 0, 36 * sizeof(double));
+    if (f <= 0.0) return;
+    double dg = f / (3 * G + H);
+    double a = 2 * G * s32 * dg / nrm;
+    double da = 2 * G * s32 / (3 * G + H) * (Y / (nrm * nrm));
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j)
+            out[6 * i + j] = (i == j ? a : 0.0) + xi[i] * (da / nrm) * ((j < 3 ? 1.0 : 2.0) * xi[j]);
+}
+
+void or_finite_stress_update(const or_mat *m, const or_kin *k, const double F[9], const double epn[6],
+                             const double bref_n[6], double an, double tau[6], double ep[6],
+                             double bref_new[6], double *aout, double D[36], double cpi_new[6],
+                             double bsp_out[6], double R_out[9], double U_out[9]) {
+    /* FiniteStrain.jl:292-346 */
+    double Z6[6] = {0}, dep[6], bsp[6] = {0}, bsp_new[6], R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1},
+           U[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    const int kin = m->Hkin > 0;
+    if (kin) { /* :312-314: beta_sp = R beta_ref R^T */
+        double B[9], t[9], Rt[9];
+        or_polar_RU(F, R, U);
+        v2s(bref_n ? bref_n : Z6, B);
+        tr3(R, Rt);
+        mm3(R, B, t);
+        mm3(t, Rt, t);
+        s2v(t, bsp);
+    }
+    or_return_map(m, k->ee, Z6, bsp, an, tau, dep, bsp_new, aout, D); /* :318 */
+    if (bref_new) {                                                  /* :320 */
+        if (kin) {
+            double B[9], t[9], Rt[9];
+            v2s(bsp_new, B);
+            tr3(R, Rt);
+            mm3(Rt, B, t);
+            mm3(t, R, t);
+            s2v(t, bref_new);
+        } else {
+            memcpy(bref_new, bsp_new, 6 * sizeof(double));
+        }
+    }
+    if (bsp_out) memcpy(bsp_out, bsp, sizeof bsp);
+    if (R_out) memcpy(R_out, R, sizeof R);
+    if (U_out) memcpy(U_out, U, sizeof U);
     for (int i = 0; i < 6; ++i) ep[i] = epn[i] + dep[i];        /* :329 */
     if (!cpi_new) return;
     double ev[6], et[9], w[3], V[9], be[9] = {0}, FiT[9], t[9];
@@ -312,9 +408,16 @@ static void dhalflog_contract(const double *Q, const double *b, const double *H,
 }
 
 void or_dPdF(const or_kin *k, const double cpi[6], const double D[36],
-             const double tau[6], const double F[9], double A[81]) {
-    /* FiniteStrain.jl:503-547, kin_active == false */
-    double C[9], Ft[9], FiT[9], T[9];
+             const double tau[6], const double F[9], const double bref[6], const double R[9],
+             const double U[9], const double dtdb[36], double A[81]) {
+    /* FiniteStrain.jl:503-547; the back-stress rotation term (:533-537) is active when
+     * bref, R, U and dtdb are all given */
+    double C[9], Ft[9], FiT[9], T[9], B[9], Rt[9];
+    const int kin = bref && R && U && dtdb;
+    if (kin) {
+        v2s(bref, B);
+        tr3(R, Rt);
+    }
     v2s(cpi, C);
     tr3(F, Ft);
     tr3(k->Finv, FiT);
@@ -335,6 +438,20 @@ void or_dPdF(const or_kin *k, const double cpi[6], const double D[36],
             double s = 0.0;
             for (int j = 0; j < 6; ++j) s += D[6 * i + j] * ev[j];
             dtv[i] = s;
+        }
+        if (kin) { /* d beta_sp = dR B R^T + R B dR^T; d tau += dtdb d beta_sp */
+            double dR[9], t1[9], t2[9], dRt[9], dbs[9], dbv[6];
+            or_dR_polar(R, U, dF, dR);
+            mm3(dR, B, t1); mm3(t1, Rt, t1);
+            tr3(dR, dRt);
+            mm3(R, B, t2); mm3(t2, dRt, t2);
+            for (int i = 0; i < 9; ++i) dbs[i] = t1[i] + t2[i];
+            s2v(dbs, dbv);
+            for (int i = 0; i < 6; ++i) {
+                double s = 0.0;
+                for (int j = 0; j < 6; ++j) s += dtdb[6 * i + j] * dbv[j];
+                dtv[i] += s;
+            }
         }
         v2s(dtv, dt);
         double b1[9], b2[9], dFiT[9], dP[9];
@@ -398,7 +515,7 @@ static void defgrad(const double *ue, const double *dNdX, double *F) {
 /* G (9x24) applied implicitly: G[(j*3+i), (3a+k)] = delta_ik dNdX[a][j]   (Elements.jl:500-510) */
 
 int or_element_finite(int fbar, const or_mat *m, const double Xe[24],
-                      const double ue[24], double *epsp, double *abar,
+                      const double ue[24], double *epsp, double *beta, double *abar,
                       double *cpinv, double *sig, int commit, double Fe[24],
                       double Ke[576], int *fail_g, double *fail_J) {
     const double gq = 1.0 / sqrt(3.0);
@@ -436,14 +553,22 @@ int or_element_finite(int fbar, const or_mat *m, const double Xe[24],
             if (fail_J) *fail_J = kin.J;
             return 1; /* :447 */
         }
-        double tau[6], epn[6], ep[6], an = abar[g], anew, D[36], cn[6];
+        double tau[6], epn[6], ep[6], an = abar[g], anew, D[36], cn[6], bn[6], bnew[6], bsp[6],
+            Rp[9], Up[9], dtdb[36];
         memcpy(epn, epsp + 6 * g, sizeof epn);
-        or_finite_stress_update(m, &kin, epn, an, tau, ep, &anew, D, commit ? cn : NULL);
+        memcpy(bn, beta + 6 * g, sizeof bn);
+        or_finite_stress_update(m, &kin, Fb, epn, bn, an, tau, ep, bnew, &anew, D, commit ? cn : NULL,
+                                bsp, Rp, Up);
         double T[9], FiT[9], P[9], A[81];
         v2s(tau, T);
         tr3(kin.Finv, FiT);
         mm3(T, FiT, P); /* first_piola :560 ; P column-major == _p9 */
-        or_dPdF(&kin, cp, D, tau, Fb, A);
+        if (m->Hkin > 0) { /* Elements.jl:467-469 */
+            or_dtau_dbeta(m, kin.ee, bsp, an, dtdb);
+            or_dPdF(&kin, cp, D, tau, Fb, bn, Rp, Up, dtdb, A);
+        } else {
+            or_dPdF(&kin, cp, D, tau, Fb, NULL, NULL, NULL, NULL, A);
+        }
         /* Fe += G^T P w   (:470) */
         for (int a = 0; a < 8; ++a)
             for (int i = 0; i < 3; ++i) {
@@ -491,11 +616,58 @@ int or_element_finite(int fbar, const or_mat *m, const double Xe[24],
         memcpy(sig + 6 * g, tau, 6 * sizeof(double)); /* :481-482 */
         if (commit) {                                  /* :483-491 */
             memcpy(epsp + 6 * g, ep, 6 * sizeof(double));
+            memcpy(beta + 6 * g, bnew, 6 * sizeof(double));
             abar[g] = anew;
             memcpy(cp, cn, 6 * sizeof(double));
         }
     }
     return 0;
+}
+
+/* Small-strain Hex8 (element_force_tangent! Elements.jl:323-360 with the B matrices of
+ * element_geometry :130-144 and bmatrix :97-118): eps = B ue (engineering shear),
+ * return_map with the committed eps_p, beta, abar; Fe += B^T sigma w, Ke += B^T D B w. */
+static const int VOIGT_OF[3][3] = {{0, 3, 5}, {3, 1, 4}, {5, 4, 2}};
+void or_element_small(const or_mat *m, const double Xe[24], const double ue[24], double *epsp,
+                      double *beta, double *abar, double *sig, int commit, double Fe[24],
+                      double Ke[576]) {
+    const double gq = 1.0 / sqrt(3.0);
+    memset(Fe, 0, 24 * sizeof(double));
+    memset(Ke, 0, 576 * sizeof(double));
+    for (int g = 0; g < 8; ++g) {
+        double xi[3] = {GSGN[g][0] * gq, GSGN[g][1] * gq, GSGN[g][2] * gq};
+        double dNdX[24], detJ, H[9] = {0};
+        or_ref_grads(Xe, xi, dNdX, &detJ);
+        const double w = detJ * 1.0;
+        for (int a = 0; a < 8; ++a)
+            for (int j = 0; j < 3; ++j)
+                for (int i = 0; i < 3; ++i) M3(H, i, j) += ue[3 * a + i] * dNdX[3 * a + j];
+        double eps[6] = {M3(H, 0, 0), M3(H, 1, 1), M3(H, 2, 2), M3(H, 0, 1) + M3(H, 1, 0),
+                         M3(H, 1, 2) + M3(H, 2, 1), M3(H, 0, 2) + M3(H, 2, 0)};
+        double s[6], ep[6], bt[6], an, D[36];
+        or_return_map(m, eps, epsp + 6 * g, beta + 6 * g, abar[g], s, ep, bt, &an, D);
+        /* B[v][(a,i)] = dNdX[a][j] for the one j with VOIGT_OF[i][j] == v */
+        for (int a = 0; a < 8; ++a)
+            for (int i = 0; i < 3; ++i) {
+                double f = 0.0;
+                for (int j = 0; j < 3; ++j) f += dNdX[3 * a + j] * s[VOIGT_OF[i][j]];
+                Fe[3 * a + i] += f * w;
+                for (int b = 0; b < 8; ++b)
+                    for (int k = 0; k < 3; ++k) {
+                        double kv = 0.0;
+                        for (int j = 0; j < 3; ++j)
+                            for (int l = 0; l < 3; ++l)
+                                kv += dNdX[3 * a + j] * D[6 * VOIGT_OF[i][j] + VOIGT_OF[k][l]] * dNdX[3 * b + l];
+                        Ke[(3 * a + i) + 24 * (3 * b + k)] += kv * w;
+                    }
+            }
+        memcpy(sig + 6 * g, s, 6 * sizeof(double));
+        if (commit) {
+            memcpy(epsp + 6 * g, ep, 6 * sizeof(double));
+            memcpy(beta + 6 * g, bt, 6 * sizeof(double));
+            abar[g] = an;
+        }
+    }
 }
 
 /* ------------------------------------------------------------------ */
@@ -637,10 +809,10 @@ static int64_t find_nz(const int64_t *rowval, int64_t lo, int64_t hi, int64_t gr
     return -1;
 }
 
-int or_assemble(int fbar, const or_mat *mats, const int32_t *matid, int64_t nnodes, int64_t nelem,
+int or_assemble(int kind, const or_mat *mats, const int32_t *matid, int64_t nnodes, int64_t nelem,
                 const double *nodes, const int64_t *conn, const int64_t *colptr,
                 const int64_t *rowval, int ncolors, const int64_t *colour_ptr,
-                const int64_t *colour_order, const double *U, double *epsp,
+                const int64_t *colour_order, const double *U, double *epsp, double *beta,
                 double *abar, double *cpinv, double *sig, int commit,
                 double *nzval, double *R, int nthreads, int64_t *fail_e,
                 int *fail_g, double *fail_J) {
@@ -669,9 +841,14 @@ int or_assemble(int fbar, const or_mat *mats, const int32_t *matid, int64_t nnod
                     ed[3 * a + d] = 3 * ce[a] + d;
                     ue[3 * a + d] = U[ed[3 * a + d]];
                 }
-            int g; double J;
+            int g = 0; double J = 0.0;
             const or_mat *m = matid ? mats + matid[e] : mats; /* per-element phase (two-phase blocks) */
-            int st = or_element_finite(fbar, m, Xe, ue, epsp + 48 * e, abar + 8 * e,
+            int st = 0;
+            if (kind == 2) /* Hex8Small (Assembly.jl:287-292) */
+                or_element_small(m, Xe, ue, epsp + 48 * e, beta + 48 * e, abar + 8 * e, sig + 48 * e,
+                                 commit, Fe, Ke);
+            else
+                st = or_element_finite(kind == 1, m, Xe, ue, epsp + 48 * e, beta + 48 * e, abar + 8 * e,
                                        cpinv + 48 * e, sig + 48 * e, commit, Fe, Ke, &g, &J);
             if (st) {
 #pragma omp critical

@@ -55,27 +55,44 @@ typedef struct or_kin {
 
 void or_finite_kinematics(const double F[9], const double cpi[6], or_kin *k);
 
-/* finite_stress_update for Hkin == 0 (FiniteStrain.jl:292-346). */
-void or_finite_stress_update(const or_mat *m, const or_kin *k, const double epn[6],
-                             double an, double tau[6], double ep[6], double *aout,
-                             double D[36], double cpi_new[6]);
+/* polar_RU (FiniteStrain.jl:190-205) and dR_polar (:215-228); 3x3 column-major. */
+void or_polar_RU(const double F[9], double R[9], double U[9]);
+void or_dR_polar(const double R[9], const double U[9], const double dF[9], double dR[9]);
+/* dtau_dbeta (FiniteStrain.jl:245-262), row-major 6x6. */
+void or_dtau_dbeta(const or_mat *m, const double ee[6], const double bsp[6], double an,
+                   double out[36]);
 
-/* dPdF, Hkin == 0 branch (FiniteStrain.jl:503-547).  A column-major 9x9. */
+/* finite_stress_update (FiniteStrain.jl:292-346).  bref_n may be NULL (zero back
+ * stress); bref_new, cpi_new, bsp_out, R_out, U_out may be NULL. */
+void or_finite_stress_update(const or_mat *m, const or_kin *k, const double F[9], const double epn[6],
+                             const double bref_n[6], double an, double tau[6], double ep[6],
+                             double bref_new[6], double *aout, double D[36], double cpi_new[6],
+                             double bsp_out[6], double R_out[9], double U_out[9]);
+
+/* dPdF (FiniteStrain.jl:503-547).  A column-major 9x9.  bref, R, U, dtdb NULL: the
+ * Hkin == 0 form without the back-stress rotation term. */
 void or_dPdF(const or_kin *k, const double cpi[6], const double D[36],
-             const double tau[6], const double F[9], double A[81]);
+             const double tau[6], const double F[9], const double bref[6], const double R[9],
+             const double U[9], const double dtdb[36], double A[81]);
 
 /* hex8 geometry (Elements.jl:62-75, 155-161, 517-519) */
 void or_hex8_dshape(const double xi[3], double dN[24]);
 void or_ref_grads(const double Xe[24], const double xi[3], double dNdX[24], double *detJ);
 
-/* element_force_tangent_finite! (Elements.jl:405-494), Hkin == 0.
+/* element_force_tangent_finite! (Elements.jl:405-494).
  * fbar: 0 = Hex8Finite, 1 = Hex8FiniteFbar.  State arrays hold the 8 GPs of
- * this element (6x8 / 8).  Returns 0, or 1 on an inverted GP (fail_g, fail_J
- * filled, nothing further written for that GP). */
+ * this element (6x8 / 8); beta is the back stress in the reference
+ * configuration.  Returns 0, or 1 on an inverted GP (fail_g, fail_J filled,
+ * nothing further written for that GP). */
 int or_element_finite(int fbar, const or_mat *m, const double Xe[24],
-                      const double ue[24], double *epsp, double *abar,
+                      const double ue[24], double *epsp, double *beta, double *abar,
                       double *cpinv, double *sig, int commit, double Fe[24],
                       double Ke[576], int *fail_g, double *fail_J);
+
+/* element_force_tangent! (Elements.jl:323-360), small strain. */
+void or_element_small(const or_mat *m, const double Xe[24], const double ue[24], double *epsp,
+                      double *beta, double *abar, double *sig, int commit, double Fe[24],
+                      double Ke[576]);
 
 /* ---------------- mesh / pattern / colours ---------------- */
 
@@ -97,15 +114,16 @@ int or_build_csc(int64_t nnodes, int64_t nelem, const int64_t *conn,
 int or_greedy_colors(int64_t nnodes, int64_t nelem, const int64_t *conn, int *color);
 
 /* Colour-ordered assembly (Assembly.jl:370-416): K.nzval and R overwritten.
+ * kind: 0 Hex8Finite, 1 Hex8FiniteFbar, 2 Hex8Small.
  * colour_order lists elements colour by colour; colour_ptr has ncolors+1
  * offsets.  nthreads <= 0 => OpenMP default.  Returns 0 or 1 (inversion).
  * mats: one material, or with matid (per element, may be NULL) an array of
  * them -- the return map is local to each Gauss point, so a two-phase block is
  * the same loop with the element's own material. */
-int or_assemble(int fbar, const or_mat *mats, const int32_t *matid, int64_t nnodes, int64_t nelem,
+int or_assemble(int kind, const or_mat *mats, const int32_t *matid, int64_t nnodes, int64_t nelem,
                 const double *nodes, const int64_t *conn, const int64_t *colptr,
                 const int64_t *rowval, int ncolors, const int64_t *colour_ptr,
-                const int64_t *colour_order, const double *U, double *epsp,
+                const int64_t *colour_order, const double *U, double *epsp, double *beta,
                 double *abar, double *cpinv, double *sig, int commit,
                 double *nzval, double *R, int nthreads, int64_t *fail_e,
                 int *fail_g, double *fail_J);

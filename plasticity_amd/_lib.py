@@ -24,6 +24,7 @@ PFEM_ERR_NO_DEVICE = 7
 
 HEX8_FINITE = 0
 HEX8_FINITE_FBAR = 1
+HEX8_SMALL = 2
 PREC_FP64 = 0
 PREC_FP32_IR = 1
 SMOOTH_JACOBI = 0
@@ -89,8 +90,8 @@ SIGNATURES = {
     "pfem_reset": (C.c_int, [_vp]),
     "pfem_set_displacement": (C.c_int, [_vp, _vp]),
     "pfem_get_displacement": (C.c_int, [_vp, _vp]),
-    "pfem_set_state": (C.c_int, [_vp, _vp, _vp, _vp]),
-    "pfem_get_state": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "pfem_set_state": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "pfem_get_state": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "pfem_assemble": (C.c_int, [_vp, _vp, C.c_int, _vp, C.POINTER(Fail)]),
     "pfem_subtract_external": (C.c_int, [_vp, C.c_double]),
     "pfem_impose_dirichlet": (C.c_int, [_vp, C.c_double]),
@@ -104,7 +105,7 @@ SIGNATURES = {
     "pfem_get_colors": (C.c_int, [_vp, _vp, _i32p]),
     "pfem_newton": (C.c_int, [_vp, C.c_double, C.c_double, C.c_int, C.POINTER(NewtonResult), C.POINTER(Fail)]),
     "pfem_commit": (C.c_int, [_vp, C.POINTER(Fail)]),
-    "pfem_element_batch": (C.c_int, [C.c_int32, C.POINTER(Material), C.c_int64, _vp, _vp, _vp, _vp, _vp,
+    "pfem_element_batch": (C.c_int, [C.c_int32, C.POINTER(Material), C.c_int64, _vp, _vp, _vp, _vp, _vp, _vp,
                                      C.c_int, _vp, _vp, _vp, C.POINTER(Fail)]),
     "pfem_get_stats": (C.c_int, [_vp, C.POINTER(Stats)]),
     "pfem_bench_spmv": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),

@@ -104,9 +104,12 @@ struct ElementArgs {
     const double *U;        // ndof
     const int32_t *matid;   // nelem or null
     const DevMat *mats;     // device array
-    int fbar;
+    int kind;               // pfem_element_kind: 0 finite, 1 finite F-bar, 2 small strain
+    int fbar;               // kind == 1
+    int kin;                // some material has Hkin > 0 (objective back-stress branch)
     int commit;
     double *epsp, *abar, *cpinv, *sig;  // Gauss state (6 x ngp / ngp)
+    double *beta;           // back stress (6 x ngp); reference configuration for finite strain
     double *packet;         // per GP packet (PK doubles)
     double *g0;             // per element 24
     unsigned long long *fail_key;  // min (8e+g) of inverted GPs, ~0ull if none

@@ -22,7 +22,6 @@ int make_devmat(const pfem_material &m, DevMat &d) {
         !(m.ssat >= m.sy0) || !(m.delta >= 0))
         return PFEM_ERR_BAD_ARG;
     if (m.delta > 0 && m.ssat > m.sy0 && m.Hkin > 0) return PFEM_ERR_BAD_ARG;
-    if (m.Hkin > 0) return PFEM_ERR_UNSUPPORTED;  // objective back-stress branch not built
     d.E = m.E; d.nu = m.nu; d.sy0 = m.sy0; d.Hiso = m.Hiso; d.Hkin = m.Hkin; d.ssat = m.ssat;
     d.delta = m.delta;
     d.G = m.E / (2 * (1 + m.nu));                       // Materials.jl:104
@@ -124,7 +123,9 @@ void pfem_default_options(pfem_options *o) {
 int pfem_create(const pfem_mesh_desc *md, const pfem_options *opts, pfem_ctx **out) {
     if (!md || !out || md->nnodes <= 0 || md->nelem <= 0 || !md->nodes || !md->conn) return PFEM_ERR_BAD_ARG;
     if (md->nmaterials < 1 || !md->materials) return PFEM_ERR_BAD_ARG;
-    if (md->element_kind != PFEM_HEX8_FINITE && md->element_kind != PFEM_HEX8_FINITE_FBAR) return PFEM_ERR_BAD_ARG;
+    if (md->element_kind != PFEM_HEX8_FINITE && md->element_kind != PFEM_HEX8_FINITE_FBAR &&
+        md->element_kind != PFEM_HEX8_SMALL)
+        return PFEM_ERR_BAD_ARG;
     *out = nullptr;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PFEM_ERR_NO_DEVICE;
@@ -149,7 +150,9 @@ int pfem_create(const pfem_mesh_desc *md, const pfem_options *opts, pfem_ctx **o
         hipStream_t s = c->stream;
         const int64_t nn = md->nnodes, ne = md->nelem;
         c->nnodes = nn; c->nelem = ne; c->ndof = 3 * nn;
+        c->kind = md->element_kind;
         c->fbar = md->element_kind == PFEM_HEX8_FINITE_FBAR;
+        for (const DevMat &m : dm) c->kin |= m.Hkin > 0;
         c->h_nodes.assign(md->nodes, md->nodes + 3 * nn);
         c->h_conn.assign(md->conn, md->conn + 8 * ne);
         c->X.alloc(3 * nn); c->X.upload(md->nodes, 3 * nn, s);
@@ -241,6 +244,7 @@ int pfem_create(const pfem_mesh_desc *md, const pfem_options *opts, pfem_ctx **o
         // ---- state (GaussState Mesh.jl:132-138) and scratch
         const int64_t ngp = 8 * ne;
         c->epsp.alloc(6 * ngp); c->abar.alloc(ngp); c->cpinv.alloc(6 * ngp); c->sig.alloc(6 * ngp);
+        if (c->kin || c->kind == PFEM_HEX8_SMALL) c->beta.alloc(6 * ngp);
         c->packet.alloc((size_t)ngp * PK);
         c->g0.alloc(24 * ne);
         c->fail_key.alloc(1);
@@ -285,6 +289,7 @@ int pfem_reset(pfem_ctx *c) {
         const int64_t ngp = 8 * c->nelem;
         c->U.zero(s); c->dU.zero(s); c->R.zero(s);
         c->epsp.zero(s); c->abar.zero(s); c->sig.zero(s);
+        if (c->beta.p) c->beta.zero(s);
         std::vector<double> cp(6 * ngp, 0.0);
         for (int64_t g = 0; g < ngp; ++g) cp[6 * g] = cp[6 * g + 1] = cp[6 * g + 2] = 1.0;
         c->cpinv.upload(cp.data(), cp.size(), s);
@@ -368,22 +373,28 @@ int pfem_get_dU(pfem_ctx *c, double *d) {
     if (!c || !d) return PFEM_ERR_BAD_ARG;
     return guarded(nullptr, [&]() -> int { c->dU.download(d, c->ndof, c->stream); c->sync(); return PFEM_OK; });
 }
-int pfem_set_state(pfem_ctx *c, const double *epsp, const double *abar, const double *cpinv) {
+int pfem_set_state(pfem_ctx *c, const double *epsp, const double *beta, const double *abar,
+                   const double *cpinv) {
     if (!c) return PFEM_ERR_BAD_ARG;
     return guarded(nullptr, [&]() -> int {
         const int64_t ngp = 8 * c->nelem;
         if (epsp) c->epsp.upload(epsp, 6 * ngp, c->stream);
+        if (beta && c->beta.p) c->beta.upload(beta, 6 * ngp, c->stream);
         if (abar) c->abar.upload(abar, ngp, c->stream);
         if (cpinv) c->cpinv.upload(cpinv, 6 * ngp, c->stream);
         c->sync();
         return PFEM_OK;
     });
 }
-int pfem_get_state(pfem_ctx *c, double *epsp, double *abar, double *cpinv, double *sig) {
+int pfem_get_state(pfem_ctx *c, double *epsp, double *beta, double *abar, double *cpinv, double *sig) {
     if (!c) return PFEM_ERR_BAD_ARG;
     return guarded(nullptr, [&]() -> int {
         const int64_t ngp = 8 * c->nelem;
         if (epsp) c->epsp.download(epsp, 6 * ngp, c->stream);
+        if (beta) {
+            if (c->beta.p) c->beta.download(beta, 6 * ngp, c->stream);
+            else std::memset(beta, 0, sizeof(double) * 6 * ngp);
+        }
         if (abar) c->abar.download(abar, ngp, c->stream);
         if (cpinv) c->cpinv.download(cpinv, 6 * ngp, c->stream);
         if (sig) c->sig.download(sig, 6 * ngp, c->stream);
@@ -532,7 +543,7 @@ int pfem_get_stats(pfem_ctx *c, pfem_stats *st) {
     if (!c || !st) return PFEM_ERR_BAD_ARG;
     *st = c->st;
     size_t bytes = c->K.v64.bytes() + c->K.v32.bytes() + c->K.colind.bytes() + c->slots.bytes() +
-                   c->packet.bytes() + c->g0.bytes() + 4 * c->epsp.bytes();
+                   c->packet.bytes() + c->g0.bytes() + 4 * c->epsp.bytes() + c->beta.bytes();
     st->device_bytes = (int64_t)bytes;
     return PFEM_OK;
 }

@@ -16,7 +16,9 @@ struct pfem_ctx {
     int device = 0;
     pfem_options opt{};
     int64_t nnodes = 0, nelem = 0, ndof = 0;
-    int fbar = 1;
+    int kind = 1;       // pfem_element_kind
+    int fbar = 1;       // kind == PFEM_HEX8_FINITE_FBAR
+    int kin = 0;        // some material has Hkin > 0 (finite strain: objective back stress)
 
     // mesh
     pfem::dbuf<double> X;
@@ -36,6 +38,7 @@ struct pfem_ctx {
 
     // Gauss state (committed history + last stress) and element scratch
     pfem::dbuf<double> epsp, abar, cpinv, sig;
+    pfem::dbuf<double> beta;   // back stress; allocated for small strain or Hkin > 0 only
     pfem::dbuf<double> packet, g0;
     int64_t packet_elems = 0;  // elements per packet chunk
     pfem::dbuf<unsigned long long> fail_key;

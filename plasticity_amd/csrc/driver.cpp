@@ -34,9 +34,12 @@ static ElementArgs element_args(pfem_ctx *c, int commit) {
     a.U = c->U.p;
     a.matid = c->matid.p;
     a.mats = c->mats.p;
+    a.kind = c->kind;
     a.fbar = c->fbar;
+    a.kin = c->kin;
     a.commit = commit;
     a.epsp = c->epsp.p;
+    a.beta = c->beta.p;
     a.abar = c->abar.p;
     a.cpinv = c->cpinv.p;
     a.sig = c->sig.p;
@@ -751,10 +754,13 @@ int pfem_commit(pfem_ctx *c, pfem_fail *fail) {
 }
 
 int pfem_element_batch(int32_t kind, const pfem_material *mat, int64_t nelem, const double *Xe, const double *ue,
-                       double *epsp, double *abar, double *cpinv, int commit, double *Fe, double *Ke, double *sig,
-                       pfem_fail *fail) {
-    if (!mat || nelem <= 0 || !Xe || !ue || !epsp || !abar || !cpinv || !Fe || !Ke || !sig) return PFEM_ERR_BAD_ARG;
-    if (kind != PFEM_HEX8_FINITE && kind != PFEM_HEX8_FINITE_FBAR) return PFEM_ERR_BAD_ARG;
+                       double *epsp, double *beta, double *abar, double *cpinv, int commit, double *Fe, double *Ke,
+                       double *sig, pfem_fail *fail) {
+    if (kind != PFEM_HEX8_FINITE && kind != PFEM_HEX8_FINITE_FBAR && kind != PFEM_HEX8_SMALL)
+        return PFEM_ERR_BAD_ARG;
+    if (!mat || nelem <= 0 || !Xe || !ue || !epsp || !abar || (!cpinv && kind != PFEM_HEX8_SMALL) || !Fe || !Ke ||
+        !sig)
+        return PFEM_ERR_BAD_ARG;
     DevMat dm;
     int rc = make_devmat(*mat, dm);
     if (rc) return rc;
@@ -765,7 +771,7 @@ int pfem_element_batch(int32_t kind, const pfem_material *mat, int64_t nelem, co
         PF_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         struct SG { hipStream_t s; ~SG() { (void)hipStreamDestroy(s); } } sg{s};
         const int64_t ngp = 8 * nelem;
-        dbuf<double> X(24 * nelem), U(24 * nelem), ep(6 * ngp), ab(ngp), cp(6 * ngp), sg6(6 * ngp),
+        dbuf<double> X(24 * nelem), U(24 * nelem), ep(6 * ngp), bt(6 * ngp), ab(ngp), cp(6 * ngp), sg6(6 * ngp),
             pk((size_t)ngp * PK), g0(24 * nelem), fe(24 * nelem), ke(576 * nelem);
         dbuf<int64_t> cn(8 * nelem);
         dbuf<DevMat> md(1);
@@ -773,13 +779,16 @@ int pfem_element_batch(int32_t kind, const pfem_material *mat, int64_t nelem, co
         std::vector<int64_t> hc(8 * nelem);
         for (int64_t k = 0; k < 8 * nelem; ++k) hc[k] = k;
         X.upload(Xe, 24 * nelem, s); U.upload(ue, 24 * nelem, s); cn.upload(hc.data(), 8 * nelem, s);
-        ep.upload(epsp, 6 * ngp, s); ab.upload(abar, ngp, s); cp.upload(cpinv, 6 * ngp, s);
+        ep.upload(epsp, 6 * ngp, s); ab.upload(abar, ngp, s);
+        if (cpinv) cp.upload(cpinv, 6 * ngp, s); else cp.zero(s);
+        if (beta) bt.upload(beta, 6 * ngp, s); else bt.zero(s);
         md.upload(&dm, 1, s);
         sg6.zero(s);
         PF_HIP(hipMemsetAsync(fk.p, 0xff, sizeof(unsigned long long), s));
         ElementArgs a{};
         a.nelem = nelem; a.X = X.p; a.conn = cn.p; a.U = U.p; a.matid = nullptr; a.mats = md.p;
-        a.fbar = kind == PFEM_HEX8_FINITE_FBAR; a.commit = commit; a.epsp = ep.p; a.abar = ab.p; a.cpinv = cp.p;
+        a.kind = kind; a.fbar = kind == PFEM_HEX8_FINITE_FBAR; a.kin = dm.Hkin > 0; a.commit = commit;
+        a.epsp = ep.p; a.beta = bt.p; a.abar = ab.p; a.cpinv = cp.p;
         a.sig = sg6.p; a.packet = pk.p; a.g0 = g0.p; a.fail_key = fk.p;
         launch_gp_kernel(a, 0, nelem, s);
         unsigned long long key = 0;
@@ -801,7 +810,8 @@ int pfem_element_batch(int32_t kind, const pfem_material *mat, int64_t nelem, co
         if (commit) {
             ep.download(epsp, 6 * ngp, s);
             ab.download(abar, ngp, s);
-            cp.download(cpinv, 6 * ngp, s);
+            if (cpinv) cp.download(cpinv, 6 * ngp, s);
+            if (beta) bt.download(beta, 6 * ngp, s);
         }
         PF_HIP(hipStreamSynchronize(s));
         return PFEM_OK;

@@ -165,6 +165,11 @@ __device__ __forceinline__ double halflog_dd(double bA, double bB) {
     return 0.5 * log1p(d / bB) / d;
 }
 
+// KIN: some material has Hkin > 0.  Those Gauss points push the stored back stress
+// forward with the polar rotation of Fbar and add the back-stress rotation term to A
+// (FiniteStrain.jl:190-262, :312-320, :533-537); the KIN = false instantiation is the
+// isotropic kernel without that code.
+template <bool KIN>
 __global__ __launch_bounds__(256) void gp_kernel(ElementArgs a, int64_t e_begin, int64_t e_end) {
     __shared__ double sX[32][24];
     __shared__ double sU[32][24];
@@ -255,7 +260,57 @@ __global__ __launch_bounds__(256) void gp_kernel(ElementArgs a, int64_t e_begin,
         }
     }
 
-    // --- J2 radial return with zero additive plastic strain and zero back stress
+    // --- objective back stress (FiniteStrain.jl:312-314): Fb = R U, beta_sp = R beta_ref R^T.
+    //     Kept in U's eigenbasis V: RV = R V = Fb V diag(1/sqrt(lp)), bv = V^T beta_ref V.
+    const bool kin = KIN && m.Hkin > 0;
+    double RV[9], Vp[9], lp[3], bv[6], bsp[6] = {0, 0, 0, 0, 0, 0};
+    if (kin) {
+        double cc[6];  // C = Fb^T Fb (00,11,22,01,12,02)
+        {
+            const int I[6] = {0, 1, 2, 0, 1, 0}, Jx[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+            for (int t = 0; t < 6; ++t)
+                cc[t] = Fb[3 * I[t]] * Fb[3 * Jx[t]] + Fb[3 * I[t] + 1] * Fb[3 * Jx[t] + 1] +
+                        Fb[3 * I[t] + 2] * Fb[3 * Jx[t] + 2];
+        }
+        double wc[3];
+        eig_sym3(cc[0], cc[1], cc[2], cc[3], cc[4], cc[5], wc, Vp);
+#pragma unroll
+        for (int A_ = 0; A_ < 3; ++A_) {
+            lp[A_] = sqrt(wc[A_] > 0.0 ? wc[A_] : 0.0);  // FiniteStrain.jl:195-197
+            const double is = lp[A_] > 0 ? 1.0 / lp[A_] : 0.0;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                RV[i + 3 * A_] = is * (Fb[i] * Vp[3 * A_] + Fb[i + 3] * Vp[3 * A_ + 1] + Fb[i + 6] * Vp[3 * A_ + 2]);
+        }
+        const double *br = a.beta + 6 * gp;
+        const double B[9] = {br[0], br[3], br[5], br[3], br[1], br[4], br[5], br[4], br[2]};
+        double BV[9];  // B V
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int A_ = 0; A_ < 3; ++A_)
+                BV[i + 3 * A_] = B[i] * Vp[3 * A_] + B[i + 3] * Vp[3 * A_ + 1] + B[i + 6] * Vp[3 * A_ + 2];
+        {
+            const int I[6] = {0, 1, 2, 0, 1, 0}, Jx[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+            for (int t = 0; t < 6; ++t)
+                bv[t] = Vp[3 * I[t]] * BV[3 * Jx[t]] + Vp[3 * I[t] + 1] * BV[3 * Jx[t] + 1] +
+                        Vp[3 * I[t] + 2] * BV[3 * Jx[t] + 2];
+            const double Bv[9] = {bv[0], bv[3], bv[5], bv[3], bv[1], bv[4], bv[5], bv[4], bv[2]};
+            double T1[9];  // RV Bv
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int A_ = 0; A_ < 3; ++A_)
+                    T1[i + 3 * A_] = RV[i] * Bv[3 * A_] + RV[i + 3] * Bv[3 * A_ + 1] + RV[i + 6] * Bv[3 * A_ + 2];
+#pragma unroll
+            for (int t = 0; t < 6; ++t)
+                bsp[t] = T1[I[t]] * RV[Jx[t]] + T1[I[t] + 3] * RV[Jx[t] + 3] + T1[I[t] + 6] * RV[Jx[t] + 6];
+        }
+    }
+
+    // --- J2 radial return with zero additive plastic strain and the spatial back stress
     //     (Materials.jl:159-250 as called at FiniteStrain.jl:318)
     const double G = m.G, lam = m.lam;
     const double tr = ee[0] + ee[1] + ee[2];
@@ -267,7 +322,7 @@ __global__ __launch_bounds__(256) void gp_kernel(ElementArgs a, int64_t e_begin,
     const double p = (tau[0] + tau[1] + tau[2]) / 3;
     double xi[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) xi[k] = tau[k] - (k < 3 ? p : 0.0);
+    for (int k = 0; k < 6; ++k) xi[k] = tau[k] - (k < 3 ? p : 0.0) - bsp[k];
     const double nrm = sqrt(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2] +
                             2 * (xi[3] * xi[3] + xi[4] * xi[4] + xi[5] * xi[5]));
     const double s32 = 1.2247448713915890491;  // sqrt(3/2)
@@ -281,21 +336,21 @@ __global__ __launch_bounds__(256) void gp_kernel(ElementArgs a, int64_t e_begin,
 #pragma unroll
         for (int k = 0; k < 6; ++k) nh[k] = xi[k] / nrm;
         if (m.delta > 0 && m.ssat > m.sy0) {  // Voce local Newton (Materials.jl:201-213)
-            dg = ftr / (3 * G + yield_slope(m, an));
+            dg = ftr / (3 * G + m.Hkin + yield_slope(m, an));
             for (int it = 0; it < 50; ++it) {
                 double Hp = yield_slope(m, an + dg);
-                double r = qtr - 3 * G * dg - yield_stress(m, an + dg);
-                double ddg = r / (3 * G + Hp);
+                double r = qtr - (3 * G + m.Hkin) * dg - yield_stress(m, an + dg);
+                double ddg = r / (3 * G + m.Hkin + Hp);
                 dg += ddg;
                 if (fabs(ddg) <= 1e-13 * (dg + 1e-14)) break;
             }
         } else {
-            dg = ftr / (3 * G + m.Hiso);  // Materials.jl:218
+            dg = ftr / (3 * G + m.Hiso + m.Hkin);  // Materials.jl:218
         }
 #pragma unroll
         for (int k = 0; k < 6; ++k) tau[k] -= (2 * G * dg * s32) * nh[k];
         const double Hp = yield_slope(m, an + dg);
-        const double b0 = 2 * G * dg / qtr, g0c = 2 * G / (3 * G + Hp);
+        const double b0 = 2 * G * dg / qtr, g0c = 2 * G / (3 * G + m.Hkin + Hp);
         cdev = 3 * G * b0;
         cnn = 3 * G * (b0 - g0c);
     }
@@ -357,8 +412,47 @@ __global__ __launch_bounds__(256) void gp_kernel(ElementArgs a, int64_t e_begin,
             cw[k] = cn[k];
         }
         a.abar[gp] = an + dg;
+        if (KIN) {  // beta_ref_new = R^T beta_sp_new R (FiniteStrain.jl:230, :320); 0 when Hkin == 0
+            double *bw = a.beta + 6 * gp;
+            if (kin) {
+                const double c = (2.0 / 3.0) * m.Hkin * (dg * s32);
+                double bn[6];
+#pragma unroll
+                for (int k = 0; k < 6; ++k) bn[k] = bsp[k] + c * nh[k];
+                const double Bn[9] = {bn[0], bn[3], bn[5], bn[3], bn[1], bn[4], bn[5], bn[4], bn[2]};
+                double Rm[9];  // R = RV V^T
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j)
+                        Rm[i + 3 * j] = RV[i] * Vp[j] + RV[i + 3] * Vp[j + 3] + RV[i + 6] * Vp[j + 6];
+                double T1[9];  // Bn R
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j)
+                        T1[i + 3 * j] = Bn[i] * Rm[3 * j] + Bn[i + 3] * Rm[3 * j + 1] + Bn[i + 6] * Rm[3 * j + 2];
+                const int I[6] = {0, 1, 2, 0, 1, 0}, Jx[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+                for (int t = 0; t < 6; ++t)
+                    bw[t] = Rm[3 * I[t]] * T1[3 * Jx[t]] + Rm[3 * I[t] + 1] * T1[3 * Jx[t] + 1] +
+                            Rm[3 * I[t] + 2] * T1[3 * Jx[t] + 2];
+            } else {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) bw[k] = 0.0;
+            }
+        }
     }
     if (!a.packet) return;  // state-only pass (the commit of a converged load step)
+
+    // dtau/dbeta_sp = ka I + kc xi (W xi)^T in the plastic branch (FiniteStrain.jl:245-262)
+    double ka = 0.0, kc = 0.0;
+    if (kin && plastic) {
+        const double Hs = m.Hiso + m.Hkin, Y = m.sy0 + m.Hiso * an;
+        const double dgk = (s32 * nrm - Y) / (3 * G + Hs);
+        ka = 2 * G * s32 * dgk / nrm;
+        kc = (2 * G * s32 / (3 * G + Hs) * (Y / (nrm * nrm))) / nrm;
+    }
 
     double *pk = a.packet + (size_t)gp * PK;
     const double sw = s * w;
@@ -413,6 +507,50 @@ __global__ __launch_bounds__(256) void gp_kernel(ElementArgs a, int64_t e_begin,
             dt[k] = lam * trd + 2 * G * de[k] - cdev * (de[k] - trd / 3) + cnn * nde * nh[k];
 #pragma unroll
         for (int k = 3; k < 6; ++k) dt[k] = G * de[k] - cdev * 0.5 * de[k] + cnn * nde * nh[k];
+        if (kin && plastic) {  // + dtau/dbeta_sp : d beta_sp (FiniteStrain.jl:533-537)
+            // Omega_b = V^T R^T dR skew: (x y^T - y x^T)_AB / (lp_A + lp_B), x = (RV)^T e_p, y = V^T e_q
+            double x[3], y[3];
+#pragma unroll
+            for (int A_ = 0; A_ < 3; ++A_) {
+                x[A_] = RV[pp + 3 * A_];
+                y[A_] = Vp[qq + 3 * A_];
+            }
+            double Om[3][3];
+#pragma unroll
+            for (int A_ = 0; A_ < 3; ++A_)
+#pragma unroll
+                for (int B_ = 0; B_ < 3; ++B_)
+                    Om[A_][B_] = (A_ == B_) ? 0.0 : (x[A_] * y[B_] - y[A_] * x[B_]) / (lp[A_] + lp[B_]);
+            const double Bv[3][3] = {{bv[0], bv[3], bv[5]}, {bv[3], bv[1], bv[4]}, {bv[5], bv[4], bv[2]}};
+            double OB[3][3];
+#pragma unroll
+            for (int A_ = 0; A_ < 3; ++A_)
+#pragma unroll
+                for (int B_ = 0; B_ < 3; ++B_)
+                    OB[A_][B_] = Om[A_][0] * Bv[0][B_] + Om[A_][1] * Bv[1][B_] + Om[A_][2] * Bv[2][B_];
+            double Mb[3][3];  // Om Bv - Bv Om = OB + OB^T (Om skew, Bv symmetric)
+#pragma unroll
+            for (int A_ = 0; A_ < 3; ++A_)
+#pragma unroll
+                for (int B_ = 0; B_ < 3; ++B_) Mb[A_][B_] = OB[A_][B_] + OB[B_][A_];
+            double T1[9];  // RV Mb
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int B_ = 0; B_ < 3; ++B_)
+                    T1[i + 3 * B_] = RV[i] * Mb[0][B_] + RV[i + 3] * Mb[1][B_] + RV[i + 6] * Mb[2][B_];
+            double db[6];  // d beta_sp = RV Mb RV^T, physical-shear Voigt
+            {
+                const int I[6] = {0, 1, 2, 0, 1, 0}, Jx[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+                for (int t = 0; t < 6; ++t)
+                    db[t] = T1[I[t]] * RV[Jx[t]] + T1[I[t] + 3] * RV[Jx[t] + 3] + T1[I[t] + 6] * RV[Jx[t] + 6];
+            }
+            const double xw = xi[0] * db[0] + xi[1] * db[1] + xi[2] * db[2] +
+                              2 * (xi[3] * db[3] + xi[4] * db[4] + xi[5] * db[5]);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) dt[k] += ka * db[k] + kc * xi[k] * xw;
+        }
         const double dT[9] = {dt[0], dt[3], dt[5], dt[3], dt[1], dt[4], dt[5], dt[4], dt[2]};
         // dP = dtau Fb^-T - (tau Fb^-T[:,q]) (Fb^-T[p,:])   (FiniteStrain.jl:539-540)
         double wq[3];
@@ -444,6 +582,128 @@ __global__ __launch_bounds__(256) void gp_kernel(ElementArgs a, int64_t e_begin,
 #pragma unroll
     for (int k = 0; k < 24; ++k) pk[108 + k] = dNdX[k];
 
+}
+
+// Small-strain Hex8 (element_force_tangent! Elements.jl:323-360): eps = B ue with the
+// reference-configuration gradients (element_geometry :130-144, bmatrix :97-118), the
+// full return map with the committed eps_p, beta, abar (Materials.jl:159-250), sigma to
+// the stress output.  The packet carries the same two-point form as the finite-strain
+// kernel so that the scatter is shared: with H = grad u, eps_v(ij) = H_ij + H_ji off the
+// diagonal, so  B^T D B = G^T A G  with  A_(i+3j),(k+3l) = D_v(ij),v(kl)  and
+// B^T sigma = G^T vec(sigma).
+__global__ __launch_bounds__(256) void gp_small_kernel(ElementArgs a, int64_t e_begin, int64_t e_end) {
+    __shared__ double sX[32][24];
+    __shared__ double sU[32][24];
+    const int le = threadIdx.x >> 3, g = threadIdx.x & 7;
+    const int64_t e = e_begin + (int64_t)blockIdx.x * 32 + le;
+    const bool valid = e < e_end;
+    if (valid) {
+        const int64_t node = a.conn[8 * e + g];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            sX[le][3 * g + d] = a.X[3 * node + d];
+            sU[le][3 * g + d] = a.U[3 * node + d];
+        }
+    }
+    __syncthreads();
+    if (!valid) return;
+    const double *Xe = sX[le];
+    const double *ue = sU[le];
+    const DevMat m = a.mats[a.matid ? a.matid[e] : 0];
+    const int64_t gp = 8 * e + g;
+    const double gq = 0.57735026918962576451;
+    const double x0 = (((g + 1) & 2) ? gq : -gq), x1 = ((g & 2) ? gq : -gq), x2 = ((g & 4) ? gq : -gq);
+    double dNdX[24], w;
+    ref_grads(Xe, x0, x1, x2, dNdX, w);  // w = detJ (GAUSS_WTS == 1)
+    double H[9];
+    defgrad(ue, dNdX, H);
+    H[0] -= 1.0; H[4] -= 1.0; H[8] -= 1.0;
+    const double eps[6] = {H[0], H[4], H[8], H[3] + H[1], H[7] + H[5], H[6] + H[2]};
+
+    // --- return_map (Materials.jl:159-250)
+    const double G = m.G, lam = m.lam;
+    const double *epn = a.epsp + 6 * gp;
+    const double *bn = a.beta + 6 * gp;
+    double d[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) d[k] = eps[k] - epn[k];
+    const double tr = d[0] + d[1] + d[2];
+    double sg[6];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) sg[k] = lam * tr + 2 * G * d[k];
+#pragma unroll
+    for (int k = 3; k < 6; ++k) sg[k] = G * d[k];
+    const double p = (sg[0] + sg[1] + sg[2]) / 3;
+    double xi[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) xi[k] = sg[k] - (k < 3 ? p : 0.0) - bn[k];
+    const double nrm = sqrt(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2] +
+                            2 * (xi[3] * xi[3] + xi[4] * xi[4] + xi[5] * xi[5]));
+    const double s32 = 1.2247448713915890491;  // sqrt(3/2)
+    const double qtr = s32 * nrm;
+    const double an = a.abar[gp];
+    const double ftr = qtr - yield_stress(m, an);
+    double cdev = 0.0, cnn = 0.0, dg = 0.0;  // D = C - cdev*Idev + cnn*nh nh^T
+    double nh[6] = {0, 0, 0, 0, 0, 0};
+    const bool plastic = ftr > 0.0;
+    if (plastic) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) nh[k] = xi[k] / nrm;
+        if (m.delta > 0 && m.ssat > m.sy0) {  // Voce local Newton (Materials.jl:201-213)
+            dg = ftr / (3 * G + m.Hkin + yield_slope(m, an));
+            for (int it = 0; it < 50; ++it) {
+                double Hp = yield_slope(m, an + dg);
+                double r = qtr - (3 * G + m.Hkin) * dg - yield_stress(m, an + dg);
+                double ddg = r / (3 * G + m.Hkin + Hp);
+                dg += ddg;
+                if (fabs(ddg) <= 1e-13 * (dg + 1e-14)) break;
+            }
+        } else {
+            dg = ftr / (3 * G + m.Hiso + m.Hkin);  // Materials.jl:218
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) sg[k] -= (2 * G * dg * s32) * nh[k];
+        const double Hp = yield_slope(m, an + dg);
+        const double b0 = 2 * G * dg / qtr, g0c = 2 * G / (3 * G + m.Hkin + Hp);
+        cdev = 3 * G * b0;
+        cnn = 3 * G * (b0 - g0c);
+    }
+    double *so = a.sig + 6 * gp;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) so[k] = sg[k];
+    if (a.commit) {  // Elements.jl:350-357
+        double *ep = a.epsp + 6 * gp;
+        double *bw = a.beta + 6 * gp;
+        const double c = (2.0 / 3.0) * m.Hkin * (dg * s32);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            ep[k] += (dg * s32) * (k < 3 ? nh[k] : 2 * nh[k]);
+            bw[k] += c * nh[k];
+        }
+        a.abar[gp] = an + dg;
+    }
+    if (!a.packet) return;
+
+    // D (6x6) = C - cdev Idev + cnn nh nh^T, scattered to A_(i+3j),(k+3l) = w D_v(ij),v(kl)
+    double Dm[6][6];
+#pragma unroll
+    for (int v = 0; v < 6; ++v)
+#pragma unroll
+        for (int u = 0; u < 6; ++u) {
+            const double Cvu = (v < 3 && u < 3) ? (lam + (v == u ? 2 * G : 0.0)) : (v == u ? G : 0.0);
+            const double Idev = (v == u ? (v < 3 ? 1.0 : 0.5) : 0.0) - ((v < 3 && u < 3) ? 1.0 / 3.0 : 0.0);
+            Dm[v][u] = w * (Cvu - cdev * Idev + cnn * nh[v] * nh[u]);
+        }
+    double *pk = a.packet + (size_t)gp * PK;
+    const int VO[9] = {0, 3, 5, 3, 1, 4, 5, 4, 2};  // Voigt index of (i, j), i + 3j
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+#pragma unroll
+        for (int r = 0; r < 9; ++r) pk[r + 9 * c] = Dm[VO[r]][VO[c]];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) pk[99 + k] = w * sg[VO[k]];
+#pragma unroll
+    for (int k = 0; k < 24; ++k) pk[108 + k] = dNdX[k];
 }
 
 // Contraction of the 8 GP packets of an element into 3x3 blocks Ke(a, b),
@@ -642,7 +902,12 @@ __global__ void detF_kernel(ElementArgs a, int64_t gp, double *out) {
 void launch_gp_kernel(const ElementArgs &a, int64_t e_begin, int64_t e_count, hipStream_t s) {
     if (e_count <= 0) return;
     const int64_t blocks = (e_count + 31) / 32;
-    hipLaunchKernelGGL(gp_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a, e_begin, e_begin + e_count);
+    if (a.kind == 2)
+        hipLaunchKernelGGL(gp_small_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a, e_begin, e_begin + e_count);
+    else if (a.kin)
+        hipLaunchKernelGGL(gp_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, s, a, e_begin, e_begin + e_count);
+    else
+        hipLaunchKernelGGL(gp_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, s, a, e_begin, e_begin + e_count);
     PF_HIP(hipGetLastError());
 }
 

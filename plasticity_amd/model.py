@@ -105,7 +105,7 @@ def default_options(**kw):
     return o
 
 
-_KIND = {"finite": L.HEX8_FINITE, "finite_fbar": L.HEX8_FINITE_FBAR}
+_KIND = {"small": L.HEX8_SMALL, "finite": L.HEX8_FINITE, "finite_fbar": L.HEX8_FINITE_FBAR}
 
 
 def _comps(comp):
@@ -119,7 +119,7 @@ class Model:
 
     def __init__(self, mesh: Mesh, material, element="finite_fbar", options=None, material_id=None):
         if element not in _KIND:
-            raise ValueError(f"unsupported element kind {element!r} (use 'finite' or 'finite_fbar')")
+            raise ValueError(f"unknown element kind {element!r} (use 'small', 'finite' or 'finite_fbar')")
         mats = list(material) if isinstance(material, (list, tuple)) else [material]
         self.mesh = mesh
         self.materials = mats
@@ -214,13 +214,17 @@ class Model:
         L.check(L.lib().pfem_set_displacement(self._h, L.ptr(u)), "set_displacement")
 
     def get_state(self):
-        ep, ab, cp, sg = (np.empty((self.ngp, 6)), np.empty(self.ngp), np.empty((self.ngp, 6)),
-                          np.empty((self.ngp, 6)))
-        L.check(L.lib().pfem_get_state(self._h, L.ptr(ep), L.ptr(ab), L.ptr(cp), L.ptr(sg)), "get_state")
-        return {"epsp": ep, "abar": ab, "cpinv": cp, "sig": sg}
+        """Committed GaussState (Mesh.jl:124-163): epsp, beta (back stress), abar, cpinv and
+        the last assembled stress sig (Kirchhoff for finite strain, Cauchy for small)."""
+        ep, bt, ab, cp, sg = (np.empty((self.ngp, 6)), np.empty((self.ngp, 6)), np.empty(self.ngp),
+                              np.empty((self.ngp, 6)), np.empty((self.ngp, 6)))
+        L.check(L.lib().pfem_get_state(self._h, L.ptr(ep), L.ptr(bt), L.ptr(ab), L.ptr(cp), L.ptr(sg)),
+                "get_state")
+        return {"epsp": ep, "beta": bt, "abar": ab, "cpinv": cp, "sig": sg}
 
-    def set_state(self, epsp=None, abar=None, cpinv=None):
-        args = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (epsp, abar, cpinv)]
+    def set_state(self, epsp=None, abar=None, cpinv=None, beta=None):
+        args = [None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+                for a in (epsp, beta, abar, cpinv)]
         L.check(L.lib().pfem_set_state(self._h, *[None if a is None else L.ptr(a) for a in args]), "set_state")
 
     def nodal_displacements(self):
@@ -234,8 +238,11 @@ class Model:
         return self.get_state()["abar"]
 
     def gauss_stress(self):
-        """Cauchy sigma = tau / det F with the unbarred F (Model.jl:188-208), ngp x 6."""
+        """Cauchy stress, ngp x 6 (Model.jl:188-208): the stored stress for the small-strain
+        element, tau / det F with the unbarred F for the finite-strain elements."""
         tau = self.gauss_kirchhoff()
+        if self.element == "small":
+            return tau
         U = self.U.reshape(-1, 3)
         X = self._nodes
         g = 1.0 / np.sqrt(3.0)
@@ -394,10 +401,12 @@ class Model:
         return sec.value
 
 
-def element_batch(kind, material: J2Material, Xe, ue, epsp=None, abar=None, cpinv=None, commit=False):
-    """Per-element Fe (n,24), Ke (n,24,24) and Kirchhoff stress (n,8,6) computed by the
-    production GPU kernels for independent elements (Elements.jl:405-494).  History
-    arrays (n,8,6)/(n,8) are updated in place when commit is true."""
+def element_batch(kind, material: J2Material, Xe, ue, epsp=None, abar=None, cpinv=None, commit=False,
+                  beta=None):
+    """Per-element Fe (n,24), Ke (n,24,24) and stress (n,8,6) computed by the production
+    GPU kernels for independent elements (Elements.jl:405-494 finite strain, :323-360 small
+    strain; Kirchhoff stress for finite strain).  History arrays (n,8,6)/(n,8) -- beta is
+    the back stress -- are updated in place when commit is true."""
     Xe = np.ascontiguousarray(Xe, dtype=np.float64).reshape(-1, 24)
     ue = np.ascontiguousarray(ue, dtype=np.float64).reshape(-1, 24)
     n = Xe.shape[0]
@@ -407,7 +416,9 @@ def element_batch(kind, material: J2Material, Xe, ue, epsp=None, abar=None, cpin
         abar = np.zeros((n, 8))
     if cpinv is None:
         cpinv = np.tile(np.array([1.0, 1, 1, 0, 0, 0]), (n, 8, 1))
-    for a in (epsp, abar, cpinv):
+    if beta is None:
+        beta = np.zeros((n, 8, 6))
+    for a in (epsp, beta, abar, cpinv):
         if a.dtype != np.float64 or not a.flags.c_contiguous:
             raise ValueError("history arrays must be C-contiguous float64")
     Fe = np.empty((n, 24))
@@ -416,7 +427,8 @@ def element_batch(kind, material: J2Material, Xe, ue, epsp=None, abar=None, cpin
     f = L.Fail()
     m = material._c()
     rc = L.lib().pfem_element_batch(_KIND[kind] if isinstance(kind, str) else int(kind), C.byref(m), n,
-                                    L.ptr(Xe), L.ptr(ue), L.ptr(epsp), L.ptr(abar), L.ptr(cpinv), int(commit),
+                                    L.ptr(Xe), L.ptr(ue), L.ptr(epsp), L.ptr(beta), L.ptr(abar), L.ptr(cpinv),
+                                    int(commit),
                                     L.ptr(Fe), L.ptr(Ke), L.ptr(sig), C.byref(f))
     if rc == L.PFEM_ERR_INVERTED:
         raise ElementInversionError(f.e, f.g, f.J)

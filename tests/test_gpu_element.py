@@ -142,3 +142,76 @@ def test_voce_hardening_matches_oracle(need_gpu):
         worst = max(worst, rel(Fe[e], f), rel(Ke[e], K), rel(sig[e], s), rel(a2[e], a1), rel(c2[e], c1))
     print(f"[voce] worst relative difference {worst:.2e}", flush=True)
     assert worst < 1e-10
+
+
+def _kin_states(n, seed, fbar, mat_kw, amp=0.05):
+    """Elements with a committed plastic state that holds a back stress, then a
+    non-coaxial second deformation (the F2(c)/F9b states of test/test_finite_strain.jl)."""
+    rng = np.random.default_rng(seed)
+    mat = O.Material(**mat_kw)
+    nodes, conn = O.box_mesh(1, 1, 1, 1, 1, 1)
+    X0 = nodes[conn[0]]
+    out = {k: [] for k in ("X", "ue", "ep", "bt", "ab", "cp")}
+    for _ in range(n):
+        X = X0 * (0.5 + rng.random(3)) + 0.05 * rng.normal(size=(8, 3))
+        F1 = np.eye(3) + amp * rng.normal(size=(3, 3))
+        e, b, a, c = np.zeros((8, 6)), np.zeros((8, 6)), np.zeros(8), np.tile([1.0, 1, 1, 0, 0, 0], (8, 1))
+        O.element_finite(fbar, mat, X.ravel(), (X @ F1.T - X).ravel(), e, a, c, commit=True, beta=b)
+        F2 = F1 + 0.5 * amp * rng.normal(size=(3, 3))
+        out["X"].append(X.ravel()); out["ue"].append((X @ F2.T - X).ravel())
+        out["ep"].append(e); out["bt"].append(b); out["ab"].append(a); out["cp"].append(c)
+    return {k: np.array(v) for k, v in out.items()}
+
+
+@pytest.mark.parametrize("fbar", [0, 1])
+def test_kinematic_hardening_matches_oracle(fbar, need_gpu):
+    """Objective back stress (FiniteStrain.jl:172-346): Fe, the non-symmetric Ke, tau and
+    the committed history (beta_ref, abar, Cp^-1) within 1e-10 relative of the oracle."""
+    kw = dict(E=210e3, nu=0.3, sy0=250.0, Hiso=200.0, Hkin=2000.0)
+    s = _kin_states(48, 21 + fbar, fbar, kw)
+    assert np.abs(s["bt"]).max() > 0 and s["ab"].max() > 0
+    print(f"[kinematic] fbar={fbar} n=48, max |beta| {np.abs(s['bt']).max():.3g}", flush=True)
+    mat = O.Material(**kw)
+    e2, b2, a2, c2 = s["ep"].copy(), s["bt"].copy(), s["ab"].copy(), s["cp"].copy()
+    kind = "finite_fbar" if fbar else "finite"
+    Fe, Ke, sig = pa.element_batch(kind, pa.J2Material(**kw), s["X"], s["ue"], e2, a2, c2, commit=True, beta=b2)
+    worst, asym = 0.0, 0.0
+    for e in range(s["X"].shape[0]):
+        e1, b1, a1, c1 = s["ep"][e].copy(), s["bt"][e].copy(), s["ab"][e].copy(), s["cp"][e].copy()
+        f, K, t = O.element_finite(fbar, mat, s["X"][e], s["ue"][e], e1, a1, c1, commit=True, beta=b1)
+        worst = max(worst, rel(Fe[e], f), rel(Ke[e], K), rel(sig[e], t), rel(b2[e], b1), rel(a2[e], a1),
+                    rel(c2[e], c1), rel(e2[e], e1))
+        asym = max(asym, rel(Ke[e], Ke[e].T))
+    print(f"  worst relative difference {worst:.2e}; max asymmetry {asym:.2e}", flush=True)
+    assert worst < 1e-10
+    assert asym > 1e-6  # the back-stress rotation term is present (non-symmetric tangent)
+
+
+def test_small_strain_element_matches_oracle(need_gpu):
+    """Hex8Small (Elements.jl:323-360) with kinematic + isotropic hardening: Fe, Ke, sigma
+    and committed eps_p, beta, abar within 1e-10 relative of the oracle."""
+    kw = dict(E=210e3, nu=0.3, sy0=250.0, Hiso=1000.0, Hkin=500.0)
+    mat = O.Material(**kw)
+    rng = np.random.default_rng(8)
+    nodes, conn = O.box_mesh(1, 1, 1, 1, 1, 1)
+    X0 = nodes[conn[0]]
+    Xs, us, eps, bts, abs_ = [], [], [], [], []
+    for _ in range(48):
+        X = X0 * (0.5 + rng.random(3)) + 0.05 * rng.normal(size=(8, 3))
+        u1 = 4e-3 * rng.normal(size=24)
+        e, b, a = np.zeros((8, 6)), np.zeros((8, 6)), np.zeros(8)
+        O.element_small(mat, X.ravel(), u1, e, b, a, commit=True)
+        Xs.append(X.ravel()); us.append(u1 + 2e-3 * rng.normal(size=24))
+        eps.append(e); bts.append(b); abs_.append(a)
+    X, ue, ep, bt, ab = map(np.array, (Xs, us, eps, bts, abs_))
+    assert ab.max() > 0 and np.abs(bt).max() > 0
+    e2, b2, a2 = ep.copy(), bt.copy(), ab.copy()
+    Fe, Ke, sig = pa.element_batch("small", pa.J2Material(**kw), X, ue, e2, a2, None, commit=True, beta=b2)
+    worst = 0.0
+    for e in range(X.shape[0]):
+        e1, b1, a1 = ep[e].copy(), bt[e].copy(), ab[e].copy()
+        f, K, s = O.element_small(mat, X[e], ue[e], e1, b1, a1, commit=True)
+        worst = max(worst, rel(Fe[e], f), rel(Ke[e], K), rel(sig[e], s), rel(e2[e], e1), rel(b2[e], b1),
+                    rel(a2[e], a1))
+    print(f"[small strain] worst relative difference {worst:.2e}", flush=True)
+    assert worst < 1e-10

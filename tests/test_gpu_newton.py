@@ -155,3 +155,52 @@ def test_two_phase_block_matches_oracle(need_gpu):
     eu, ea = rel(m.U, keep[3][0]), rel(m.get_state()["abar"], keep[3][1])
     print(f"  newton {res.iters} krylov {res.cg_iters}; U rel err {eu:.2e}, abar rel err {ea:.2e}", flush=True)
     assert eu <= 1e-8 and ea <= 1e-8 and np.linalg.norm(keep[3][1]) > 0
+
+
+def _oracle_vs_gpu(element, mat_kw, nsteps, stretch):
+    """A jittered 4x3x3 block clamped at x = 0 whose x = 1 face is pulled by `stretch` and
+    sheared by 0.3 stretch in y (non-homogeneous, rotating states), solved on the oracle
+    (direct solves) and on the device.  Returns (relative U error, relative abar error,
+    oracle model, device model, device result, device state)."""
+    import oracle as O
+    mesh = pa.box_mesh(1, 1, 1, 4, 3, 3, jitter=0.1, seed=3)
+    M = O.OracleModel(mesh.nodes, mesh.elements, O.Material(**mat_kw), element=element)
+    m = pa.Model(mesh, pa.J2Material(**mat_kw), element=element,
+                 options=pa.default_options(eta_min=1e-10, eta_max=1e-10))
+    for mod in (M, m):
+        mod.fix(pa.on_face(mesh, "xmin"), "all")
+        mod.prescribe(pa.on_face(mesh, "xmax"), "x", stretch)
+        mod.prescribe(pa.on_face(mesh, "xmax"), "y", 0.3 * stretch)
+    conv, iters, *_ = M.solve(nsteps=nsteps, tol=1e-10)
+    assert conv
+    res = m.solve(nsteps=nsteps, tol=1e-10)
+    assert res.converged
+    st = m.get_state()
+    return rel(m.U, M.U), rel(st["abar"], M.st_c["abar"]), M, m, res, st
+
+
+def test_small_strain_newton_matches_oracle(need_gpu):
+    """Hex8Small (the reference's default element, Model.jl:63-75) with combined hardening:
+    U, abar, beta and the stress within 1e-8 of the oracle; symmetric tangent, so PCG."""
+    kw = dict(E=210e3, nu=0.3, sy0=250.0, Hiso=1000.0, Hkin=500.0)
+    eu, ea, M, m, res, st = _oracle_vs_gpu("small", kw, 4, 0.01)
+    eb = rel(st["beta"], M.st_c["beta"])
+    es = rel(m.gauss_stress(), M.st_c["sig"])
+    print(f"[small strain] newton {res.iters} krylov {res.cg_iters} solver {res.solver_used}; "
+          f"U {eu:.2e} abar {ea:.2e} beta {eb:.2e} sigma {es:.2e}", flush=True)
+    assert M.st_c["abar"].max() > 0
+    assert eu <= 1e-8 and ea <= 1e-8 and eb <= 1e-8 and es <= 1e-8
+
+
+@pytest.mark.parametrize("element", ["finite", "finite_fbar"])
+def test_kinematic_hardening_newton_matches_oracle(element, need_gpu):
+    """Finite strain with Hkin = 2000 (the material of F11(b), test/test_finite_strain.jl:
+    364-369): the objective back stress makes the tangent non-symmetric; the device run
+    (PCG with the BiCGStab fallback) equals the oracle's direct-solve run within 1e-8."""
+    kw = dict(E=210e3, nu=0.3, sy0=250.0, Hiso=200.0, Hkin=2000.0)
+    eu, ea, M, m, res, st = _oracle_vs_gpu(element, kw, 4, 0.08)
+    eb = rel(st["beta"], M.st_c["beta"])
+    print(f"[kinematic {element}] newton {res.iters} krylov {res.cg_iters} solver {res.solver_used}; "
+          f"U {eu:.2e} abar {ea:.2e} beta {eb:.2e}", flush=True)
+    assert np.abs(M.st_c["beta"]).max() > 0
+    assert eu <= 1e-8 and ea <= 1e-8 and eb <= 1e-8
