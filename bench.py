@@ -158,7 +158,9 @@ def main():
     import plasticity_amd as pa
     from plasticity_amd.workloads import workload
 
-    wl = workload(args.workload, n=args.n, nsteps=args.nsteps)
+    t0 = time.perf_counter()
+    wl = workload(args.workload, n=args.n, nsteps=args.nsteps, device=True)
+    log(f"[bench] workload generated in {time.perf_counter() - t0:.2f}s (mesh on the device)")
     if args.mode == "spmv":
         m = pa.Model(wl.mesh, pa.J2Material(**wl.materials[0]), options=pa.default_options())
         m.assemble(np.zeros(m.ndof))
@@ -182,7 +184,10 @@ def main():
         from plasticity_amd.learned import UNet, grid_of
         net = UNet(grid_of(wl.mesh)).load(args.learned)
         m.attach_net(net, "precondition")
-    log(f"[bench] model set-up {time.perf_counter() - t0:.2f}s")
+    st0 = m.stats()
+    log(f"[bench] model set-up {time.perf_counter() - t0:.2f}s (pattern, slots, colours "
+        f"{st0.t_create_pattern:.3f}s, built on the device: {'pattern' if st0.built_on_device & 1 else 'host'}"
+        f"/{'colours' if st0.built_on_device & 2 else 'host'})")
 
     def load_step(k):
         lam = k / wl.nsteps

@@ -99,7 +99,9 @@ typedef struct pfem_options {
                                    Newton step, Solver.jl:383);
                                    reserved[4], reserved[5]: power-iteration steps of the
                                    rho(D^-1 A) estimate behind the Chebyshev bounds, first
-                                   set-up and warm-started rebuilds (<= 0: 25 and 4) */
+                                   set-up and warm-started rebuilds (<= 0: 25 and 4);
+                                   reserved[6]: 1 = build the pattern, slots and colours
+                                   on the host (pfem_create) */
 } pfem_options;
 
 typedef struct pfem_fail {
@@ -114,10 +116,21 @@ typedef struct pfem_ctx pfem_ctx;
 /* Fills `o` with the reference defaults (Solver.jl:103-111). */
 void pfem_default_options(pfem_options *o);
 
+/* box_mesh(lx, ly, lz, nx, ny, nz) (Mesh.jl:30-59) generated on the device and copied
+ * to nodes (3 x nnodes) and conn (8 x nelem, 0-based), with the seeded jitter of the
+ * interior nodes: each coordinate moves by (2u - 1) * jitter * h, u =
+ * splitmix64(splitmix64(seed) ^ (3 node + comp)) >> 11 scaled by 2^-53 (jitter = 0:
+ * the reference's box_mesh exactly). */
+int pfem_box_mesh(double lx, double ly, double lz, int64_t nx, int64_t ny, int64_t nz, double jitter,
+                  uint64_t seed, double *nodes, int64_t *conn);
+
 /* Model(mesh, material; element) set-up (Model.jl:63-75): uploads the mesh,
- * builds the block pattern (Assembly.jl:62-162, as 3x3 BSR), the greedy element
- * colouring (Assembly.jl:172-188 on box meshes), scatter slots, and resets the
- * Gauss state (Mesh.jl:132-138). */
+ * builds the block pattern (Assembly.jl:62-162, as 3x3 BSR) and the scatter slots
+ * on the device, colours the elements (the parity colours of Assembly.jl:172-188
+ * for box numbering, on the device; greedy first-fit in element order otherwise,
+ * on the host), and resets the Gauss state (Mesh.jl:132-138).  The pattern is
+ * built on the host instead when opts->reserved[6] == 1 or when a node is shared
+ * by more than 32 elements. */
 int pfem_create(const pfem_mesh_desc *mesh, const pfem_options *opts, pfem_ctx **out);
 void pfem_destroy(pfem_ctx *ctx);
 
@@ -226,6 +239,8 @@ typedef struct pfem_stats {
     double t_setup_rho, t_setup_galerkin, t_setup_coarse;  /* AMG numeric set-up phases (s) */
     int64_t n_assemble, n_setup, n_solve, n_krylov;
     int64_t device_bytes;                    /* device memory held by the context */
+    double t_create_pattern;                 /* pfem_create: pattern, scatter slots and colours (s) */
+    int32_t built_on_device;                 /* bit 0: pattern + slots built on the device, bit 1: colours */
 } pfem_stats;
 
 int pfem_get_stats(pfem_ctx *ctx, pfem_stats *st);

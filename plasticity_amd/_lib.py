@@ -68,7 +68,8 @@ class Stats(C.Structure):
                 ("t_pcg_spmv", C.c_double), ("n_pcg_spmv", C.c_int64),
                 ("t_setup_rho", C.c_double), ("t_setup_galerkin", C.c_double), ("t_setup_coarse", C.c_double),
                 ("n_assemble", C.c_int64), ("n_setup", C.c_int64), ("n_solve", C.c_int64),
-                ("n_krylov", C.c_int64), ("device_bytes", C.c_int64)]
+                ("n_krylov", C.c_int64), ("device_bytes", C.c_int64), ("t_create_pattern", C.c_double),
+                ("built_on_device", C.c_int32)]
 
 
 class NetDesc(C.Structure):
@@ -84,6 +85,8 @@ _dp = C.POINTER(C.c_double)
 SIGNATURES = {
     "pfem_default_options": (None, [C.POINTER(Options)]),
     "pfem_create": (C.c_int, [C.POINTER(MeshDesc), C.POINTER(Options), C.POINTER(_vp)]),
+    "pfem_box_mesh": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_int64, C.c_int64, C.c_int64, C.c_double,
+                                C.c_uint64, _vp, _vp]),
     "pfem_destroy": (None, [_vp]),
     "pfem_set_dirichlet": (C.c_int, [_vp, C.c_int64, _vp, _vp, _vp]),
     "pfem_set_neumann": (C.c_int, [_vp, C.c_int64, _vp, _vp]),

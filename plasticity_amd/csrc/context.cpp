@@ -6,6 +6,7 @@
 // on box meshes), and the per-element scatter slot map that replaces the
 // reference's 576 binary searches per element (Assembly.jl:210-224, :401-409).
 #include "context.h"
+#include "build.h"
 
 #include <algorithm>
 #include <cmath>
@@ -154,71 +155,108 @@ int pfem_create(const pfem_mesh_desc *md, const pfem_options *opts, pfem_ctx **o
         c->fbar = md->element_kind == PFEM_HEX8_FINITE_FBAR;
         for (const DevMat &m : dm) c->kin |= m.Hkin > 0;
         c->h_nodes.assign(md->nodes, md->nodes + 3 * nn);
-        c->h_conn.assign(md->conn, md->conn + 8 * ne);
         c->X.alloc(3 * nn); c->X.upload(md->nodes, 3 * nn, s);
         c->conn.alloc(8 * ne); c->conn.upload(md->conn, 8 * ne, s);
         if (md->material_id) { c->matid.alloc(ne); c->matid.upload(md->material_id, ne, s); }
         c->mats.alloc(dm.size()); c->mats.upload(dm.data(), dm.size(), s);
 
-        // ---- pattern (Assembly.jl:62-162) as SELL-64 BSR
-        std::vector<int64_t> aptr;
-        std::vector<int32_t> aidx;
-        node_adjacency(nn, ne, md->conn, aptr, aidx);
+        // ---- pattern (Assembly.jl:62-162) as SELL-64 BSR, scatter slots (on the device
+        //      unless reserved[6] == 1 or a node is too dense for the device build)
+        const double tb0 = now();
+        const bool host_build = c->opt.reserved[6] == 1;
         FineMatrix &K = c->K;
-        K.nb = nn;
-        K.nnzb = aptr[nn];
-        K.nslices = (nn + SELL_C - 1) / SELL_C;
-        K.h_slice_ptr.assign(K.nslices + 1, 0);
-        K.h_rowlen.resize(nn);
-        for (int64_t i = 0; i < nn; ++i) K.h_rowlen[i] = (int32_t)(aptr[i + 1] - aptr[i]);
-        for (int64_t sl = 0; sl < K.nslices; ++sl) {
-            int32_t w = 0;
-            for (int64_t i = sl * SELL_C; i < std::min<int64_t>(nn, (sl + 1) * SELL_C); ++i)
-                w = std::max(w, K.h_rowlen[i]);
-            K.h_slice_ptr[sl + 1] = K.h_slice_ptr[sl] + (int64_t)SELL_C * w;
-        }
-        K.nslots = K.h_slice_ptr[K.nslices];
-        K.h_colind.assign(K.nslots, 0);
-        std::vector<int32_t> diag(nn, -1);
-        for (int64_t sl = 0; sl < K.nslices; ++sl) {
-            const int64_t w = (K.h_slice_ptr[sl + 1] - K.h_slice_ptr[sl]) / SELL_C;
-            for (int64_t l = 0; l < SELL_C; ++l) {
-                const int64_t i = sl * SELL_C + l;
-                for (int64_t k = 0; k < w; ++k) {
-                    const int64_t pos = K.h_slice_ptr[sl] + k * SELL_C + l;
-                    if (i < nn && k < K.h_rowlen[i]) {
-                        K.h_colind[pos] = aidx[aptr[i] + k];
-                        if (aidx[aptr[i] + k] == i) diag[i] = (int32_t)pos;
-                    } else {
-                        K.h_colind[pos] = (int32_t)std::min<int64_t>(i, nn - 1);  // padding: value 0
+        DevicePattern P;
+        const int brc = host_build ? BUILD_TOO_DENSE : build_pattern_device(nn, ne, c->conn.p, P, s);
+        if (brc == BUILD_OK) {
+            K.nb = nn;
+            K.nslices = P.nslices;
+            K.nslots = P.nslots;
+            K.slice_ptr = std::move(P.slice_ptr);
+            K.rowlen = std::move(P.rowlen);
+            K.colind = std::move(P.colind);
+            K.diagpos = std::move(P.diagpos);
+            c->slots = std::move(P.slots);
+            // host copies of the pattern (AMG symbolic phase, exports, Dirichlet rows)
+            K.h_slice_ptr.resize(K.nslices + 1);
+            K.h_rowlen.resize(nn);
+            K.h_colind.resize(K.nslots);
+            K.slice_ptr.download(K.h_slice_ptr.data(), K.nslices + 1, s);
+            K.rowlen.download(K.h_rowlen.data(), nn, s);
+            K.colind.download(K.h_colind.data(), K.nslots, s);
+            c->sync();
+            K.nnzb = 0;
+            for (int64_t i = 0; i < nn; ++i) K.nnzb += K.h_rowlen[i];
+            c->built_on_device |= 1;
+        } else {
+            std::vector<int64_t> aptr;
+            std::vector<int32_t> aidx;
+            node_adjacency(nn, ne, md->conn, aptr, aidx);
+            K.nb = nn;
+            K.nnzb = aptr[nn];
+            K.nslices = (nn + SELL_C - 1) / SELL_C;
+            K.h_slice_ptr.assign(K.nslices + 1, 0);
+            K.h_rowlen.resize(nn);
+            for (int64_t i = 0; i < nn; ++i) K.h_rowlen[i] = (int32_t)(aptr[i + 1] - aptr[i]);
+            for (int64_t sl = 0; sl < K.nslices; ++sl) {
+                int32_t w = 0;
+                for (int64_t i = sl * SELL_C; i < std::min<int64_t>(nn, (sl + 1) * SELL_C); ++i)
+                    w = std::max(w, K.h_rowlen[i]);
+                K.h_slice_ptr[sl + 1] = K.h_slice_ptr[sl] + (int64_t)SELL_C * w;
+            }
+            K.nslots = K.h_slice_ptr[K.nslices];
+            K.h_colind.assign(K.nslots, 0);
+            std::vector<int32_t> diag(nn, -1);
+            for (int64_t sl = 0; sl < K.nslices; ++sl) {
+                const int64_t w = (K.h_slice_ptr[sl + 1] - K.h_slice_ptr[sl]) / SELL_C;
+                for (int64_t l = 0; l < SELL_C; ++l) {
+                    const int64_t i = sl * SELL_C + l;
+                    for (int64_t k = 0; k < w; ++k) {
+                        const int64_t pos = K.h_slice_ptr[sl] + k * SELL_C + l;
+                        if (i < nn && k < K.h_rowlen[i]) {
+                            K.h_colind[pos] = aidx[aptr[i] + k];
+                            if (aidx[aptr[i] + k] == i) diag[i] = (int32_t)pos;
+                        } else {
+                            K.h_colind[pos] = (int32_t)std::min<int64_t>(i, nn - 1);  // padding: value 0
+                        }
                     }
                 }
             }
+            if (K.nslots >= (int64_t)1 << 31) throw status_failure(PFEM_ERR_BAD_ARG, "pattern exceeds int32 slots");
+            K.slice_ptr.alloc(K.nslices + 1); K.slice_ptr.upload(K.h_slice_ptr.data(), K.nslices + 1, s);
+            K.rowlen.alloc(nn); K.rowlen.upload(K.h_rowlen.data(), nn, s);
+            K.colind.alloc(K.nslots); K.colind.upload(K.h_colind.data(), K.nslots, s);
+            K.diagpos.alloc(nn); K.diagpos.upload(diag.data(), nn, s);
+
+            // ---- scatter slots: block (a, b) of element e -> slot in row conn[a]
+            std::vector<int32_t> hs(64 * ne);
+            for (int64_t e = 0; e < ne; ++e)
+                for (int a = 0; a < 8; ++a) {
+                    const int64_t na = md->conn[8 * e + a];
+                    const int32_t *lo = aidx.data() + aptr[na], *hi = aidx.data() + aptr[na + 1];
+                    const int64_t sb = K.h_slice_ptr[na / SELL_C] + na % SELL_C;
+                    for (int b = 0; b < 8; ++b) {
+                        const int32_t nbn = (int32_t)md->conn[8 * e + b];
+                        const int64_t k = std::lower_bound(lo, hi, nbn) - lo;
+                        hs[64 * e + 8 * a + b] = (int32_t)(sb + k * SELL_C);
+                    }
+                }
+            c->slots.alloc(64 * ne); c->slots.upload(hs.data(), 64 * ne, s);
         }
-        if (K.nslots >= (int64_t)1 << 31) throw status_failure(PFEM_ERR_BAD_ARG, "pattern exceeds int32 slots");
-        K.slice_ptr.alloc(K.nslices + 1); K.slice_ptr.upload(K.h_slice_ptr.data(), K.nslices + 1, s);
-        K.rowlen.alloc(nn); K.rowlen.upload(K.h_rowlen.data(), nn, s);
-        K.colind.alloc(K.nslots); K.colind.upload(K.h_colind.data(), K.nslots, s);
-        K.diagpos.alloc(nn); K.diagpos.upload(diag.data(), nn, s);
         K.v64.alloc(9 * K.nslots); K.v64.zero(s);
 
-        // ---- scatter slots: block (a, b) of element e -> slot in row conn[a]
-        std::vector<int32_t> hs(64 * ne);
-        for (int64_t e = 0; e < ne; ++e)
-            for (int a = 0; a < 8; ++a) {
-                const int64_t na = md->conn[8 * e + a];
-                const int32_t *lo = aidx.data() + aptr[na], *hi = aidx.data() + aptr[na + 1];
-                const int64_t sb = K.h_slice_ptr[na / SELL_C] + na % SELL_C;
-                for (int b = 0; b < 8; ++b) {
-                    const int32_t nbn = (int32_t)md->conn[8 * e + b];
-                    const int64_t k = std::lower_bound(lo, hi, nbn) - lo;
-                    hs[64 * e + 8 * a + b] = (int32_t)(sb + k * SELL_C);
-                }
-            }
-        c->slots.alloc(64 * ne); c->slots.upload(hs.data(), 64 * ne, s);
-
-        // ---- greedy first-fit colouring in element order
-        {
+        // ---- colours: parity classes of element_colors (Assembly.jl:172-188) computed on
+        //      the device for box numbering, else greedy first-fit in element order (equal
+        //      to the parity colours on box meshes)
+        int64_t dims[3];
+        if (!host_build && box_dims_from_conn(nn, ne, md->conn, c->conn.p, dims, s)) {
+            c->color_order.alloc(ne);
+            dbuf<int32_t> col(ne);
+            c->ncolors = parity_colours_device(dims, col.p, c->color_order.p, c->color_ptr, s);
+            c->h_color.resize(ne);
+            col.download(c->h_color.data(), ne, s);
+            c->sync();
+            c->built_on_device |= 2;
+        } else {
             std::vector<uint64_t> mask(nn, 0);
             c->h_color.resize(ne);
             int nc = 0;
@@ -239,7 +277,8 @@ int pfem_create(const pfem_mesh_desc *md, const pfem_options *opts, pfem_ctx **o
             std::vector<int64_t> order(ne), f(c->color_ptr.begin(), c->color_ptr.end() - 1);
             for (int64_t e = 0; e < ne; ++e) order[f[c->h_color[e]]++] = e;
             c->color_order.alloc(ne); c->color_order.upload(order.data(), ne, s);
-        }
+    }
+        c->t_create_pattern = now() - tb0;
 
         // ---- state (GaussState Mesh.jl:132-138) and scratch
         const int64_t ngp = 8 * ne;
@@ -290,9 +329,7 @@ int pfem_reset(pfem_ctx *c) {
         c->U.zero(s); c->dU.zero(s); c->R.zero(s);
         c->epsp.zero(s); c->abar.zero(s); c->sig.zero(s);
         if (c->beta.p) c->beta.zero(s);
-        std::vector<double> cp(6 * ngp, 0.0);
-        for (int64_t g = 0; g < ngp; ++g) cp[6 * g] = cp[6 * g + 1] = cp[6 * g + 2] = 1.0;
-        c->cpinv.upload(cp.data(), cp.size(), s);
+        fill_identity6(ngp, c->cpinv.p, s);
         c->sync();
         return PFEM_OK;
     });
@@ -545,7 +582,32 @@ int pfem_get_stats(pfem_ctx *c, pfem_stats *st) {
     size_t bytes = c->K.v64.bytes() + c->K.v32.bytes() + c->K.colind.bytes() + c->slots.bytes() +
                    c->packet.bytes() + c->g0.bytes() + 4 * c->epsp.bytes() + c->beta.bytes();
     st->device_bytes = (int64_t)bytes;
+    st->t_create_pattern = c->t_create_pattern;
+    st->built_on_device = c->built_on_device;
     return PFEM_OK;
+}
+
+int pfem_box_mesh(double lx, double ly, double lz, int64_t nx, int64_t ny, int64_t nz, double jitter, uint64_t seed,
+                  double *nodes, int64_t *conn) {
+    if (!(lx > 0) || !(ly > 0) || !(lz > 0) || nx < 1 || ny < 1 || nz < 1 || !(jitter >= 0) || !nodes || !conn)
+        return PFEM_ERR_BAD_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PFEM_ERR_NO_DEVICE;
+    try {
+        hipStream_t s;
+        PF_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        struct SG { hipStream_t s; ~SG() { (void)hipStreamDestroy(s); } } sg{s};
+        const int64_t nn = (nx + 1) * (ny + 1) * (nz + 1), ne = nx * ny * nz;
+        dbuf<double> dn(3 * nn);
+        dbuf<int64_t> dc(8 * ne);
+        box_mesh_device(lx, ly, lz, nx, ny, nz, jitter, seed, dn.p, dc.p, s);
+        dn.download(nodes, 3 * nn, s);
+        dc.download(conn, 8 * ne, s);
+        PF_HIP(hipStreamSynchronize(s));
+        return PFEM_OK;
+    } catch (const hip_failure &) {
+        return PFEM_ERR_HIP;
+    }
 }
 
 }  // extern "C"

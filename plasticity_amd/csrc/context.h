@@ -26,7 +26,8 @@ struct pfem_ctx {
     pfem::dbuf<int32_t> matid;
     pfem::dbuf<pfem::DevMat> mats;
     std::vector<double> h_nodes;
-    std::vector<int64_t> h_conn;
+    int built_on_device = 0;       // bit 0: pattern and slots, bit 1: colours
+    double t_create_pattern = 0;   // seconds spent on pattern, slots and colours
 
     // pattern, colours, scatter slots
     pfem::FineMatrix K;

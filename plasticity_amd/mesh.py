@@ -51,7 +51,15 @@ class Mesh:
         return f"Mesh(nnodes={self.nnodes}, nelem={self.nelem})"
 
 
-def box_mesh(lx, ly, lz, nx, ny, nz, jitter=0.0, seed=0) -> Mesh:
+def box_mesh(lx, ly, lz, nx, ny, nz, jitter=0.0, seed=0, device=False) -> Mesh:
+    """device=True generates the same mesh (bit for bit) with pfem_box_mesh on the GPU."""
+    if device:
+        from . import _lib as L
+        nodes = np.empty(((nx + 1) * (ny + 1) * (nz + 1), 3))
+        el = np.empty((nx * ny * nz, 8), dtype=np.int64)
+        L.check(L.lib().pfem_box_mesh(float(lx), float(ly), float(lz), int(nx), int(ny), int(nz), float(jitter),
+                                      int(seed), L.ptr(nodes), L.ptr(el)), "pfem_box_mesh")
+        return Mesh(nodes, el, dims=(nx, ny, nz, lx, ly, lz))
     npx, npy, npz = nx + 1, ny + 1, nz + 1
     k, j, i = np.meshgrid(np.arange(npz), np.arange(npy), np.arange(npx), indexing="ij")
     i, j, k = i.ravel(), j.ravel(), k.ravel()
@@ -143,7 +151,17 @@ def ellipsoid_inclusions(mesh: Mesh, volume_fraction, seed, semi_axes=(0.08, 0.0
     a = np.asarray(semi_axes, dtype=np.float64) * (hi - lo)
     k = 0
     target = volume_fraction * mesh.nelem
-    while mid.sum() < target and k < 100000:
+    count = 0
+    # Box numbering (dims known): only centroids in the index box around an ellipsoid
+    # can lie inside it (|centroid - c| <= max semi-axis; jitter moves a centroid by less
+    # than one cell), so each ellipsoid tests that block instead of every element.
+    box = mesh.dims is not None and mesh.nelem == mesh.dims[0] * mesh.dims[1] * mesh.dims[2]
+    if box:
+        nd = np.array(mesh.dims[:3])
+        h = (hi - lo) / nd
+        cg = cent.reshape(nd[2], nd[1], nd[0], 3)
+        ids = np.arange(mesh.nelem).reshape(nd[2], nd[1], nd[0])
+    while count < target and k < 100000:
         u = jitter_u01(seed, np.arange(6 * k, 6 * k + 6, dtype=np.uint64) + np.uint64(1 << 40), 0)
         c = lo + u[:3] * (hi - lo)
         th, ph, ps = u[3] * 2 * np.pi, u[4] * np.pi, u[5] * 2 * np.pi
@@ -151,8 +169,17 @@ def ellipsoid_inclusions(mesh: Mesh, volume_fraction, seed, semi_axes=(0.08, 0.0
         Ry = np.array([[np.cos(ph), 0, np.sin(ph)], [0, 1, 0], [-np.sin(ph), 0, np.cos(ph)]])
         Rx = np.array([[1, 0, 0], [0, np.cos(ps), -np.sin(ps)], [0, np.sin(ps), np.cos(ps)]])
         R = Rz @ Ry @ Rx
-        d = (cent - c) @ R
-        inside = ((d / a) ** 2).sum(axis=1) <= 1.0
+        if box:
+            r = a.max()
+            i0 = np.maximum(np.floor((c - r - lo) / h).astype(int) - 1, 0)
+            i1 = np.minimum(np.ceil((c + r - lo) / h).astype(int) + 1, nd)
+            sub = cg[i0[2]:i1[2], i0[1]:i1[1], i0[0]:i1[0]].reshape(-1, 3)
+            sid = ids[i0[2]:i1[2], i0[1]:i1[1], i0[0]:i1[0]].ravel()
+        else:
+            sub, sid = cent, np.arange(mesh.nelem)
+        d = (sub - c) @ R
+        inside = sid[((d / a) ** 2).sum(axis=1) <= 1.0]
+        count += int(np.count_nonzero(mid[inside] == 0))
         mid[inside] = 1
         k += 1
     return mid

@@ -75,7 +75,8 @@ def uniaxial_constraints(mesh: Mesh, disp: float):
 
 
 def workload(k: int, n: int | None = None, nsteps: int | None = None, seed: int | None = None,
-             inclusion: float = 0.0) -> Workload:
+             inclusion: float = 0.0, device: bool = False) -> Workload:
+    """Workload k at size n; device=True generates the (bit-identical) mesh on the GPU."""
     if k == 0:
         n = 8 if n is None else n
         mesh = box_mesh(1.0, 1.0, 1.0, n, n, n)
@@ -83,13 +84,13 @@ def workload(k: int, n: int | None = None, nsteps: int | None = None, seed: int 
         return Workload("tension cube 8^3 F-bar", mesh, [STEEL], None, d, v, r, nsteps or 20, 1e-10, 1e-10)
     if k == 1:
         n = 100 if n is None else n
-        mesh = box_mesh(1.0, 1.0, 1.0, n, n, n, jitter=0.1, seed=0)
+        mesh = box_mesh(1.0, 1.0, 1.0, n, n, n, jitter=0.1, seed=0, device=device)
         d, v, r = uniaxial_constraints(mesh, -0.2)
         return Workload(f"jittered block {n}^3 compression 20%", mesh, [STEEL], None, d, v, r, nsteps or 40,
                         1e-8, 1e-8)
     if k == 2:
         n = 200 if n is None else n
-        mesh = box_mesh(1.0, 1.0, 1.0, n, n, n, jitter=0.1, seed=1)
+        mesh = box_mesh(1.0, 1.0, 1.0, n, n, n, jitter=0.1, seed=1, device=device)
         mid = ellipsoid_inclusions(mesh, 0.2, seed=1)
         d, v, r = uniaxial_constraints(mesh, 0.1)
         hard = dict(STEEL, sy0=750e6)

@@ -50,3 +50,30 @@ def test_inclusions_volume_fraction():
     mid = pa.ellipsoid_inclusions(m, 0.2, seed=1)
     assert 0.2 <= mid.mean() < 0.35
     assert np.array_equal(mid, pa.ellipsoid_inclusions(m, 0.2, seed=1))
+
+
+def test_ellipsoid_inclusions_equal_brute_force():
+    """The index-box search returns exactly the element set of testing every centroid."""
+    from plasticity_amd.mesh import ellipsoid_inclusions, jitter_u01
+
+    def brute(mesh, vf, seed, semi_axes=(0.08, 0.05, 0.05)):
+        cent = mesh.nodes[mesh.elements].mean(axis=1)
+        lo, hi = mesh.nodes.min(axis=0), mesh.nodes.max(axis=0)
+        mid = np.zeros(mesh.nelem, dtype=np.int32)
+        a = np.asarray(semi_axes) * (hi - lo)
+        k = 0
+        while mid.sum() < vf * mesh.nelem:
+            u = jitter_u01(seed, np.arange(6 * k, 6 * k + 6, dtype=np.uint64) + np.uint64(1 << 40), 0)
+            c = lo + u[:3] * (hi - lo)
+            th, ph, ps = u[3] * 2 * np.pi, u[4] * np.pi, u[5] * 2 * np.pi
+            Rz = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]])
+            Ry = np.array([[np.cos(ph), 0, np.sin(ph)], [0, 1, 0], [-np.sin(ph), 0, np.cos(ph)]])
+            Rx = np.array([[1, 0, 0], [0, np.cos(ps), -np.sin(ps)], [0, np.sin(ps), np.cos(ps)]])
+            d = (cent - c) @ (Rz @ Ry @ Rx)
+            mid[((d / a) ** 2).sum(axis=1) <= 1.0] = 1
+            k += 1
+        return mid
+
+    for n, seed, vf in [(24, 1, 0.2), (17, 5, 0.3), (30, 2, 0.1)]:
+        m = pa.box_mesh(1, 1, 1, n, n, n, jitter=0.1, seed=seed)
+        assert np.array_equal(ellipsoid_inclusions(m, vf, seed), brute(m, vf, seed))
