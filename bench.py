@@ -32,8 +32,8 @@ HBM_PEAK_GBS = 8000.0
 ELEMENT_FLOPS = 160073
 # HBM bytes per fine-SpMV launch at 100^3 (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors
 # from a 1 GiB device copy), measured with rocprofv3 --pmc on `--mode spmv --n 100`:
-# 1,291,914.9 KiB fetched + 24,148.5 KiB written per dispatch.
-SPMV_TRAFFIC_100 = (1291914.9 * 2.0 + 24148.5) * 1024
+# 1,034,757.0 KiB fetched + 24,148.0 KiB written per dispatch.
+SPMV_TRAFFIC_100 = (1034757.0 * 2.0 + 24148.0) * 1024
 FP64_PEAK_TFLOPS = 78.6
 
 

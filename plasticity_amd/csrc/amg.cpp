@@ -549,8 +549,8 @@ void vcycle(pfem_ctx *c, Amg *a, int l, const T *b, T *x) {
     AmgLevel &N = a->L[l + 1];
     smooth<T>(c, a, l, b, x, true);
     level_residual<T>(c, L, l, x, b, Vec<T>::r(L), false);
-    if (L.bs == 3) bsr_spmv<6, 3, T, T>(L.nagg, L.R_rowptr.p, L.R_colind.p, Vec<T>::R(L), Vec<T>::r(L), Vec<T>::b(N), 0.0, s);
-    else bsr_spmv<6, 6, T, T>(L.nagg, L.R_rowptr.p, L.R_colind.p, Vec<T>::R(L), Vec<T>::r(L), Vec<T>::b(N), 0.0, s);
+    if (L.bs == 3) bsr_spmv_wave<6, 3, T, T>(L.nagg, L.R_rowptr.p, L.R_colind.p, Vec<T>::R(L), Vec<T>::r(L), Vec<T>::b(N), s);
+    else bsr_spmv_wave<6, 6, T, T>(L.nagg, L.R_rowptr.p, L.R_colind.p, Vec<T>::R(L), Vec<T>::r(L), Vec<T>::b(N), s);
     vcycle<T>(c, a, l + 1, Vec<T>::b(N), Vec<T>::x(N));
     if (L.bs == 3) bsr_spmv<3, 6, T, T>(L.nb, L.P_rowptr.p, L.P_colind.p, Vec<T>::P(L), Vec<T>::x(N), x, 1.0, s);
     else bsr_spmv<6, 6, T, T>(L.nb, L.P_rowptr.p, L.P_colind.p, Vec<T>::P(L), Vec<T>::x(N), x, 1.0, s);

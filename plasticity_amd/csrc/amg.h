@@ -91,6 +91,9 @@ void amg_transpose_P(int64_t nR, const int32_t *R_src, const double *P, double *
 template <int BR, int BC, class VT, class XT>
 void bsr_spmv(int64_t nb, const int32_t *rowptr, const int32_t *colind, const VT *vals, const XT *x, XT *y,
               double beta, hipStream_t s);  // y = A x + beta*y
+template <int BR, int BC, class VT, class XT>
+void bsr_spmv_wave(int64_t nb, const int32_t *rowptr, const int32_t *colind, const VT *vals, const XT *x, XT *y,
+                   hipStream_t s);  // y = A x, one wave per block row (long rows)
 template <int BS, class VT, class XT>
 void bsr_residual(int64_t nb, const int32_t *rowptr, const int32_t *colind, const VT *vals, const XT *x,
                   const XT *b, XT *r, hipStream_t s);

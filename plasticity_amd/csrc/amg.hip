@@ -230,6 +230,41 @@ __global__ void bsr_spmv_kernel(int64_t nb, const int32_t *__restrict__ rowptr, 
     y[o] = (XT)(beta == 0.0 ? (double)s : (double)s + beta * (double)y[o]);
 }
 
+// One 64-lane wave per block row, for long rows (the restriction R = P^T holds ~10^2
+// blocks per coarse row): lanes stride over the row's blocks, so consecutive lanes read
+// consecutive blocks, and the BR partial sums are reduced across the wave.  y = A x.
+template <int BR, int BC, class VT, class XT>
+__global__ __launch_bounds__(256) void bsr_spmv_wave_kernel(int64_t nb, const int32_t *__restrict__ rowptr,
+                                                            const int32_t *__restrict__ colind,
+                                                            const VT *__restrict__ vals, const XT *__restrict__ x,
+                                                            XT *__restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= nb) return;
+    VT acc[BR];
+#pragma unroll
+    for (int r = 0; r < BR; ++r) acc[r] = 0;
+    const int k1 = rowptr[i + 1];
+    for (int k = rowptr[i] + lane; k < k1; k += 64) {
+        const VT *v = vals + (int64_t)BR * BC * k;
+        const XT *xx = x + (int64_t)BC * colind[k];
+        VT xv[BC];
+#pragma unroll
+        for (int c = 0; c < BC; ++c) xv[c] = (VT)xx[c];
+#pragma unroll
+        for (int r = 0; r < BR; ++r)
+#pragma unroll
+            for (int c = 0; c < BC; ++c) acc[r] += v[BC * r + c] * xv[c];
+    }
+#pragma unroll
+    for (int r = 0; r < BR; ++r)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) acc[r] += __shfl_xor(acc[r], off, 64);
+    if (lane == 0)
+#pragma unroll
+        for (int r = 0; r < BR; ++r) y[(int64_t)BR * i + r] = (XT)acc[r];
+}
+
 template <int BS, class VT, class XT>
 __global__ void bsr_residual_kernel(int64_t nb, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colind,
                                     const VT *__restrict__ vals, const XT *__restrict__ x, const XT *__restrict__ b,
@@ -455,6 +490,14 @@ void bsr_spmv(int64_t nb, const int32_t *rowptr, const int32_t *colind, const VT
                        colind, vals, x, y, beta);
     PF_HIP(hipGetLastError());
 }
+template <int BR, int BC, class VT, class XT>
+void bsr_spmv_wave(int64_t nb, const int32_t *rowptr, const int32_t *colind, const VT *vals, const XT *x, XT *y,
+                   hipStream_t s) {
+    if (nb <= 0) return;
+    hipLaunchKernelGGL((bsr_spmv_wave_kernel<BR, BC, VT, XT>), dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, s, nb,
+                       rowptr, colind, vals, x, y);
+    PF_HIP(hipGetLastError());
+}
 template <int BS, class VT, class XT>
 void bsr_residual(int64_t nb, const int32_t *rowptr, const int32_t *colind, const VT *vals, const XT *x,
                   const XT *b, XT *r, hipStream_t s) {
@@ -528,6 +571,10 @@ template void amg_transpose_P<6>(int64_t, const int32_t *, const double *, doubl
                                          double, hipStream_t);                                                     \
     template void bsr_spmv<6, 3, VT, XT>(int64_t, const int32_t *, const int32_t *, const VT *, const XT *, XT *,  \
                                          double, hipStream_t);                                                     \
+    template void bsr_spmv_wave<6, 3, VT, XT>(int64_t, const int32_t *, const int32_t *, const VT *, const XT *,    \
+                                              XT *, hipStream_t);                                                  \
+    template void bsr_spmv_wave<6, 6, VT, XT>(int64_t, const int32_t *, const int32_t *, const VT *, const XT *,    \
+                                              XT *, hipStream_t);                                                  \
     template void bsr_residual<6, VT, XT>(int64_t, const int32_t *, const int32_t *, const VT *, const XT *,       \
                                           const XT *, XT *, hipStream_t);                                          \
     template void sell_residual<3, VT, XT>(const FineMatrix &, const VT *, const XT *, const XT *, XT *,           \

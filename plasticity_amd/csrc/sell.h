@@ -3,8 +3,8 @@
 //
 // One 64-lane workgroup per slice of 64 block rows.  For each slice column k the
 // workgroup first copies the 64 contiguous 3x3 blocks (64 x 72 B in FP64, 64 x 36 B in
-// FP32) into LDS with 16-byte loads -- every global load instruction covers 1 KB of
-// consecutive bytes -- then each lane takes its own block from LDS and gathers x.
+// FP32) into LDS with 16-byte non-temporal loads -- every global load instruction covers
+// 1 KB of consecutive bytes -- then each lane takes its own block from LDS and gathers x.
 // Padding slots carry zero values and a valid column, so lanes never branch on the
 // row length.
 #pragma once
@@ -13,6 +13,14 @@
 namespace pfem {
 
 enum SellMode { SELL_Y = 0, SELL_RES = 1, SELL_DOT = 2 };
+
+// Matrix values and indices are read once per product: load them non-temporally so the
+// stream does not evict the gathered vector from L2 (fine FP64 SpMV at 100^3:
+// 0.385 -> 0.341 ms).
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void stream_copy16(void *dst, const void *src, int t) {
+    reinterpret_cast<u32x4_t *>(dst)[t] = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t *>(src) + t);
+}
 
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2): group b
 // runs on XCD b % 8.  Remap so that XCD x works through one contiguous eighth of the
@@ -39,11 +47,9 @@ __global__ __launch_bounds__(64) void sell_lds_kernel(int64_t nb, const int64_t 
     VT y0 = 0, y1 = 0, y2 = 0;
     for (int k = 0; k < w; ++k) {
         const int64_t col0 = base + (int64_t)k * SELL_C;
-        const uint4 *src = reinterpret_cast<const uint4 *>(vals + 9 * col0);
-        uint4 *dst = reinterpret_cast<uint4 *>(tile);
 #pragma unroll
-        for (int t = lane; t < NV; t += SELL_C) dst[t] = src[t];
-        const int64_t j = colind[col0 + lane];
+        for (int t = lane; t < NV; t += SELL_C) stream_copy16(tile, vals + 9 * col0, t);
+        const int64_t j = __builtin_nontemporal_load(colind + col0 + lane);
         const XT x0 = x[3 * j], x1 = x[3 * j + 1], x2 = x[3 * j + 2];
         __syncthreads();
         const VT *v = tile + 9 * lane;
@@ -95,10 +101,8 @@ __global__ __launch_bounds__(64) void sell_half_kernel(int64_t nb, const int64_t
     float y0 = 0.f, y1 = 0.f, y2 = 0.f;
     for (int k = 0; k < w; ++k) {
         const int64_t col0 = base + (int64_t)k * SELL_C;
-        const uint4 *src = reinterpret_cast<const uint4 *>(vals + 9 * col0);
-        uint4 *dst = reinterpret_cast<uint4 *>(tile);
-        for (int t = lane; t < NV; t += SELL_C) dst[t] = src[t];
-        const int64_t j = colind[col0 + lane];
+        for (int t = lane; t < NV; t += SELL_C) stream_copy16(tile, vals + 9 * col0, t);
+        const int64_t j = __builtin_nontemporal_load(colind + col0 + lane);
         const float x0 = x[3 * j] * sinv[3 * j], x1 = x[3 * j + 1] * sinv[3 * j + 1],
                     x2 = x[3 * j + 2] * sinv[3 * j + 2];
         __syncthreads();
