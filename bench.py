@@ -139,6 +139,9 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_ir"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--amg-rebuild", type=int, default=1, help="0: once per load step, 1: every Newton iteration")
+    ap.add_argument("--smoother", default="chebyshev", choices=["chebyshev", "jacobi"])
+    ap.add_argument("--smoother-degree", type=int, default=0,
+                    help="Chebyshev degree / Jacobi sweeps (0: library default, 2 / 1)")
     ap.add_argument("--verbose", type=int, default=0)
     ap.add_argument("--fp16-fine", action="store_true",
                     help="FP32-IR: fine-level smoother products with an FP16 copy of the scaled K")
@@ -172,7 +175,8 @@ def main():
     log(f"[bench] workload: {wl.name}, nodes {wl.mesh.nnodes}, dof {3 * wl.mesh.nnodes}")
     opts = pa.default_options(eta_min=wl.krylov_rtol, eta_max=wl.krylov_rtol, precision=args.precision,
                               verbose=args.verbose, amg_rebuild=args.amg_rebuild, fp16_fine=args.fp16_fine,
-                              line_search=args.line_search, rho_iters=args.rho_iters)
+                              line_search=args.line_search, rho_iters=args.rho_iters, smoother=args.smoother,
+                              smoother_degree=args.smoother_degree)
     t0 = time.perf_counter()
     mats = [pa.J2Material(**d) for d in wl.materials]
     m = pa.Model(wl.mesh, mats if len(mats) > 1 else mats[0], options=opts, material_id=wl.material_id)

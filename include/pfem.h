@@ -245,6 +245,12 @@ typedef struct pfem_stats {
 
 int pfem_get_stats(pfem_ctx *ctx, pfem_stats *st);
 
+/* Dense inverse of the n x n row-major A on the device with the coarsest-level AMG
+ * solver (Gauss-Jordan with partial pivoting, blocked panels when they fit in LDS;
+ * an all-zero row is decoupled with a unit diagonal).  No counterpart in the
+ * reference; exported so the coarse solve can be checked on its own. */
+int pfem_dense_inverse(int64_t n, const double *A, double *inv);
+
 /* Time `reps` fine-level SpMVs y = K x (FP64 if prec == 0, FP32 if 1) on the
  * context's stream with events; returns mean seconds per SpMV. */
 int pfem_bench_spmv(pfem_ctx *ctx, int prec, int reps, double *sec_per_spmv);
@@ -257,7 +263,7 @@ int pfem_bench_spmv(pfem_ctx *ctx, int prec, int reps, double *sec_per_spmv);
  * x + nx*(y + ny*z)); BF16 weights and activations on MFMA with FP32
  * accumulation; hand-written forward, backward and Adam kernels.  It maps the
  * 3-component nodal residual to a smooth correction n; the preconditioner
- * z = g n + D^-1 (r - g K n) (g = r'n / n'Kn, D the 3x3 block diagonal of K)
+ * z = g n + D^-1 (r - g K n) (g = (Kn)'r / (Kn)'(Kn), D the 3x3 block diagonal of K)
  * replaces the V-cycle inside FP64 flexible CG, and AMG takes over from the
  * current iterate when the iteration stalls.  Trained on (residual, V-cycle correction) samples that
  * the solver logs from its own AMG-preconditioned solves, with the

@@ -112,6 +112,7 @@ SIGNATURES = {
                                      C.c_int, _vp, _vp, _vp, C.POINTER(Fail)]),
     "pfem_get_stats": (C.c_int, [_vp, C.POINTER(Stats)]),
     "pfem_bench_spmv": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
+    "pfem_dense_inverse": (C.c_int, [C.c_int64, _vp, _vp]),
     "pfem_status_string": (C.c_char_p, [C.c_int]),
     "pfem_net_create": (C.c_int, [C.POINTER(NetDesc), C.POINTER(_vp)]),
     "pfem_net_destroy": (None, [_vp]),

@@ -342,13 +342,16 @@ void coarse_inverse(pfem_ctx *c, Amg *a) {
         a->gj_work.alloc((size_t)2 * n * n);
         a->gj_prow.alloc(2 * n);
         a->gj_fcol.alloc(n);
+        a->gj_iw.alloc(gj_scratch_ints(n));
+        a->gj_dw.alloc(gj_scratch_doubles(n));
     }
     if (a->L.size() == 1)
         dense_coarse_inverse(L.nb, 3, true, c->K.slice_ptr.p, c->K.rowlen.p, nullptr, c->K.colind.p, c->K.v64.p,
-                             a->gj_work.p, a->gj_prow.p, a->gj_fcol.p, a->coarse_inv.p, c->stream);
+                             a->gj_work.p, a->gj_prow.p, a->gj_fcol.p, a->gj_iw.p, a->gj_dw.p, a->coarse_inv.p,
+                             c->stream);
     else
         dense_coarse_inverse(L.nb, 6, false, nullptr, nullptr, L.rowptr.p, L.colind.p, L.A.p, a->gj_work.p,
-                             a->gj_prow.p, a->gj_fcol.p, a->coarse_inv.p, c->stream);
+                             a->gj_prow.p, a->gj_fcol.p, a->gj_iw.p, a->gj_dw.p, a->coarse_inv.p, c->stream);
 }
 
 }  // namespace

@@ -57,7 +57,8 @@ struct Amg {
     // coarsest level: dense inverse (FP64)
     int64_t nc = 0;
     dbuf<double> coarse_inv;
-    dbuf<double> gj_work, gj_prow, gj_fcol;  // Gauss-Jordan scratch
+    dbuf<double> gj_work, gj_prow, gj_fcol, gj_dw;  // Gauss-Jordan scratch
+    dbuf<int> gj_iw;
     int smoother = PFEM_SMOOTH_CHEBYSHEV;
     int degree = 2;
     bool have32 = false;
@@ -113,6 +114,13 @@ void scale_by_rnorm(int64_t n, const double *nrm2, double *x, hipStream_t s);
 // Gauss-Jordan with partial pivoting; W is 2 n^2, prow 2n, fcol n scratch
 void dense_coarse_inverse(int64_t nb, int bs, bool sell, const int64_t *slice_ptr, const int32_t *rowlen,
                           const int32_t *rowptr, const int32_t *colind, const double *A, double *W, double *prow,
-                          double *fcol, double *inv, hipStream_t s);
+                          double *fcol, int *iw, double *dw, double *inv, hipStream_t s);
+// inv = M^-1 for the dense n x n M stored in the left half of W (n x 2n, row-major);
+// zero rows of M are decoupled with a unit diagonal.  W, prow (2n), fcol (n), iw
+// (gj_scratch_ints(n)) and dw (gj_scratch_doubles(n)) are scratch.
+size_t gj_scratch_ints(int64_t n);
+size_t gj_scratch_doubles(int64_t n);
+void gauss_jordan_inverse(int64_t n, double *W, double *prow, double *fcol, int *iw, double *dw, double *inv,
+                          hipStream_t s);
 
 }  // namespace pfem

@@ -3,6 +3,8 @@
 // exactly one thread, so repeated set-ups give bit-identical hierarchies.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "amg.h"
 #include "sell.h"
 
@@ -364,15 +366,17 @@ __global__ void dense_from_blocks_kernel(int64_t nb, int bs, const int64_t *__re
             for (int q = 0; q < bs; ++q) W[(bs * i + r) * 2 * n + bs * j + q] = a[bs * r + q];
     }
 }
-__global__ void dense_prepare_kernel(int64_t n, double *__restrict__ W) {
-    // unit right half; zero rows (dropped coarse dofs) decoupled with a unit diagonal
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
+__global__ __launch_bounds__(64) void dense_prepare_kernel(int64_t n, double *__restrict__ W) {
+    // one wave per row: unit right half; a zero row (dropped coarse dof) is decoupled
+    // with a unit diagonal
+    const int64_t r = blockIdx.x;
+    const int lane = threadIdx.x;
     double *row = W + r * 2 * n;
-    bool zero = true;
-    for (int64_t c = 0; c < n && zero; ++c) zero = row[c] == 0.0;
-    if (zero) row[r] = 1.0;
-    for (int64_t c = 0; c < n; ++c) row[n + c] = (c == r) ? 1.0 : 0.0;
+    int nz = 0;
+    for (int64_t c = lane; c < n; c += 64) nz |= row[c] != 0.0;
+    for (int64_t c = lane; c < n; c += 64) row[n + c] = (c == r) ? 1.0 : 0.0;
+    nz = __any(nz);
+    if (lane == 0 && !nz) row[r] = 1.0;
 }
 __global__ __launch_bounds__(1024) void gj_pivot_kernel(int64_t n, int64_t k, double *__restrict__ W,
                                                         double *__restrict__ prow, double *__restrict__ fcol) {
@@ -423,11 +427,155 @@ __global__ void gj_extract_kernel(int64_t n, const double *__restrict__ W, doubl
     inv[t] = W[r * 2 * n + n + c];
 }
 
+// ---- blocked Gauss-Jordan: b pivots per panel, 3 launches per panel instead of 2 per pivot.
+// Panel [k0, k0+b): one workgroup selects the pivot rows S by partial pivoting on the
+// panel (kept in LDS) and inverts M = W[S, panel]; then Y = M^-1 W[S, :] replaces the
+// pivot rows and every other row r takes W[r, :] -= W[r, panel] Y.  Rows are never
+// swapped: the inverse's row c is the right half of the pivot row chosen for column c.
+// M^-1 comes out of the panel elimination itself: every panel row carries b tracking
+// columns E (zero), the row chosen as the j-th pivot gets E_j += 1 before it is scaled,
+// and at the end E[S_j] is row j of M^-1 (Gauss-Jordan on [M | I] with the rows of M in
+// pivot order; an unchosen row's E only ever involves columns of earlier pivots).
+constexpr int GJ_MAXB = 16;
+constexpr int GJ_THREADS = 1024;
+
+template <int B>
+__global__ __launch_bounds__(GJ_THREADS) void gj_panel_kernel(int64_t n, int64_t k0, int bb, int panel,
+                                                              const double *__restrict__ W, int *__restrict__ used,
+                                                              int *__restrict__ ptag, int *__restrict__ pivrow,
+                                                              double *__restrict__ L, double *__restrict__ Minv,
+                                                              int *__restrict__ S) {
+    extern __shared__ double gsm[];
+    // the last panel may be narrower (bb < B): its extra columns are zero and never pivoted
+    const int b = B;
+    const int ld = 2 * B + 1;              // padded row stride (LDS banks)
+    double *P = gsm;                       // n x (2b) [panel | E], stride ld
+    double *prow = P + n * ld;             // scaled pivot row (2b)
+    int *usedL = reinterpret_cast<int *>(prow + 2 * GJ_MAXB);
+    __shared__ double wv[GJ_THREADS / 64];
+    __shared__ int wi[GJ_THREADS / 64];
+    __shared__ int Ss[GJ_MAXB];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int64_t n2 = 2 * n;
+    const int b2 = 2 * b;
+    for (int64_t idx = t; idx < n * b; idx += GJ_THREADS) {
+        const int64_t r = idx / b, j = idx % b;
+        const double v = j < bb ? W[r * n2 + k0 + j] : 0.0;
+        P[r * ld + j] = v;
+        P[r * ld + b + j] = 0.0;
+        if (j < bb) L[r * bb + j] = v;
+    }
+    for (int64_t r = t; r < n; r += GJ_THREADS) usedL[r] = used[r];
+    __syncthreads();
+    for (int j = 0; j < bb; ++j) {
+        // pivot: largest |P[r, j]| over unused rows (any unused row beats none, so a NaN
+        // column still yields a valid row); every thread reduces the per-wave winners
+        double best = -1.0;
+        int arg = (int)n;
+        for (int64_t r = t; r < n; r += GJ_THREADS)
+            if (!usedL[r]) {
+                const double v = fabs(P[r * ld + j]);
+                if (arg == (int)n || v > best) { best = v; arg = (int)r; }
+            }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ob = __shfl_xor(best, off, 64);
+            const int oa = __shfl_xor(arg, off, 64);
+            if (oa < (int)n && (arg == (int)n || ob > best || (ob == best && oa < arg))) { best = ob; arg = oa; }
+        }
+        if (lane == 0) { wv[w] = best; wi[w] = arg; }
+        __syncthreads();
+        int p = (int)n;
+        double pb = -1.0;
+        for (int q = 0; q < GJ_THREADS / 64; ++q)
+            if (wi[q] < (int)n && (p == (int)n || wv[q] > pb || (wv[q] == pb && wi[q] < p))) { pb = wv[q]; p = wi[q]; }
+        const double pv = P[(int64_t)p * ld + j];
+        const double ip = pv != 0.0 ? 1.0 / pv : 0.0;
+        if (t < b2) prow[t] = (P[(int64_t)p * ld + t] + (t == b + j ? 1.0 : 0.0)) * ip;
+        if (t == 0) { usedL[p] = 1; Ss[j] = p; }
+        __syncthreads();
+        for (int64_t r = t; r < n; r += GJ_THREADS) {  // one row per thread
+            double *Pr = P + r * ld;
+            if (r == p) {
+#pragma unroll
+                for (int c = 0; c < 2 * B; ++c) Pr[c] = prow[c];
+            } else {
+                const double f = Pr[j];
+                double v[2 * B];
+#pragma unroll
+                for (int c = 0; c < 2 * B; ++c) v[c] = Pr[c];
+#pragma unroll
+                for (int c = 0; c < 2 * B; ++c) Pr[c] = v[c] - f * prow[c];
+            }
+        }
+        __syncthreads();
+    }
+    for (int idx = t; idx < bb * bb; idx += GJ_THREADS) Minv[idx] = P[(int64_t)Ss[idx / bb] * ld + b + idx % bb];
+    if (t < bb) {
+        S[t] = Ss[t];
+        pivrow[k0 + t] = Ss[t];
+        ptag[Ss[t]] = panel * 64 + t + 1;
+    }
+    for (int64_t r = t; r < n; r += GJ_THREADS) used[r] = usedL[r];
+}
+
+// Y[jj, c] = sum_i Minv[jj, i] W[S_i, c] for c in [k0, 2n)
+__global__ void gj_rows_kernel(int64_t n, int64_t k0, int b, const int *__restrict__ S,
+                               const double *__restrict__ Minv, const double *__restrict__ W,
+                               double *__restrict__ Y) {
+    const int64_t w = 2 * n - k0;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= b * w) return;
+    const int64_t jj = idx / w, c = k0 + idx % w;
+    double s = 0.0;
+    for (int i = 0; i < b; ++i) s += Minv[jj * b + i] * W[(int64_t)S[i] * 2 * n + c];
+    Y[jj * 2 * n + c] = s;
+}
+
+// W[r, c] -= L[r, :] Y[:, c] (pivot rows of this panel: W[S_jj, c] = Y[jj, c]), c in [k0, 2n);
+// the columns left of k0 are already eliminated and Y is zero there.
+__global__ void gj_apply_kernel(int64_t n, int64_t k0, int b, int panel, const int *__restrict__ ptag,
+                                const double *__restrict__ L, const double *__restrict__ Y, double *__restrict__ W) {
+    const int64_t w = 2 * n - k0;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * w) return;
+    const int64_t r = idx / w, c = k0 + idx % w;
+    const int tg = ptag[r];
+    if (tg > 0 && (tg - 1) / 64 == panel) {
+        W[r * 2 * n + c] = Y[(int64_t)((tg - 1) % 64) * 2 * n + c];
+        return;
+    }
+    double s = 0.0;
+    for (int jj = 0; jj < b; ++jj) s += L[r * b + jj] * Y[(int64_t)jj * 2 * n + c];
+    W[r * 2 * n + c] -= s;
+}
+
+__global__ void gj_extract_perm_kernel(int64_t n, const int *__restrict__ pivrow, const double *__restrict__ W,
+                                       double *__restrict__ inv) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * n) return;
+    const int64_t c = t / n, q = t % n;
+    inv[t] = W[(int64_t)pivrow[c] * 2 * n + n + q];
+}
+
+size_t gj_panel_lds(int64_t n, int b) {
+    return (size_t)(n * (2 * b + 1) + 2 * GJ_MAXB) * sizeof(double) + n * sizeof(int);
+}
+// Widest panel whose LDS footprint fits (0: none, use the per-pivot path).
+int gj_panel_width(int64_t n) {
+    for (int b = GJ_MAXB; b >= 4; b /= 2)
+        if (gj_panel_lds(n, b) <= 150 * 1024) return b;
+    return 0;
+}
+
 }  // namespace
+
+size_t gj_scratch_ints(int64_t n) { return (size_t)3 * n + GJ_MAXB; }
+size_t gj_scratch_doubles(int64_t n) { return (size_t)n * GJ_MAXB + GJ_MAXB * GJ_MAXB + (size_t)GJ_MAXB * 2 * n; }
 
 void dense_coarse_inverse(int64_t nb, int bs, bool sell, const int64_t *slice_ptr, const int32_t *rowlen,
                           const int32_t *rowptr, const int32_t *colind, const double *A, double *W, double *prow,
-                          double *fcol, double *inv, hipStream_t s) {
+                          double *fcol, int *iw, double *dw, double *inv, hipStream_t s) {
     const int64_t n = nb * bs;
     PF_HIP(hipMemsetAsync(W, 0, sizeof(double) * 2 * n * n, s));
     if (sell)
@@ -436,7 +584,43 @@ void dense_coarse_inverse(int64_t nb, int bs, bool sell, const int64_t *slice_pt
     else
         hipLaunchKernelGGL((dense_from_blocks_kernel<false>), dim3(grid_for(nb, 64)), dim3(64), 0, s, nb, bs,
                            slice_ptr, rowlen, rowptr, colind, A, n, W);
-    hipLaunchKernelGGL(dense_prepare_kernel, dim3(grid_for(n, 64)), dim3(64), 0, s, n, W);
+    gauss_jordan_inverse(n, W, prow, fcol, iw, dw, inv, s);
+}
+
+void gauss_jordan_inverse(int64_t n, double *W, double *prow, double *fcol, int *iw, double *dw, double *inv,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(dense_prepare_kernel, dim3((unsigned)n), dim3(64), 0, s, n, W);
+    const int b = gj_panel_width(n);
+    if (b > 0) {
+        // scratch: used, ptag, pivrow (n ints each), S (b ints), L (n x b), Minv (b x b), Y (b x 2n)
+        int *used = iw, *ptag = iw + n, *pivrow = iw + 2 * n, *S = iw + 3 * n;
+        double *L = dw, *Minv = dw + n * GJ_MAXB, *Y = Minv + GJ_MAXB * GJ_MAXB;
+        PF_HIP(hipMemsetAsync(iw, 0, sizeof(int) * 2 * n, s));
+        const size_t lds = gj_panel_lds(n, b);
+        const void *pk = b == 16 ? (const void *)gj_panel_kernel<16>
+                                 : b == 8 ? (const void *)gj_panel_kernel<8> : (const void *)gj_panel_kernel<4>;
+        PF_HIP(hipFuncSetAttribute(pk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        int panel = 0;
+        for (int64_t k0 = 0; k0 < n; k0 += b, ++panel) {
+            const int bb = (int)std::min<int64_t>(b, n - k0);
+            if (b == 16)
+                hipLaunchKernelGGL(gj_panel_kernel<16>, dim3(1), dim3(GJ_THREADS), lds, s, n, k0, bb, panel, W, used,
+                                   ptag, pivrow, L, Minv, S);
+            else if (b == 8)
+                hipLaunchKernelGGL(gj_panel_kernel<8>, dim3(1), dim3(GJ_THREADS), lds, s, n, k0, bb, panel, W, used,
+                                   ptag, pivrow, L, Minv, S);
+            else
+                hipLaunchKernelGGL(gj_panel_kernel<4>, dim3(1), dim3(GJ_THREADS), lds, s, n, k0, bb, panel, W, used,
+                                   ptag, pivrow, L, Minv, S);
+            const int64_t w = 2 * n - k0;
+            hipLaunchKernelGGL(gj_rows_kernel, dim3(grid_for(bb * w, 256)), dim3(256), 0, s, n, k0, bb, S, Minv, W, Y);
+            hipLaunchKernelGGL(gj_apply_kernel, dim3(grid_for(n * w, 256)), dim3(256), 0, s, n, k0, bb, panel, ptag,
+                               L, Y, W);
+        }
+        hipLaunchKernelGGL(gj_extract_perm_kernel, dim3(grid_for(n * n, 256)), dim3(256), 0, s, n, pivrow, W, inv);
+        PF_HIP(hipGetLastError());
+        return;
+    }
     for (int64_t k = 0; k < n; ++k) {
         hipLaunchKernelGGL(gj_pivot_kernel, dim3(1), dim3(1024), 0, s, n, k, W, prow, fcol);
         hipLaunchKernelGGL(gj_update_kernel, dim3(grid_for(2 * n * n, 256)), dim3(256), 0, s, n, k, W, prow, fcol);

@@ -406,10 +406,14 @@ struct LearnedPc {
             block_jacobi3<double>(c->nnodes, c->net_dinv.p, r, z, s);
             return;
         }
+        // g minimises ||r - g K n|| (g = (Kn)'r / (Kn)'(Kn)), so the correction never
+        // increases the residual the Jacobi sweep then works on, whatever the network
+        // returns; the Galerkin choice r'n / n'Kn blew up when n'Kn was small.
         double *nv = c->w_t.p, *q = c->w_v.p, *r1 = c->w_rh.p;
         net_apply(c->net, r, c->bcflag.p, nv, s);
-        spmv_fine64_dot(c->K, nv, q, c->partial.p, sc + 10, s);  // q = K n, sc[10] = n'Kn
-        dot_dev<double, double>(n, r, nv, c->partial.p, sc + 11, s);  // sc[11] = r'n
+        spmv_fine64(c->K, nv, q, s);                                 // q = K n
+        dot_dev<double, double>(n, q, q, c->partial.p, sc + 10, s);  // sc[10] = q'q
+        dot_dev<double, double>(n, q, r, c->partial.p, sc + 11, s);  // sc[11] = q'r
         PF_HIP(hipMemcpyAsync(r1, r, n * sizeof(double), hipMemcpyDeviceToDevice, s));
         axpy_safe_ratio(n, sc + 11, sc + 10, -1.0, q, r1, s);  // r1 = r - g K n
         block_jacobi3<double>(c->nnodes, c->net_dinv.p, r1, z, s);
@@ -817,6 +821,27 @@ int pfem_element_batch(int32_t kind, const pfem_material *mat, int64_t nelem, co
         return PFEM_OK;
     } catch (const hip_failure &e) {
         if (fail) fail->hip = (int32_t)e.code;
+        return PFEM_ERR_HIP;
+    }
+}
+
+int pfem_dense_inverse(int64_t n, const double *A, double *inv) {
+    if (n <= 0 || !A || !inv) return PFEM_ERR_BAD_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PFEM_ERR_NO_DEVICE;
+    try {
+        hipStream_t s;
+        PF_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        struct SG { hipStream_t s; ~SG() { (void)hipStreamDestroy(s); } } sg{s};
+        dbuf<double> W((size_t)2 * n * n), prow(2 * n), fcol(n), out((size_t)n * n), dw(gj_scratch_doubles(n));
+        dbuf<int> iw(gj_scratch_ints(n));
+        PF_HIP(hipMemcpy2DAsync(W.p, 2 * n * sizeof(double), A, n * sizeof(double), n * sizeof(double), n,
+                                hipMemcpyHostToDevice, s));
+        gauss_jordan_inverse(n, W.p, prow.p, fcol.p, iw.p, dw.p, out.p, s);
+        out.download(inv, (size_t)n * n, s);
+        PF_HIP(hipStreamSynchronize(s));
+        return PFEM_OK;
+    } catch (const hip_failure &) {
         return PFEM_ERR_HIP;
     }
 }
