@@ -272,6 +272,10 @@ Amg *amg_create(pfem_ctx *c) {
         L.Q.alloc(std::max<size_t>(1, Qidx.size() * bs * 6));
         up(L.C_row, Crow, s);
         c->sync();
+        if (c->opt.verbose)
+            fprintf(stderr, "[amg] level %d: %lld block rows -> %lld aggregates; blocks P %lld, Q = AP %lld, "
+                            "coarse %lld\n", lev, (long long)L.nb, (long long)nagg, (long long)L.nP, (long long)L.nQ,
+                    (long long)Cidx.size());
         // the next level's graph is the coarse pattern
         Graph gn;
         gn.nb = nagg;

@@ -93,7 +93,7 @@ __global__ void block_inverse_kernel(int64_t nb, const int32_t *__restrict__ dia
 // P block (i, J) = Pt_i [if J = agg(i)] - omega * Dinv_i * sum_{k: agg(col k) = J} A_ik Pt_{col k};
 // one thread per P block, accumulated in registers.
 template <int BS, bool SELL>
-__global__ void smooth_P_kernel(int64_t nP, const int32_t *__restrict__ P_row, const int32_t *__restrict__ P_colind,
+__global__ __launch_bounds__(128) void smooth_P_kernel(int64_t nP, const int32_t *__restrict__ P_row, const int32_t *__restrict__ P_colind,
                                 const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ rowlen,
                                 const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colind,
                                 const double *__restrict__ A, const int32_t *__restrict__ agg,
@@ -140,7 +140,7 @@ __global__ void smooth_P_kernel(int64_t nP, const int32_t *__restrict__ P_row, c
 
 // Q block (i, J) = sum_k A_ik P_{col k, J}
 template <int BS, bool SELL>
-__global__ void AP_kernel(int64_t nQ, const int32_t *__restrict__ Q_row, const int32_t *__restrict__ Q_colind,
+__global__ __launch_bounds__(128) void AP_kernel(int64_t nQ, const int32_t *__restrict__ Q_row, const int32_t *__restrict__ Q_colind,
                           const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ rowlen,
                           const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colind,
                           const double *__restrict__ A, const int32_t *__restrict__ P_rowptr,
@@ -174,7 +174,7 @@ __global__ void AP_kernel(int64_t nQ, const int32_t *__restrict__ Q_row, const i
 
 // Coarse block (I, J) = sum_{m in R row I} R_m Q_{R_col m, J}
 template <int BS>
-__global__ void RAP_kernel(int64_t nC, const int32_t *__restrict__ C_row, const int32_t *__restrict__ C_colind,
+__global__ __launch_bounds__(128) void RAP_kernel(int64_t nC, const int32_t *__restrict__ C_row, const int32_t *__restrict__ C_colind,
                            const int32_t *__restrict__ R_rowptr, const int32_t *__restrict__ R_colind,
                            const double *__restrict__ R, const int32_t *__restrict__ Q_rowptr,
                            const int32_t *__restrict__ Q_colind, const double *__restrict__ Q,
