@@ -100,8 +100,9 @@ typedef struct pfem_options {
                                    reserved[4], reserved[5]: power-iteration steps of the
                                    rho(D^-1 A) estimate behind the Chebyshev bounds, first
                                    set-up and warm-started rebuilds (<= 0: 25 and 4);
-                                   reserved[6]: 1 = build the pattern, slots and colours
-                                   on the host (pfem_create) */
+                                   reserved[6]: bit 0 = build the pattern, slots and
+                                   colours on the host (pfem_create); bit 1 = run the
+                                   V-cycle eagerly instead of replaying it as a HIP graph */
 } pfem_options;
 
 typedef struct pfem_fail {
@@ -129,7 +130,7 @@ int pfem_box_mesh(double lx, double ly, double lz, int64_t nx, int64_t ny, int64
  * on the device, colours the elements (the parity colours of Assembly.jl:172-188
  * for box numbering, on the device; greedy first-fit in element order otherwise,
  * on the host), and resets the Gauss state (Mesh.jl:132-138).  The pattern is
- * built on the host instead when opts->reserved[6] == 1 or when a node is shared
+ * built on the host instead when bit 0 of opts->reserved[6] is set or when a node is shared
  * by more than 32 elements. */
 int pfem_create(const pfem_mesh_desc *mesh, const pfem_options *opts, pfem_ctx **out);
 void pfem_destroy(pfem_ctx *ctx);

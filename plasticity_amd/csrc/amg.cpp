@@ -310,7 +310,21 @@ Amg *amg_create(pfem_ctx *c) {
     return a;
 }
 
-void amg_destroy(Amg *a) { delete a; }
+namespace {
+void graph_reset(Amg *a) {
+    for (int k = 0; k < 2; ++k)
+        if (a->gexec[k]) {
+            (void)hipGraphExecDestroy(a->gexec[k]);
+            a->gexec[k] = nullptr;
+        }
+}
+}  // namespace
+
+void amg_destroy(Amg *a) {
+    if (!a) return;
+    graph_reset(a);
+    delete a;
+}
 
 // ---------------------------------------------------------------- numeric set-up
 namespace {
@@ -361,6 +375,7 @@ void coarse_inverse(pfem_ctx *c, Amg *a) {
 }  // namespace
 
 void amg_numeric_setup(pfem_ctx *c, Amg *a) {
+    graph_reset(a);  // smoother coefficients change with rho: captured V-cycles are stale
     hipStream_t s = c->stream;
     const bool want32 = c->opt.precision == PFEM_PREC_FP32_IR;
     double t_rho = 0.0;
@@ -433,6 +448,7 @@ void amg_numeric_setup(pfem_ctx *c, Amg *a) {
 // Fine-level refresh: the level-0 smoother follows the current K (block-diagonal inverse,
 // warm-started rho), the interpolation and the coarse operators stay as last built.
 void amg_refresh_fine(pfem_ctx *c, Amg *a) {
+    graph_reset(a);
     hipStream_t s = c->stream;
     AmgLevel &L = a->L[0];
     amg_block_inverse<3>(L.nb, c->K.diagpos.p, c->K.v64.p, L.dinv.p, s);
@@ -566,7 +582,43 @@ void vcycle(pfem_ctx *c, Amg *a, int l, const T *b, T *x) {
 
 }  // namespace
 
-void amg_apply64(pfem_ctx *c, Amg *a, const double *r, double *z) { vcycle<double>(c, a, 0, r, z); }
-void amg_apply32(pfem_ctx *c, Amg *a, const float *r, float *z) { vcycle<float>(c, a, 0, r, z); }
+namespace {
+// The V-cycle is a fixed sequence of ~50 launches (most of them small, on the coarse
+// levels) between rebuilds: capture it once on the context's stream and replay it as
+// one graph launch.  reserved[6] bit 1 runs it eagerly instead.  It records kernels and
+// device-to-device copies only (no memset nodes).
+template <class T>
+void apply_vcycle(pfem_ctx *c, Amg *a, int k, const T *r, T *z) {
+    if (c->opt.reserved[6] & 2) {
+        vcycle<T>(c, a, 0, r, z);
+        return;
+    }
+    if (a->gexec[k] && (a->g_in[k] != r || a->g_out[k] != z)) {
+        (void)hipGraphExecDestroy(a->gexec[k]);
+        a->gexec[k] = nullptr;
+    }
+    if (!a->gexec[k]) {
+        hipGraph_t g = nullptr;
+        PF_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+        try {
+            vcycle<T>(c, a, 0, r, z);
+        } catch (...) {
+            (void)hipStreamEndCapture(c->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        PF_HIP(hipStreamEndCapture(c->stream, &g));
+        const hipError_t e = hipGraphInstantiate(&a->gexec[k], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        PF_HIP(e);
+        a->g_in[k] = r;
+        a->g_out[k] = z;
+    }
+    PF_HIP(hipGraphLaunch(a->gexec[k], c->stream));
+}
+}  // namespace
+
+void amg_apply64(pfem_ctx *c, Amg *a, const double *r, double *z) { apply_vcycle<double>(c, a, 0, r, z); }
+void amg_apply32(pfem_ctx *c, Amg *a, const float *r, float *z) { apply_vcycle<float>(c, a, 1, r, z); }
 
 }  // namespace pfem

@@ -54,6 +54,10 @@ struct AmgLevel {
 
 struct Amg {
     std::vector<AmgLevel> L;
+    // V-cycle replayed as a HIP graph ([0] FP64, [1] FP32), captured at the first
+    // application after a (re)build for one (input, output) pair
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    const void *g_in[2] = {nullptr, nullptr}, *g_out[2] = {nullptr, nullptr};
     // coarsest level: dense inverse (FP64)
     int64_t nc = 0;
     dbuf<double> coarse_inv;

@@ -161,9 +161,9 @@ int pfem_create(const pfem_mesh_desc *md, const pfem_options *opts, pfem_ctx **o
         c->mats.alloc(dm.size()); c->mats.upload(dm.data(), dm.size(), s);
 
         // ---- pattern (Assembly.jl:62-162) as SELL-64 BSR, scatter slots (on the device
-        //      unless reserved[6] == 1 or a node is too dense for the device build)
+        //      unless reserved[6] bit 0 is set or a node is too dense for the device build)
         const double tb0 = now();
-        const bool host_build = c->opt.reserved[6] == 1;
+        const bool host_build = (c->opt.reserved[6] & 1) != 0;
         FineMatrix &K = c->K;
         DevicePattern P;
         const int brc = host_build ? BUILD_TOO_DENSE : build_pattern_device(nn, ne, c->conn.p, P, s);

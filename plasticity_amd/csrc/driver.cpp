@@ -391,8 +391,8 @@ struct Amg64Pc {
     void operator()(const double *r, double *z) const { amg_apply64(c, c->amg.get(), r, z); }
 };
 // The learned preconditioner, a two-level method whose coarse correction is the
-// network: n = N(r) (constrained DOFs 0), scaled by the exact line search
-// g = r'n / n'Kn (0 if n'Kn <= 0), then one block-Jacobi sweep on what is left:
+// network: n = N(r) (constrained DOFs 0), scaled by g = (Kn)'r / (Kn)'(Kn) (0 if
+// Kn = 0), then one block-Jacobi sweep on what is left:
 //   z = g n + D^-1 (r - g K n).
 // Without a network (attach mode 3) it is block-Jacobi alone, z = D^-1 r: the same
 // method without its learned part.
