@@ -202,6 +202,7 @@ typedef struct pfem_newton_result {
     int32_t solver_used[64];
     double residuals[64];
     double t_assemble, t_bc, t_setup, t_solve, t_total; /* seconds */
+    double eta[64];             /* Eisenstat-Walker forcing term = Krylov rtol per iteration (Solver.jl:371) */
 } pfem_newton_result;
 
 /* One load step of newton! at load factor lambda (Solver.jl:317-389). */

@@ -56,7 +56,8 @@ class NewtonResult(C.Structure):
     _fields_ = [("converged", C.c_int32), ("iters", C.c_int32),
                 ("krylov_iters", C.c_int32 * 64), ("solver_used", C.c_int32 * 64),
                 ("residuals", C.c_double * 64), ("t_assemble", C.c_double), ("t_bc", C.c_double),
-                ("t_setup", C.c_double), ("t_solve", C.c_double), ("t_total", C.c_double)]
+                ("t_setup", C.c_double), ("t_solve", C.c_double), ("t_total", C.c_double),
+                ("eta", C.c_double * 64)]
 
 
 class Stats(C.Structure):

@@ -679,6 +679,7 @@ int pfem_newton(pfem_ctx *c, double lambda, double tol, int maxiter, pfem_newton
             }
             if (it == maxiter) break;
             const double eta = forcing_term(c->opt, it, rnorm, rkm1, eta_prev);  // :371
+            res->eta[it - 1] = eta;
             // :374 rebuilds once per load step; amg_rebuild = 1 rebuilds from every
             // Newton iteration's tangent (the plastic iterations then need about half the
             // Krylov iterations of a hierarchy built from the elastic predictor tangent);

@@ -238,3 +238,35 @@ def test_vcycle_graph_replay_equals_eager_launches(precision, need_gpu):
     for (a, ia, ua), (b, ib, ub) in zip(out[True], out[False]):
         assert ia == ib and ua == ub
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("gamma,alpha", [(0.9, 1.5), (0.5, 2.0)])
+def test_eisenstat_walker_forcing_terms_match_oracle(gamma, alpha, need_gpu):
+    """The Krylov tolerance of every Newton iteration (Solver.jl:371) is the oracle's
+    forcing term applied to the device's own residual history, starting from
+    r_0 = 0 and eta_0 = eta_max (Solver.jl:338-339, 384-385)."""
+    import oracle as O
+    mesh = pa.box_mesh(1, 1, 1, 5, 5, 5, jitter=0.1, seed=1)
+    d, v, r = uniaxial_constraints(mesh, 0.03)
+    o = pa.default_options(coarse_max=60)
+    o.ew_gamma, o.ew_alpha = gamma, alpha
+    m = pa.Model(mesh, pa.J2Material(**MAT), options=o)
+    m.constrain(d, v, r)
+    m.reset()
+    m._freeze()
+    seen = 0
+    for n in (1, 2, 3):
+        conv, hist, kry, res = m.newton(n / 3, tol=1e-10)
+        assert conv
+        m.commit()
+        eta_prev, rkm1 = o.eta_max, 0.0
+        for it in range(1, len(hist)):
+            want = O.forcing_term(it, hist[it - 1], rkm1, eta_prev, eta_min=o.eta_min, eta_max=o.eta_max,
+                                  gamma=gamma, alpha=alpha)
+            assert res.eta[it - 1] == pytest.approx(want, rel=1e-12, abs=0)
+            rkm1, eta_prev = hist[it - 1], want
+            seen += 1
+        print(f"[EW gamma {gamma} alpha {alpha}] step {n}: |R| {['%.2e' % h for h in hist]} "
+              f"eta {['%.2e' % res.eta[i] for i in range(len(hist) - 1)]} krylov {kry}", flush=True)
+        assert all(res.eta[i] == 0.0 for i in range(len(hist) - 1, 64))
+    assert seen >= 6
