@@ -98,6 +98,19 @@ def cpu_baseline(newton_per_step, krylov_per_newton, n_sample=50):
     }
 
 
+def measured_peaks():
+    """Achievable device peaks (pfem_device_peaks): best-of-10 STREAM-style copy of 2 GiB
+    (read + write bytes), a read-only stream of the same 2 GiB and an FP64 FMA loop.  Reported beside the published peaks the
+    roofline fractions use."""
+    import ctypes as C
+    from plasticity_amd import _lib as L
+    gbs, rd, tf = C.c_double(), C.c_double(), C.c_double()
+    L.check(L.lib().pfem_device_peaks(2 << 30, 10, C.byref(gbs), C.byref(rd), C.byref(tf)), "pfem_device_peaks")
+    return {"copy_GBps": gbs.value, "copy_frac_of_hbm_peak": gbs.value / HBM_PEAK_GBS,
+            "read_GBps": rd.value, "read_frac_of_hbm_peak": rd.value / HBM_PEAK_GBS,
+            "fp64_fma_TFLOPs": tf.value, "fma_frac_of_fp64_peak": tf.value / FP64_PEAK_TFLOPS}
+
+
 def copy_calibration(nbytes=1 << 30, reps=10):
     """Plain device-to-device copy of a known size through the HIP runtime (its blit
     kernel reads nbytes and writes nbytes per call): the reference point for the
@@ -256,6 +269,9 @@ def main():
         log(f"[bench] vs FP64 path: displacement rel err {compare['u_rel_err']:.2e}, "
             f"abar rel err {compare['abar_rel_err']:.2e}")
         m64.close()
+    peaks = measured_peaks()
+    log(f"[bench] measured peaks: copy {peaks['copy_GBps']:.0f} GB/s, read {peaks['read_GBps']:.0f} GB/s, "
+        f"FP64 FMA {peaks['fp64_fma_TFLOPs']:.1f} TFLOP/s")
     cpu = None
     if not args.no_cpu and args.workload == 1:
         try:
@@ -282,6 +298,7 @@ def main():
                      "traffic": SPMV_TRAFFIC_100 if (args.workload == 1 and ndof == 3090903) else None,
                      "bytes_per_launch": bytes_sp, "avg_launch_ms": t_sp * 1e3},
         "cpu_baseline": cpu,
+        "measured_peaks": peaks,
         "element_roofline": {"kernel": "gp_kernel + scatter_kernel (Hex8 F-bar integration and tangent "
                                        "contraction, FP64)", "bound": "fp64",
                              "achieved": ELEMENT_FLOPS * wl.mesh.nelem / ((gp_ms + sc_ms) * 1e-3) / 1e12,

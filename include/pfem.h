@@ -253,6 +253,13 @@ int pfem_get_stats(pfem_ctx *ctx, pfem_stats *st);
  * reference; exported so the coarse solve can be checked on its own. */
 int pfem_dense_inverse(int64_t n, const double *A, double *inv);
 
+/* Achievable device peaks for the rooflines: the best of `reps` launches of a
+ * STREAM-style copy of `copy_bytes` (>= 1 MiB; read + write bytes counted) and of a
+ * read-only stream over the same bytes, in GB/s (read_gbs may be NULL), and of an
+ * FP64 FMA loop (8 independent chains per lane) in TFLOP/s.  Allocates 2 x copy_bytes
+ * of device memory for the call.  No counterpart in the reference. */
+int pfem_device_peaks(int64_t copy_bytes, int reps, double *copy_gbs, double *read_gbs, double *fp64_tflops);
+
 /* Time `reps` fine-level SpMVs y = K x (FP64 if prec == 0, FP32 if 1) on the
  * context's stream with events; returns mean seconds per SpMV. */
 int pfem_bench_spmv(pfem_ctx *ctx, int prec, int reps, double *sec_per_spmv);

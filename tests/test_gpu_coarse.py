@@ -1,4 +1,4 @@
-"""Coarsest-level AMG solve: the device Gauss-Jordan inverse (blocked panels of 32/16/8/4
+"""Coarsest-level AMG solve: the device Gauss-Jordan inverse (blocked panels of 16/8/4
 pivots while a panel fits in LDS, one launch pair per pivot beyond that) against LAPACK,
 including a matrix that needs pivoting in every column and an all-zero row (decoupled
 with a unit diagonal)."""
