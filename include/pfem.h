@@ -87,8 +87,10 @@ typedef struct pfem_options {
     int32_t max_levels;         /* AMG level cap (default 10) */
     int32_t verbose;            /* print per-iteration lines to stderr */
     int32_t reserved[7];        /* reserved[0]: AMG rebuild policy, 0 = once per load step
-                                   (Solver.jl:374), 1 = every Newton iteration, 2 = at the
-                                   first two Newton iterations of a load step;
+                                   (Solver.jl:374; default), 1 = every Newton iteration,
+                                   2 = at the first two Newton iterations of a load step,
+                                   3 = at the first, then only the fine-level smoother
+                                   from each new tangent;
                                    reserved[1]: directions kept by the flexible GCR of the
                                    learned preconditioner (<= 0: 16);
                                    reserved[2]: FP32-IR only, 1 = the fine-level products

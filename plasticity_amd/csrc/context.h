@@ -59,6 +59,7 @@ struct pfem_ctx {
     pfem::dbuf<double> w_r, w_z, w_p, w_q, w_x, w_s, w_t, w_v, w_rh;
     pfem::dbuf<float> f_r, f_z, f_p, f_q, f_x, f_b, f_rh, f_v, f_s, f_t;
     pfem::dbuf<double> partial, scal;
+    pfem::dbuf<double> mpartial;      // partials of the fused multi-dot (flexible GCR)
     double *h_scal = nullptr;  // pinned
 
     // preconditioner

@@ -305,11 +305,12 @@ static int fpcg(pfem_ctx *c, const double *b, double *x, double rtol, int itmax,
 // product, then alpha = r'q / q'q minimises ||r|| along p.  It needs neither a
 // symmetric K (the F-bar tangent is not symmetric) nor a linear preconditioner (a
 // network), and costs one SpMV per iteration.  Stops on ||r|| <= rtol ||b||;
-// 0 solved, 1 itmax, 2 breakdown (q = 0), 3 stall (no 0.1% gain of the best residual
-// over `stall` iterations).
+// 0 solved, 1 itmax, 2 breakdown (q = 0), 3 stall: no 0.1% gain of the best residual
+// over `stall` iterations, or (slow_window > 0) the last slow_window iterations
+// together reduced ||r|| by less than slow_rate^slow_window.
 template <class PC>
 static int gcr_m(pfem_ctx *c, const double *b, double *x, double rtol, int itmax, int m, int stall, int *iters,
-                 PC &&pcfun) {
+                 PC &&pcfun, int slow_window = 0, double slow_rate = 1.0) {
     hipStream_t s = c->stream;
     const int64_t n = c->ndof;
     const int slots = m + 1;
@@ -325,7 +326,9 @@ static int gcr_m(pfem_ctx *c, const double *b, double *x, double rtol, int itmax
     *iters = 0;
     if (bn == 0.0) return 0;
     const double eps = rtol * bn;
-    std::vector<double> qq(slots, 0.0);
+    std::vector<double> qq(slots, 0.0), beta(m, 0.0), hist(1, bn);
+    std::vector<const double *> vp(m, nullptr), vq(m, nullptr);
+    if (c->mpartial.n < multi_dot_partials()) c->mpartial.alloc(multi_dot_partials());
     double best = bn;
     int best_it = 0;
     for (int it = 1; it <= itmax; ++it) {
@@ -337,22 +340,22 @@ static int gcr_m(pfem_ctx *c, const double *b, double *x, double rtol, int itmax
         PF_HIP(hipEventRecord(c->ev0, s));
         spmv_fine64(c->K, pk, qk, s);  // q = K z
         PF_HIP(hipEventRecord(c->ev1, s));
+        // classical Gram-Schmidt against the kept directions: all (K z)' (K p_j) in one
+        // pass, then p -= sum beta_j p_j and K p -= sum beta_j K p_j in one pass
         for (int j = 1; j <= nold; ++j) {
             const int sj = (k - j + slots) % slots;
-            dot_dev<double, double>(n, qk, c->fcg_Q.p + (size_t)sj * n, c->partial.p, sc + 16 + j, s);
+            vq[j - 1] = c->fcg_Q.p + (size_t)sj * n;
+            vp[j - 1] = c->fcg_P.p + (size_t)sj * n;
         }
         if (nold > 0) {
-            PF_HIP(hipMemcpyAsync(c->h_scal + 16, sc + 16, (nold + 1) * sizeof(double), hipMemcpyDeviceToHost, s));
+            multi_dot(n, qk, vq.data(), nold, c->mpartial.p, sc + 17, s);
+            PF_HIP(hipMemcpyAsync(c->h_scal + 17, sc + 17, nold * sizeof(double), hipMemcpyDeviceToHost, s));
             c->sync();
-            for (int j = 1; j <= nold; ++j) {
-                const int sj = (k - j + slots) % slots;
-                const double beta = c->h_scal[16 + j] / qq[sj];
-                axpby<double>(n, -beta, c->fcg_P.p + (size_t)sj * n, 1.0, pk, s);
-                axpby<double>(n, -beta, c->fcg_Q.p + (size_t)sj * n, 1.0, qk, s);
-            }
+            for (int j = 1; j <= nold; ++j) beta[j - 1] = c->h_scal[16 + j] / qq[(k - j + slots) % slots];
+            multi_axpy2(n, vp.data(), vq.data(), beta.data(), nold, pk, qk, s);
         }
-        dot_dev<double, double>(n, qk, qk, c->partial.p, sc + 1, s);
-        dot_dev<double, double>(n, r, qk, c->partial.p, sc + 2, s);
+        const double *qr[2] = {qk, r};
+        multi_dot(n, qk, qr, 2, c->mpartial.p, sc + 1, s);  // sc[1] = q'q, sc[2] = q'r
         PF_HIP(hipMemcpyAsync(c->h_scal, sc, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
         c->sync();
         float ms = 0.f;
@@ -363,12 +366,13 @@ static int gcr_m(pfem_ctx *c, const double *b, double *x, double rtol, int itmax
         if (!(qqk > 0)) return 2;
         qq[k] = qqk;
         const double alpha = rq / qqk;
-        axpby<double>(n, alpha, pk, 1.0, x, s);
-        axpby<double>(n, -alpha, qk, 1.0, r, s);
-        dot_dev<double, double>(n, r, r, c->partial.p, sc + 3, s);
+        update_dot(n, alpha, pk, qk, x, r, c->partial.p, sc + 3, s);  // x += alpha p, r -= alpha K p
         const double rn = std::sqrt(c->sync_read(sc + 3));
         if (c->opt.verbose > 2) fprintf(stderr, "      gcr(%d) %3d  %.3e  (target %.3e)\n", m, it, rn, eps);
         if (rn <= eps) return 0;
+        hist.push_back(rn);
+        if (slow_window > 0 && it >= slow_window && rn > std::pow(slow_rate, slow_window) * hist[it - slow_window])
+            return 3;
         if (rn < 0.999 * best) { best = rn; best_it = it; }
         else if (stall > 0 && it - best_it >= stall) return 3;
     }
@@ -490,8 +494,9 @@ static void prepare_fp32(pfem_ctx *c) {
 }
 
 // GCR(m) with the learned preconditioner; when it stalls (no 0.1% gain in the best
-// residual over 40 iterations), breaks down or hits itmax, AMG solves for the
-// remaining residual b - K x and the correction is added (the progress is kept).
+// residual over 40 iterations, or less than a 0.9^10 reduction over the last 10),
+// breaks down or hits itmax, AMG solves for the remaining residual b - K x and the
+// correction is added (the progress is kept).
 static int learned_solve(pfem_ctx *c, double rtol, int itmax, int *total, int *which) {
     hipStream_t s = c->stream;
     int it = 0;
@@ -499,7 +504,8 @@ static int learned_solve(pfem_ctx *c, double rtol, int itmax, int *total, int *w
     if (!c->net_dinv.p) c->net_dinv.alloc(9 * c->nnodes);
     block_diag_inverse3(c->K, c->net_dinv.p, s);
     const int m = c->opt.reserved[1] > 0 ? std::min(c->opt.reserved[1], 40) : 16;
-    int rc = gcr_m(c, c->rhs.p, c->dU.p, rtol, itmax, m, 40, &it, LearnedPc{c});
+    // hand over to AMG once the last 10 iterations together gained less than 0.9^10
+    int rc = gcr_m(c, c->rhs.p, c->dU.p, rtol, itmax, m, 40, &it, LearnedPc{c}, 10, 0.9);
     *total += it;
     if (c->opt.verbose > 1) fprintf(stderr, "      learned gcr: rc %d after %d its\n", rc, it);
     if (rc == 0) return 0;

@@ -19,6 +19,16 @@ void spmv_fine_half(const FineMatrix &K, const float *x, float *y, hipStream_t s
 void residual_fine_half(const FineMatrix &K, const float *x, const float *b, float *r, hipStream_t s);
 template <class A, class B>
 void dot_dev(int64_t n, const A *a, const B *b, double *partial, double *out, hipStream_t s);
+// fused flexible-GCR vector work: out[j] = a . v[j] (nv dots in passes of 16; partial
+// holds multi_dot_partials() doubles); p -= sum c_j P_j, q -= sum c_j Q_j; and
+// x += a p, r -= a q with out = r . r.  Results equal the separate dot/axpby calls.
+size_t multi_dot_partials();
+void multi_dot(int64_t n, const double *a, const double *const *v, int nv, double *partial, double *out,
+               hipStream_t s);
+void multi_axpy2(int64_t n, const double *const *P, const double *const *Q, const double *c, int nv, double *p,
+                 double *q, hipStream_t s);
+void update_dot(int64_t n, double a, const double *p, const double *q, double *x, double *r, double *partial,
+                double *out, hipStream_t s);
 template <class T>
 void axpy_ratio(int64_t n, const double *num, const double *den, double sign, const T *x, T *y, hipStream_t s);
 template <class T>

@@ -10,6 +10,8 @@
 //  * Block-diagonal (3x3) inverse and block-Jacobi application.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "common.h"
 #include "kernels.h"
 #include "sell.h"
@@ -42,6 +44,7 @@ __device__ __forceinline__ double block_sum(double v) {
 
 // ---------------------------------------------------------------- reductions
 constexpr int RED_BLOCKS = 1024;
+constexpr int MV_MAX = 16;  // vectors per fused Gram-Schmidt pass
 
 template <class A, class B>
 __global__ __launch_bounds__(256) void dot_partial_kernel(int64_t n, const A *__restrict__ a,
@@ -59,6 +62,77 @@ __global__ __launch_bounds__(1024) void finish_sum_kernel(int n, const double *_
     for (int i = threadIdx.x; i < n; i += 1024) s += partial[i];
     s = block_sum(s);
     if (threadIdx.x == 0) *out = s;
+}
+
+// Up to MV_MAX dot products a . v_j in one pass over a (the Gram-Schmidt step of the
+// flexible GCR); per-thread order, grid and finishing sum as dot_partial_kernel /
+// finish_sum_kernel, so each result equals the separate dot product bit for bit.
+struct MultiVec {
+    const double *v[MV_MAX];
+    int nv;
+};
+__global__ __launch_bounds__(256) void multi_dot_partial_kernel(int64_t n, const double *__restrict__ a, MultiVec V,
+                                                                double *__restrict__ partial) {
+    double s[MV_MAX];
+#pragma unroll
+    for (int j = 0; j < MV_MAX; ++j) s[j] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const double ai = a[i];
+#pragma unroll
+        for (int j = 0; j < MV_MAX; ++j)
+            if (j < V.nv) s[j] += ai * V.v[j][i];
+    }
+#pragma unroll
+    for (int j = 0; j < MV_MAX; ++j)
+        if (j < V.nv) {
+            const double t = block_sum(s[j]);
+            if (threadIdx.x == 0) partial[(int64_t)j * gridDim.x + blockIdx.x] = t;
+        }
+}
+// block j sums row j of the partials
+__global__ __launch_bounds__(1024) void multi_finish_kernel(int nblk, const double *__restrict__ partial,
+                                                            double *__restrict__ out) {
+    const double *row = partial + (int64_t)blockIdx.x * nblk;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += 1024) s += row[i];
+    s = block_sum(s);
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+// p -= sum_j c_j P_j and q -= sum_j c_j Q_j in one pass (the j-th step is the axpby
+// y = -c_j x + y, in the same order)
+struct MultiAxpy {
+    const double *P[MV_MAX];
+    const double *Q[MV_MAX];
+    double c[MV_MAX];
+    int nv;
+};
+__global__ __launch_bounds__(256) void multi_axpy2_kernel(int64_t n, MultiAxpy V, double *__restrict__ p,
+                                                          double *__restrict__ q) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double pa = p[i], qa = q[i];
+#pragma unroll
+        for (int j = 0; j < MV_MAX; ++j)
+            if (j < V.nv) {
+                pa = -V.c[j] * V.P[j][i] + pa;
+                qa = -V.c[j] * V.Q[j][i] + qa;
+            }
+        p[i] = pa;
+        q[i] = qa;
+    }
+}
+// x += a p; r -= a q; per-block partials of r . r (grid and order of dot_partial_kernel)
+__global__ __launch_bounds__(256) void update_dot_kernel(int64_t n, double a, const double *__restrict__ p,
+                                                         const double *__restrict__ q, double *__restrict__ x,
+                                                         double *__restrict__ r, double *__restrict__ partial) {
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        x[i] = a * p[i] + x[i];
+        const double ri = -a * q[i] + r[i];
+        r[i] = ri;
+        s += ri * ri;
+    }
+    s = block_sum(s);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
 // ---------------------------------------------------------------- vector ops
@@ -280,6 +354,42 @@ void dot_dev(int64_t n, const A *a, const B *b, double *partial, double *out, hi
 }
 template void dot_dev<double, double>(int64_t, const double *, const double *, double *, double *, hipStream_t);
 template void dot_dev<float, float>(int64_t, const float *, const float *, double *, double *, hipStream_t);
+
+size_t multi_dot_partials() { return (size_t)MV_MAX * RED_BLOCKS; }
+
+void multi_dot(int64_t n, const double *a, const double *const *v, int nv, double *partial, double *out,
+               hipStream_t s) {
+    for (int j0 = 0; j0 < nv; j0 += MV_MAX) {
+        MultiVec V{};
+        V.nv = std::min(MV_MAX, nv - j0);
+        for (int j = 0; j < V.nv; ++j) V.v[j] = v[j0 + j];
+        hipLaunchKernelGGL(multi_dot_partial_kernel, dim3(RED_BLOCKS), dim3(256), 0, s, n, a, V, partial);
+        hipLaunchKernelGGL(multi_finish_kernel, dim3(V.nv), dim3(1024), 0, s, RED_BLOCKS, partial, out + j0);
+    }
+    PF_HIP(hipGetLastError());
+}
+
+void multi_axpy2(int64_t n, const double *const *P, const double *const *Q, const double *c, int nv, double *p,
+                 double *q, hipStream_t s) {
+    for (int j0 = 0; j0 < nv; j0 += MV_MAX) {
+        MultiAxpy V{};
+        V.nv = std::min(MV_MAX, nv - j0);
+        for (int j = 0; j < V.nv; ++j) {
+            V.P[j] = P[j0 + j];
+            V.Q[j] = Q[j0 + j];
+            V.c[j] = c[j0 + j];
+        }
+        hipLaunchKernelGGL(multi_axpy2_kernel, dim3(grid_strided(n)), dim3(256), 0, s, n, V, p, q);
+    }
+    PF_HIP(hipGetLastError());
+}
+
+void update_dot(int64_t n, double a, const double *p, const double *q, double *x, double *r, double *partial,
+                double *out, hipStream_t s) {
+    hipLaunchKernelGGL(update_dot_kernel, dim3(RED_BLOCKS), dim3(256), 0, s, n, a, p, q, x, r, partial);
+    hipLaunchKernelGGL(finish_sum_kernel, dim3(1), dim3(1024), 0, s, RED_BLOCKS, partial, out);
+    PF_HIP(hipGetLastError());
+}
 
 template <class T>
 void axpy_ratio(int64_t n, const double *num, const double *den, double sign, const T *x, T *y, hipStream_t s) {
