@@ -152,9 +152,11 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_ir"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--amg-rebuild", type=int, default=1, help="0: once per load step, 1: every Newton iteration")
-    ap.add_argument("--smoother", default="chebyshev", choices=["chebyshev", "jacobi"])
+    ap.add_argument("--smoother", default="chebyshev", choices=["chebyshev", "chebyshev4", "jacobi"])
     ap.add_argument("--smoother-degree", type=int, default=0,
                     help="Chebyshev degree / Jacobi sweeps (0: library default, 2 / 1)")
+    ap.add_argument("--cycle", default="default", choices=["default", "v", "w"],
+                    help="AMG cycle (default: W under FP32-IR, V under FP64)")
     ap.add_argument("--verbose", type=int, default=0)
     ap.add_argument("--fp16-fine", action="store_true",
                     help="FP32-IR: fine-level smoother products with an FP16 copy of the scaled K")
@@ -189,7 +191,8 @@ def main():
     opts = pa.default_options(eta_min=wl.krylov_rtol, eta_max=wl.krylov_rtol, precision=args.precision,
                               verbose=args.verbose, amg_rebuild=args.amg_rebuild, fp16_fine=args.fp16_fine,
                               line_search=args.line_search, rho_iters=args.rho_iters, smoother=args.smoother,
-                              smoother_degree=args.smoother_degree)
+                              smoother_degree=args.smoother_degree,
+                              wcycle={"default": None, "v": False, "w": True}[args.cycle])
     t0 = time.perf_counter()
     mats = [pa.J2Material(**d) for d in wl.materials]
     m = pa.Model(wl.mesh, mats if len(mats) > 1 else mats[0], options=opts, material_id=wl.material_id)
@@ -288,6 +291,8 @@ def main():
                                f"{'GCR + learned U-net/block-Jacobi, AMG fallback' if net else 'PCG+SA-AMG'} "
                                f"rtol {wl.krylov_rtol:g}, precision {args.precision}, "
                                f"AMG rebuilt {'every Newton iteration' if args.amg_rebuild else 'once per load step'}",
+                   "amg_cycle": ("W" if args.precision == "fp32_ir" else "V") if args.cycle == "default"
+                   else args.cycle.upper(),
                    "newton_steps_timed": newton, "krylov_its_timed": krylov,
                    "t_assemble_s": t_asm, "t_amg_setup_s": t_setup, "t_solve_s": t_solve,
                    "amg_levels": list(s1.amg_ndof[:s1.amg_levels]),

@@ -72,7 +72,8 @@ typedef enum pfem_precision {
 
 typedef enum pfem_smoother {
     PFEM_SMOOTH_JACOBI = 0,     /* damped block-Jacobi, one sweep (Solver.jl:138 Jacobi(2/3; iter=1)) */
-    PFEM_SMOOTH_CHEBYSHEV = 1   /* Chebyshev polynomial in D^-1 A (block diagonal D) */
+    PFEM_SMOOTH_CHEBYSHEV = 1,  /* Chebyshev polynomial in D^-1 A (block diagonal D) on [lmax/30, lmax] */
+    PFEM_SMOOTH_CHEBYSHEV4 = 2  /* fourth-kind Chebyshev smoother in D^-1 A (Lottes 2022), no lower bound */
 } pfem_smoother;
 
 /* Linear-solve options (LinearSolveState, Solver.jl:86-111). */
@@ -104,7 +105,9 @@ typedef struct pfem_options {
                                    set-up and warm-started rebuilds (<= 0: 25 and 4);
                                    reserved[6]: bit 0 = build the pattern, slots and
                                    colours on the host (pfem_create); bit 1 = run the
-                                   V-cycle eagerly instead of replaying it as a HIP graph */
+                                   V-cycle eagerly instead of replaying it as a HIP graph;
+                                   bit 4 = AMG W-cycle, bit 5 = AMG V-cycle (default:
+                                   W under FP32-IR, V under FP64) */
 } pfem_options;
 
 typedef struct pfem_fail {

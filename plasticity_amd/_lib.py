@@ -29,6 +29,7 @@ PREC_FP64 = 0
 PREC_FP32_IR = 1
 SMOOTH_JACOBI = 0
 SMOOTH_CHEBYSHEV = 1
+SMOOTH_CHEBYSHEV4 = 2
 
 
 class Material(C.Structure):

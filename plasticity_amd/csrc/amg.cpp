@@ -158,7 +158,14 @@ Amg *amg_create(pfem_ctx *c) {
     hipStream_t s = c->stream;
     auto *a = new Amg();
     a->smoother = c->opt.smoother;
-    a->degree = c->opt.smoother_degree > 0 ? c->opt.smoother_degree : (a->smoother == PFEM_SMOOTH_CHEBYSHEV ? 2 : 1);
+    // W-cycle by default under FP32-IR (its flexible CG needs 20 instead of 28 iterations
+    // per solve at 100^3 for about 10% more work per application); V-cycle under FP64,
+    // whose PCG stops on the preconditioned norm like Krylov.jl's cg! and then needed
+    // more Newton iterations with the W-cycle.  reserved[6] bit 4 forces W, bit 5 V.
+    a->wcycle = (c->opt.reserved[6] & 16) ? true
+                : (c->opt.reserved[6] & 32) ? false
+                                            : c->opt.precision == PFEM_PREC_FP32_IR;
+    a->degree = c->opt.smoother_degree > 0 ? c->opt.smoother_degree : (a->smoother == PFEM_SMOOTH_JACOBI ? 1 : 2);
     const int64_t coarse_max = c->opt.coarse_max > 0 ? c->opt.coarse_max : 1500;
     const int max_levels = c->opt.max_levels > 1 ? c->opt.max_levels : 10;
 
@@ -206,6 +213,7 @@ Amg *amg_create(pfem_ctx *c) {
             L.A.alloc(std::max<int64_t>(1, L.nnzb * 36));
         }
         L.x.alloc(L.n); L.b.alloc(L.n); L.r.alloc(L.n); L.d.alloc(L.n); L.t.alloc(L.n);
+        L.wr.alloc(L.n); L.we.alloc(L.n);
         L.dinv.alloc((size_t)L.nb * bs * bs);
         if (L.n <= coarse_max || lev + 1 >= max_levels) break;
         std::vector<int32_t> agg;
@@ -427,6 +435,7 @@ void amg_numeric_setup(pfem_ctx *c, Amg *a) {
             AmgLevel &L = a->L[l];
             if (!L.x32.p) {
                 L.x32.alloc(L.n); L.b32.alloc(L.n); L.r32.alloc(L.n); L.d32.alloc(L.n); L.t32.alloc(L.n);
+                L.wr32.alloc(L.n); L.we32.alloc(L.n);
             }
             if (!L.dinv32.p) L.dinv32.alloc(L.dinv.n);
             convert<double, float>((int64_t)L.dinv.n, L.dinv.p, L.dinv32.p, s);
@@ -474,6 +483,8 @@ template <> struct Vec<double> {
     static double *r(AmgLevel &L) { return L.r.p; }
     static double *d(AmgLevel &L) { return L.d.p; }
     static double *t(AmgLevel &L) { return L.t.p; }
+    static double *wr(AmgLevel &L) { return L.wr.p; }
+    static double *we(AmgLevel &L) { return L.we.p; }
     static const double *A(AmgLevel &L) { return L.A.p; }
     static const double *P(AmgLevel &L) { return L.P.p; }
     static const double *R(AmgLevel &L) { return L.R.p; }
@@ -486,6 +497,8 @@ template <> struct Vec<float> {
     static float *r(AmgLevel &L) { return L.r32.p; }
     static float *d(AmgLevel &L) { return L.d32.p; }
     static float *t(AmgLevel &L) { return L.t32.p; }
+    static float *wr(AmgLevel &L) { return L.wr32.p; }
+    static float *we(AmgLevel &L) { return L.we32.p; }
     static const float *A(AmgLevel &L) { return L.A32.p; }
     static const float *P(AmgLevel &L) { return L.P32.p; }
     static const float *R(AmgLevel &L) { return L.R32.p; }
@@ -523,7 +536,8 @@ void diag_apply(AmgLevel &L, const T *r, T *z, double scale, hipStream_t s) {
 
 // Polynomial smoother on level l: x <- x + p(D^-1 A) D^-1 (b - A x).
 // Chebyshev on [lmax/30, lmax] with lmax = 1.1 rho (Adams, Brezina, Hu & Tuminaro,
-// J. Comput. Phys. 188, 2003), or damped block Jacobi with w = 4/(3 rho).
+// J. Comput. Phys. 188, 2003), fourth-kind Chebyshev on (0, lmax], or damped block
+// Jacobi with w = 4/(3 rho).
 template <class T>
 void smooth(pfem_ctx *c, Amg *a, int l, const T *b, T *x, bool zero_init) {
     AmgLevel &L = a->L[l];
@@ -539,6 +553,23 @@ void smooth(pfem_ctx *c, Amg *a, int l, const T *b, T *x, bool zero_init) {
             level_residual<T>(c, L, l, x, b, r);
             diag_apply<T>(L, r, d, w, s);
             axpby<T>(L.n, 1.0, d, 1.0, x, s);
+        }
+        return;
+    }
+    if (a->smoother == PFEM_SMOOTH_CHEBYSHEV4) {
+        // fourth-kind Chebyshev (Lottes, Numer. Linear Algebra Appl. 30, 2023, Alg. 1):
+        // d_0 = 4/(3 lmax) D^-1 r_0, d_i = (2i-1)/(2i+3) d_{i-1} + (8i+4)/((2i+3) lmax) D^-1 r_i
+        const double lam = 1.1 * L.rho;
+        if (zero_init) PF_HIP(hipMemcpyAsync(r, b, L.n * sizeof(T), hipMemcpyDeviceToDevice, s));
+        else level_residual<T>(c, L, l, x, b, r);
+        diag_apply<T>(L, r, d, 4.0 / (3.0 * lam), s);
+        if (zero_init) PF_HIP(hipMemcpyAsync(x, d, L.n * sizeof(T), hipMemcpyDeviceToDevice, s));
+        else axpby<T>(L.n, 1.0, d, 1.0, x, s);
+        for (int i = 1; i < a->degree; ++i) {
+            level_matvec<T>(c, L, l, d, t);
+            const double c1 = (2.0 * i - 1.0) / (2.0 * i + 3.0), c2 = (8.0 * i + 4.0) / ((2.0 * i + 3.0) * lam);
+            if (L.bs == 3) cheb_step<3, T, T>(L.nb, Vec<T>::Dinv(L), t, r, d, x, c1, c2, s);
+            else cheb_step<6, T, T>(L.nb, Vec<T>::Dinv(L), t, r, d, x, c1, c2, s);
         }
         return;
     }
@@ -575,6 +606,11 @@ void vcycle(pfem_ctx *c, Amg *a, int l, const T *b, T *x) {
     if (L.bs == 3) bsr_spmv_wave<6, 3, T, T>(L.nagg, L.R_rowptr.p, L.R_colind.p, Vec<T>::R(L), Vec<T>::r(L), Vec<T>::b(N), s);
     else bsr_spmv_wave<6, 6, T, T>(L.nagg, L.R_rowptr.p, L.R_colind.p, Vec<T>::R(L), Vec<T>::r(L), Vec<T>::b(N), s);
     vcycle<T>(c, a, l + 1, Vec<T>::b(N), Vec<T>::x(N));
+    if (a->wcycle && (size_t)l + 2 < a->L.size()) {  // W-cycle: a second coarse cycle on its residual
+        level_residual<T>(c, N, l + 1, Vec<T>::x(N), Vec<T>::b(N), Vec<T>::wr(N), false);
+        vcycle<T>(c, a, l + 1, Vec<T>::wr(N), Vec<T>::we(N));
+        axpby<T>(N.n, 1.0, Vec<T>::we(N), 1.0, Vec<T>::x(N), s);
+    }
     if (L.bs == 3) bsr_spmv<3, 6, T, T>(L.nb, L.P_rowptr.p, L.P_colind.p, Vec<T>::P(L), Vec<T>::x(N), x, 1.0, s);
     else bsr_spmv<6, 6, T, T>(L.nb, L.P_rowptr.p, L.P_colind.p, Vec<T>::P(L), Vec<T>::x(N), x, 1.0, s);
     smooth<T>(c, a, l, b, x, false);

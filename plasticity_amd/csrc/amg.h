@@ -50,6 +50,8 @@ struct AmgLevel {
     dbuf<double> pw;             // power-iteration vector, kept between set-ups (warm start)
     bool pw_warm = false;
     dbuf<float> x32, b32, r32, d32, t32;
+    dbuf<double> wr, we;         // W-cycle: coarse residual and correction of the second visit
+    dbuf<float> wr32, we32;
 };
 
 struct Amg {
@@ -65,6 +67,7 @@ struct Amg {
     dbuf<int> gj_iw;
     int smoother = PFEM_SMOOTH_CHEBYSHEV;
     int degree = 2;
+    bool wcycle = false;
     bool have32 = false;
     double op_complexity = 1.0;
 };

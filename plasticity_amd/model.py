@@ -94,13 +94,18 @@ def default_options(**kw):
         if k == "line_search":  # Newton backtracking on ||R||: most step halvings (0 = full steps)
             o.reserved[3] = int(v)
             continue
+        if k == "wcycle":  # AMG cycle: True W, False V, None the default (W under FP32-IR, V under FP64)
+            o.reserved[6] &= ~48
+            if v is not None:
+                o.reserved[6] |= 16 if v else 32
+            continue
         if k == "fp16_fine":  # FP32-IR: smoother products on the fine level with the FP16 scaled K
             o.reserved[2] = 1 if v else 0
             continue
         if k == "precision" and isinstance(v, str):
             v = {"fp64": L.PREC_FP64, "fp32_ir": L.PREC_FP32_IR}[v]
         if k == "smoother" and isinstance(v, str):
-            v = {"jacobi": L.SMOOTH_JACOBI, "chebyshev": L.SMOOTH_CHEBYSHEV}[v]
+            v = {"jacobi": L.SMOOTH_JACOBI, "chebyshev": L.SMOOTH_CHEBYSHEV, "chebyshev4": L.SMOOTH_CHEBYSHEV4}[v]
         setattr(o, k, v)
     return o
 
