@@ -129,6 +129,39 @@ def copy_calibration(nbytes=1 << 30, reps=10):
     log(f"[bench] copy calibration: {reps} x {nbytes} B device-to-device")
 
 
+def time_variant(pa, wl, precision, learned, warmup, steps):
+    """The same load steps of the workload on a fresh model with another linear solver:
+    precision "fp32_ir", learned = path of U-net weights (GCR with the network, AMG
+    fallback) or None.  Returns s/Newton step over the timed load steps and the counts."""
+    opts = pa.default_options(eta_min=wl.krylov_rtol, eta_max=wl.krylov_rtol, precision=precision, amg_rebuild=1)
+    mats = [pa.J2Material(**d) for d in wl.materials]
+    m = pa.Model(wl.mesh, mats if len(mats) > 1 else mats[0], options=opts, material_id=wl.material_id)
+    m.constrain(wl.dir_dofs, wl.dir_vals, wl.dir_ramp)
+    m.reset()
+    m._freeze()
+    if learned:
+        from plasticity_amd.learned import UNet, grid_of
+        net = UNet(grid_of(wl.mesh)).load(learned)
+        m.attach_net(net, "precondition")
+    newton = krylov = 0
+    t0 = None
+    for k in range(1, warmup + steps + 1):
+        if k == warmup + 1:
+            t0 = time.perf_counter()
+        conv, hist, kry, res = m.newton(k / wl.nsteps, tol=wl.newton_tol, maxiter=25)
+        if not conv:
+            raise RuntimeError(f"{precision} learned={bool(learned)}: load step {k} did not converge")
+        m.commit()
+        if k > warmup:
+            newton += res.iters
+            krylov += sum(res.krylov_iters[:res.iters])
+    elapsed = time.perf_counter() - t0
+    if learned:
+        m.attach_net(None, "off")
+    m.close()
+    return {"value": elapsed / newton, "newton_steps": newton, "krylov_its": krylov, "seconds": elapsed}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,6 +184,9 @@ def main():
                     help="precondition with the learned U-net (weights .npz) in flexible GCR, AMG fallback")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_ir"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="FP64 run of workload 1: skip timing the FP32-IR and FP32-IR + learned solvers "
+                         "on the same load steps")
     ap.add_argument("--amg-rebuild", type=int, default=1, help="0: once per load step, 1: every Newton iteration")
     ap.add_argument("--smoother", default="chebyshev", choices=["chebyshev", "chebyshev4", "jacobi"])
     ap.add_argument("--smoother-degree", type=int, default=0,
@@ -272,6 +308,20 @@ def main():
         log(f"[bench] vs FP64 path: displacement rel err {compare['u_rel_err']:.2e}, "
             f"abar rel err {compare['abar_rel_err']:.2e}")
         m64.close()
+    variants = None
+    if (args.workload == 1 and args.precision == "fp64" and not args.learned and not args.no_variants
+            and not args.compare):
+        # the metric compares the FP64 drop-in with FP32-IR + the learned preconditioner:
+        # time both on the same load steps, each on a fresh model
+        m.close()
+        weights = os.path.join(ROOT, "plasticity_amd", "weights", "unet_16_32_64_64.npz")
+        variants = {"fp32_ir": time_variant(pa, wl, "fp32_ir", None, args.warmup, args.steps)}
+        if os.path.exists(weights):
+            variants["fp32_ir_learned"] = time_variant(pa, wl, "fp32_ir", weights, args.warmup, args.steps)
+        for k, v in variants.items():
+            v["speedup_vs_fp64"] = per_newton / v["value"]
+            log(f"[bench] {k}: {v['value']:.4f} s/Newton step ({v['newton_steps']} Newton, {v['krylov_its']} "
+                f"Krylov), {v['speedup_vs_fp64']:.2f}x the FP64 path")
     peaks = measured_peaks()
     log(f"[bench] measured peaks: copy {peaks['copy_GBps']:.0f} GB/s, read {peaks['read_GBps']:.0f} GB/s, "
         f"FP64 FMA {peaks['fp64_fma_TFLOPs']:.1f} TFLOP/s")
@@ -303,6 +353,7 @@ def main():
                      "traffic": SPMV_TRAFFIC_100 if (args.workload == 1 and ndof == 3090903) else None,
                      "bytes_per_launch": bytes_sp, "avg_launch_ms": t_sp * 1e3},
         "cpu_baseline": cpu,
+        "solver_variants": variants,
         "measured_peaks": peaks,
         "element_roofline": {"kernel": "gp_kernel + scatter_kernel (Hex8 F-bar integration and tangent "
                                        "contraction, FP64)", "bound": "fp64",
