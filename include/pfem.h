@@ -91,7 +91,8 @@ typedef struct pfem_options {
                                    (Solver.jl:374; default), 1 = every Newton iteration,
                                    2 = at the first two Newton iterations of a load step,
                                    3 = at the first, then only the fine-level smoother
-                                   from each new tangent;
+                                   from each new tangent, 4 = at the first, then all
+                                   but the prolongators from each new tangent;
                                    reserved[1]: directions kept by the flexible GCR of the
                                    learned preconditioner (<= 0: 16);
                                    reserved[2]: FP32-IR only, 1 = the fine-level products

@@ -382,7 +382,7 @@ void coarse_inverse(pfem_ctx *c, Amg *a) {
 
 }  // namespace
 
-void amg_numeric_setup(pfem_ctx *c, Amg *a) {
+void amg_numeric_setup(pfem_ctx *c, Amg *a, bool keep_P) {
     graph_reset(a);  // smoother coefficients change with rho: captured V-cycles are stale
     hipStream_t s = c->stream;
     const bool want32 = c->opt.precision == PFEM_PREC_FP32_IR;
@@ -406,17 +406,21 @@ void amg_numeric_setup(pfem_ctx *c, Amg *a) {
         const double omega = (4.0 / 3.0) / L.rho;
         AmgLevel &N = a->L[l + 1];
         if (L.bs == 3) {
-            amg_smooth_P<3, true>(L.nP, L.P_row.p, L.P_colind.p, c->K.slice_ptr.p, c->K.rowlen.p,
-                                  nullptr, c->K.colind.p, Av, L.agg.p, L.Pt.p, L.dinv.p, omega, L.P.p, s);
-            amg_transpose_P<3>(L.nR, L.R_src.p, L.P.p, L.R.p, s);
+            if (!keep_P) {
+                amg_smooth_P<3, true>(L.nP, L.P_row.p, L.P_colind.p, c->K.slice_ptr.p, c->K.rowlen.p,
+                                      nullptr, c->K.colind.p, Av, L.agg.p, L.Pt.p, L.dinv.p, omega, L.P.p, s);
+                amg_transpose_P<3>(L.nR, L.R_src.p, L.P.p, L.R.p, s);
+            }
             amg_AP<3, true>(L.nQ, L.Q_row.p, L.Q_colind.p, c->K.slice_ptr.p, c->K.rowlen.p, nullptr,
                             c->K.colind.p, Av, L.P_rowptr.p, L.P_colind.p, L.P.p, L.Q.p, s);
             amg_RAP<3>(N.nnzb, L.C_row.p, N.colind.p, L.R_rowptr.p, L.R_colind.p, L.R.p, L.Q_rowptr.p, L.Q_colind.p,
                        L.Q.p, N.A.p, s);
         } else {
-            amg_smooth_P<6, false>(L.nP, L.P_row.p, L.P_colind.p, nullptr, nullptr, L.rowptr.p,
-                                   L.colind.p, Av, L.agg.p, L.Pt.p, L.dinv.p, omega, L.P.p, s);
-            amg_transpose_P<6>(L.nR, L.R_src.p, L.P.p, L.R.p, s);
+            if (!keep_P) {
+                amg_smooth_P<6, false>(L.nP, L.P_row.p, L.P_colind.p, nullptr, nullptr, L.rowptr.p,
+                                       L.colind.p, Av, L.agg.p, L.Pt.p, L.dinv.p, omega, L.P.p, s);
+                amg_transpose_P<6>(L.nR, L.R_src.p, L.P.p, L.R.p, s);
+            }
             amg_AP<6, false>(L.nQ, L.Q_row.p, L.Q_colind.p, nullptr, nullptr, L.rowptr.p,
                              L.colind.p, Av, L.P_rowptr.p, L.P_colind.p, L.P.p, L.Q.p, s);
             amg_RAP<6>(N.nnzb, L.C_row.p, N.colind.p, L.R_rowptr.p, L.R_colind.p, L.R.p, L.Q_rowptr.p, L.Q_colind.p,
@@ -443,7 +447,7 @@ void amg_numeric_setup(pfem_ctx *c, Amg *a) {
                 if (!L.A32.p) L.A32.alloc(L.A.n);
                 convert<double, float>((int64_t)L.A.n, L.A.p, L.A32.p, s);
             }
-            if (l + 1 < a->L.size()) {
+            if (l + 1 < a->L.size() && !(keep_P && L.P32.p)) {
                 if (!L.P32.p) { L.P32.alloc(L.P.n); L.R32.alloc(L.R.n); }
                 convert<double, float>((int64_t)L.P.n, L.P.p, L.P32.p, s);
                 convert<double, float>((int64_t)L.R.n, L.R.p, L.R32.p, s);

@@ -74,7 +74,10 @@ struct Amg {
 
 Amg *amg_create(pfem_ctx *c);                 // symbolic set-up (host)
 void amg_destroy(Amg *a);
-void amg_numeric_setup(pfem_ctx *c, Amg *a);  // numeric set-up (device)
+// numeric set-up (device); keep_P: the prolongator and restriction of the last set-up
+// are kept, everything that depends on the operator (smoothers, A P, coarse operators,
+// coarse inverse) is recomputed from the current one
+void amg_numeric_setup(pfem_ctx *c, Amg *a, bool keep_P = false);
 void amg_refresh_fine(pfem_ctx *c, Amg *a);  // level-0 smoother from the current K only
 void amg_apply64(pfem_ctx *c, Amg *a, const double *r, double *z);  // z = V-cycle(r)
 void amg_apply32(pfem_ctx *c, Amg *a, const float *r, float *z);

@@ -70,7 +70,7 @@ struct pfem_ctx {
     pfem_net *net = nullptr;
     int net_mode = 0, net_stride = 1;
     int64_t net_calls = 0;
-    bool pc_stale = false;  // AMG rebuild requested while the network preconditions
+    int pc_stale = 0;       // AMG rebuild requested while the network preconditions (1 full, 3 keep P)
     pfem::dbuf<double> net_dinv;  // block-diagonal inverse of K for the learned preconditioner
     bool half_fine = false;       // FP32 V-cycle: fine-level products with the FP16 scaled K
     pfem::dbuf<double> fcg_P, fcg_Q; // truncated flexible CG: search directions and K times them

@@ -90,10 +90,10 @@ int assemble_impl(pfem_ctx *c, int commit, pfem_fail *fail, bool state_only) {
 }
 
 // ---------------------------------------------------------------- preconditioner
-static void pc_setup(pfem_ctx *c) {
+static void pc_setup(pfem_ctx *c, bool keep_P = false) {
     const double t0 = now();
     if (!c->amg) c->amg.reset(amg_create(c));
-    amg_numeric_setup(c, c->amg.get());
+    amg_numeric_setup(c, c->amg.get(), keep_P && c->pc_ready);
     c->pc_ready = true;
     c->st.t_setup += now() - t0;
     c->st.n_setup++;
@@ -511,8 +511,8 @@ static int learned_solve(pfem_ctx *c, double rtol, int itmax, int *total, int *w
     if (rc == 0) return 0;
     *which = 6;
     if (c->pc_stale || !c->pc_ready) {
-        pc_setup(c);
-        c->pc_stale = false;
+        pc_setup(c, c->pc_stale == 3);
+        c->pc_stale = 0;
     }
     // remaining residual b2 = b - K x and the AMG correction x2 live in the GCR direction
     // store (free now): PCG and BiCGStab use every w_* workspace themselves
@@ -541,18 +541,23 @@ static int learned_solve(pfem_ctx *c, double rtol, int itmax, int *total, int *w
 }
 
 // rebuild: 0 reuse the preconditioner, 1 rebuild it, 2 refresh only the fine-level
-// smoother from the current K (interpolation and coarse operators kept)
+// smoother from the current K (interpolation and coarse operators kept), 3 rebuild it
+// keeping the prolongators of the last full set-up
 int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *used, double *relres) {
     const bool learned = c->net_mode >= 2;
     if (learned) {
         if (c->opt.precision == PFEM_PREC_FP32_IR) prepare_fp32(c);
-        if (rebuild) c->pc_stale = true;  // the AMG hierarchy is rebuilt only if it is needed
+        // the AMG hierarchy is rebuilt only if it is needed (a full rebuild request wins
+        // over a keep-the-prolongators one)
+        if (rebuild) c->pc_stale = (c->pc_stale == 1 || rebuild == 1) ? 1 : rebuild;
     } else if (rebuild == 1 || !c->pc_ready) {
         pc_setup(c);
     } else if (rebuild == 2) {
         const double t0 = now();
         amg_refresh_fine(c, c->amg.get());
         c->st.t_setup += now() - t0;
+    } else if (rebuild == 3) {
+        pc_setup(c, true);
     }
     const double t1 = now();
     int it = 0, total = 0, which = 0, rc;
@@ -693,8 +698,14 @@ int pfem_newton(pfem_ctx *c, double lambda, double tol, int maxiter, pfem_newton
             // first plastic tangent) and reuses that hierarchy afterwards
             // amg_rebuild = 3 builds at the first iteration and afterwards refreshes only
             // the fine-level smoother from the current tangent
+            // amg_rebuild = 4 builds at the first iteration and afterwards recomputes
+            // everything but the prolongators from the current tangent (fewer Krylov
+            // iterations than rebuilding the prolongators too, at less set-up cost)
             const int policy = c->opt.reserved[0];
-            const int rebuild = (first || policy == 1 || (policy == 2 && it == 2)) ? 1 : (policy == 3 ? 2 : 0);
+            const int rebuild = (first || policy == 1 || (policy == 2 && it == 2)) ? 1
+                                : policy == 3                                   ? 2
+                                : policy == 4                                   ? 3
+                                                                                : 0;
             first = false;
             axpby<double>(c->ndof, -1.0, c->R.p, 0.0, c->rhs.p, s);  // :378
             const double t = now();

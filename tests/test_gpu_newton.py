@@ -85,10 +85,19 @@ def test_config0_matches_oracle_golden(need_gpu):
     assert abs(sum(res.iters) - int(g["newton_iters"].sum())) <= 2
 
 
-def test_jittered_compression_matches_oracle_golden(need_gpu):
+# (AMG rebuild policy, coarsest size): the library default (one level at 6^3: the dense
+# coarse solve), and a 3-level hierarchy under the benchmark's policy 4 (prolongators kept
+# within a load step)
+POLICIES = [(0, 1500), (4, 60)]
+
+
+@pytest.mark.parametrize("policy,coarse_max", POLICIES)
+def test_jittered_compression_matches_oracle_golden(policy, coarse_max, need_gpu):
     g = np.load(os.path.join(HERE, "golden", "jitter6.npz"))
-    print("[newton] 6^3 jittered F-bar compression 20% in 80 steps", flush=True)
-    m, res, out = run_gpu(jitter6(), {20, 40, 80})
+    print(f"[newton] 6^3 jittered F-bar compression 20% in 80 steps, AMG policy {policy}, coarse_max {coarse_max}",
+          flush=True)
+    m, res, out = run_gpu(jitter6(), {20, 40, 80}, amg_rebuild=policy, coarse_max=coarse_max)
+    print(f"  AMG levels {m.stats().amg_levels}", flush=True)
     assert res.converged
     print("  newton iters total gpu", sum(res.iters), "oracle", int(g["newton_iters"].sum()), flush=True)
     for n in (20, 40, 80):
@@ -98,10 +107,11 @@ def test_jittered_compression_matches_oracle_golden(need_gpu):
         assert eu < 1e-8 and ea < 1e-8
 
 
-def test_fp32_ir_matches_fp64_oracle(need_gpu):
+@pytest.mark.parametrize("policy,coarse_max", POLICIES)
+def test_fp32_ir_matches_fp64_oracle(policy, coarse_max, need_gpu):
     g = np.load(os.path.join(HERE, "golden", "jitter6.npz"))
-    print("[newton fp32-IR] 6^3 jittered compression", flush=True)
-    m, res, out = run_gpu(jitter6(), {20, 80}, precision="fp32_ir")
+    print(f"[newton fp32-IR] 6^3 jittered compression, AMG policy {policy}, coarse_max {coarse_max}", flush=True)
+    m, res, out = run_gpu(jitter6(), {20, 80}, precision="fp32_ir", amg_rebuild=policy, coarse_max=coarse_max)
     assert res.converged
     for n in (20, 80):
         eu = rel(out[f"U_{n}"], g[f"U_{n}"])
