@@ -345,7 +345,8 @@ def main():
                                f"AMG rebuild policy {args.amg_rebuild} ("
                                + {0: "once per load step", 1: "every Newton iteration", 2: "first two iterations",
                                   3: "fine smoother refreshed", 4: "prolongators kept within a load step, the rest "
-                                  "rebuilt every Newton iteration"}.get(args.amg_rebuild, "?") + ")",
+                                  "rebuilt every Newton iteration", 5: "prolongators built once, the rest rebuilt "
+                                  "every Newton iteration"}.get(args.amg_rebuild, "?") + ")",
                    "amg_cycle": ("W" if args.precision == "fp32_ir" else "V") if args.cycle == "default"
                    else args.cycle.upper(),
                    "newton_steps_timed": newton, "krylov_its_timed": krylov,

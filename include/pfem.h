@@ -92,7 +92,9 @@ typedef struct pfem_options {
                                    2 = at the first two Newton iterations of a load step,
                                    3 = at the first, then only the fine-level smoother
                                    from each new tangent, 4 = at the first, then all
-                                   but the prolongators from each new tangent;
+                                   but the prolongators from each new tangent, 5 = the
+                                   prolongators once (first set-up), all the rest from
+                                   every new tangent;
                                    reserved[1]: directions kept by the flexible GCR of the
                                    learned preconditioner (<= 0: 16);
                                    reserved[2]: FP32-IR only, 1 = the fine-level products

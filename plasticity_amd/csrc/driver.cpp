@@ -701,8 +701,11 @@ int pfem_newton(pfem_ctx *c, double lambda, double tol, int maxiter, pfem_newton
             // amg_rebuild = 4 builds at the first iteration and afterwards recomputes
             // everything but the prolongators from the current tangent (fewer Krylov
             // iterations than rebuilding the prolongators too, at less set-up cost)
+            // amg_rebuild = 5 builds the prolongators once (first set-up of the context)
+            // and everything else from every Newton iteration's tangent
             const int policy = c->opt.reserved[0];
-            const int rebuild = (first || policy == 1 || (policy == 2 && it == 2)) ? 1
+            const int rebuild = policy == 5                                     ? (c->pc_ready ? 3 : 1)
+                                : (first || policy == 1 || (policy == 2 && it == 2)) ? 1
                                 : policy == 3                                   ? 2
                                 : policy == 4                                   ? 3
                                                                                 : 0;
