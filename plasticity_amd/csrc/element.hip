@@ -766,33 +766,35 @@ __device__ __forceinline__ void contract_blocks(const double *__restrict__ pe,
 // One 64-lane wave per element of one colour; lane (a, b) = (lane / 8, lane % 8)
 // accumulates the 3x3 block Ke(a, b) over the 8 GP packets and adds it into the fine
 // BSR K through the precomputed slot map (same-colour elements share no node, so no
-// atomics; colours run serially, Assembly.jl:370-385).  The wave first copies the
-// element's 8 packets into LDS with coalesced 16-byte loads.
+// atomics; colours run serially, Assembly.jl:370-385).
 //   Ke(a,b)_ik = sum_g [ sum_jl dNa_j dNb_l SA_(i+3j),(k+3l) + h_i (g0_b - gJ_b)_k ]
 //   h_i = sum_j dNa_j q_(i+3j),  gJ_b,k = sum_j dNb_j (F^-1)_jk        (Elements.jl:470-478)
 // The main term is contracted in two stages per Gauss point: the 64 lanes first form,
 // for every node a, the 27 partial sums T_a[i][k + 3l] = sum_j dNa_j SA_(i+3j),(k+3l)
 // (216 values, at most four per lane, into LDS), then lane (a, b) finishes its block as
-// sum_l T_a[i][k + 3l] dNb_l: 12 + 27 multiply-adds per lane instead of 81 + 9 (91
-// instead of 143 VGPRs; 7.5 instead of 8.9 ms per 100^3 assembly).
+// sum_l T_a[i][k + 3l] dNb_l: 12 + 27 multiply-adds per lane instead of 81 + 9 (96
+// instead of 143 VGPRs).  The packets are staged one Gauss point at a time in a double
+// buffer (the next point's 16-byte loads are in flight during the contraction), so a
+// workgroup needs 3.8 KB of LDS instead of 10 KB and occupancy is set by registers.
+// 6.6 ms per 100^3 assembly (8.9 ms with the one-stage contraction and all 8 packets
+// in LDS, 7.5 ms with the two stages and all 8 packets).
 __global__ __launch_bounds__(64) void scatter_kernel(const double *__restrict__ packet,
                                                       const double *__restrict__ g0,
                                                       const int64_t *__restrict__ elist, int64_t count,
                                                       const int64_t *__restrict__ conn,
                                                       const int32_t *__restrict__ slots,
                                                       double *__restrict__ Kv, double *__restrict__ R, int fbar) {
-    __shared__ __attribute__((aligned(16))) double sp[8 * PK];
+    constexpr int NC = PK / 2;  // 16-byte chunks per Gauss-point packet (66)
+    __shared__ __attribute__((aligned(16))) double sp[2][PK];
     __shared__ __attribute__((aligned(16))) double T[8 * 27];
     const int64_t li = blockIdx.x;
     if (li >= count) return;
     const int64_t e = elist[li];
     const int lane = threadIdx.x, a = lane >> 3, b = lane & 7;
-    {  // the element's 8 packets are contiguous (8448 B): 16-byte loads, all in flight at once
-        const double2 *src = reinterpret_cast<const double2 *>(packet + (size_t)e * 8 * PK);
-        double2 *dst = reinterpret_cast<double2 *>(sp);
-        for (int q = lane; q < 4 * PK; q += 64) dst[q] = src[q];
-    }
-    // stage-1 work of this lane: values v = lane + 64 q (< 216), v = 27 a' + 9 i + kl
+    const double2 *src = reinterpret_cast<const double2 *>(packet + (size_t)e * 8 * PK);
+    // lanes 0 and 1 also fetch the two chunks beyond 64; the other lanes re-read their own
+    // chunk into r1 and never store it
+    double2 r0 = src[lane], r1 = src[lane < NC - 64 ? 64 + lane : lane];
     int s1_dn[4], s1_sa[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -801,7 +803,6 @@ __global__ __launch_bounds__(64) void scatter_kernel(const double *__restrict__ 
         s1_dn[q] = 108 + 3 * ap;
         s1_sa[q] = i + 9 * kl;
     }
-    __syncthreads();
     double acc[9];
 #pragma unroll
     for (int t = 0; t < 9; ++t) acc[t] = 0.0;
@@ -810,9 +811,16 @@ __global__ __launch_bounds__(64) void scatter_kernel(const double *__restrict__ 
     if (fbar)
 #pragma unroll
         for (int k = 0; k < 3; ++k) g0b[k] = g0[24 * e + 3 * b + k];
+    reinterpret_cast<double2 *>(sp[0])[lane] = r0;
+    if (lane < NC - 64) reinterpret_cast<double2 *>(sp[0])[64 + lane] = r1;
+    __syncthreads();
 #pragma unroll 1
     for (int g = 0; g < 8; ++g) {
-        const double *pk = sp + g * PK;
+        if (g < 7) {  // the next Gauss point's packet, written to the other buffer at the end
+            r0 = src[(g + 1) * NC + lane];
+            r1 = src[(g + 1) * NC + (lane < NC - 64 ? 64 + lane : lane)];
+        }
+        const double *pk = sp[g & 1];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             if (lane + 64 * q < 216) {
@@ -848,7 +856,11 @@ __global__ __launch_bounds__(64) void scatter_kernel(const double *__restrict__ 
         if (b == 0)
 #pragma unroll
             for (int i = 0; i < 3; ++i) fe[i] += dNa[0] * pk[99 + i] + dNa[1] * pk[102 + i] + dNa[2] * pk[105 + i];
-        __syncthreads();  // T is rewritten for the next Gauss point
+        if (g < 7) {
+            reinterpret_cast<double2 *>(sp[(g + 1) & 1])[lane] = r0;
+            if (lane < NC - 64) reinterpret_cast<double2 *>(sp[(g + 1) & 1])[64 + lane] = r1;
+        }
+        __syncthreads();
     }
     double *kb = Kv + 9 * (int64_t)slots[64 * e + 8 * a + b];
 #pragma unroll
