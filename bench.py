@@ -144,6 +144,7 @@ def time_variant(pa, wl, precision, learned, warmup, steps):
         net = UNet(grid_of(wl.mesh)).load(learned)
         m.attach_net(net, "precondition")
     newton = krylov = 0
+    t_asm = t_setup = t_solve = 0.0
     t0 = None
     for k in range(1, warmup + steps + 1):
         if k == warmup + 1:
@@ -155,11 +156,15 @@ def time_variant(pa, wl, precision, learned, warmup, steps):
         if k > warmup:
             newton += res.iters
             krylov += sum(res.krylov_iters[:res.iters])
+            t_asm += res.t_assemble
+            t_setup += res.t_setup
+            t_solve += res.t_solve
     elapsed = time.perf_counter() - t0
     if learned:
         m.attach_net(None, "off")
     m.close()
-    return {"value": elapsed / newton, "newton_steps": newton, "krylov_its": krylov, "seconds": elapsed}
+    return {"value": elapsed / newton, "newton_steps": newton, "krylov_its": krylov, "seconds": elapsed,
+            "t_assemble_s": t_asm, "t_amg_setup_s": t_setup, "t_solve_s": t_solve}
 
 
 def main():
