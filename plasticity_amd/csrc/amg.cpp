@@ -537,6 +537,12 @@ void diag_apply(AmgLevel &L, const T *r, T *z, double scale, hipStream_t s) {
     if (L.bs == 3) block_diag_apply<3, T, T>(L.nb, Vec<T>::Dinv(L), r, z, scale, s);
     else block_diag_apply<6, T, T>(L.nb, Vec<T>::Dinv(L), r, z, scale, s);
 }
+// d = scale Dinv v and, from a zero guess (v = b), r = b and x = d; else (v = r) x += d
+template <class T>
+void diag_first_step(AmgLevel &L, const T *v, T *r, T *d, T *x, double scale, bool zero_init, hipStream_t s) {
+    if (L.bs == 3) diag_first<3, T, T>(L.nb, Vec<T>::Dinv(L), v, r, d, x, scale, zero_init, s);
+    else diag_first<6, T, T>(L.nb, Vec<T>::Dinv(L), v, r, d, x, scale, zero_init, s);
+}
 
 // Polynomial smoother on level l: x <- x + p(D^-1 A) D^-1 (b - A x).
 // Chebyshev on [lmax/30, lmax] with lmax = 1.1 rho (Adams, Brezina, Hu & Tuminaro,
@@ -555,8 +561,7 @@ void smooth(pfem_ctx *c, Amg *a, int l, const T *b, T *x, bool zero_init) {
                 continue;
             }
             level_residual<T>(c, L, l, x, b, r);
-            diag_apply<T>(L, r, d, w, s);
-            axpby<T>(L.n, 1.0, d, 1.0, x, s);
+            diag_first_step<T>(L, r, nullptr, d, x, w, false, s);
         }
         return;
     }
@@ -564,11 +569,8 @@ void smooth(pfem_ctx *c, Amg *a, int l, const T *b, T *x, bool zero_init) {
         // fourth-kind Chebyshev (Lottes, Numer. Linear Algebra Appl. 30, 2023, Alg. 1):
         // d_0 = 4/(3 lmax) D^-1 r_0, d_i = (2i-1)/(2i+3) d_{i-1} + (8i+4)/((2i+3) lmax) D^-1 r_i
         const double lam = 1.1 * L.rho;
-        if (zero_init) PF_HIP(hipMemcpyAsync(r, b, L.n * sizeof(T), hipMemcpyDeviceToDevice, s));
-        else level_residual<T>(c, L, l, x, b, r);
-        diag_apply<T>(L, r, d, 4.0 / (3.0 * lam), s);
-        if (zero_init) PF_HIP(hipMemcpyAsync(x, d, L.n * sizeof(T), hipMemcpyDeviceToDevice, s));
-        else axpby<T>(L.n, 1.0, d, 1.0, x, s);
+        if (!zero_init) level_residual<T>(c, L, l, x, b, r);
+        diag_first_step<T>(L, zero_init ? b : r, zero_init ? r : nullptr, d, x, 4.0 / (3.0 * lam), zero_init, s);
         for (int i = 1; i < a->degree; ++i) {
             level_matvec<T>(c, L, l, d, t);
             const double c1 = (2.0 * i - 1.0) / (2.0 * i + 3.0), c2 = (8.0 * i + 4.0) / ((2.0 * i + 3.0) * lam);
@@ -581,11 +583,8 @@ void smooth(pfem_ctx *c, Amg *a, int l, const T *b, T *x, bool zero_init) {
     const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
     const double sigma = theta / delta;
     double rho_old = 1.0 / sigma;
-    if (zero_init) PF_HIP(hipMemcpyAsync(r, b, L.n * sizeof(T), hipMemcpyDeviceToDevice, s));
-    else level_residual<T>(c, L, l, x, b, r);
-    diag_apply<T>(L, r, d, 1.0 / theta, s);
-    if (zero_init) PF_HIP(hipMemcpyAsync(x, d, L.n * sizeof(T), hipMemcpyDeviceToDevice, s));
-    else axpby<T>(L.n, 1.0, d, 1.0, x, s);
+    if (!zero_init) level_residual<T>(c, L, l, x, b, r);
+    diag_first_step<T>(L, zero_init ? b : r, zero_init ? r : nullptr, d, x, 1.0 / theta, zero_init, s);
     for (int k = 2; k <= a->degree; ++k) {
         level_matvec<T>(c, L, l, d, t);
         const double rho_new = 1.0 / (2.0 * sigma - rho_old);

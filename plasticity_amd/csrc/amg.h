@@ -114,6 +114,10 @@ template <int BS, class VT, class XT>
 void sell_spmv(const FineMatrix &K, const VT *vals, const XT *x, XT *y, hipStream_t s);
 template <int BS, class DT, class XT>
 void block_diag_apply(int64_t nb, const DT *dinv, const XT *r, XT *z, double scale, hipStream_t s);
+// d = scale Dinv v; zero_init: r = v, x = d; else x += d
+template <int BS, class DT, class XT>
+void diag_first(int64_t nb, const DT *dinv, const XT *v, XT *r, XT *d, XT *x, double scale, bool zero_init,
+                hipStream_t s);
 template <int BS, class DT, class XT>
 void cheb_step(int64_t nb, const DT *dinv, const XT *Ad, XT *r, XT *d, XT *x, double c1, double c2,
                hipStream_t s);  // r -= Ad; d = c1 d + c2 Dinv r; x += d

@@ -301,6 +301,35 @@ __global__ void block_diag_apply_kernel(int64_t nb, const DT *__restrict__ dinv,
     z[(int64_t)BS * i + rr] = (XT)(scale * (double)s);
 }
 
+// First step of a polynomial smoother, d = scale Dinv v, fused with what surrounds it:
+// zero initial guess (v = b): also r = b and x = d; otherwise (v = r): x += d.  Same
+// arithmetic as block_diag_apply followed by the copies / the axpby.
+template <int BS, class DT, class XT, bool ZERO>
+__global__ void diag_first_kernel(int64_t nb, const DT *__restrict__ dinv, const XT *__restrict__ v,
+                                  XT *__restrict__ r, XT *__restrict__ d, XT *__restrict__ x, double scale) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = t / BS;
+    const int rr = (int)(t % BS);
+    if (i >= nb) return;
+    const DT *m = dinv + (int64_t)BS * BS * i + BS * rr;
+    const XT *rv = v + (int64_t)BS * i;
+    DT s = 0;
+#pragma unroll
+    for (int c = 0; c < BS; ++c) s += m[c] * (DT)rv[c];
+    const int64_t o = (int64_t)BS * i + rr;
+    const XT dv = (XT)(scale * (double)s);
+    d[o] = dv;
+    if (ZERO) {
+        r[o] = v[o];
+        x[o] = dv;
+    } else {
+        // x += d with d already rounded (no fused multiply-add of scale * s into x, which
+        // the compiler would otherwise form in FP64)
+#pragma clang fp contract(off)
+        x[o] = (XT)((double)dv + (double)x[o]);
+    }
+}
+
 // r -= Ad; d = c1 d + c2 Dinv r; x += d
 template <int BS, class DT, class XT>
 __global__ void cheb_step_kernel(int64_t nb, const DT *__restrict__ dinv, const XT *__restrict__ Ad,
@@ -727,6 +756,26 @@ void dense_gemv(int64_t n, const double *Ainv, const XT *b, XT *x, hipStream_t s
     hipLaunchKernelGGL((dense_gemv_kernel<XT>), dim3(grid_for(n, 4)), dim3(256), 0, s, n, Ainv, b, x);
     PF_HIP(hipGetLastError());
 }
+
+template <int BS, class DT, class XT>
+void diag_first(int64_t nb, const DT *dinv, const XT *v, XT *r, XT *d, XT *x, double scale, bool zero_init,
+                hipStream_t s) {
+    if (zero_init)
+        hipLaunchKernelGGL((diag_first_kernel<BS, DT, XT, true>), dim3(grid_for(nb * BS, 256)), dim3(256), 0, s, nb,
+                           dinv, v, r, d, x, scale);
+    else
+        hipLaunchKernelGGL((diag_first_kernel<BS, DT, XT, false>), dim3(grid_for(nb * BS, 256)), dim3(256), 0, s, nb,
+                           dinv, v, r, d, x, scale);
+    PF_HIP(hipGetLastError());
+}
+template void diag_first<3, double, double>(int64_t, const double *, const double *, double *, double *, double *,
+                                            double, bool, hipStream_t);
+template void diag_first<6, double, double>(int64_t, const double *, const double *, double *, double *, double *,
+                                            double, bool, hipStream_t);
+template void diag_first<3, float, float>(int64_t, const float *, const float *, float *, float *, float *, double,
+                                          bool, hipStream_t);
+template void diag_first<6, float, float>(int64_t, const float *, const float *, float *, float *, float *, double,
+                                          bool, hipStream_t);
 
 // explicit instantiations
 template void amg_block_inverse<3>(int64_t, const int32_t *, const double *, double *, hipStream_t);
