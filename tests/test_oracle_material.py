@@ -52,7 +52,7 @@ def test_T2_uniaxial_closed_form(exx):
     assert abs(s[1]) < 1e-6 * s[0]
     # abar is the equivalent plastic strain (Materials.jl:183-193): = eps_p axial
     assert a == pytest.approx(ep[0], rel=1e-9)
-    # 258.77 MPa anchor (docs/DESIGN.md:829, README.md:60)
+    # 258.77 MPa anchor (lagrangian/docs/DESIGN.md:829, lagrangian/README.md:60 of the reference)
     if exx == 0.01:
         assert s[0] == pytest.approx(258.77, abs=5e-3)
 
