@@ -170,7 +170,9 @@ def time_variant(pa, wl, precision, learned, warmup, steps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3, help="timed load steps")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed load steps (default: the rest of the workload's load path after the warm-up, "
+                         "39 of workload 1's 40)")
     ap.add_argument("--warmup", type=int, default=1, help="untimed load steps")
     ap.add_argument("--n", type=int, default=None, help="elements per edge (default: the workload's own)")
     ap.add_argument("--workload", type=int, default=1, choices=[1, 2, 4],
@@ -221,6 +223,8 @@ def main():
 
     t0 = time.perf_counter()
     wl = workload(args.workload, n=args.n, nsteps=args.nsteps, device=True)
+    if args.steps is None:
+        args.steps = max(1, wl.nsteps - args.warmup) if args.workload == 1 else 3
     log(f"[bench] workload generated in {time.perf_counter() - t0:.2f}s (mesh on the device)")
     if args.mode == "spmv":
         m = pa.Model(wl.mesh, pa.J2Material(**wl.materials[0]), options=pa.default_options())
@@ -322,11 +326,18 @@ def main():
         # time both on the same load steps, each on a fresh model
         m.close()
         weights = os.path.join(ROOT, "plasticity_amd", "weights", "unet_16_32_64_64.npz")
-        variants = {"fp32_ir": time_variant(pa, wl, "fp32_ir", None, args.warmup, args.steps)}
-        if os.path.exists(weights):
-            variants["fp32_ir_learned"] = time_variant(pa, wl, "fp32_ir", weights, args.warmup, args.steps)
-        for k, v in variants.items():
+        variants = {}
+        for k, learned in (("fp32_ir", None), ("fp32_ir_learned", weights if os.path.exists(weights) else None)):
+            if k == "fp32_ir_learned" and learned is None:
+                continue
+            try:
+                v = time_variant(pa, wl, "fp32_ir", learned, args.warmup, args.steps)
+            except Exception as e:  # a failed variant is reported, the FP64 line still printed
+                variants[k] = {"error": repr(e)}
+                log(f"[bench] {k} failed: {e!r}")
+                continue
             v["speedup_vs_fp64"] = per_newton / v["value"]
+            variants[k] = v
             log(f"[bench] {k}: {v['value']:.4f} s/Newton step ({v['newton_steps']} Newton, {v['krylov_its']} "
                 f"Krylov), {v['speedup_vs_fp64']:.2f}x the FP64 path")
     peaks = measured_peaks()
