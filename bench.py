@@ -133,7 +133,7 @@ def time_variant(pa, wl, precision, learned, warmup, steps):
     """The same load steps of the workload on a fresh model with another linear solver:
     precision "fp32_ir", learned = path of U-net weights (GCR with the network, AMG
     fallback) or None.  Returns s/Newton step over the timed load steps and the counts."""
-    opts = pa.default_options(eta_min=wl.krylov_rtol, eta_max=wl.krylov_rtol, precision=precision, amg_rebuild=4)
+    opts = pa.default_options(eta_min=wl.krylov_rtol, eta_max=wl.krylov_rtol, precision=precision, amg_rebuild=5)
     mats = [pa.J2Material(**d) for d in wl.materials]
     m = pa.Model(wl.mesh, mats if len(mats) > 1 else mats[0], options=opts, material_id=wl.material_id)
     m.constrain(wl.dir_dofs, wl.dir_vals, wl.dir_ramp)
@@ -194,9 +194,10 @@ def main():
     ap.add_argument("--no-variants", action="store_true",
                     help="FP64 run of workload 1: skip timing the FP32-IR and FP32-IR + learned solvers "
                          "on the same load steps")
-    ap.add_argument("--amg-rebuild", type=int, default=4,
+    ap.add_argument("--amg-rebuild", type=int, default=5,
                     help="0: once per load step (the reference), 1: every Newton iteration, 4: at the first "
-                         "Newton iteration of a load step, then all but the prolongators at every iteration")
+                         "Newton iteration of a load step, then all but the prolongators at every iteration, "
+                         "5: the prolongators once, everything else at every Newton iteration")
     ap.add_argument("--smoother", default="chebyshev", choices=["chebyshev", "chebyshev4", "jacobi"])
     ap.add_argument("--smoother-degree", type=int, default=0,
                     help="Chebyshev degree / Jacobi sweeps (0: library default, 2 / 1)")
