@@ -133,7 +133,7 @@ def time_variant(pa, wl, precision, learned, warmup, steps):
     """The same load steps of the workload on a fresh model with another linear solver:
     precision "fp32_ir", learned = path of U-net weights (GCR with the network, AMG
     fallback) or None.  Returns s/Newton step over the timed load steps and the counts."""
-    opts = pa.default_options(eta_min=wl.krylov_rtol, eta_max=wl.krylov_rtol, precision=precision, amg_rebuild=5)
+    opts = pa.default_options(eta_min=wl.krylov_rtol, eta_max=wl.krylov_rtol, precision=precision, amg_rebuild=6)
     mats = [pa.J2Material(**d) for d in wl.materials]
     m = pa.Model(wl.mesh, mats if len(mats) > 1 else mats[0], options=opts, material_id=wl.material_id)
     m.constrain(wl.dir_dofs, wl.dir_vals, wl.dir_ramp)
@@ -194,10 +194,11 @@ def main():
     ap.add_argument("--no-variants", action="store_true",
                     help="FP64 run of workload 1: skip timing the FP32-IR and FP32-IR + learned solvers "
                          "on the same load steps")
-    ap.add_argument("--amg-rebuild", type=int, default=5,
+    ap.add_argument("--amg-rebuild", type=int, default=6,
                     help="0: once per load step (the reference), 1: every Newton iteration, 4: at the first "
                          "Newton iteration of a load step, then all but the prolongators at every iteration, "
-                         "5: the prolongators once, everything else at every Newton iteration")
+                         "5: the prolongators once, everything else at every Newton iteration, 6: as 5, the "
+                         "prolongators rebuilt after a solve that needed a fallback")
     ap.add_argument("--smoother", default="chebyshev", choices=["chebyshev", "chebyshev4", "jacobi"])
     ap.add_argument("--smoother-degree", type=int, default=0,
                     help="Chebyshev degree / Jacobi sweeps (0: library default, 2 / 1)")
@@ -363,7 +364,8 @@ def main():
                                + {0: "once per load step", 1: "every Newton iteration", 2: "first two iterations",
                                   3: "fine smoother refreshed", 4: "prolongators kept within a load step, the rest "
                                   "rebuilt every Newton iteration", 5: "prolongators built once, the rest rebuilt "
-                                  "every Newton iteration"}.get(args.amg_rebuild, "?") + ")",
+                                  "every Newton iteration", 6: "prolongators kept until a solve falls back, the rest "
+                                  "rebuilt every Newton iteration"}.get(args.amg_rebuild, "?") + ")",
                    "amg_cycle": ("W" if args.precision == "fp32_ir" else "V") if args.cycle == "default"
                    else args.cycle.upper(),
                    "newton_steps_timed": newton, "krylov_its_timed": krylov,

@@ -94,7 +94,8 @@ typedef struct pfem_options {
                                    from each new tangent, 4 = at the first, then all
                                    but the prolongators from each new tangent, 5 = the
                                    prolongators once (first set-up), all the rest from
-                                   every new tangent;
+                                   every new tangent, 6 = as 5, the prolongators rebuilt
+                                   after a solve that needed a fallback;
                                    reserved[1]: directions kept by the flexible GCR of the
                                    learned preconditioner (<= 0: 16);
                                    reserved[2]: FP32-IR only, 1 = the fine-level products

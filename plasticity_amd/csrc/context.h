@@ -71,6 +71,7 @@ struct pfem_ctx {
     int net_mode = 0, net_stride = 1;
     int64_t net_calls = 0;
     int pc_stale = 0;       // AMG rebuild requested while the network preconditions (1 full, 3 keep P)
+    bool refresh_P = false;  // AMG policy 6: the last solve fell back, rebuild the prolongators
     pfem::dbuf<double> net_dinv;  // block-diagonal inverse of K for the learned preconditioner
     bool half_fine = false;       // FP32 V-cycle: fine-level products with the FP16 scaled K
     pfem::dbuf<double> fcg_P, fcg_Q; // truncated flexible CG: search directions and K times them

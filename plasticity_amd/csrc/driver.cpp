@@ -703,8 +703,11 @@ int pfem_newton(pfem_ctx *c, double lambda, double tol, int maxiter, pfem_newton
             // iterations than rebuilding the prolongators too, at less set-up cost)
             // amg_rebuild = 5 builds the prolongators once (first set-up of the context)
             // and everything else from every Newton iteration's tangent
+            // amg_rebuild = 6 is 5, except that after a solve that needed a fallback the
+            // next set-up also rebuilds the prolongators (from the current tangent)
             const int policy = c->opt.reserved[0];
             const int rebuild = policy == 5                                     ? (c->pc_ready ? 3 : 1)
+                                : policy == 6 ? ((c->pc_ready && !c->refresh_P) ? 3 : 1)
                                 : (first || policy == 1 || (policy == 2 && it == 2)) ? 1
                                 : policy == 3                                   ? 2
                                 : policy == 4                                   ? 3
@@ -720,6 +723,9 @@ int pfem_newton(pfem_ctx *c, double lambda, double tol, int maxiter, pfem_newton
             res->t_solve += now() - t - (c->st.t_setup - setup0);
             res->krylov_iters[it - 1] = niter;
             res->solver_used[it - 1] = used;
+            // policy 6: a solve that needed a fallback (PCG retried after a refresh, the
+            // BiCGStab ladder, or BiCGStab refinement) rebuilds the prolongators next time
+            if (policy == 6) c->refresh_P = (used == 1 || used == 2 || used == 4);
             if (c->opt.verbose)
                 fprintf(stderr, "    iter %d  eta = %.2e  krylov = %d (solver %d, true relres %.2e)\n", it, eta, niter,
                         used, rel);

@@ -86,9 +86,10 @@ def test_config0_matches_oracle_golden(need_gpu):
 
 
 # (AMG rebuild policy, coarsest size): the library default (one level at 6^3: the dense
-# coarse solve), and a 3-level hierarchy under policy 4 (the benchmark's: prolongators
-# kept within a load step) and policy 5 (prolongators built once)
-POLICIES = [(0, 1500), (4, 60), (5, 60)]
+# coarse solve), and a 3-level hierarchy under policy 4 (prolongators kept within a load
+# step), 5 (prolongators built once) and 6 (the benchmark's: as 5, rebuilt after a
+# fallback)
+POLICIES = [(0, 1500), (4, 60), (5, 60), (6, 60)]
 
 
 @pytest.mark.parametrize("policy,coarse_max", POLICIES)
