@@ -204,6 +204,8 @@ def main():
                     help="Chebyshev degree / Jacobi sweeps (0: library default, 2 / 1)")
     ap.add_argument("--cycle", default="default", choices=["default", "v", "w"],
                     help="AMG cycle (default: W under FP32-IR, V under FP64)")
+    ap.add_argument("--fallback", default="gcr", choices=["bicgstab", "gcr"],
+                    help="Krylov method after a CG breakdown")
     ap.add_argument("--verbose", type=int, default=0)
     ap.add_argument("--fp16-fine", action="store_true",
                     help="FP32-IR: fine-level smoother products with an FP16 copy of the scaled K")
@@ -241,7 +243,8 @@ def main():
                               verbose=args.verbose, amg_rebuild=args.amg_rebuild, fp16_fine=args.fp16_fine,
                               line_search=args.line_search, rho_iters=args.rho_iters, smoother=args.smoother,
                               smoother_degree=args.smoother_degree,
-                              wcycle={"default": None, "v": False, "w": True}[args.cycle])
+                              wcycle={"default": None, "v": False, "w": True}[args.cycle],
+                              gcr_fallback=args.fallback == "gcr")
     t0 = time.perf_counter()
     mats = [pa.J2Material(**d) for d in wl.materials]
     m = pa.Model(wl.mesh, mats if len(mats) > 1 else mats[0], options=opts, material_id=wl.material_id)

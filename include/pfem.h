@@ -111,7 +111,8 @@ typedef struct pfem_options {
                                    colours on the host (pfem_create); bit 1 = run the
                                    V-cycle eagerly instead of replaying it as a HIP graph;
                                    bit 4 = AMG W-cycle, bit 5 = AMG V-cycle (default:
-                                   W under FP32-IR, V under FP64) */
+                                   W under FP32-IR, V under FP64); bit 7 = BiCGStab
+                                   instead of flexible GCR after a CG breakdown */
 } pfem_options;
 
 typedef struct pfem_fail {
@@ -186,7 +187,10 @@ int pfem_residual_norm(pfem_ctx *ctx, double *out);
  * dU stays on the device; niter = Krylov iterations (all attempts);
  * solver_used: 0 PCG, 1 PCG after AMG refresh, 2 BiCGStab fallback (or the FP64
  * ladder after FP32-IR failed), 3 FP32-IR flexible PCG, 4 FP32-IR with BiCGStab,
- * 5 flexible CG with the learned preconditioner, 6 the same finished by AMG.
+ * 5 flexible GCR with the learned preconditioner, 6 the same finished by AMG,
+ * 7 FP32-IR finished by flexible GCR(16) with the FP32 AMG cycle after a CG breakdown,
+ * 8 FP64 finished by flexible GCR(16) with the AMG cycle after PCG failed.  The GCR
+ * fallbacks (7, 8) are the default; reserved[6] bit 7 selects BiCGStab (2, 4).
  * true_relres = ||b - K dU|| / ||b||. */
 int pfem_linear_solve(pfem_ctx *ctx, double rtol, int rebuild_pc, int *niter,
                       int *solver_used, double *true_relres);

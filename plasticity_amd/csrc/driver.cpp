@@ -581,7 +581,15 @@ int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *us
         else
             rc = fpcg(c, c->rhs.p, c->dU.p, rtol, itmax, true, 0, &it, Amg32Pc{c});
         total += it;
-        if (rc != 0) {  // indefinite (2) or stalled (1): refinement sweeps, BiCGStab after a breakdown
+        // after a breakdown or a stall: flexible GCR(16) with the same FP32 cycle (it needs
+        // no symmetry, minimises ||r|| and costs one product and one cycle per
+        // iteration); reserved[6] bit 7: refinement with BiCGStab instead
+        if (rc != 0 && !(c->opt.reserved[6] & 128)) {
+            which = 7;
+            rc = gcr_m(c, c->rhs.p, c->dU.p, rtol, 4 * itmax, 16, 40, &it, Amg32Pc{c});
+            if (rc == 3) rc = 1;  // stalled: not converged
+            total += it;
+        } else if (rc != 0) {  // indefinite (2) or stalled (1): refinement sweeps, BiCGStab after a breakdown
             rc = ir_solve(c, c->rhs.p, c->dU.p, rtol, 4 * itmax, rc == 2, &it, &which);
             total += it;
         }
@@ -603,7 +611,12 @@ int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *us
             rc = pcg<double>(c, c->rhs.p, c->dU.p, rtol, itmax, &it);
             total += it;
         }
-        if (rc != 0) {  // in place of the direct fallback (Solver.jl:262, :284)
+        if (rc != 0 && !(c->opt.reserved[6] & 128)) {  // flexible GCR(16) with the FP64 cycle
+            which = 8;
+            rc = gcr_m(c, c->rhs.p, c->dU.p, rtol, 2 * itmax, 16, 40, &it, Amg64Pc{c});
+            if (rc == 3) rc = 1;  // stalled: not converged
+            total += it;
+        } else if (rc != 0) {  // in place of the direct fallback (Solver.jl:262, :284)
             which = 2;
             rc = bicgstab<double>(c, c->rhs.p, c->dU.p, rtol, 2 * itmax, &it);
             total += it;
@@ -725,7 +738,7 @@ int pfem_newton(pfem_ctx *c, double lambda, double tol, int maxiter, pfem_newton
             res->solver_used[it - 1] = used;
             // policy 6: a solve that needed a fallback (PCG retried after a refresh, the
             // BiCGStab ladder, or BiCGStab refinement) rebuilds the prolongators next time
-            if (policy == 6) c->refresh_P = (used == 1 || used == 2 || used == 4);
+            if (policy == 6) c->refresh_P = (used == 1 || used == 2 || used == 4 || used == 7 || used == 8);
             if (c->opt.verbose)
                 fprintf(stderr, "    iter %d  eta = %.2e  krylov = %d (solver %d, true relres %.2e)\n", it, eta, niter,
                         used, rel);

@@ -94,6 +94,9 @@ def default_options(**kw):
         if k == "line_search":  # Newton backtracking on ||R||: most step halvings (0 = full steps)
             o.reserved[3] = int(v)
             continue
+        if k == "gcr_fallback":  # after a Krylov breakdown: flexible GCR(16) with the AMG cycle (default) or BiCGStab
+            o.reserved[6] = (o.reserved[6] & ~128) if v else (o.reserved[6] | 128)
+            continue
         if k == "wcycle":  # AMG cycle: True W, False V, None the default (W under FP32-IR, V under FP64)
             o.reserved[6] &= ~48
             if v is not None:
