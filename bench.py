@@ -202,8 +202,7 @@ def main():
     ap.add_argument("--smoother", default="chebyshev", choices=["chebyshev", "chebyshev4", "jacobi"])
     ap.add_argument("--smoother-degree", type=int, default=0,
                     help="Chebyshev degree / Jacobi sweeps (0: library default, 2 / 1)")
-    ap.add_argument("--cycle", default="default", choices=["default", "v", "w"],
-                    help="AMG cycle (default: W under FP32-IR, V under FP64)")
+    ap.add_argument("--cycle", default="v", choices=["v", "w"], help="AMG cycle")
     ap.add_argument("--fallback", default="gcr", choices=["bicgstab", "gcr"],
                     help="Krylov method after a CG breakdown")
     ap.add_argument("--verbose", type=int, default=0)
@@ -243,7 +242,7 @@ def main():
                               verbose=args.verbose, amg_rebuild=args.amg_rebuild, fp16_fine=args.fp16_fine,
                               line_search=args.line_search, rho_iters=args.rho_iters, smoother=args.smoother,
                               smoother_degree=args.smoother_degree,
-                              wcycle={"default": None, "v": False, "w": True}[args.cycle],
+                              wcycle=args.cycle == "w",
                               gcr_fallback=args.fallback == "gcr")
     t0 = time.perf_counter()
     mats = [pa.J2Material(**d) for d in wl.materials]
@@ -369,8 +368,7 @@ def main():
                                   "rebuilt every Newton iteration", 5: "prolongators built once, the rest rebuilt "
                                   "every Newton iteration", 6: "prolongators kept until a solve falls back, the rest "
                                   "rebuilt every Newton iteration"}.get(args.amg_rebuild, "?") + ")",
-                   "amg_cycle": ("W" if args.precision == "fp32_ir" else "V") if args.cycle == "default"
-                   else args.cycle.upper(),
+                   "amg_cycle": args.cycle.upper(),
                    "newton_steps_timed": newton, "krylov_its_timed": krylov,
                    "t_assemble_s": t_asm, "t_amg_setup_s": t_setup, "t_solve_s": t_solve,
                    "amg_levels": list(s1.amg_ndof[:s1.amg_levels]),

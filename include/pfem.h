@@ -110,8 +110,7 @@ typedef struct pfem_options {
                                    reserved[6]: bit 0 = build the pattern, slots and
                                    colours on the host (pfem_create); bit 1 = run the
                                    V-cycle eagerly instead of replaying it as a HIP graph;
-                                   bit 4 = AMG W-cycle, bit 5 = AMG V-cycle (default:
-                                   W under FP32-IR, V under FP64); bit 7 = BiCGStab
+                                   bit 4 = AMG W-cycle (default: V-cycle); bit 7 = BiCGStab
                                    instead of flexible GCR after a CG breakdown */
 } pfem_options;
 

@@ -158,13 +158,11 @@ Amg *amg_create(pfem_ctx *c) {
     hipStream_t s = c->stream;
     auto *a = new Amg();
     a->smoother = c->opt.smoother;
-    // W-cycle by default under FP32-IR (its flexible CG needs 20 instead of 28 iterations
-    // per solve at 100^3 for about 10% more work per application); V-cycle under FP64,
-    // whose PCG stops on the preconditioned norm like Krylov.jl's cg! and then needed
-    // more Newton iterations with the W-cycle.  reserved[6] bit 4 forces W, bit 5 V.
-    a->wcycle = (c->opt.reserved[6] & 16) ? true
-                : (c->opt.reserved[6] & 32) ? false
-                                            : c->opt.precision == PFEM_PREC_FP32_IR;
+    // V-cycle by default; reserved[6] bit 4 selects the W-cycle (a second coarse cycle at
+    // every level below the finest).  Under FP32-IR the W-cycle needed 20 instead of 28
+    // Krylov iterations per solve when the prolongators were rebuilt at every Newton
+    // iteration, but with them kept (rebuild policies 4-6) the V-cycle is faster overall.
+    a->wcycle = (c->opt.reserved[6] & 16) != 0;
     a->degree = c->opt.smoother_degree > 0 ? c->opt.smoother_degree : (a->smoother == PFEM_SMOOTH_JACOBI ? 1 : 2);
     const int64_t coarse_max = c->opt.coarse_max > 0 ? c->opt.coarse_max : 1500;
     const int max_levels = c->opt.max_levels > 1 ? c->opt.max_levels : 10;

@@ -97,10 +97,8 @@ def default_options(**kw):
         if k == "gcr_fallback":  # after a Krylov breakdown: flexible GCR(16) with the AMG cycle (default) or BiCGStab
             o.reserved[6] = (o.reserved[6] & ~128) if v else (o.reserved[6] | 128)
             continue
-        if k == "wcycle":  # AMG cycle: True W, False V, None the default (W under FP32-IR, V under FP64)
-            o.reserved[6] &= ~48
-            if v is not None:
-                o.reserved[6] |= 16 if v else 32
+        if k == "wcycle":  # AMG cycle: True W, False / None V (the default)
+            o.reserved[6] = (o.reserved[6] | 16) if v else (o.reserved[6] & ~16)
             continue
         if k == "fp16_fine":  # FP32-IR: smoother products on the fine level with the FP16 scaled K
             o.reserved[2] = 1 if v else 0
