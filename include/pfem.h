@@ -111,7 +111,9 @@ typedef struct pfem_options {
                                    colours on the host (pfem_create); bit 1 = run the
                                    V-cycle eagerly instead of replaying it as a HIP graph;
                                    bit 4 = AMG W-cycle (default: V-cycle); bit 7 = BiCGStab
-                                   instead of flexible GCR after a CG breakdown */
+                                   instead of flexible GCR after a CG breakdown; bit 8 =
+                                   fine-level prolongation with one thread per row instead
+                                   of over chunks of consecutive rows */
 } pfem_options;
 
 typedef struct pfem_fail {

@@ -271,6 +271,22 @@ Amg *amg_create(pfem_ctx *c) {
         for (int64_t i = 0; i < L.nb; ++i)
             for (int32_t m = Pptr[i]; m < Pptr[i + 1]; ++m) Prow[m] = (int32_t)i;
         up(L.P_rowptr, Pptr, s); up(L.P_colind, Pidx, s); up(L.P_row, Prow, s);
+        // chunks of the streamed prolongation (fine level; reserved[6] bit 8: one thread
+        // per row instead): greedy runs of whole rows of at most P_STREAM_CAP blocks
+        if (bs == 3 && !(c->opt.reserved[6] & 256)) {
+            std::vector<int32_t> ch{0};
+            bool fits = true;
+            for (int64_t i = 0; i < L.nb; ++i) {
+                const int32_t len = Pptr[i + 1] - Pptr[i];
+                if (len > P_STREAM_CAP) { fits = false; break; }
+                if (Pptr[i + 1] - Pptr[ch.back()] > P_STREAM_CAP) ch.push_back((int32_t)i);
+            }
+            if (fits) {
+                ch.push_back((int32_t)L.nb);
+                L.nPchunk = (int64_t)ch.size() - 1;
+                up(L.P_chunk, ch, s);
+            }
+        }
         L.P.alloc(std::max<size_t>(1, Pidx.size() * bs * 6));
         up(L.R_rowptr, Rptr, s); up(L.R_colind, Ridx, s); up(L.R_src, Rsrc, s);
         L.R.alloc(std::max<size_t>(1, Ridx.size() * bs * 6));
@@ -612,7 +628,10 @@ void vcycle(pfem_ctx *c, Amg *a, int l, const T *b, T *x) {
         vcycle<T>(c, a, l + 1, Vec<T>::wr(N), Vec<T>::we(N));
         axpby<T>(N.n, 1.0, Vec<T>::we(N), 1.0, Vec<T>::x(N), s);
     }
-    if (L.bs == 3) bsr_spmv<3, 6, T, T>(L.nb, L.P_rowptr.p, L.P_colind.p, Vec<T>::P(L), Vec<T>::x(N), x, 1.0, s);
+    if (L.bs == 3 && L.nPchunk > 0)
+        bsr_spmv_stream<3, 6, T, T>(L.nPchunk, L.P_chunk.p, L.P_rowptr.p, L.P_colind.p, Vec<T>::P(L), Vec<T>::x(N), x,
+                                    1.0, s);
+    else if (L.bs == 3) bsr_spmv<3, 6, T, T>(L.nb, L.P_rowptr.p, L.P_colind.p, Vec<T>::P(L), Vec<T>::x(N), x, 1.0, s);
     else bsr_spmv<6, 6, T, T>(L.nb, L.P_rowptr.p, L.P_colind.p, Vec<T>::P(L), Vec<T>::x(N), x, 1.0, s);
     smooth<T>(c, a, l, b, x, false);
 }

@@ -15,6 +15,9 @@ struct pfem_ctx;
 
 namespace pfem {
 
+// most blocks per chunk of the streamed prolongation (bsr_spmv_stream)
+constexpr int P_STREAM_CAP = 256;
+
 struct AmgLevel {
     int bs = 3;                  // dofs per block row on this level (3 fine, 6 coarse)
     int64_t nb = 0;              // block rows
@@ -35,6 +38,8 @@ struct AmgLevel {
     dbuf<int32_t> agg;           // node -> aggregate (-1 none)
     dbuf<double> Pt;             // tentative block per node (bs x 6), zero if agg < 0
     dbuf<int32_t> P_rowptr, P_colind, P_row;
+    dbuf<int32_t> P_chunk;       // row boundaries of the streamed prolongation's chunks
+    int64_t nPchunk = 0;         // 0: thread-per-row prolongation
     dbuf<double> P;
     dbuf<float> P32;
     // restriction R = P^T (nagg x nb, 6 x bs blocks)
@@ -102,6 +107,11 @@ void amg_transpose_P(int64_t nR, const int32_t *R_src, const double *P, double *
 template <int BR, int BC, class VT, class XT>
 void bsr_spmv(int64_t nb, const int32_t *rowptr, const int32_t *colind, const VT *vals, const XT *x, XT *y,
               double beta, hipStream_t s);  // y = A x + beta*y
+// y = A x + beta*y over chunks of consecutive block rows (chunk_row: nchunk + 1 row
+// boundaries, at most P_STREAM_CAP blocks per chunk)
+template <int BR, int BC, class VT, class XT>
+void bsr_spmv_stream(int64_t nchunk, const int32_t *chunk_row, const int32_t *rowptr, const int32_t *colind,
+                     const VT *vals, const XT *x, XT *y, double beta, hipStream_t s);
 template <int BR, int BC, class VT, class XT>
 void bsr_spmv_wave(int64_t nb, const int32_t *rowptr, const int32_t *colind, const VT *vals, const XT *x, XT *y,
                    hipStream_t s);  // y = A x, one wave per block row (long rows)

@@ -267,6 +267,46 @@ __global__ __launch_bounds__(256) void bsr_spmv_wave_kernel(int64_t nb, const in
         for (int r = 0; r < BR; ++r) y[(int64_t)BR * i + r] = (XT)acc[r];
 }
 
+// y = A x + beta*y over runs of consecutive block rows ("chunks" of at most
+// P_STREAM_CAP blocks, row boundaries listed on the host), for short rows of varying
+// length (the fine prolongator: 1-8 blocks per row).  One 64-lane wave per chunk:
+// pass 1 gives lane t row r = t % BR of block b = t / BR, so the wave reads the chunk's
+// blocks as one contiguous stream and keeps each block row's product with x in LDS;
+// pass 2 gives each lane one output row, which sums its blocks' products in order.
+template <int BR, int BC, class VT, class XT>
+__global__ __launch_bounds__(64) void bsr_stream_kernel(const int32_t *__restrict__ chunk_row,
+                                                        const int32_t *__restrict__ rowptr,
+                                                        const int32_t *__restrict__ colind,
+                                                        const VT *__restrict__ vals, const XT *__restrict__ x,
+                                                        XT *__restrict__ y, double beta) {
+    __shared__ VT part[P_STREAM_CAP * BR];
+    const int lane = threadIdx.x;
+    const int32_t r0 = chunk_row[blockIdx.x], r1 = chunk_row[blockIdx.x + 1];
+    const int32_t b0 = rowptr[r0];
+    const int nt = (rowptr[r1] - b0) * BR;
+    for (int t = lane; t < nt; t += 64) {
+        const int b = t / BR, r = t - BR * b;
+        const int64_t k = (int64_t)b0 + b;
+        const VT *v = vals + (int64_t)BR * BC * k + BC * r;
+        const XT *xx = x + (int64_t)BC * colind[k];
+        VT s = 0;
+#pragma unroll
+        for (int c = 0; c < BC; ++c) s += v[c] * (VT)xx[c];
+        part[t] = s;
+    }
+    __syncthreads();
+    const int no = (r1 - r0) * BR;
+    for (int o = lane; o < no; o += 64) {
+        const int32_t i = r0 + o / BR;
+        const int r = o - BR * (o / BR);
+        VT s = 0;
+        const int32_t k1 = rowptr[i + 1];
+        for (int32_t k = rowptr[i]; k < k1; ++k) s += part[(k - b0) * BR + r];
+        const int64_t oi = (int64_t)BR * i + r;
+        y[oi] = (XT)(beta == 0.0 ? (double)s : (double)s + beta * (double)y[oi]);
+    }
+}
+
 template <int BS, class VT, class XT>
 __global__ void bsr_residual_kernel(int64_t nb, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colind,
                                     const VT *__restrict__ vals, const XT *__restrict__ x, const XT *__restrict__ b,
@@ -704,6 +744,14 @@ void bsr_spmv(int64_t nb, const int32_t *rowptr, const int32_t *colind, const VT
     PF_HIP(hipGetLastError());
 }
 template <int BR, int BC, class VT, class XT>
+void bsr_spmv_stream(int64_t nchunk, const int32_t *chunk_row, const int32_t *rowptr, const int32_t *colind,
+                     const VT *vals, const XT *x, XT *y, double beta, hipStream_t s) {
+    if (nchunk <= 0) return;
+    hipLaunchKernelGGL((bsr_stream_kernel<BR, BC, VT, XT>), dim3((unsigned)nchunk), dim3(64), 0, s, chunk_row, rowptr,
+                       colind, vals, x, y, beta);
+    PF_HIP(hipGetLastError());
+}
+template <int BR, int BC, class VT, class XT>
 void bsr_spmv_wave(int64_t nb, const int32_t *rowptr, const int32_t *colind, const VT *vals, const XT *x, XT *y,
                    hipStream_t s) {
     if (nb <= 0) return;
@@ -804,6 +852,8 @@ template void amg_transpose_P<6>(int64_t, const int32_t *, const double *, doubl
                                          double, hipStream_t);                                                     \
     template void bsr_spmv<6, 3, VT, XT>(int64_t, const int32_t *, const int32_t *, const VT *, const XT *, XT *,  \
                                          double, hipStream_t);                                                     \
+    template void bsr_spmv_stream<3, 6, VT, XT>(int64_t, const int32_t *, const int32_t *, const int32_t *,        \
+                                                const VT *, const XT *, XT *, double, hipStream_t);                \
     template void bsr_spmv_wave<6, 3, VT, XT>(int64_t, const int32_t *, const int32_t *, const VT *, const XT *,    \
                                               XT *, hipStream_t);                                                  \
     template void bsr_spmv_wave<6, 6, VT, XT>(int64_t, const int32_t *, const int32_t *, const VT *, const XT *,    \

@@ -100,6 +100,9 @@ def default_options(**kw):
         if k == "wcycle":  # AMG cycle: True W, False / None V (the default)
             o.reserved[6] = (o.reserved[6] | 16) if v else (o.reserved[6] & ~16)
             continue
+        if k == "stream_prolongation":  # fine prolongation over row chunks (default) or one thread per row
+            o.reserved[6] = (o.reserved[6] & ~256) if v else (o.reserved[6] | 256)
+            continue
         if k == "fp16_fine":  # FP32-IR: smoother products on the fine level with the FP16 scaled K
             o.reserved[2] = 1 if v else 0
             continue

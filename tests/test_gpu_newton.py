@@ -251,6 +251,43 @@ def test_vcycle_graph_replay_equals_eager_launches(precision, need_gpu):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+def test_streamed_prolongation_equals_thread_per_row(precision, need_gpu):
+    """The fine-level prolongation over chunks of consecutive rows (default) against one
+    thread per row (reserved[6] bit 8).  The two sum each row's block products in a
+    different association, so the V-cycles agree to rounding: three solves on three
+    tangents to rtol 1e-10 take the same iterations (+-1) and give corrections within
+    1e-9.  At 6^3 the fine prolongator (343 rows, 1-8 blocks each) spans several chunks."""
+    mesh = pa.box_mesh(1, 1, 1, 6, 6, 6, jitter=0.1, seed=0)
+    d, v, r = uniaxial_constraints(mesh, -0.02)
+    U1 = (np.array([-0.004, 0.0015, 0.0015]) * mesh.nodes).ravel()
+    out = {}
+    for stream in (True, False):
+        o = pa.default_options(precision=precision, coarse_max=60, stream_prolongation=stream)
+        m = pa.Model(mesh, pa.J2Material(**MAT), options=o)
+        m.constrain(d, v, r)
+        m._freeze()
+        m.assemble(U1, commit=True)
+        res = []
+        for k in (1, 2, 3):
+            U = U1 * (1 + 0.5 * k)
+            m.U = U
+            m.assemble(U)
+            m.impose_dirichlet(1.0)
+            dU, it, used, relres = m.linear_solve(1e-10, rebuild_pc=True)
+            res.append((dU, it, used, relres))
+        out[stream] = (res, m.stats().amg_levels)
+        m.close()
+    (ra, la), (rb, lb) = out[True], out[False]
+    print(f"[streamed prolongation] {precision}: levels {la}, iterations {[x[1] for x in ra]} / "
+          f"{[x[1] for x in rb]}, max rel diff {max(rel(a[0], b[0]) for a, b in zip(ra, rb)):.1e}", flush=True)
+    assert la == lb and la >= 3
+    for (a, ia, ua, rra), (b, ib, ub, rrb) in zip(ra, rb):
+        assert abs(ia - ib) <= 1 and ua == ub
+        assert abs(np.log10(rra / rrb)) < 1
+        assert rel(a, b) <= 1e-9
+
+
 @pytest.mark.parametrize("gamma,alpha", [(0.9, 1.5), (0.5, 2.0)])
 def test_eisenstat_walker_forcing_terms_match_oracle(gamma, alpha, need_gpu):
     """The Krylov tolerance of every Newton iteration (Solver.jl:371) is the oracle's
