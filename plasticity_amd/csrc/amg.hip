@@ -46,6 +46,43 @@ __device__ __forceinline__ int find_sorted(const int32_t *a, int lo, int hi, int
     return -1;
 }
 
+// G independent searches of sorted rows advanced in lockstep, so that their dependent
+// loads overlap: m[u] = index in [lo[u], hi[u]) with a[index] == key, or -1.  The
+// Galerkin kernels take SEARCH_G terms of a row at a time (AP at 100^3: 4.74 -> 3.84 ms;
+// 8 at a time cost occupancy: 4.18 ms).
+constexpr int SEARCH_G = 4;
+template <int G>
+__device__ __forceinline__ void find_sorted_lockstep(const int32_t *__restrict__ a, int (&lo)[G], int (&hi)[G],
+                                                     int32_t key, int (&m)[G]) {
+#pragma unroll
+    for (int u = 0; u < G; ++u) m[u] = -1;
+    bool more = true;
+    while (more) {
+        int mid[G];
+        int32_t v[G];
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            mid[u] = (lo[u] + hi[u]) >> 1;
+            v[u] = lo[u] < hi[u] ? a[mid[u]] : 0;
+        }
+        more = false;
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            if (lo[u] < hi[u]) {
+                if (v[u] == key) {
+                    m[u] = mid[u];
+                    hi[u] = lo[u];
+                } else if (v[u] < key) {
+                    lo[u] = mid[u] + 1;
+                } else {
+                    hi[u] = mid[u];
+                }
+            }
+            more = more || lo[u] < hi[u];
+        }
+    }
+}
+
 // Gauss-Jordan inverse with partial pivoting of a BS x BS diagonal block.  A zero
 // row (a coarse dof whose rigid-body column was dropped by the QR) is treated as
 // decoupled: its diagonal is taken as 1.
@@ -154,19 +191,31 @@ __global__ __launch_bounds__(128) void AP_kernel(int64_t nQ, const int32_t *__re
     double acc[B6];
     for (int t = 0; t < B6; ++t) acc[t] = 0.0;
     RowIt<SELL> row(i, slice_ptr, rowlen, rowptr);
-    for (int k = 0; k < row.len; ++k) {
-        const int64_t pos = row.pos(k);
-        const int32_t j = colind[pos];
-        const int m = find_sorted(P_colind, P_rowptr[j], P_rowptr[j + 1], J);
-        if (m < 0) continue;
-        const double *a = A + (int64_t)BS * BS * pos;
-        const double *p = P + (int64_t)B6 * m;
-        for (int r = 0; r < BS; ++r)
-            for (int c = 0; c < 6; ++c) {
-                double s = 0.0;
-                for (int t = 0; t < BS; ++t) s += a[BS * r + t] * p[6 * t + c];
-                acc[6 * r + c] += s;
-            }
+    // G terms of the row at a time: their searches run in lockstep, then the hits are
+    // added in row order
+    for (int k0 = 0; k0 < row.len; k0 += SEARCH_G) {
+        int lo[SEARCH_G], hi[SEARCH_G], m[SEARCH_G];
+        int32_t j[SEARCH_G];
+#pragma unroll
+        for (int u = 0; u < SEARCH_G; ++u) j[u] = k0 + u < row.len ? colind[row.pos(k0 + u)] : -1;
+#pragma unroll
+        for (int u = 0; u < SEARCH_G; ++u) {
+            lo[u] = j[u] >= 0 ? P_rowptr[j[u]] : 0;
+            hi[u] = j[u] >= 0 ? P_rowptr[j[u] + 1] : 0;
+        }
+        find_sorted_lockstep(P_colind, lo, hi, J, m);
+#pragma unroll
+        for (int u = 0; u < SEARCH_G; ++u) {
+            if (m[u] < 0) continue;
+            const double *a = A + (int64_t)BS * BS * row.pos(k0 + u);
+            const double *p = P + (int64_t)B6 * m[u];
+            for (int r = 0; r < BS; ++r)
+                for (int c = 0; c < 6; ++c) {
+                    double s = 0.0;
+                    for (int t = 0; t < BS; ++t) s += a[BS * r + t] * p[6 * t + c];
+                    acc[6 * r + c] += s;
+                }
+        }
     }
     double *o = Q + (int64_t)B6 * q;
     for (int t = 0; t < B6; ++t) o[t] = acc[t];
@@ -184,18 +233,30 @@ __global__ __launch_bounds__(128) void RAP_kernel(int64_t nC, const int32_t *__r
     const int32_t I = C_row[cb], J = C_colind[cb];
     double acc[36];
     for (int t = 0; t < 36; ++t) acc[t] = 0.0;
-    for (int m = R_rowptr[I]; m < R_rowptr[I + 1]; ++m) {
-        const int32_t i = R_colind[m];
-        const int q = find_sorted(Q_colind, Q_rowptr[i], Q_rowptr[i + 1], J);
-        if (q < 0) continue;
-        const double *rb = R + (int64_t)6 * BS * m;
-        const double *qb = Q + (int64_t)BS * 6 * q;
-        for (int r = 0; r < 6; ++r)
-            for (int c = 0; c < 6; ++c) {
-                double s = 0.0;
-                for (int t = 0; t < BS; ++t) s += rb[BS * r + t] * qb[6 * t + c];
-                acc[6 * r + c] += s;
-            }
+    const int m1 = R_rowptr[I + 1];
+    for (int m0 = R_rowptr[I]; m0 < m1; m0 += SEARCH_G) {  // lockstep searches, hits in order
+        int lo[SEARCH_G], hi[SEARCH_G], q[SEARCH_G];
+        int32_t i[SEARCH_G];
+#pragma unroll
+        for (int u = 0; u < SEARCH_G; ++u) i[u] = m0 + u < m1 ? R_colind[m0 + u] : -1;
+#pragma unroll
+        for (int u = 0; u < SEARCH_G; ++u) {
+            lo[u] = i[u] >= 0 ? Q_rowptr[i[u]] : 0;
+            hi[u] = i[u] >= 0 ? Q_rowptr[i[u] + 1] : 0;
+        }
+        find_sorted_lockstep(Q_colind, lo, hi, J, q);
+#pragma unroll
+        for (int u = 0; u < SEARCH_G; ++u) {
+            if (q[u] < 0) continue;
+            const double *rb = R + (int64_t)6 * BS * (m0 + u);
+            const double *qb = Q + (int64_t)BS * 6 * q[u];
+            for (int r = 0; r < 6; ++r)
+                for (int c = 0; c < 6; ++c) {
+                    double s = 0.0;
+                    for (int t = 0; t < BS; ++t) s += rb[BS * r + t] * qb[6 * t + c];
+                    acc[6 * r + c] += s;
+                }
+        }
     }
     double *o = C + 36 * cb;
     for (int t = 0; t < 36; ++t) o[t] = acc[t];
@@ -212,6 +273,41 @@ __global__ void transpose_P_kernel(int64_t nR, const int32_t *__restrict__ R_src
         for (int c = 0; c < BS; ++c) o[BS * r + c] = p[6 * c + r];
 }
 
+// Row r of the blocks k0..k1-1 of a block row times x at their columns, summed in
+// storage order; ROW_U blocks per trip, all loaded before any is added, so each thread
+// keeps ROW_U blocks' loads in flight.
+constexpr int ROW_U = 4;
+template <int BR, int BC, class VT, class XT>
+__device__ __forceinline__ VT block_row_dot(int k0, int k1, int r, const int32_t *__restrict__ colind,
+                                            const VT *__restrict__ vals, const XT *__restrict__ x) {
+    VT s = 0;
+    int k = k0;
+    for (; k + ROW_U <= k1; k += ROW_U) {
+        VT a[ROW_U][BC], p[ROW_U][BC];
+#pragma unroll
+        for (int u = 0; u < ROW_U; ++u) {
+            const VT *v = vals + (int64_t)BR * BC * (k + u) + BC * r;
+            const XT *xx = x + (int64_t)BC * colind[k + u];
+#pragma unroll
+            for (int c = 0; c < BC; ++c) {
+                a[u][c] = v[c];
+                p[u][c] = (VT)xx[c];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < ROW_U; ++u)
+#pragma unroll
+            for (int c = 0; c < BC; ++c) s += a[u][c] * p[u][c];
+    }
+    for (; k < k1; ++k) {
+        const VT *v = vals + (int64_t)BR * BC * k + BC * r;
+        const XT *xx = x + (int64_t)BC * colind[k];
+#pragma unroll
+        for (int c = 0; c < BC; ++c) s += v[c] * (VT)xx[c];
+    }
+    return s;
+}
+
 // y = A x + beta*y; one thread per (block row, row-in-block)
 template <int BR, int BC, class VT, class XT>
 __global__ void bsr_spmv_kernel(int64_t nb, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colind,
@@ -221,13 +317,7 @@ __global__ void bsr_spmv_kernel(int64_t nb, const int32_t *__restrict__ rowptr, 
     const int64_t i = t / BR;
     const int r = (int)(t % BR);
     if (i >= nb) return;
-    VT s = 0;
-    for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
-        const VT *v = vals + (int64_t)BR * BC * k + BC * r;
-        const XT *xx = x + (int64_t)BC * colind[k];
-#pragma unroll
-        for (int c = 0; c < BC; ++c) s += v[c] * (VT)xx[c];
-    }
+    const VT s = block_row_dot<BR, BC, VT, XT>(rowptr[i], rowptr[i + 1], r, colind, vals, x);
     const int64_t o = (int64_t)BR * i + r;
     y[o] = (XT)(beta == 0.0 ? (double)s : (double)s + beta * (double)y[o]);
 }
@@ -246,17 +336,37 @@ __global__ __launch_bounds__(256) void bsr_spmv_wave_kernel(int64_t nb, const in
     VT acc[BR];
 #pragma unroll
     for (int r = 0; r < BR; ++r) acc[r] = 0;
+    // two of the lane's blocks per trip, both loaded before either is used (a row has
+    // ~10^2 blocks: two trips); the lane still adds its blocks in storage order
     const int k1 = rowptr[i + 1];
-    for (int k = rowptr[i] + lane; k < k1; k += 64) {
-        const VT *v = vals + (int64_t)BR * BC * k;
-        const XT *xx = x + (int64_t)BC * colind[k];
-        VT xv[BC];
+    for (int k = rowptr[i] + lane; k < k1; k += 128) {
+        const bool two = k + 64 < k1;
+        VT va[BR * BC], vb[BR * BC], xa[BC], xb[BC];
+        {
+            const VT *v = vals + (int64_t)BR * BC * k;
+            const XT *xx = x + (int64_t)BC * colind[k];
 #pragma unroll
-        for (int c = 0; c < BC; ++c) xv[c] = (VT)xx[c];
+            for (int t = 0; t < BR * BC; ++t) va[t] = v[t];
+#pragma unroll
+            for (int c = 0; c < BC; ++c) xa[c] = (VT)xx[c];
+        }
+        if (two) {
+            const VT *v = vals + (int64_t)BR * BC * (k + 64);
+            const XT *xx = x + (int64_t)BC * colind[k + 64];
+#pragma unroll
+            for (int t = 0; t < BR * BC; ++t) vb[t] = v[t];
+#pragma unroll
+            for (int c = 0; c < BC; ++c) xb[c] = (VT)xx[c];
+        }
 #pragma unroll
         for (int r = 0; r < BR; ++r)
 #pragma unroll
-            for (int c = 0; c < BC; ++c) acc[r] += v[BC * r + c] * xv[c];
+            for (int c = 0; c < BC; ++c) acc[r] += va[BC * r + c] * xa[c];
+        if (two)
+#pragma unroll
+            for (int r = 0; r < BR; ++r)
+#pragma unroll
+                for (int c = 0; c < BC; ++c) acc[r] += vb[BC * r + c] * xb[c];
     }
 #pragma unroll
     for (int r = 0; r < BR; ++r)
@@ -284,15 +394,33 @@ __global__ __launch_bounds__(64) void bsr_stream_kernel(const int32_t *__restric
     const int32_t r0 = chunk_row[blockIdx.x], r1 = chunk_row[blockIdx.x + 1];
     const int32_t b0 = rowptr[r0];
     const int nt = (rowptr[r1] - b0) * BR;
-    for (int t = lane; t < nt; t += 64) {
-        const int b = t / BR, r = t - BR * b;
-        const int64_t k = (int64_t)b0 + b;
-        const VT *v = vals + (int64_t)BR * BC * k + BC * r;
-        const XT *xx = x + (int64_t)BC * colind[k];
-        VT s = 0;
+    for (int t0 = lane; t0 < nt; t0 += 64 * ROW_U) {  // ROW_U products per trip, loads first
+        VT a[ROW_U][BC], p[ROW_U][BC];
 #pragma unroll
-        for (int c = 0; c < BC; ++c) s += v[c] * (VT)xx[c];
-        part[t] = s;
+        for (int u = 0; u < ROW_U; ++u) {
+            const int t = t0 + 64 * u;
+            if (t < nt) {
+                const int b = t / BR, r = t - BR * b;
+                const int64_t k = (int64_t)b0 + b;
+                const VT *v = vals + (int64_t)BR * BC * k + BC * r;
+                const XT *xx = x + (int64_t)BC * colind[k];
+#pragma unroll
+                for (int c = 0; c < BC; ++c) {
+                    a[u][c] = v[c];
+                    p[u][c] = (VT)xx[c];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < ROW_U; ++u) {
+            const int t = t0 + 64 * u;
+            if (t < nt) {
+                VT sum = 0;
+#pragma unroll
+                for (int c = 0; c < BC; ++c) sum += a[u][c] * p[u][c];
+                part[t] = sum;
+            }
+        }
     }
     __syncthreads();
     const int no = (r1 - r0) * BR;
@@ -315,13 +443,7 @@ __global__ void bsr_residual_kernel(int64_t nb, const int32_t *__restrict__ rowp
     const int64_t i = t / BS;
     const int rr = (int)(t % BS);
     if (i >= nb) return;
-    VT s = 0;
-    for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
-        const VT *v = vals + (int64_t)BS * BS * k + BS * rr;
-        const XT *xx = x + (int64_t)BS * colind[k];
-#pragma unroll
-        for (int c = 0; c < BS; ++c) s += v[c] * (VT)xx[c];
-    }
+    const VT s = block_row_dot<BS, BS, VT, XT>(rowptr[i], rowptr[i + 1], rr, colind, vals, x);
     const int64_t o = (int64_t)BS * i + rr;
     r[o] = (XT)((VT)b[o] - s);
 }
