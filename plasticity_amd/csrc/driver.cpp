@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "amg.h"
 #include "context.h"
@@ -243,12 +244,16 @@ static int bicgstab(pfem_ctx *c, const T *b, T *x, double rtol, int itmax, int *
 // `pc_norm`, else ||r|| <= rtol ||b|| (preconditioners that need not be SPD).
 // 0 solved, 1 itmax, 2 breakdown; stalls (no residual decrease over `stall` iterations)
 // return 3 so that the caller can fall back.
+struct Amg32Pc;
 template <class PC>
 static int fpcg(pfem_ctx *c, const double *b, double *x, double rtol, int itmax, bool pc_norm, int stall,
                 int *iters, PC &&pcfun) {
     hipStream_t s = c->stream;
     const int64_t n = c->ndof;
     double *r = c->w_r.p, *z = c->w_z.p, *p = c->w_p.p, *q = c->w_q.p, *sc = c->scal.p;
+    // the FP32 V-cycle: the update writes its FP32 input, the dot pass widens its output
+    constexpr bool amg32 = std::is_same<std::decay_t<PC>, Amg32Pc>::value;
+    if (c->mpartial.n < multi_dot_partials()) c->mpartial.alloc(multi_dot_partials());
     fill<double>(n, 0.0, x, s);
     PF_HIP(hipMemcpyAsync(r, b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
     pcfun(r, z);
@@ -269,11 +274,16 @@ static int fpcg(pfem_ctx *c, const double *b, double *x, double rtol, int itmax,
         PF_HIP(hipEventRecord(c->ev0, s));
         Prec<double>::spmv_dot(c, p, q, sc + 1);  // q = K p, sc[1] = p'q
         PF_HIP(hipEventRecord(c->ev1, s));
-        cg_update<double>(n, sc + 0, sc + 1, p, q, x, r, s);  // x += a p, r -= a q
-        pcfun(r, z);
-        dot_dev<double, double>(n, r, z, c->partial.p, sc + 2, s);  // r'z
-        dot_dev<double, double>(n, z, q, c->partial.p, sc + 4, s);  // z'q
-        dot_dev<double, double>(n, r, r, c->partial.p, sc + 3, s);  // r'r
+        // x += a p, r -= a q; z = M(r); sc[2] = r'z, sc[3] = r'r, sc[4] = z'q
+        if constexpr (amg32) {
+            cg_update_r32(n, sc + 0, sc + 1, p, q, x, r, c->f_r.p, s);
+            amg_apply32(c, c->amg.get(), c->f_r.p, c->f_z.p);
+            fcg_dots<float>(n, r, c->f_z.p, z, q, c->mpartial.p, sc + 2, s);
+        } else {
+            cg_update<double>(n, sc + 0, sc + 1, p, q, x, r, s);
+            pcfun(r, z);
+            fcg_dots<double>(n, r, z, nullptr, q, c->mpartial.p, sc + 2, s);
+        }
         PF_HIP(hipMemcpyAsync(c->h_scal, sc, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
         c->sync();
         const double pq = c->h_scal[1], rzn = c->h_scal[2], rrn = c->h_scal[3];

@@ -36,6 +36,15 @@ void xpby_ratio(int64_t n, const double *num, const double *den, const T *z, T *
 template <class T>
 void cg_update(int64_t n, const double *num, const double *den, const T *p, const T *q, T *x, T *r,
                hipStream_t s);
+// cg_update<double> that also writes r32 = (float) r
+void cg_update_r32(int64_t n, const double *num, const double *den, const double *p, const double *q, double *x,
+                   double *r, float *r32, hipStream_t s);
+// out[0] = r'z, out[1] = r'r, out[2] = z'q in one pass (z read as ZT and, when zout is
+// not null, written to zout as FP64); partial holds 3 * RED_BLOCKS doubles.  Each
+// equals the separate dot product bit for bit.
+template <class ZT>
+void fcg_dots(int64_t n, const double *r, const ZT *z, double *zout, const double *q, double *partial, double *out,
+              hipStream_t s);
 template <class T>
 void axpby(int64_t n, double a, const T *x, double b, T *y, hipStream_t s);
 void axpy_safe_ratio(int64_t n, const double *num, const double *den, double sign, const double *x, double *y,

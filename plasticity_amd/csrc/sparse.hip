@@ -179,6 +179,44 @@ __global__ void cg_update_mixed_kernel(int64_t n, const double *num, const doubl
         r[i] = (float)((double)r[i] - al * (double)q[i]);
     }
 }
+// cg_update_kernel<double> that also writes r32 = (float) r: the FP32 copy of the
+// residual the FP32 V-cycle takes, without a separate conversion pass
+__global__ void cg_update_r32_kernel(int64_t n, const double *num, const double *den, const double *__restrict__ p,
+                                     const double *__restrict__ q, double *__restrict__ x, double *__restrict__ r,
+                                     float *__restrict__ r32) {
+    const double al = *num / *den;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        x[i] = x[i] + al * p[i];
+        const double ri = r[i] - al * q[i];
+        r[i] = ri;
+        r32[i] = (float)ri;
+    }
+}
+// The three dot products of a flexible-CG step in one pass, r'z, r'r and z'q, with z
+// read as ZT (FP32: the V-cycle's output, written out as FP64 z); per-thread order and
+// grid as dot_partial_kernel, so each equals the separate dot product bit for bit.
+// partial: 3 rows of gridDim.x.
+template <class ZT>
+__global__ __launch_bounds__(256) void fcg_dots_kernel(int64_t n, const double *__restrict__ r,
+                                                       const ZT *__restrict__ zin, double *__restrict__ zout,
+                                                       const double *__restrict__ q, double *__restrict__ partial) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const double ri = r[i], zi = (double)zin[i];
+        if (zout) zout[i] = zi;
+        s0 += ri * zi;
+        s1 += ri * ri;
+        s2 += zi * q[i];
+    }
+    s0 = block_sum(s0);
+    s1 = block_sum(s1);
+    s2 = block_sum(s2);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = s0;
+        partial[gridDim.x + blockIdx.x] = s1;
+        partial[2 * gridDim.x + blockIdx.x] = s2;
+    }
+}
 template <class T>
 __global__ void axpby_kernel(int64_t n, double a, const T *__restrict__ x, double b, T *__restrict__ y) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -401,6 +439,22 @@ void xpby_ratio(int64_t n, const double *num, const double *den, const T *z, T *
     hipLaunchKernelGGL((xpby_ratio_kernel<T>), dim3(grid_strided(n)), dim3(256), 0, s, n, num, den, z, p);
     PF_HIP(hipGetLastError());
 }
+void cg_update_r32(int64_t n, const double *num, const double *den, const double *p, const double *q, double *x,
+                   double *r, float *r32, hipStream_t s) {
+    hipLaunchKernelGGL(cg_update_r32_kernel, dim3(grid_strided(n)), dim3(256), 0, s, n, num, den, p, q, x, r, r32);
+    PF_HIP(hipGetLastError());
+}
+template <class ZT>
+void fcg_dots(int64_t n, const double *r, const ZT *z, double *zout, const double *q, double *partial, double *out,
+              hipStream_t s) {
+    hipLaunchKernelGGL((fcg_dots_kernel<ZT>), dim3(RED_BLOCKS), dim3(256), 0, s, n, r, z, zout, q, partial);
+    hipLaunchKernelGGL(multi_finish_kernel, dim3(3), dim3(1024), 0, s, RED_BLOCKS, partial, out);
+    PF_HIP(hipGetLastError());
+}
+template void fcg_dots<double>(int64_t, const double *, const double *, double *, const double *, double *,
+                               double *, hipStream_t);
+template void fcg_dots<float>(int64_t, const double *, const float *, double *, const double *, double *, double *,
+                              hipStream_t);
 template <class T>
 void cg_update(int64_t n, const double *num, const double *den, const T *p, const T *q, T *x, T *r,
                hipStream_t s) {
