@@ -150,23 +150,26 @@ static int pcg(pfem_ctx *c, const T *b, T *x, double rtol, int itmax, int *iters
         PF_HIP(hipEventRecord(c->ev0, s));
         Prec<T>::spmv_dot(c, p, q, sc + 1);  // q = K p, sc[1] = p'q
         PF_HIP(hipEventRecord(c->ev1, s));
-        const double pq = c->sync_read(sc + 1);
+        // p'q is read with r'z, one host round trip per iteration: after a breakdown
+        // (p'q <= 0) this iteration's update is discarded with the solve (every caller's
+        // next solver restarts from x = 0)
+        cg_update<T>(n, sc + 0, sc + 1, p, q, x, r, s);  // x += a p, r -= a q
+        Prec<T>::pc(c, r, z);
+        dot_dev<T, T>(n, r, z, c->partial.p, sc + 2, s);
+        PF_HIP(hipMemcpyAsync(c->h_scal + 1, sc + 1, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+        c->sync();
+        const double pq = c->h_scal[1], rzn = c->h_scal[2];
         if (sizeof(T) == 8) {
             float ms = 0.f;
             PF_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
             c->st.t_pcg_spmv += 1e-3 * ms;
             c->st.n_pcg_spmv++;
         }
+        *iters = it;
         if (!(pq > 0)) {
-            *iters = it;
             if (c->opt.verbose > 1) fprintf(stderr, "      pcg breakdown p'Kp = %.3e at it %d\n", pq, it);
             return 2;
         }
-        cg_update<T>(n, sc + 0, sc + 1, p, q, x, r, s);  // x += a p, r -= a q
-        Prec<T>::pc(c, r, z);
-        dot_dev<T, T>(n, r, z, c->partial.p, sc + 2, s);
-        const double rzn = c->sync_read(sc + 2);
-        *iters = it;
         if (!(rzn >= 0)) {
             if (c->opt.verbose > 1) fprintf(stderr, "      pcg breakdown r'z = %.3e at it %d\n", rzn, it);
             return 2;
