@@ -117,8 +117,12 @@ struct ElementArgs {
 constexpr int PK = 132;     // doubles per Gauss-point packet
 
 void launch_gp_kernel(const ElementArgs &a, int64_t e_begin, int64_t e_count, hipStream_t s);
+// first: per element, bit 8a+b set where block (a, b) is the first contribution to its
+// slot in colour order (stored, not added: K needs no clearing before the scatter)
 void launch_scatter_kernel(const ElementArgs &a, const int64_t *elem_list, int64_t count,
-                           const int32_t *slots, double *Kv, double *R, hipStream_t s);
+                           const int32_t *slots, const uint64_t *first, double *Kv, double *R, hipStream_t s);
+void build_first_touch(int64_t nslots, int ncolors, const int64_t *order, const int64_t *color_ptr,
+                       const int32_t *slots, uint64_t *mask, hipStream_t s);
 void launch_dense_ke_kernel(const ElementArgs &a, int64_t nelem, double *Fe, double *Ke,
                             hipStream_t s);
 void launch_detF_kernel(const ElementArgs &a, int64_t gp, double *out, hipStream_t s);

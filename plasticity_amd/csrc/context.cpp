@@ -278,6 +278,8 @@ int pfem_create(const pfem_mesh_desc *md, const pfem_options *opts, pfem_ctx **o
             for (int64_t e = 0; e < ne; ++e) order[f[c->h_color[e]]++] = e;
             c->color_order.alloc(ne); c->color_order.upload(order.data(), ne, s);
     }
+        c->first_touch.alloc(ne);
+        build_first_touch(K.nslots, c->ncolors, c->color_order.p, c->color_ptr.data(), c->slots.p, c->first_touch.p, s);
         c->t_create_pattern = now() - tb0;
 
         // ---- state (GaussState Mesh.jl:132-138) and scratch

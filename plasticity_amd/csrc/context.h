@@ -32,6 +32,7 @@ struct pfem_ctx {
     // pattern, colours, scatter slots
     pfem::FineMatrix K;
     pfem::dbuf<int32_t> slots;  // 64 per element: slot of block (a, b)
+    pfem::dbuf<uint64_t> first_touch;  // per element: blocks that open their slot in colour order
     int ncolors = 0;
     std::vector<int64_t> color_ptr;
     std::vector<int32_t> h_color;

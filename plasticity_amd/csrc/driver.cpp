@@ -56,10 +56,9 @@ int assemble_impl(pfem_ctx *c, int commit, pfem_fail *fail, bool state_only) {
     ElementArgs a = element_args(c, commit);
     if (state_only) a.packet = nullptr;  // stress and history only: no tangent, no K or R
     PF_HIP(hipMemsetAsync(c->fail_key.p, 0xff, sizeof(unsigned long long), s));
-    if (!state_only) {
-        c->K.v64.zero(s);  // Assembly.jl:361-362
-        c->R.zero(s);
-    }
+    // Assembly.jl:361-362 zeroes K and R: R here, K through the scatter's first-touch
+    // stores
+    if (!state_only) c->R.zero(s);
     PF_HIP(hipEventRecord(c->ev0, s));
     launch_gp_kernel(a, 0, c->nelem, s);
     PF_HIP(hipEventRecord(c->ev1, s));
@@ -79,7 +78,7 @@ int assemble_impl(pfem_ctx *c, int commit, pfem_fail *fail, bool state_only) {
     PF_HIP(hipEventRecord(c->ev0, s));
     for (int col = 0; col < c->ncolors; ++col)
         launch_scatter_kernel(a, c->color_order.p + c->color_ptr[col], c->color_ptr[col + 1] - c->color_ptr[col],
-                              c->slots.p, c->K.v64.p, c->R.p, s);
+                              c->slots.p, c->first_touch.p, c->K.v64.p, c->R.p, s);
     PF_HIP(hipEventRecord(c->ev1, s));
     c->sync();
     float ms_sc = 0.f;

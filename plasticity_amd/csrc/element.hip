@@ -783,6 +783,7 @@ __global__ __launch_bounds__(64) void scatter_kernel(const double *__restrict__ 
                                                       const int64_t *__restrict__ elist, int64_t count,
                                                       const int64_t *__restrict__ conn,
                                                       const int32_t *__restrict__ slots,
+                                                      const uint64_t *__restrict__ first,
                                                       double *__restrict__ Kv, double *__restrict__ R, int fbar) {
     constexpr int NC = PK / 2;  // 16-byte chunks per Gauss-point packet (66)
     __shared__ __attribute__((aligned(16))) double sp[2][PK];
@@ -862,9 +863,16 @@ __global__ __launch_bounds__(64) void scatter_kernel(const double *__restrict__ 
         }
         __syncthreads();
     }
+    // the block's first contribution in colour order is stored (K is not cleared
+    // before the assembly), every later one added
     double *kb = Kv + 9 * (int64_t)slots[64 * e + 8 * a + b];
+    if ((first[e] >> (8 * a + b)) & 1) {
 #pragma unroll
-    for (int t = 0; t < 9; ++t) kb[t] += acc[t];
+        for (int t = 0; t < 9; ++t) kb[t] = acc[t];
+    } else {
+#pragma unroll
+        for (int t = 0; t < 9; ++t) kb[t] += acc[t];
+    }
     if (b == 0) {
         const int64_t n = conn[8 * e + a];
 #pragma unroll
@@ -938,11 +946,49 @@ void launch_gp_kernel(const ElementArgs &a, int64_t e_begin, int64_t e_count, hi
 }
 
 void launch_scatter_kernel(const ElementArgs &a, const int64_t *elem_list, int64_t count,
-                           const int32_t *slots, double *Kv, double *R, hipStream_t s) {
+                           const int32_t *slots, const uint64_t *first, double *Kv, double *R, hipStream_t s) {
     if (count <= 0) return;
     hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)count), dim3(64), 0, s, a.packet, a.g0, elem_list, count,
-                       a.conn, slots, Kv, R, a.fbar);
+                       a.conn, slots, first, Kv, R, a.fbar);
     PF_HIP(hipGetLastError());
+}
+
+// First-touch masks of the scatter.  Elements of one colour share no node, so no two
+// of them touch one K block: the block's first contribution comes from the element of
+// the lowest colour that holds both its nodes.  Pass 1 takes the lowest colour per slot,
+// pass 2 sets bit 8a+b of element e where (a, b) is that first touch.
+__global__ void slot_fill_kernel(int64_t n, int32_t v, int32_t *__restrict__ d) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        d[i] = v;
+}
+__global__ __launch_bounds__(64) void first_colour_kernel(const int64_t *__restrict__ elist,
+                                                          const int32_t *__restrict__ slots, int32_t col,
+                                                          int32_t *__restrict__ minc) {
+    const int64_t e = elist[blockIdx.x];
+    atomicMin(minc + slots[64 * e + threadIdx.x], col);
+}
+__global__ __launch_bounds__(64) void first_mask_kernel(const int64_t *__restrict__ elist,
+                                                        const int32_t *__restrict__ slots, int32_t col,
+                                                        const int32_t *__restrict__ minc,
+                                                        uint64_t *__restrict__ mask) {
+    const int64_t e = elist[blockIdx.x];
+    const uint64_t m = __ballot(minc[slots[64 * e + threadIdx.x]] == col);
+    if (threadIdx.x == 0) mask[e] = m;
+}
+void build_first_touch(int64_t nslots, int ncolors, const int64_t *order, const int64_t *color_ptr,
+                       const int32_t *slots, uint64_t *mask, hipStream_t s) {
+    dbuf<int32_t> minc(std::max<int64_t>(1, nslots));
+    hipLaunchKernelGGL(slot_fill_kernel, dim3(1024), dim3(256), 0, s, nslots, INT32_MAX, minc.p);
+    for (int col = 0; col < ncolors; ++col)
+        if (color_ptr[col + 1] > color_ptr[col])
+            hipLaunchKernelGGL(first_colour_kernel, dim3((unsigned)(color_ptr[col + 1] - color_ptr[col])), dim3(64), 0,
+                               s, order + color_ptr[col], slots, col, minc.p);
+    for (int col = 0; col < ncolors; ++col)
+        if (color_ptr[col + 1] > color_ptr[col])
+            hipLaunchKernelGGL(first_mask_kernel, dim3((unsigned)(color_ptr[col + 1] - color_ptr[col])), dim3(64), 0,
+                               s, order + color_ptr[col], slots, col, minc.p, mask);
+    PF_HIP(hipGetLastError());
+    PF_HIP(hipStreamSynchronize(s));  // minc is freed on return
 }
 
 void launch_dense_ke_kernel(const ElementArgs &a, int64_t nelem, double *Fe, double *Ke, hipStream_t s) {
