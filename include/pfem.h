@@ -76,7 +76,22 @@ typedef enum pfem_smoother {
     PFEM_SMOOTH_CHEBYSHEV4 = 2  /* fourth-kind Chebyshev smoother in D^-1 A (Lottes 2022), no lower bound */
 } pfem_smoother;
 
-/* Linear-solve options (LinearSolveState, Solver.jl:86-111). */
+/* AMG rebuild policy of pfem_newton (pfem_options.amg_rebuild).  The reference
+ * rebuilds once per load step (Solver.jl:374); the others trade set-up time
+ * against Krylov iterations. */
+typedef enum pfem_amg_rebuild {
+    PFEM_AMG_PER_LOAD_STEP = 0,     /* full set-up at the first Newton iteration of a load step (the reference) */
+    PFEM_AMG_EVERY_ITERATION = 1,   /* full set-up from every Newton iteration's tangent */
+    PFEM_AMG_FIRST_TWO = 2,         /* at the first and second Newton iteration of a load step */
+    PFEM_AMG_FINE_SMOOTHER = 3,     /* at the first, then only the fine-level smoother from each tangent */
+    PFEM_AMG_KEEP_P_IN_STEP = 4,    /* at the first, then all but the prolongators from each tangent */
+    PFEM_AMG_KEEP_P = 5,            /* prolongators from the first set-up, the rest from every tangent */
+    PFEM_AMG_KEEP_P_UNTIL_FALLBACK = 6  /* as 5, the prolongators rebuilt after a solve that needed a fallback
+                                           (the library default) */
+} pfem_amg_rebuild;
+
+/* Linear-solve options (LinearSolveState, Solver.jl:86-111) and the switches of the
+ * device driver.  pfem_default_options fills every field. */
 typedef struct pfem_options {
     int32_t precision;          /* pfem_precision */
     int32_t smoother;           /* pfem_smoother */
@@ -87,33 +102,29 @@ typedef struct pfem_options {
     int32_t coarse_max;         /* coarsest-level size target in DOF (default 1500) */
     int32_t max_levels;         /* AMG level cap (default 10) */
     int32_t verbose;            /* print per-iteration lines to stderr */
-    int32_t reserved[7];        /* reserved[0]: AMG rebuild policy, 0 = once per load step
-                                   (Solver.jl:374; default), 1 = every Newton iteration,
-                                   2 = at the first two Newton iterations of a load step,
-                                   3 = at the first, then only the fine-level smoother
-                                   from each new tangent, 4 = at the first, then all
-                                   but the prolongators from each new tangent, 5 = the
-                                   prolongators once (first set-up), all the rest from
-                                   every new tangent, 6 = as 5, the prolongators rebuilt
-                                   after a solve that needed a fallback;
-                                   reserved[1]: directions kept by the flexible GCR of the
-                                   learned preconditioner (<= 0: 16);
-                                   reserved[2]: FP32-IR only, 1 = the fine-level products
-                                   inside the V-cycle's smoother use an FP16 copy of S K S
-                                   (S = diag|K_ii|^-1/2), 0 = the FP32 copy of K (default);
-                                   reserved[3]: pfem_newton backtracking on ||R|| with at
-                                   most this many step halvings (0: the reference's full
-                                   Newton step, Solver.jl:383);
-                                   reserved[4], reserved[5]: power-iteration steps of the
-                                   rho(D^-1 A) estimate behind the Chebyshev bounds, first
-                                   set-up and warm-started rebuilds (<= 0: 25 and 4);
-                                   reserved[6]: bit 0 = build the pattern, slots and
-                                   colours on the host (pfem_create); bit 1 = run the
-                                   V-cycle eagerly instead of replaying it as a HIP graph;
-                                   bit 4 = AMG W-cycle (default: V-cycle); bit 7 = BiCGStab
-                                   instead of flexible GCR after a CG breakdown; bit 8 =
-                                   fine-level prolongation with one thread per row instead
-                                   of over chunks of consecutive rows */
+    int32_t amg_rebuild;        /* pfem_amg_rebuild (default PFEM_AMG_KEEP_P_UNTIL_FALLBACK) */
+    int32_t amg_wcycle;         /* 1 = W-cycle (levels below the finest visited twice), 0 = V-cycle (default) */
+    int32_t krylov_fallback;    /* after a CG breakdown or stall: 0 = flexible GCR(16) with the same AMG cycle
+                                   (default), 1 = BiCGStab */
+    int32_t gcr_directions;     /* directions kept by the learned preconditioner's flexible GCR (<= 0: 16) */
+    int32_t fp16_fine;          /* FP32-IR: 1 = the V-cycle's fine-level smoother products use an FP16 copy of
+                                   S K S (S = diag|K_ii|^-1/2); 0 = the FP32 copy of K (default) */
+    int32_t line_search;        /* pfem_newton backtracking on ||R|| with at most this many step halvings
+                                   (0: the reference's full Newton step, Solver.jl:383; default) */
+    int32_t rho_iters_cold;     /* power-iteration steps of the rho(D^-1 A) estimate behind the Chebyshev */
+    int32_t rho_iters_warm;     /* bounds: first set-up / warm-started rebuilds (<= 0: 25 / 4) */
+    int32_t host_build;         /* 1 = build the pattern, slots and colours on the host in pfem_create */
+    int32_t vcycle_eager;       /* 1 = launch the V-cycle's kernels one by one instead of replaying a HIP graph */
+    int32_t prolong_rowwise;    /* 1 = fine-level prolongation with one thread per row instead of over chunks
+                                   of consecutive rows */
+    int32_t fp32_krylov;        /* FP32-IR: 1 = the Krylov iteration itself runs in FP32 (FP32 K p and FP32
+                                   vectors) inside FP64 iterative refinement (FP64 residual b - K x once per
+                                   refinement sweep); 0 = FP64 flexible CG whose preconditioner is the FP32
+                                   V-cycle (default) */
+    double newton_diverge;      /* pfem_newton stops a load step early (not converged) when, from the third
+                                   iteration on, ||R_k|| > newton_diverge * ||R_1||; 0 = never (the reference
+                                   runs maxiter iterations; default).  Used with load-step cut-back. */
+    int32_t reserved[8];        /* zero */
 } pfem_options;
 
 typedef struct pfem_fail {
@@ -158,43 +169,85 @@ int pfem_reset(pfem_ctx *ctx);
 
 int pfem_set_displacement(pfem_ctx *ctx, const double *U);
 int pfem_get_displacement(pfem_ctx *ctx, double *U);
-/* Committed Gauss state (GaussState, Mesh.jl:124-163); any pointer may be NULL.
- * epsp, beta, cpinv, sig: 6 x ngp; abar: ngp.  beta is the back stress (for
- * finite strain in the reference configuration); it is stored only for the
+/* Committed Gauss state (state_committed, GaussState, Mesh.jl:124-163); any pointer
+ * may be NULL.  epsp, beta, cpinv, sig: 6 x ngp; abar: ngp.  beta is the back stress
+ * (for finite strain in the reference configuration); it is stored only for the
  * small-strain element or when a material has Hkin > 0 -- otherwise it is
- * identically zero (set ignores it, get returns zeros).  sig holds the last
- * assembled stress: Kirchhoff for finite strain, Cauchy for small strain. */
+ * identically zero (set ignores it, get returns zeros).  sig is the committed
+ * stress (Kirchhoff for finite strain, Cauchy for small strain), written by
+ * pfem_commit only (Solver.jl:470-471); it is zero after pfem_reset. */
 int pfem_set_state(pfem_ctx *ctx, const double *epsp, const double *beta, const double *abar,
                    const double *cpinv);
 int pfem_get_state(pfem_ctx *ctx, double *epsp, double *beta, double *abar, double *cpinv, double *sig);
 
+/* The stress of the last assembly (the trial state's sigma that assemble! overwrites,
+ * Assembly.jl:238-257): 6 x ngp, Kirchhoff for finite strain. */
+int pfem_get_trial_stress(pfem_ctx *ctx, double *sig);
+
+/* gauss_stress (Model.jl:188-208): the committed stress as Cauchy stress, 6 x ngp.
+ * Finite strain: tau / det F with the un-barred F = I + grad U at each Gauss point
+ * and the device-resident U (1/det F replaced by 1 where det F <= 0); small strain:
+ * the committed stress itself.  Computed on the device. */
+int pfem_gauss_stress(pfem_ctx *ctx, double *sig);
+
+/* U <- U at the last pfem_commit (zero after pfem_reset or pfem_create): returns the
+ * model to its committed state after a load step that failed, for the load-step
+ * bisection of lagrangian/docs/SCALING.md:268-274.  The committed history is never
+ * touched by a trial assembly, so U is all that has to be restored. */
+int pfem_rollback(pfem_ctx *ctx);
+
 /* assemble! (Assembly.jl:238-257): K (device) and R = F_int overwritten from U
  * (host array, or NULL to use the device-resident U) and the committed
- * history; sig overwritten; commit != 0 writes the new history.  R_out may be
- * NULL.  Returns PFEM_ERR_INVERTED with `fail` filled on det F <= 0. */
+ * history; the trial stress overwritten; commit != 0 also writes the new history
+ * and the committed stress.  R_out may be NULL.  Returns PFEM_ERR_INVERTED with
+ * `fail` filled on det F <= 0; K is then marked invalid (pfem_get_K_* values,
+ * pfem_impose_dirichlet* and pfem_linear_solve refuse it with PFEM_ERR_BAD_ARG)
+ * until an assembly succeeds.  Before the first assembly K is the zero matrix on
+ * the pattern. */
 int pfem_assemble(pfem_ctx *ctx, const double *U, int commit, double *R_out, pfem_fail *fail);
 
 /* R -= lambda * Fext (Solver.jl:326-327, :349). */
 int pfem_subtract_external(pfem_ctx *ctx, double lambda);
 
 /* Both impose_dirichlet! calls of newton! (Solver.jl:354-355,
- * BoundaryConditions.jl:62-109) on the device K and R at load factor lambda. */
+ * BoundaryConditions.jl:62-109) on the device K and R at load factor lambda:
+ * the ramped set, then the fixed set. */
 int pfem_impose_dirichlet(pfem_ctx *ctx, double lambda);
+
+typedef enum pfem_bc_set {
+    PFEM_BC_RAMPED = 0,         /* dir_ramp of _freeze_bcs (Solver.jl:192-218): g = lambda * value */
+    PFEM_BC_FIXED = 1           /* dir_fix: g = value */
+} pfem_bc_set;
+
+/* One impose_dirichlet!(K, R, bc, lambda, U) call (BoundaryConditions.jl:62-109) for one
+ * of the two frozen sets registered by pfem_set_dirichlet: delta g = g - U[d];
+ * R[r] += K[r,d] delta g on rows r outside the set; the set's rows and columns of K
+ * zeroed with a unit diagonal; R[d] = -delta g.  U (ndof, or NULL: the device U)
+ * is uploaded first.  R (ndof, or NULL) is in/out on the host: when given, it
+ * replaces the device R before the imposition and receives the result, so a host
+ * driver that keeps R itself (newton! with R -= Fext on the host) can call it
+ * unchanged. */
+int pfem_impose_dirichlet_set(pfem_ctx *ctx, int set, double lambda, const double *U, double *R);
 
 /* ||R||_2 (Solver.jl:358). */
 int pfem_residual_norm(pfem_ctx *ctx, double *out);
 
-/* _linear_solve!(dU, K, b = -R, ls, rtol, rebuild_pc) (Solver.jl:235-290).
- * dU stays on the device; niter = Krylov iterations (all attempts);
+/* _linear_solve!(dU, K, b, ls, rtol, rebuild_pc) (Solver.jl:235-290).
+ * b: host right-hand side (ndof), or NULL for b = -R from the device R.  The solution
+ * stays on the device (pfem_update_displacement, pfem_get_dU) and is also copied to
+ * the host array dU when it is not NULL.  niter = Krylov iterations (all attempts);
  * solver_used: 0 PCG, 1 PCG after AMG refresh, 2 BiCGStab fallback (or the FP64
  * ladder after FP32-IR failed), 3 FP32-IR flexible PCG, 4 FP32-IR with BiCGStab,
  * 5 flexible GCR with the learned preconditioner, 6 the same finished by AMG,
  * 7 FP32-IR finished by flexible GCR(16) with the FP32 AMG cycle after a CG breakdown,
  * 8 FP64 finished by flexible GCR(16) with the AMG cycle after PCG failed.  The GCR
- * fallbacks (7, 8) are the default; reserved[6] bit 7 selects BiCGStab (2, 4).
- * true_relres = ||b - K dU|| / ||b||. */
-int pfem_linear_solve(pfem_ctx *ctx, double rtol, int rebuild_pc, int *niter,
-                      int *solver_used, double *true_relres);
+ * fallbacks (7, 8) are the default; krylov_fallback = 1 selects BiCGStab (2, 4).
+ * true_relres = ||b - K dU|| / ||b||.
+ * A solve that ends with ||b - K dU|| > ||b|| returns dU = 0 (not applied); its status
+ * is then PFEM_ERR_NOT_CONVERGED or PFEM_ERR_NON_SPD.  Both statuses leave a usable dU
+ * otherwise (the reference's fallbacks always produce a step, Solver.jl:257-286). */
+int pfem_linear_solve(pfem_ctx *ctx, const double *b, double rtol, int rebuild_pc, int *niter,
+                      int *solver_used, double *true_relres, double *dU);
 
 /* U += dU (Solver.jl:383). */
 int pfem_update_displacement(pfem_ctx *ctx);
@@ -212,14 +265,24 @@ int pfem_get_colors(pfem_ctx *ctx, int32_t *colour, int32_t *ncolors);
 typedef struct pfem_newton_result {
     int32_t converged;
     int32_t iters;              /* Newton iterations (length of the residual history) */
-    int32_t krylov_iters[64];   /* per Newton iteration */
+    int32_t krylov_iters[64];   /* per Newton iteration that solved (iters - converged of them) */
     int32_t solver_used[64];
     double residuals[64];
     double t_assemble, t_bc, t_setup, t_solve, t_total; /* seconds */
     double eta[64];             /* Eisenstat-Walker forcing term = Krylov rtol per iteration (Solver.jl:371) */
+    int32_t solves;             /* linear solves done (= iters - converged unless stopped early) */
+    int32_t discarded;          /* solves whose correction was discarded (||b - K dU|| > ||b||) */
+    int32_t stopped;            /* 0 ran to convergence or maxiter; 1 stopped by newton_diverge; 2 stopped
+                                   after two discarded solves in a row (no usable correction) */
 } pfem_newton_result;
 
-/* One load step of newton! at load factor lambda (Solver.jl:317-389). */
+/* One load step of newton! at load factor lambda (Solver.jl:317-389): at each iteration
+ * assemble, R -= lambda Fext, impose the ramped then the fixed Dirichlet set, test
+ * ||R|| <= tol * max(||R_1||, ||Fext||, 1); otherwise solve with the Eisenstat-Walker
+ * tolerance and U += dU -- also at iteration maxiter, as the reference does, so a
+ * step that does not converge leaves U where the reference leaves it.  A solve whose
+ * correction is discarded makes the next iteration rebuild the whole preconditioner;
+ * a second one in a row ends the step (not converged, stopped = 2). */
 int pfem_newton(pfem_ctx *ctx, double lambda, double tol, int maxiter,
                 pfem_newton_result *res, pfem_fail *fail);
 

@@ -98,6 +98,7 @@ lib.or_amg_vcycle.argtypes = [C.c_void_p, _dp, _dp]
 lib.or_amg_apply_A.argtypes = [C.c_void_p, _dp, _dp]
 lib.or_impose_dirichlet.argtypes = [C.c_int64, _ip, _ip, _dp, _dp, C.c_int64, _ip, _dp, C.c_int,
                                     C.c_double, _dp]
+lib.or_gauss_stress.argtypes = [C.c_int, C.c_int64, _dp, _ip, _dp, _dp, _dp]
 
 
 class ElementInversionError(Exception):
@@ -405,6 +406,7 @@ class OracleModel:
         self.cptr, self.corder = colour_lists(col, nc)
         self.nthreads = nthreads
         self.U = np.zeros(self.ndof)
+        self.U_c = np.zeros(self.ndof)   # U at the last commit (restored after a failed increment)
         self.R = np.zeros(self.ndof)
         ngp = 8 * self.ne
         self.st_c = self._new_state(ngp)
@@ -470,8 +472,10 @@ class OracleModel:
             raise ElementInversionError(fe.value, fg.value, fJ.value)
         return self.K, self.R
 
-    def newton(self, lam, Fext, bcs, tol=1e-8, maxiter=25, solver=None):
-        """newton! (Solver.jl:317-389).  `solver(K, b, eta)` -> (dU, niter)."""
+    def newton(self, lam, Fext, bcs, tol=1e-8, maxiter=25, solver=None, diverge=0.0):
+        """newton! (Solver.jl:317-389).  `solver(K, b, eta)` -> (dU, niter).  diverge > 0 stops
+        the step (not converged) when, from the third iteration on, ||R|| > diverge * ||R_1||
+        (the device driver's optional early stop; the reference runs maxiter iterations)."""
         import scipy.sparse.linalg as spla
         if solver is None:
             solver = lambda K, b, eta: (spla.spsolve(K.tocsc(), b), 0)  # noqa: E731
@@ -490,40 +494,96 @@ class OracleModel:
             hist.append(rnorm)
             if it == 1:
                 ref = max(rnorm, fext_norm, 1.0)
+                r1 = rnorm
             if rnorm <= tol * ref:
                 conv = True
                 break
+            if diverge > 0 and it >= 3 and not rnorm <= diverge * r1:
+                break
             eta = forcing_term(it, rnorm, rkm1, eta_prev)
-            dU, n = solver(K, -R, eta)
+            dU, n = solver(K, -R, eta)                                # :378-379 (also at maxiter)
             cg.append(n)
-            self.U += dU
+            self.U += dU                                              # :383
             rkm1, eta_prev = rnorm, eta
         return conv, hist, cg
 
-    def solve(self, nsteps=10, tol=1e-8, maxiter=25, solver=None, callback=None):
-        """solve! (Solver.jl:419-475) with the direct linear solve."""
+    def commit(self):
+        """solve!'s commit (Solver.jl:470-471)."""
+        trial = {k: v.copy() for k, v in self.st_c.items()}
+        self.assemble(self.U, trial, commit=True)                     # :470
+        self.st_c = trial                                              # :471
+        self.U_c = self.U.copy()
+
+    def gauss_stress(self):
+        """gauss_stress (Model.jl:188-208): committed tau / det F (un-barred F at the current U)."""
+        out = np.zeros((8 * self.ne, 6))
+        lib.or_gauss_stress(self.kind, self.ne, self.nodes, self.conn, _v(self.U),
+                            np.ascontiguousarray(self.st_c["sig"]), out)
+        return out
+
+    def solve(self, nsteps=10, tol=1e-8, maxiter=25, solver=None, callback=None, cutback=0, grow_after=2,
+              diverge=0.0, steps=None, records=None):
+        """solve! (Solver.jl:419-475) with the direct linear solve.
+
+        cutback > 0: load-step bisection (lagrangian/docs/SCALING.md:268-274).  A failed
+        increment (Newton not converged, or an inverted element) is undone (U back to the
+        committed U; the committed history is untouched) and halved, at most `cutback`
+        times below 1/nsteps; after `grow_after` converged increments in a row it doubles
+        again, never above 1/nsteps, and an increment that would pass the load step's end
+        n/nsteps is cut to end there.  `records` (a list) receives one
+        (lam, converged, newton iterations) tuple per attempt."""
         self.U[:] = 0.0
+        self.U_c = self.U.copy()
         self.st_c = self._new_state(8 * self.ne)
         bcs = self.freeze_bcs()
         Fext0 = np.zeros(self.ndof)
         for d, v in zip(self.neu_dofs, self.neu_vals):
             Fext0[d] += v
         iters, res, cgs, allconv = [], [], [], True
-        for n in range(1, nsteps + 1):
-            lam = n / nsteps
-            Fext = np.zeros(self.ndof)
-            for d, v in zip(self.neu_dofs, self.neu_vals):
-                Fext[d] += lam * v
-            conv, hist, cg = self.newton(lam, Fext, bcs, tol=tol, maxiter=maxiter, solver=solver)
-            iters.append(len(hist))
-            res.append(hist)
-            cgs.append(cg)
-            if not conv:
+        nominal = 1.0 / nsteps
+        dl, streak, lam_c = nominal, 0, 0.0
+        dl_min = nominal / 2.0 ** cutback
+        for n in (range(1, nsteps + 1) if steps is None else steps):
+            target = n / nsteps
+            ok = True
+            while True:
+                lam = target if lam_c + dl >= target - 1e-12 * nominal else lam_c + dl
+                Fext = np.zeros(self.ndof)
+                for d, v in zip(self.neu_dofs, self.neu_vals):
+                    Fext[d] += lam * v                                 # assemble_neumann! :44-49
+                try:
+                    conv, hist, cg = self.newton(lam, Fext, bcs, tol=tol, maxiter=maxiter, solver=solver,
+                                                 diverge=diverge)
+                except ElementInversionError:
+                    conv, hist, cg = False, [], []
+                if records is not None:
+                    records.append((lam, conv, len(hist)))
+                if conv:
+                    iters.append(len(hist))
+                    res.append(hist)
+                    cgs.append(cg)
+                    self.commit()
+                    lam_c = lam
+                    streak += 1
+                    if streak >= grow_after and dl < nominal:
+                        dl, streak = min(2.0 * dl, nominal), 0
+                    if lam == target:
+                        break
+                    continue
+                half = 0.5 * (lam - lam_c)
+                if cutback <= 0 or half < dl_min * (1.0 - 1e-9):
+                    if cutback > 0:
+                        self.U = self.U_c.copy()
+                    iters.append(len(hist))
+                    res.append(hist)
+                    cgs.append(cg)
+                    ok = False
+                    break
+                self.U = self.U_c.copy()
+                dl, streak = half, 0
+            if not ok:
                 allconv = False
                 break
-            trial = {k: v.copy() for k, v in self.st_c.items()}
-            self.assemble(self.U, trial, commit=True)                 # :470
-            self.st_c = trial                                          # :471
             if callback is not None:
                 callback(n, self)
         return allconv, iters, res, cgs

@@ -906,3 +906,35 @@ void or_impose_dirichlet(int64_t n, const int64_t *colptr, const int64_t *rowval
     free(isbc);
     free(dg);
 }
+
+/* ------------------------------------------------------------------ */
+/* gauss_stress (Model.jl:188-208): committed Kirchhoff stress tau -> */
+/* Cauchy sigma = tau / J with J = det F of the un-barred              */
+/* F = I + grad U (_defgrad, Model.jl:212-222) at each Gauss point,    */
+/* 1/J replaced by 1 where J <= 0.  kind 2 (small strain) copies tau.   */
+/* ------------------------------------------------------------------ */
+void or_gauss_stress(int kind, int64_t nelem, const double *nodes, const int64_t *conn,
+                     const double *U, const double *tau, double *sig) {
+    for (int64_t e = 0; e < nelem; ++e) {
+        double Xe[24], ue[24];
+        for (int a = 0; a < 8; ++a)
+            for (int d = 0; d < 3; ++d) {
+                Xe[3 * a + d] = nodes[3 * conn[8 * e + a] + d];
+                ue[3 * a + d] = U[3 * conn[8 * e + a] + d];
+            }
+        for (int g = 0; g < 8; ++g) {
+            const int64_t gp = 8 * e + g;
+            double invJ = 1.0;
+            if (kind != 2) {
+                const double q = 0.57735026918962576451;
+                const double xi[3] = {GSGN[g][0] * q, GSGN[g][1] * q, GSGN[g][2] * q};
+                double dNdX[24], dj, F[9];
+                or_ref_grads(Xe, xi, dNdX, &dj);
+                defgrad(ue, dNdX, F);
+                const double J = det3(F);
+                invJ = J > 0 ? 1.0 / J : 1.0;
+            }
+            for (int k = 0; k < 6; ++k) sig[6 * gp + k] = tau[6 * gp + k] * invJ;
+        }
+    }
+}

@@ -134,6 +134,11 @@ void or_impose_dirichlet(int64_t n, const int64_t *colptr, const int64_t *rowval
                          const double *values, int ramp, double lambda,
                          const double *U);
 
+/* gauss_stress (Model.jl:188-208): sig = tau / det F (un-barred F at each Gauss point);
+ * kind 2 (small strain) copies tau. */
+void or_gauss_stress(int kind, int64_t nelem, const double *nodes, const int64_t *conn,
+                     const double *U, const double *tau, double *sig);
+
 #ifdef __cplusplus
 }
 #endif

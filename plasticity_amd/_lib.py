@@ -42,11 +42,27 @@ class MeshDesc(C.Structure):
                 ("materials", C.c_void_p), ("element_kind", C.c_int32)]
 
 
+AMG_PER_LOAD_STEP = 0
+AMG_EVERY_ITERATION = 1
+AMG_FIRST_TWO = 2
+AMG_FINE_SMOOTHER = 3
+AMG_KEEP_P_IN_STEP = 4
+AMG_KEEP_P = 5
+AMG_KEEP_P_UNTIL_FALLBACK = 6
+BC_RAMPED = 0
+BC_FIXED = 1
+
+
 class Options(C.Structure):
     _fields_ = [("precision", C.c_int32), ("smoother", C.c_int32), ("smoother_degree", C.c_int32),
                 ("cg_itmax", C.c_int32), ("eta_min", C.c_double), ("eta_max", C.c_double),
                 ("ew_gamma", C.c_double), ("ew_alpha", C.c_double), ("coarse_max", C.c_int32),
-                ("max_levels", C.c_int32), ("verbose", C.c_int32), ("reserved", C.c_int32 * 7)]
+                ("max_levels", C.c_int32), ("verbose", C.c_int32), ("amg_rebuild", C.c_int32),
+                ("amg_wcycle", C.c_int32), ("krylov_fallback", C.c_int32), ("gcr_directions", C.c_int32),
+                ("fp16_fine", C.c_int32), ("line_search", C.c_int32), ("rho_iters_cold", C.c_int32),
+                ("rho_iters_warm", C.c_int32), ("host_build", C.c_int32), ("vcycle_eager", C.c_int32),
+                ("prolong_rowwise", C.c_int32), ("fp32_krylov", C.c_int32), ("newton_diverge", C.c_double),
+                ("reserved", C.c_int32 * 8)]
 
 
 class Fail(C.Structure):
@@ -58,7 +74,8 @@ class NewtonResult(C.Structure):
                 ("krylov_iters", C.c_int32 * 64), ("solver_used", C.c_int32 * 64),
                 ("residuals", C.c_double * 64), ("t_assemble", C.c_double), ("t_bc", C.c_double),
                 ("t_setup", C.c_double), ("t_solve", C.c_double), ("t_total", C.c_double),
-                ("eta", C.c_double * 64)]
+                ("eta", C.c_double * 64), ("solves", C.c_int32), ("discarded", C.c_int32),
+                ("stopped", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -97,11 +114,16 @@ SIGNATURES = {
     "pfem_get_displacement": (C.c_int, [_vp, _vp]),
     "pfem_set_state": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "pfem_get_state": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "pfem_get_trial_stress": (C.c_int, [_vp, _vp]),
+    "pfem_gauss_stress": (C.c_int, [_vp, _vp]),
+    "pfem_rollback": (C.c_int, [_vp]),
     "pfem_assemble": (C.c_int, [_vp, _vp, C.c_int, _vp, C.POINTER(Fail)]),
     "pfem_subtract_external": (C.c_int, [_vp, C.c_double]),
     "pfem_impose_dirichlet": (C.c_int, [_vp, C.c_double]),
+    "pfem_impose_dirichlet_set": (C.c_int, [_vp, C.c_int, C.c_double, _vp, _vp]),
     "pfem_residual_norm": (C.c_int, [_vp, _dp]),
-    "pfem_linear_solve": (C.c_int, [_vp, C.c_double, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp]),
+    "pfem_linear_solve": (C.c_int, [_vp, _vp, C.c_double, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp,
+                                    _vp]),
     "pfem_update_displacement": (C.c_int, [_vp]),
     "pfem_get_K_bsr": (C.c_int, [_vp, _i64p, _vp, _vp, _vp]),
     "pfem_get_K_csc": (C.c_int, [_vp, _i64p, _vp, _vp, _vp]),

@@ -158,11 +158,11 @@ Amg *amg_create(pfem_ctx *c) {
     hipStream_t s = c->stream;
     auto *a = new Amg();
     a->smoother = c->opt.smoother;
-    // V-cycle by default; reserved[6] bit 4 selects the W-cycle (a second coarse cycle at
+    // V-cycle by default; amg_wcycle selects the W-cycle (a second coarse cycle at
     // every level below the finest).  Under FP32-IR the W-cycle needed 20 instead of 28
     // Krylov iterations per solve when the prolongators were rebuilt at every Newton
     // iteration, but with them kept (rebuild policies 4-6) the V-cycle is faster overall.
-    a->wcycle = (c->opt.reserved[6] & 16) != 0;
+    a->wcycle = c->opt.amg_wcycle != 0;
     a->degree = c->opt.smoother_degree > 0 ? c->opt.smoother_degree : (a->smoother == PFEM_SMOOTH_JACOBI ? 1 : 2);
     const int64_t coarse_max = c->opt.coarse_max > 0 ? c->opt.coarse_max : 1500;
     const int max_levels = c->opt.max_levels > 1 ? c->opt.max_levels : 10;
@@ -271,9 +271,9 @@ Amg *amg_create(pfem_ctx *c) {
         for (int64_t i = 0; i < L.nb; ++i)
             for (int32_t m = Pptr[i]; m < Pptr[i + 1]; ++m) Prow[m] = (int32_t)i;
         up(L.P_rowptr, Pptr, s); up(L.P_colind, Pidx, s); up(L.P_row, Prow, s);
-        // chunks of the streamed prolongation (fine level; reserved[6] bit 8: one thread
+        // chunks of the streamed prolongation (fine level; prolong_rowwise: one thread
         // per row instead): greedy runs of whole rows of at most P_STREAM_CAP blocks
-        if (bs == 3 && !(c->opt.reserved[6] & 256)) {
+        if (bs == 3 && !c->opt.prolong_rowwise) {
             std::vector<int32_t> ch{0};
             bool fits = true;
             for (int64_t i = 0; i < L.nb; ++i) {
@@ -413,8 +413,8 @@ void amg_numeric_setup(pfem_ctx *c, Amg *a, bool keep_P) {
         const double tr = now();
         // power iteration converges from below, slowly where the top of the spectrum is
         // crowded (12 steps gave 0.91 rho on a notched bar): 25 steps cold, 4 warm
-        const int cold = c->opt.reserved[4] > 0 ? c->opt.reserved[4] : 25;
-        const int warm = c->opt.reserved[5] > 0 ? c->opt.reserved[5] : 4;
+        const int cold = c->opt.rho_iters_cold > 0 ? c->opt.rho_iters_cold : 25;
+        const int warm = c->opt.rho_iters_warm > 0 ? c->opt.rho_iters_warm : 4;
         L.rho = spectral_radius(c, a, (int)l, L.pw_warm ? warm : cold);
         t_rho += now() - tr;
         const double omega = (4.0 / 3.0) / L.rho;
@@ -481,7 +481,7 @@ void amg_refresh_fine(pfem_ctx *c, Amg *a) {
     amg_block_inverse<3>(L.nb, c->K.diagpos.p, c->K.v64.p, L.dinv.p, s);
     if (a->L.size() > 1) {
         const double tr = now();
-        const int warm = c->opt.reserved[5] > 0 ? c->opt.reserved[5] : 4;
+        const int warm = c->opt.rho_iters_warm > 0 ? c->opt.rho_iters_warm : 4;
         L.rho = spectral_radius(c, a, 0, warm);
         c->st.t_setup_rho += now() - tr;
     } else {
@@ -641,11 +641,11 @@ void vcycle(pfem_ctx *c, Amg *a, int l, const T *b, T *x) {
 namespace {
 // The V-cycle is a fixed sequence of ~50 launches (most of them small, on the coarse
 // levels) between rebuilds: capture it once on the context's stream and replay it as
-// one graph launch.  reserved[6] bit 1 runs it eagerly instead.  It records kernels and
+// one graph launch.  vcycle_eager runs it eagerly instead.  It records kernels and
 // device-to-device copies only (no memset nodes).
 template <class T>
 void apply_vcycle(pfem_ctx *c, Amg *a, int k, const T *r, T *z) {
-    if (c->opt.reserved[6] & 2) {
+    if (c->opt.vcycle_eager) {
         vcycle<T>(c, a, 0, r, z);
         return;
     }

@@ -126,5 +126,7 @@ void build_first_touch(int64_t nslots, int ncolors, const int64_t *order, const 
 void launch_dense_ke_kernel(const ElementArgs &a, int64_t nelem, double *Fe, double *Ke,
                             hipStream_t s);
 void launch_detF_kernel(const ElementArgs &a, int64_t gp, double *out, hipStream_t s);
+// gauss_stress: out = tau / det F (un-barred F) per Gauss point; copy for small strain
+void launch_cauchy_kernel(const ElementArgs &a, const double *tau, double *out, hipStream_t s);
 
 }  // namespace pfem

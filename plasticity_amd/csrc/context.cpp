@@ -119,6 +119,7 @@ void pfem_default_options(pfem_options *o) {
     o->coarse_max = 1500;
     o->max_levels = 10;
     o->verbose = 0;
+    o->amg_rebuild = PFEM_AMG_KEEP_P_UNTIL_FALLBACK;
 }
 
 int pfem_create(const pfem_mesh_desc *md, const pfem_options *opts, pfem_ctx **out) {
@@ -128,8 +129,6 @@ int pfem_create(const pfem_mesh_desc *md, const pfem_options *opts, pfem_ctx **o
         md->element_kind != PFEM_HEX8_SMALL)
         return PFEM_ERR_BAD_ARG;
     *out = nullptr;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PFEM_ERR_NO_DEVICE;
     std::vector<DevMat> dm(md->nmaterials);
     for (int i = 0; i < md->nmaterials; ++i) {
         int rc = make_devmat(md->materials[i], dm[i]);
@@ -137,9 +136,20 @@ int pfem_create(const pfem_mesh_desc *md, const pfem_options *opts, pfem_ctx **o
     }
     for (int64_t k = 0; k < 8 * md->nelem; ++k)
         if (md->conn[k] < 0 || md->conn[k] >= md->nnodes) return PFEM_ERR_BAD_ARG;
+    // eight distinct nodes per element: the scatter's slots (one per node pair) and its
+    // first-touch stores assume that no two lanes of an element share a K block
+    for (int64_t e = 0; e < md->nelem; ++e)
+        for (int a = 1; a < 8; ++a)
+            for (int b = 0; b < a; ++b)
+                if (md->conn[8 * e + a] == md->conn[8 * e + b]) {
+                    g_last_error = "element " + std::to_string(e) + " repeats node " + std::to_string(md->conn[8 * e + a]);
+                    return PFEM_ERR_BAD_ARG;
+                }
     if (md->material_id)
         for (int64_t e = 0; e < md->nelem; ++e)
             if (md->material_id[e] < 0 || md->material_id[e] >= md->nmaterials) return PFEM_ERR_BAD_ARG;
+    int ndev = 0;  // argument checks first: they need no device
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PFEM_ERR_NO_DEVICE;
     auto *c = new pfem_ctx();
     int rc = guarded(nullptr, [&]() -> int {
         if (opts) c->opt = *opts; else pfem_default_options(&c->opt);
@@ -161,9 +171,9 @@ int pfem_create(const pfem_mesh_desc *md, const pfem_options *opts, pfem_ctx **o
         c->mats.alloc(dm.size()); c->mats.upload(dm.data(), dm.size(), s);
 
         // ---- pattern (Assembly.jl:62-162) as SELL-64 BSR, scatter slots (on the device
-        //      unless reserved[6] bit 0 is set or a node is too dense for the device build)
+        //      unless host_build is set or a node is too dense for the device build)
         const double tb0 = now();
-        const bool host_build = (c->opt.reserved[6] & 1) != 0;
+        const bool host_build = c->opt.host_build != 0;
         FineMatrix &K = c->K;
         DevicePattern P;
         const int brc = host_build ? BUILD_TOO_DENSE : build_pattern_device(nn, ne, c->conn.p, P, s);
@@ -285,12 +295,14 @@ int pfem_create(const pfem_mesh_desc *md, const pfem_options *opts, pfem_ctx **o
         // ---- state (GaussState Mesh.jl:132-138) and scratch
         const int64_t ngp = 8 * ne;
         c->epsp.alloc(6 * ngp); c->abar.alloc(ngp); c->cpinv.alloc(6 * ngp); c->sig.alloc(6 * ngp);
+        c->sig_c.alloc(6 * ngp);
         if (c->kin || c->kind == PFEM_HEX8_SMALL) c->beta.alloc(6 * ngp);
         c->packet.alloc((size_t)ngp * PK);
         c->g0.alloc(24 * ne);
         c->fail_key.alloc(1);
         const int64_t nd = c->ndof;
         c->U.alloc(nd); c->R.alloc(nd); c->dU.alloc(nd); c->rhs.alloc(nd); c->Fext0.alloc(nd); c->dg.alloc(nd);
+        c->U_c.alloc(nd);
         c->Fext0.zero(s);
         c->bcflag.alloc(nd); c->bcflag.zero(s); c->bcval.alloc(nd); c->bcval.zero(s);
         for (auto *b : {&c->w_r, &c->w_z, &c->w_p, &c->w_q, &c->w_x, &c->w_s, &c->w_t, &c->w_v, &c->w_rh})
@@ -328,8 +340,8 @@ int pfem_reset(pfem_ctx *c) {
     return guarded(nullptr, [&]() -> int {
         hipStream_t s = c->stream;
         const int64_t ngp = 8 * c->nelem;
-        c->U.zero(s); c->dU.zero(s); c->R.zero(s);
-        c->epsp.zero(s); c->abar.zero(s); c->sig.zero(s);
+        c->U.zero(s); c->dU.zero(s); c->R.zero(s); c->U_c.zero(s);
+        c->epsp.zero(s); c->abar.zero(s); c->sig.zero(s); c->sig_c.zero(s);
         if (c->beta.p) c->beta.zero(s);
         fill_identity6(ngp, c->cpinv.p, s);
         c->sync();
@@ -436,7 +448,36 @@ int pfem_get_state(pfem_ctx *c, double *epsp, double *beta, double *abar, double
         }
         if (abar) c->abar.download(abar, ngp, c->stream);
         if (cpinv) c->cpinv.download(cpinv, 6 * ngp, c->stream);
-        if (sig) c->sig.download(sig, 6 * ngp, c->stream);
+        if (sig) c->sig_c.download(sig, 6 * ngp, c->stream);
+        c->sync();
+        return PFEM_OK;
+    });
+}
+int pfem_get_trial_stress(pfem_ctx *c, double *sig) {
+    if (!c || !sig) return PFEM_ERR_BAD_ARG;
+    return guarded(nullptr, [&]() -> int {
+        c->sig.download(sig, 6 * 8 * c->nelem, c->stream);
+        c->sync();
+        return PFEM_OK;
+    });
+}
+int pfem_gauss_stress(pfem_ctx *c, double *sig) {
+    if (!c || !sig) return PFEM_ERR_BAD_ARG;
+    return guarded(nullptr, [&]() -> int {
+        ElementArgs a{};
+        a.nelem = c->nelem; a.X = c->X.p; a.conn = c->conn.p; a.U = c->U.p; a.kind = c->kind;
+        // the Gauss-point packet buffer (element scratch, rewritten by every assembly)
+        // holds the result for the copy out
+        launch_cauchy_kernel(a, c->sig_c.p, c->packet.p, c->stream);
+        PF_HIP(hipMemcpyAsync(sig, c->packet.p, c->sig.bytes(), hipMemcpyDeviceToHost, c->stream));
+        c->sync();
+        return PFEM_OK;
+    });
+}
+int pfem_rollback(pfem_ctx *c) {
+    if (!c) return PFEM_ERR_BAD_ARG;
+    return guarded(nullptr, [&]() -> int {
+        PF_HIP(hipMemcpyAsync(c->U.p, c->U_c.p, c->ndof * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
         c->sync();
         return PFEM_OK;
     });
@@ -468,11 +509,31 @@ int pfem_subtract_external(pfem_ctx *c, double lambda) {
     });
 }
 
+static int need_K(pfem_ctx *c) {
+    if (c->K_valid) return PFEM_OK;
+    g_last_error = "K is invalid: the last assembly failed (inverted element)";
+    return PFEM_ERR_BAD_ARG;
+}
+
 int pfem_impose_dirichlet(pfem_ctx *c, double lambda) {
     if (!c) return PFEM_ERR_BAD_ARG;
+    if (int rc = need_K(c)) return rc;
     return guarded(nullptr, [&]() -> int {
-        impose_dirichlet_dev(c->K, c->nbc, c->bcdofs.p, c->bcflag.p, c->bcval.p, lambda, c->U.p, c->dg.p,
-                             c->nbcrows, c->bcrows.p, c->R.p, c->stream);
+        impose_set(c, 2, lambda);  // Solver.jl:354 dir_ramp
+        impose_set(c, 1, lambda);  // :355 dir_fix
+        c->sync();
+        return PFEM_OK;
+    });
+}
+
+int pfem_impose_dirichlet_set(pfem_ctx *c, int set, double lambda, const double *U, double *R) {
+    if (!c || (set != PFEM_BC_RAMPED && set != PFEM_BC_FIXED)) return PFEM_ERR_BAD_ARG;
+    if (int rc = need_K(c)) return rc;
+    return guarded(nullptr, [&]() -> int {
+        if (U) c->U.upload(U, c->ndof, c->stream);
+        if (R) c->R.upload(R, c->ndof, c->stream);
+        impose_set(c, set == PFEM_BC_RAMPED ? 2 : 1, lambda);
+        if (R) c->R.download(R, c->ndof, c->stream);
         c->sync();
         return PFEM_OK;
     });
@@ -496,17 +557,28 @@ int pfem_update_displacement(pfem_ctx *c) {
     });
 }
 
-int pfem_linear_solve(pfem_ctx *c, double rtol, int rebuild_pc, int *niter, int *used, double *relres) {
+int pfem_linear_solve(pfem_ctx *c, const double *b, double rtol, int rebuild_pc, int *niter, int *used,
+                      double *relres, double *dU) {
     if (!c || !(rtol > 0)) return PFEM_ERR_BAD_ARG;
+    if (int rc = need_K(c)) return rc;
     return guarded(nullptr, [&]() -> int {
-        // b = -R (Solver.jl:378)
-        axpby<double>(c->ndof, -1.0, c->R.p, 0.0, c->rhs.p, c->stream);
-        return linear_solve_impl(c, rtol, rebuild_pc ? 1 : 0, niter, used, relres);
+        if (b)
+            c->rhs.upload(b, c->ndof, c->stream);
+        else  // b = -R (Solver.jl:378)
+            axpby<double>(c->ndof, -1.0, c->R.p, 0.0, c->rhs.p, c->stream);
+        const int rc = linear_solve_impl(c, rtol, rebuild_pc ? 1 : 0, niter, used, relres);
+        if (dU && (rc == PFEM_OK || rc == PFEM_ERR_NOT_CONVERGED || rc == PFEM_ERR_NON_SPD)) {
+            c->dU.download(dU, c->ndof, c->stream);
+            c->sync();
+        }
+        return rc;
     });
 }
 
 int pfem_get_K_bsr(pfem_ctx *c, int64_t *nnzb, int32_t *rowptr, int32_t *colind, double *vals) {
     if (!c) return PFEM_ERR_BAD_ARG;
+    if (vals)
+        if (int rc = need_K(c)) return rc;
     return guarded(nullptr, [&]() -> int {
         const FineMatrix &K = c->K;
         if (nnzb) *nnzb = K.nnzb;
@@ -537,6 +609,8 @@ int pfem_get_K_csc(pfem_ctx *c, int64_t *nnz, int64_t *colptr, int64_t *rowval, 
     // neighbour nb of node j ascending.  The pattern is symmetric, so the blocks of
     // block column j are the blocks (nb, j) of block rows nb in adj(j).
     if (!c) return PFEM_ERR_BAD_ARG;
+    if (nzval)
+        if (int rc = need_K(c)) return rc;
     return guarded(nullptr, [&]() -> int {
         const FineMatrix &K = c->K;
         if (nnz) *nnz = 9 * K.nnzb;

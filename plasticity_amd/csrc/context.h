@@ -38,15 +38,17 @@ struct pfem_ctx {
     std::vector<int32_t> h_color;
     pfem::dbuf<int64_t> color_order;
 
-    // Gauss state (committed history + last stress) and element scratch
-    pfem::dbuf<double> epsp, abar, cpinv, sig;
+    // Gauss state: committed history (epsp, abar, cpinv, beta) and stress (sig_c), the
+    // stress of the last assembly (sig: the trial state's sigma), element scratch
+    pfem::dbuf<double> epsp, abar, cpinv, sig, sig_c;
     pfem::dbuf<double> beta;   // back stress; allocated for small strain or Hkin > 0 only
     pfem::dbuf<double> packet, g0;
     int64_t packet_elems = 0;  // elements per packet chunk
     pfem::dbuf<unsigned long long> fail_key;
 
-    // vectors
-    pfem::dbuf<double> U, R, dU, rhs, Fext0, dg;
+    // vectors (U_c: U at the last commit, restored by pfem_rollback)
+    pfem::dbuf<double> U, R, dU, rhs, Fext0, dg, U_c;
+    bool K_valid = true;   // false after a failed assembly (K holds a partial or stale tangent)
     double fext0_norm = 0.0;
 
     // boundary conditions
@@ -90,5 +92,9 @@ inline double now() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 int assemble_impl(pfem_ctx *c, int commit, pfem_fail *fail, bool state_only = false);
-int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *used, double *relres);
+int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *used, double *relres,
+                      bool *discarded = nullptr);
+// Dirichlet imposition on the device K and R for the constrained set `sel` (bit 0: the
+// fixed set, bit 1: the ramped set) at load factor lambda, against the device U.
+void impose_set(pfem_ctx *c, int sel, double lambda);
 }  // namespace pfem

@@ -54,7 +54,10 @@ int assemble_impl(pfem_ctx *c, int commit, pfem_fail *fail, bool state_only) {
     hipStream_t s = c->stream;
     const double t0 = now();
     ElementArgs a = element_args(c, commit);
+    // a commit writes the committed stress; a trial assembly the trial state's sigma
+    if (commit) a.sig = c->sig_c.p;
     if (state_only) a.packet = nullptr;  // stress and history only: no tangent, no K or R
+    else c->K_valid = false;             // until the scatter has run
     PF_HIP(hipMemsetAsync(c->fail_key.p, 0xff, sizeof(unsigned long long), s));
     // Assembly.jl:361-362 zeroes K and R: R here, K through the scatter's first-touch
     // stores
@@ -74,7 +77,15 @@ int assemble_impl(pfem_ctx *c, int commit, pfem_fail *fail, bool state_only) {
         if (fail) { fail->e = (int64_t)(key / 8); fail->g = (int32_t)(key % 8); fail->J = J; }
         return PFEM_ERR_INVERTED;
     }
-    if (state_only) return PFEM_OK;
+    if (commit) {  // the committed displacement, restored by pfem_rollback
+        PF_HIP(hipMemcpyAsync(c->U_c.p, c->U.p, c->ndof * sizeof(double), hipMemcpyDeviceToDevice, s));
+        if (!state_only)  // this assembly's stress is also the trial stress
+            PF_HIP(hipMemcpyAsync(c->sig.p, c->sig_c.p, c->sig.bytes(), hipMemcpyDeviceToDevice, s));
+    }
+    if (state_only) {
+        c->sync();
+        return PFEM_OK;
+    }
     PF_HIP(hipEventRecord(c->ev0, s));
     for (int col = 0; col < c->ncolors; ++col)
         launch_scatter_kernel(a, c->color_order.p + c->color_ptr[col], c->color_ptr[col + 1] - c->color_ptr[col],
@@ -86,7 +97,13 @@ int assemble_impl(pfem_ctx *c, int commit, pfem_fail *fail, bool state_only) {
     c->st.t_scatter_kernel += 1e-3 * ms_sc;
     c->st.t_assemble += now() - t0;
     c->st.n_assemble++;
+    c->K_valid = true;
     return PFEM_OK;
+}
+
+void impose_set(pfem_ctx *c, int sel, double lambda) {
+    impose_dirichlet_dev(c->K, c->nbc, c->bcdofs.p, c->bcflag.p, c->bcval.p, lambda, c->U.p, c->dg.p, c->nbcrows,
+                         c->bcrows.p, c->R.p, sel, c->stream);
 }
 
 // ---------------------------------------------------------------- preconditioner
@@ -501,7 +518,7 @@ static void prepare_fp32(pfem_ctx *c) {
             b->alloc(c->ndof);
     if (!c->K.v32.p) c->K.v32.alloc(9 * c->K.nslots);
     convert<double, float>(9 * c->K.nslots, c->K.v64.p, c->K.v32.p, c->stream);
-    c->half_fine = c->opt.reserved[2] == 1;  // FP16 scaled fine level inside the V-cycle's smoother
+    c->half_fine = c->opt.fp16_fine == 1;  // FP16 scaled fine level inside the V-cycle's smoother
     if (c->half_fine) half_scaled_copy(c->K, c->stream);
 }
 
@@ -515,7 +532,7 @@ static int learned_solve(pfem_ctx *c, double rtol, int itmax, int *total, int *w
     *which = 5;
     if (!c->net_dinv.p) c->net_dinv.alloc(9 * c->nnodes);
     block_diag_inverse3(c->K, c->net_dinv.p, s);
-    const int m = c->opt.reserved[1] > 0 ? std::min(c->opt.reserved[1], 40) : 16;
+    const int m = c->opt.gcr_directions > 0 ? std::min(c->opt.gcr_directions, 40) : 16;
     // hand over to AMG once the last 10 iterations together gained less than 0.9^10
     int rc = gcr_m(c, c->rhs.p, c->dU.p, rtol, itmax, m, 40, &it, LearnedPc{c}, 10, 0.9);
     *total += it;
@@ -555,7 +572,8 @@ static int learned_solve(pfem_ctx *c, double rtol, int itmax, int *total, int *w
 // rebuild: 0 reuse the preconditioner, 1 rebuild it, 2 refresh only the fine-level
 // smoother from the current K (interpolation and coarse operators kept), 3 rebuild it
 // keeping the prolongators of the last full set-up
-int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *used, double *relres) {
+int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *used, double *relres,
+                      bool *discarded) {
     const bool learned = c->net_mode >= 2;
     if (learned) {
         if (c->opt.precision == PFEM_PREC_FP32_IR) prepare_fp32(c);
@@ -595,8 +613,8 @@ int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *us
         total += it;
         // after a breakdown or a stall: flexible GCR(16) with the same FP32 cycle (it needs
         // no symmetry, minimises ||r|| and costs one product and one cycle per
-        // iteration); reserved[6] bit 7: refinement with BiCGStab instead
-        if (rc != 0 && !(c->opt.reserved[6] & 128)) {
+        // iteration); krylov_fallback = 1: refinement with BiCGStab instead
+        if (rc != 0 && c->opt.krylov_fallback == 0) {
             which = 7;
             rc = gcr_m(c, c->rhs.p, c->dU.p, rtol, 4 * itmax, 16, 40, &it, Amg32Pc{c});
             if (rc == 3) rc = 1;  // stalled: not converged
@@ -623,7 +641,7 @@ int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *us
             rc = pcg<double>(c, c->rhs.p, c->dU.p, rtol, itmax, &it);
             total += it;
         }
-        if (rc != 0 && !(c->opt.reserved[6] & 128)) {  // flexible GCR(16) with the FP64 cycle
+        if (rc != 0 && c->opt.krylov_fallback == 0) {  // flexible GCR(16) with the FP64 cycle
             which = 8;
             rc = gcr_m(c, c->rhs.p, c->dU.p, rtol, 2 * itmax, 16, 40, &it, Amg64Pc{c});
             if (rc == 3) rc = 1;  // stalled: not converged
@@ -642,10 +660,12 @@ int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *us
     c->sync();
     const double rr = std::sqrt(c->h_scal[5]), bb = std::sqrt(c->h_scal[6]);
     if (relres) *relres = bb > 0 ? rr / bb : 0.0;
-    if (!(rr <= bb)) {  // a failed solve that made things worse is not applied
+    const bool drop = !(rr <= bb);
+    if (drop) {  // a failed solve that made things worse is not applied
         fill<double>(c->ndof, 0.0, c->dU.p, s);
         c->sync();
     }
+    if (discarded) *discarded = drop;
     if (niter) *niter = total;
     if (used) *used = which;
     c->st.t_solve += now() - t1;
@@ -676,9 +696,10 @@ int pfem_newton(pfem_ctx *c, double lambda, double tol, int maxiter, pfem_newton
         hipStream_t s = c->stream;
         const double T0 = now();
         const double fext_norm = std::fabs(lambda) * c->fext0_norm;  // Solver.jl:336
-        double ref = 1.0, rkm1 = 0.0, eta_prev = c->opt.eta_max;
+        double ref = 1.0, rkm1 = 0.0, eta_prev = c->opt.eta_max, r1 = 0.0;
         bool first = true;
-        // K, R and ||R|| at the current U (assemble, R -= lambda Fext, both Dirichlet passes)
+        // K, R and ||R|| at the current U (assemble, R -= lambda Fext, impose_dirichlet! for
+        // the ramped set, then for the fixed set)
         auto evaluate = [&](double *rn) -> int {
             double t = now();
             int rc = assemble_impl(c, 0, fail);  // Solver.jl:347
@@ -686,18 +707,18 @@ int pfem_newton(pfem_ctx *c, double lambda, double tol, int maxiter, pfem_newton
             res->t_assemble += now() - t;
             t = now();
             sub_external(c->ndof, lambda, c->Fext0.p, c->R.p, s);  // :349
-            impose_dirichlet_dev(c->K, c->nbc, c->bcdofs.p, c->bcflag.p, c->bcval.p, lambda, c->U.p, c->dg.p,
-                                 c->nbcrows, c->bcrows.p, c->R.p, s);  // :354-355
+            impose_set(c, 2, lambda);                              // :354 dir_ramp
+            impose_set(c, 1, lambda);                              // :355 dir_fix
             dot_dev<double, double>(c->ndof, c->R.p, c->R.p, c->partial.p, c->scal.p + 7, s);
             *rn = std::sqrt(c->sync_read(c->scal.p + 7));  // :358
             res->t_bc += now() - t;
             return PFEM_OK;
         };
-        // Optional backtracking on ||R|| (reserved[3] = most halvings; 0 = the reference's
+        // Optional backtracking on ||R|| (line_search = most halvings; 0 = the reference's
         // full Newton step).  The residual at the accepted point is the next iteration's.
-        const int ls_max = std::max(0, std::min(c->opt.reserved[3], 20));
+        const int ls_max = std::max(0, std::min(c->opt.line_search, 20));
         double rnorm = 0.0;
-        bool have_residual = false;
+        bool have_residual = false, force_rebuild = false, last_discarded = false;
         for (int it = 1; it <= maxiter; ++it) {
             if (!have_residual) {
                 const int rc0 = evaluate(&rnorm);
@@ -707,59 +728,71 @@ int pfem_newton(pfem_ctx *c, double lambda, double tol, int maxiter, pfem_newton
             int rc = PFEM_OK;
             res->residuals[it - 1] = rnorm;
             res->iters = it;
-            if (it == 1) ref = std::max(std::max(rnorm, fext_norm), 1.0);  // :360-362
+            if (it == 1) {
+                ref = std::max(std::max(rnorm, fext_norm), 1.0);  // :360-362
+                r1 = rnorm;
+            }
             if (c->opt.verbose) fprintf(stderr, "    iter %d  |R| = %.6e\n", it, rnorm);
             if (rnorm <= tol * ref) {  // :364
                 res->converged = 1;
                 break;
             }
-            if (it == maxiter) break;
+            // optional early stop of a diverging load step (not in the reference, which runs
+            // maxiter iterations): from the third iteration on, ||R|| above newton_diverge
+            // times the first residual
+            if (c->opt.newton_diverge > 0 && it >= 3 && !(rnorm <= c->opt.newton_diverge * r1)) {
+                res->stopped = 1;
+                break;
+            }
             const double eta = forcing_term(c->opt, it, rnorm, rkm1, eta_prev);  // :371
             res->eta[it - 1] = eta;
-            // :374 rebuilds once per load step; amg_rebuild = 1 rebuilds from every
-            // Newton iteration's tangent (the plastic iterations then need about half the
-            // Krylov iterations of a hierarchy built from the elastic predictor tangent);
-            // amg_rebuild = 2 rebuilds at the first and second iteration (the second is the
-            // first plastic tangent) and reuses that hierarchy afterwards
-            // amg_rebuild = 3 builds at the first iteration and afterwards refreshes only
-            // the fine-level smoother from the current tangent
-            // amg_rebuild = 4 builds at the first iteration and afterwards recomputes
-            // everything but the prolongators from the current tangent (fewer Krylov
-            // iterations than rebuilding the prolongators too, at less set-up cost)
-            // amg_rebuild = 5 builds the prolongators once (first set-up of the context)
-            // and everything else from every Newton iteration's tangent
-            // amg_rebuild = 6 is 5, except that after a solve that needed a fallback the
-            // next set-up also rebuilds the prolongators (from the current tangent)
-            const int policy = c->opt.reserved[0];
-            const int rebuild = policy == 5                                     ? (c->pc_ready ? 3 : 1)
-                                : policy == 6 ? ((c->pc_ready && !c->refresh_P) ? 3 : 1)
-                                : (first || policy == 1 || (policy == 2 && it == 2)) ? 1
-                                : policy == 3                                   ? 2
-                                : policy == 4                                   ? 3
-                                                                                : 0;
+            // :374 rebuilds once per load step; the other policies (pfem_amg_rebuild) rebuild
+            // from later tangents too, keep the prolongators (3 = rebuild all but P), or
+            // refresh only the fine-level smoother (2)
+            const int policy = c->opt.amg_rebuild;
+            int rebuild = policy == 5                                            ? (c->pc_ready ? 3 : 1)
+                          : policy == 6 ? ((c->pc_ready && !c->refresh_P) ? 3 : 1)
+                          : (first || policy == 1 || (policy == 2 && it == 2)) ? 1
+                          : policy == 3                                          ? 2
+                          : policy == 4                                          ? 3
+                                                                                 : 0;
+            // a discarded correction is retried once with a preconditioner built from scratch
+            if (force_rebuild) rebuild = 1;
+            force_rebuild = false;
             first = false;
             axpby<double>(c->ndof, -1.0, c->R.p, 0.0, c->rhs.p, s);  // :378
             const double t = now();
             const double setup0 = c->st.t_setup;
             int niter = 0, used = 0;
             double rel = 0;
-            rc = linear_solve_impl(c, eta, rebuild, &niter, &used, &rel);  // :379
+            bool dropped = false;
+            rc = linear_solve_impl(c, eta, rebuild, &niter, &used, &rel, &dropped);  // :379
             res->t_setup += c->st.t_setup - setup0;
             res->t_solve += now() - t - (c->st.t_setup - setup0);
             res->krylov_iters[it - 1] = niter;
             res->solver_used[it - 1] = used;
+            res->solves++;
             // policy 6: a solve that needed a fallback (PCG retried after a refresh, the
             // BiCGStab ladder, or BiCGStab refinement) rebuilds the prolongators next time
             if (policy == 6) c->refresh_P = (used == 1 || used == 2 || used == 4 || used == 7 || used == 8);
             if (c->opt.verbose)
-                fprintf(stderr, "    iter %d  eta = %.2e  krylov = %d (solver %d, true relres %.2e)\n", it, eta, niter,
-                        used, rel);
+                fprintf(stderr, "    iter %d  eta = %.2e  krylov = %d (solver %d, true relres %.2e)%s\n", it, eta,
+                        niter, used, rel, dropped ? " discarded" : "");
             if (rc == PFEM_ERR_NOT_CONVERGED || rc == PFEM_ERR_NON_SPD) {
                 // keep the (inexact) correction, as the reference keeps the direct fallback's
             } else if (rc) {
                 return rc;
             }
-            axpby<double>(c->ndof, 1.0, c->dU.p, 1.0, c->U.p, s);  // :383
+            if (dropped) {
+                res->discarded++;
+                if (last_discarded) {  // no usable correction twice in a row: give up the step
+                    res->stopped = 2;
+                    break;
+                }
+                force_rebuild = true;
+            }
+            last_discarded = dropped;
+            axpby<double>(c->ndof, 1.0, c->dU.p, 1.0, c->U.p, s);  // :383 (also at it == maxiter)
             if (ls_max > 0 && it < maxiter) {
                 double alpha = 1.0, rn_new = 0.0;
                 for (int h = 0;; ++h) {

@@ -931,7 +931,44 @@ __global__ void detF_kernel(ElementArgs a, int64_t gp, double *out) {
     *out = det3(F);
 }
 
+// gauss_stress (Model.jl:188-208): the committed Kirchhoff stress divided by det F of
+// the un-barred F = I + sum_a u_a (dN_a/dX)^T at the Gauss point (1 where det F <= 0),
+// one thread per Gauss point; the small-strain element stores Cauchy stress already.
+__global__ __launch_bounds__(256) void cauchy_kernel(ElementArgs a, const double *__restrict__ tau,
+                                                     double *__restrict__ out) {
+    const int64_t gp = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (gp >= 8 * a.nelem) return;
+    double inv = 1.0;
+    if (a.kind != 2) {
+        const int64_t e = gp >> 3;
+        const int g = (int)(gp & 7);
+        double Xe[24], ue[24];
+        for (int n = 0; n < 8; ++n) {
+            const int64_t node = a.conn[8 * e + n];
+            for (int d = 0; d < 3; ++d) {
+                Xe[3 * n + d] = a.X[3 * node + d];
+                ue[3 * n + d] = a.U[3 * node + d];
+            }
+        }
+        const double gq = 0.57735026918962576451;
+        double dNdX[24], dj, F[9];
+        ref_grads(Xe, (((g + 1) & 2) ? gq : -gq), ((g & 2) ? gq : -gq), ((g & 4) ? gq : -gq), dNdX, dj);
+        defgrad(ue, dNdX, F);
+        const double J = det3(F);
+        inv = J > 0 ? 1.0 / J : 1.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) out[6 * gp + k] = tau[6 * gp + k] * inv;
+}
+
 }  // namespace
+
+void launch_cauchy_kernel(const ElementArgs &a, const double *tau, double *out, hipStream_t s) {
+    const int64_t ngp = 8 * a.nelem;
+    if (ngp <= 0) return;
+    hipLaunchKernelGGL(cauchy_kernel, dim3((unsigned)((ngp + 255) / 256)), dim3(256), 0, s, a, tau, out);
+    PF_HIP(hipGetLastError());
+}
 
 void launch_gp_kernel(const ElementArgs &a, int64_t e_begin, int64_t e_count, hipStream_t s) {
     if (e_count <= 0) return;

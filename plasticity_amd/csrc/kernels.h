@@ -56,9 +56,10 @@ void convert(int64_t n, const S *src, D *dst, hipStream_t s);
 template <class T>
 void fill(int64_t n, T v, T *d, hipStream_t s);
 void sub_external(int64_t n, double lambda, const double *fext, double *R, hipStream_t s);
+// sel: constrained set of this call, a mask of bcflag values (1 fixed, 2 ramped, 3 both)
 void impose_dirichlet_dev(FineMatrix &K, int64_t nbc, const int64_t *bcdofs, const uint8_t *flag,
                           const double *val, double lambda, const double *U, double *dg, int64_t nrows,
-                          const int32_t *rows, double *R, hipStream_t s);
+                          const int32_t *rows, double *R, int sel, hipStream_t s);
 void block_diag_inverse3(const FineMatrix &K, double *dinv, hipStream_t s);
 template <class T>
 void block_jacobi3(int64_t nb, const double *dinv, const T *r, T *z, hipStream_t s);

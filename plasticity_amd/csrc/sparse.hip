@@ -241,8 +241,10 @@ __global__ __launch_bounds__(256) void resid_ext_kernel(int64_t n, double lambda
 }
 
 // ---------------------------------------------------------------- Dirichlet
-// bcflag[d]: 0 free, 1 fixed (value as given), 2 ramped (lambda * value).
-__device__ __forceinline__ bool isbc(const uint8_t *f, int64_t d) { return f[d] != 0; }
+// bcflag[d]: 0 free, 1 fixed (value as given), 2 ramped (lambda * value).  One
+// impose_dirichlet! call treats the dofs whose flag is in `sel` as constrained
+// (sel = 2: the ramped set, 1: the fixed set, 3: both at once).
+__device__ __forceinline__ bool isbc(const uint8_t *f, int64_t d, int sel) { return (f[d] & sel) != 0; }
 
 __global__ void bc_dg_kernel(int64_t n, const int64_t *__restrict__ bcdofs, const uint8_t *__restrict__ flag,
                              const double *__restrict__ val, double lambda, const double *__restrict__ U,
@@ -260,7 +262,7 @@ __global__ void bc_rows_kernel(int64_t nrows, const int32_t *__restrict__ rows,
                                const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ rowlen,
                                const int32_t *__restrict__ colind, double *__restrict__ vals,
                                const uint8_t *__restrict__ flag, const double *__restrict__ dg,
-                               double *__restrict__ R) {
+                               double *__restrict__ R, int sel) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nrows) return;
     const int64_t i = rows[t];
@@ -268,14 +270,14 @@ __global__ void bc_rows_kernel(int64_t nrows, const int32_t *__restrict__ rows,
     const int len = rowlen[i];
     bool rb[3];
     double acc[3] = {0, 0, 0};
-    for (int r = 0; r < 3; ++r) rb[r] = isbc(flag, 3 * i + r);
+    for (int r = 0; r < 3; ++r) rb[r] = isbc(flag, 3 * i + r, sel);
     for (int k = 0; k < len; ++k) {
         const int64_t pos = base + (int64_t)k * SELL_C;
         const int64_t j = colind[pos];
         double *v = vals + 9 * pos;
         for (int c = 0; c < 3; ++c) {
             const int64_t cd = 3 * j + c;
-            if (!isbc(flag, cd)) continue;
+            if (!isbc(flag, cd, sel)) continue;
             for (int r = 0; r < 3; ++r)
                 if (!rb[r]) acc[r] += v[3 * r + c] * dg[cd];
         }
@@ -289,17 +291,17 @@ __global__ void bc_rows_kernel(int64_t nrows, const int32_t *__restrict__ rows,
         for (int r = 0; r < 3; ++r)
             for (int c = 0; c < 3; ++c) {
                 const int64_t rd = 3 * i + r, cd = 3 * j + c;
-                if (rb[r] || isbc(flag, cd)) v[3 * r + c] = (rd == cd && rb[r]) ? 1.0 : 0.0;
+                if (rb[r] || isbc(flag, cd, sel)) v[3 * r + c] = (rd == cd && rb[r]) ? 1.0 : 0.0;
             }
     }
 }
 
-__global__ void bc_rhs_kernel(int64_t n, const int64_t *__restrict__ bcdofs, const double *__restrict__ dg,
-                              double *__restrict__ R) {
+__global__ void bc_rhs_kernel(int64_t n, const int64_t *__restrict__ bcdofs, const uint8_t *__restrict__ flag,
+                              const double *__restrict__ dg, double *__restrict__ R, int sel) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int64_t d = bcdofs[t];
-    R[d] = -dg[d];
+    if (isbc(flag, d, sel)) R[d] = -dg[d];
 }
 
 // ---------------------------------------------------------------- block Jacobi (3x3)
@@ -557,13 +559,13 @@ void sub_external(int64_t n, double lambda, const double *fext, double *R, hipSt
 
 void impose_dirichlet_dev(FineMatrix &K, int64_t nbc, const int64_t *bcdofs, const uint8_t *flag,
                           const double *val, double lambda, const double *U, double *dg, int64_t nrows,
-                          const int32_t *rows, double *R, hipStream_t s) {
+                          const int32_t *rows, double *R, int sel, hipStream_t s) {
     if (nbc == 0) return;
     hipLaunchKernelGGL(bc_dg_kernel, dim3(grid_for(nbc, 256)), dim3(256), 0, s, nbc, bcdofs, flag, val, lambda, U,
                        dg);
     hipLaunchKernelGGL(bc_rows_kernel, dim3(grid_for(nrows, 64)), dim3(64), 0, s, nrows, rows, K.slice_ptr.p,
-                       K.rowlen.p, K.colind.p, K.v64.p, flag, dg, R);
-    hipLaunchKernelGGL(bc_rhs_kernel, dim3(grid_for(nbc, 256)), dim3(256), 0, s, nbc, bcdofs, dg, R);
+                       K.rowlen.p, K.colind.p, K.v64.p, flag, dg, R, sel);
+    hipLaunchKernelGGL(bc_rhs_kernel, dim3(grid_for(nbc, 256)), dim3(256), 0, s, nbc, bcdofs, flag, dg, R, sel);
     PF_HIP(hipGetLastError());
 }
 

@@ -78,38 +78,41 @@ class SolveResult:
     times: list = field(default_factory=list)
 
 
+_OPTION_ALIASES = {
+    "fcg_directions": "gcr_directions",   # directions kept by the learned preconditioner's flexible GCR
+    "line_search": "line_search",
+    "wcycle": "amg_wcycle",
+}
+
+
 def default_options(**kw):
+    """pfem_default_options with fields overridden by keyword (names of pfem_options), plus:
+    rho_iters=(cold, warm); gcr_fallback=True/False (flexible GCR or BiCGStab after a CG
+    breakdown); stream_prolongation=True/False; precision / smoother / amg_rebuild by name."""
     o = L.Options()
     L.lib().pfem_default_options(C.byref(o))
     for k, v in kw.items():
-        if k == "amg_rebuild":  # 0: once per load step (Solver.jl:374), 1: every Newton iteration
-            o.reserved[0] = int(v)
+        k = _OPTION_ALIASES.get(k, k)
+        if k == "rho_iters":
+            o.rho_iters_cold, o.rho_iters_warm = int(v[0]), int(v[1])
             continue
-        if k == "fcg_directions":  # directions kept by the learned preconditioner's flexible GCR
-            o.reserved[1] = int(v)
+        if k == "gcr_fallback":
+            o.krylov_fallback = 0 if v else 1
             continue
-        if k == "rho_iters":  # (cold, warm) power-iteration steps of the Chebyshev bound estimate
-            o.reserved[4], o.reserved[5] = int(v[0]), int(v[1])
-            continue
-        if k == "line_search":  # Newton backtracking on ||R||: most step halvings (0 = full steps)
-            o.reserved[3] = int(v)
-            continue
-        if k == "gcr_fallback":  # after a Krylov breakdown: flexible GCR(16) with the AMG cycle (default) or BiCGStab
-            o.reserved[6] = (o.reserved[6] & ~128) if v else (o.reserved[6] | 128)
-            continue
-        if k == "wcycle":  # AMG cycle: True W, False / None V (the default)
-            o.reserved[6] = (o.reserved[6] | 16) if v else (o.reserved[6] & ~16)
-            continue
-        if k == "stream_prolongation":  # fine prolongation over row chunks (default) or one thread per row
-            o.reserved[6] = (o.reserved[6] & ~256) if v else (o.reserved[6] | 256)
-            continue
-        if k == "fp16_fine":  # FP32-IR: smoother products on the fine level with the FP16 scaled K
-            o.reserved[2] = 1 if v else 0
+        if k == "stream_prolongation":
+            o.prolong_rowwise = 0 if v else 1
             continue
         if k == "precision" and isinstance(v, str):
             v = {"fp64": L.PREC_FP64, "fp32_ir": L.PREC_FP32_IR}[v]
         if k == "smoother" and isinstance(v, str):
             v = {"jacobi": L.SMOOTH_JACOBI, "chebyshev": L.SMOOTH_CHEBYSHEV, "chebyshev4": L.SMOOTH_CHEBYSHEV4}[v]
+        if k == "amg_rebuild" and isinstance(v, str):
+            v = {"per_load_step": L.AMG_PER_LOAD_STEP, "every_iteration": L.AMG_EVERY_ITERATION,
+                 "keep_p": L.AMG_KEEP_P, "keep_p_until_fallback": L.AMG_KEEP_P_UNTIL_FALLBACK}[v]
+        if isinstance(v, bool):
+            v = int(v)
+        if not hasattr(o, k):
+            raise TypeError(f"unknown option {k!r}")
         setattr(o, k, v)
     return o
 
@@ -150,6 +153,7 @@ class Model:
         self._h = h
         self.dir_dofs, self.dir_vals, self.dir_ramp = [], [], []
         self.neu_dofs, self.neu_vals = [], []
+        self._lam_c, self._dl, self._streak = 0.0, None, 0  # committed load factor, increment (advance)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -209,6 +213,7 @@ class Model:
     # ---------------------------------------------------------------- state
     def reset(self):
         L.check(L.lib().pfem_reset(self._h), "reset")
+        self._lam_c, self._dl, self._streak = 0.0, None, 0
         return self
 
     @property
@@ -224,7 +229,7 @@ class Model:
 
     def get_state(self):
         """Committed GaussState (Mesh.jl:124-163): epsp, beta (back stress), abar, cpinv and
-        the last assembled stress sig (Kirchhoff for finite strain, Cauchy for small)."""
+        the committed stress sig (Kirchhoff for finite strain, Cauchy for small)."""
         ep, bt, ab, cp, sg = (np.empty((self.ngp, 6)), np.empty((self.ngp, 6)), np.empty(self.ngp),
                               np.empty((self.ngp, 6)), np.empty((self.ngp, 6)))
         L.check(L.lib().pfem_get_state(self._h, L.ptr(ep), L.ptr(bt), L.ptr(ab), L.ptr(cp), L.ptr(sg)),
@@ -236,44 +241,30 @@ class Model:
                 for a in (epsp, beta, abar, cpinv)]
         L.check(L.lib().pfem_set_state(self._h, *[None if a is None else L.ptr(a) for a in args]), "set_state")
 
+    def trial_stress(self):
+        """Stress of the last assembly (the trial state's sigma written by assemble!), ngp x 6."""
+        out = np.empty((self.ngp, 6))
+        L.check(L.lib().pfem_get_trial_stress(self._h, L.ptr(out)), "get_trial_stress")
+        return out
+
     def nodal_displacements(self):
         """3 x nnodes (Model.jl:169-178)."""
         return self.U.reshape(-1, 3).T.copy()
 
     def gauss_kirchhoff(self):
+        """Committed Kirchhoff stress tau, ngp x 6 (Model.jl:226)."""
         return self.get_state()["sig"]
 
     def equivalent_plastic_strain(self):
+        """Committed abar (Model.jl:231)."""
         return self.get_state()["abar"]
 
     def gauss_stress(self):
-        """Cauchy stress, ngp x 6 (Model.jl:188-208): the stored stress for the small-strain
-        element, tau / det F with the unbarred F for the finite-strain elements."""
-        tau = self.gauss_kirchhoff()
-        if self.element == "small":
-            return tau
-        U = self.U.reshape(-1, 3)
-        X = self._nodes
-        g = 1.0 / np.sqrt(3.0)
-        sgn = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
-                        [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]], dtype=float)
-        out = np.empty_like(tau)
-        Xe = X[self._conn]      # nelem x 8 x 3
-        ue = U[self._conn]
-        for gp in range(8):
-            xi = sgn[gp] * g
-            dN = np.empty((8, 3))
-            for a in range(8):
-                s = sgn[a]
-                dN[a] = 0.125 * np.array([s[0] * (1 + s[1] * xi[1]) * (1 + s[2] * xi[2]),
-                                          s[1] * (1 + s[0] * xi[0]) * (1 + s[2] * xi[2]),
-                                          s[2] * (1 + s[0] * xi[0]) * (1 + s[1] * xi[1])])
-            J = np.einsum("eai,aj->eij", Xe, dN)
-            dNdX = np.einsum("ak,ekj->eaj", dN, np.linalg.inv(J))
-            F = np.eye(3) + np.einsum("eai,eaj->eij", ue, dNdX)
-            det = np.linalg.det(F)
-            inv = np.where(det > 0, 1.0 / det, 1.0)
-            out[gp::8] = tau[gp::8] * inv[:, None]
+        """Committed Cauchy stress, ngp x 6 (Model.jl:188-208), computed on the device: the
+        stored stress for the small-strain element, tau / det F with the un-barred F at the
+        current U for the finite-strain elements."""
+        out = np.empty((self.ngp, 6))
+        L.check(L.lib().pfem_gauss_stress(self._h, L.ptr(out)), "gauss_stress")
         return out
 
     # ---------------------------------------------------------------- hot path
@@ -293,8 +284,21 @@ class Model:
         L.check(L.lib().pfem_get_R(self._h, L.ptr(R)), "get_R")
         return R
 
-    def impose_dirichlet(self, lam):
-        L.check(L.lib().pfem_impose_dirichlet(self._h, float(lam)), "impose_dirichlet")
+    def impose_dirichlet(self, lam, which=None, U=None, R=None):
+        """impose_dirichlet! on the device K and R.  which=None: both frozen sets (the two
+        calls of newton!, Solver.jl:354-355); "ramp" / "fixed": one call for that set.  A host
+        R (ndof array) is sent in and overwritten with the result; U (ndof) replaces the
+        device U first."""
+        if which is None:
+            L.check(L.lib().pfem_impose_dirichlet(self._h, float(lam)), "impose_dirichlet")
+            return R
+        Ua = None if U is None else np.ascontiguousarray(U, dtype=np.float64)
+        if R is not None:
+            assert R.dtype == np.float64 and R.flags.c_contiguous
+        st = {"ramp": L.BC_RAMPED, "fixed": L.BC_FIXED}[which]
+        L.check(L.lib().pfem_impose_dirichlet_set(self._h, st, float(lam), None if Ua is None else L.ptr(Ua),
+                                                  None if R is None else L.ptr(R)), "impose_dirichlet_set")
+        return R
 
     def subtract_external(self, lam):
         L.check(L.lib().pfem_subtract_external(self._h, float(lam)), "subtract_external")
@@ -304,18 +308,23 @@ class Model:
         L.check(L.lib().pfem_residual_norm(self._h, C.byref(out)), "residual_norm")
         return out.value
 
-    def linear_solve(self, rtol, rebuild_pc=True, strict=False):
-        """_linear_solve!(dU, K, -R, ...) -> (dU, niter, solver_used, true_relres)."""
+    def linear_solve(self, rtol, rebuild_pc=True, strict=False, b=None):
+        """_linear_solve!(dU, K, b, ...) -> (dU, niter, solver_used, true_relres); b=None
+        solves against -R on the device.  Statuses 2 (not converged) and 3 (breakdown) keep
+        the correction the fallbacks produced, as the reference does, unless strict."""
         it, used, rel = C.c_int(), C.c_int(), C.c_double()
-        rc = L.lib().pfem_linear_solve(self._h, float(rtol), int(rebuild_pc), C.byref(it), C.byref(used),
-                                       C.byref(rel))
+        dU = np.empty(self.ndof)
+        ba = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
+        rc = L.lib().pfem_linear_solve(self._h, None if ba is None else L.ptr(ba), float(rtol), int(rebuild_pc),
+                                       C.byref(it), C.byref(used), C.byref(rel), L.ptr(dU))
         if rc in (L.PFEM_ERR_NOT_CONVERGED, L.PFEM_ERR_NON_SPD) and not strict:
             warnings.warn(f"linear solve: {L.lib().pfem_status_string(rc).decode()}")
         else:
             L.check(rc, "linear_solve")
-        dU = np.empty(self.ndof)
-        L.check(L.lib().pfem_get_dU(self._h, L.ptr(dU)), "get_dU")
         return dU, it.value, used.value, rel.value
+
+    def update_displacement(self):
+        L.check(L.lib().pfem_update_displacement(self._h), "update_displacement")
 
     def get_K_csc(self):
         import scipy.sparse as sp
@@ -352,7 +361,7 @@ class Model:
         L.check(rc, "newton")
         n = res.iters
         hist = list(res.residuals[:n])
-        kry = [res.krylov_iters[i] for i in range(n) if not (res.converged and i == n - 1)]
+        kry = list(res.krylov_iters[:res.solves])
         return bool(res.converged), hist, kry, res
 
     def commit(self):
@@ -362,28 +371,101 @@ class Model:
             raise ElementInversionError(f.e, f.g, f.J)
         L.check(rc, "commit")
 
-    def solve(self, nsteps=10, tol=1e-8, maxiter=25, verbose=False, callback=None, steps=None):
-        """solve! (Solver.jl:419-475): reset, freeze BCs, ramp lambda = n/nsteps."""
+    def rollback(self):
+        """U back to its value at the last commit (the committed history is never changed by a
+        trial assembly)."""
+        L.check(L.lib().pfem_rollback(self._h), "rollback")
+
+    def advance(self, target, nsteps, tol=1e-8, maxiter=25, cutback=0, grow_after=2, verbose=False):
+        """Advance the committed load factor to `target` (a point n/nsteps of the load path)
+        with newton! + commit, bisecting the increment on failure (lagrangian/docs/SCALING.md
+        :268-274; the reference's solve! only warns and stops, Solver.jl:463-467).
+
+        An increment whose Newton loop does not converge, or inverts an element, is undone
+        (pfem_rollback) and halved -- at most `cutback` halvings below the nominal increment
+        1/nsteps.  After `grow_after` converged increments in a row the increment doubles again,
+        never above 1/nsteps; an increment that would pass `target` is cut to end on it.
+        Returns the list of attempted increments, each a dict (lam, converged, inverted,
+        newton iterations, residuals, krylov, times); raises RuntimeError when the increment
+        would have to fall below its limit (or at the first failure when cutback = 0)."""
+        nominal = 1.0 / nsteps
+        if self._dl is None:
+            self._dl = nominal
+        dl_min = nominal / 2.0 ** cutback
+        out = []
+        while True:
+            lam_c = self._lam_c
+            lam = target if lam_c + self._dl >= target - 1e-12 * nominal else lam_c + self._dl
+            rec = {"lam": lam, "converged": False, "inverted": False}
+            try:
+                conv, hist, kry, res = self.newton(lam, tol=tol, maxiter=maxiter)
+                rec.update(converged=conv, iters=len(hist), residuals=hist, krylov=kry,
+                           solver_used=list(res.solver_used[:res.solves]), stopped=res.stopped,
+                           times={"assemble": res.t_assemble, "bc": res.t_bc, "setup": res.t_setup,
+                                  "solve": res.t_solve, "total": res.t_total})
+            except ElementInversionError as e:
+                rec.update(inverted=True, error=str(e), iters=0, residuals=[], krylov=[], solver_used=[],
+                           times={})
+            out.append(rec)
+            if verbose:
+                print(f"  lambda {lam:.6f} (+{lam - lam_c:.3e}): {'ok' if rec['converged'] else 'failed'} "
+                      f"newton={rec['iters']} krylov={rec['krylov']}" + (" inverted" if rec["inverted"] else ""))
+            if rec["converged"]:
+                self.commit()
+                self._lam_c = lam
+                self._streak += 1
+                if self._streak >= grow_after and self._dl < nominal:
+                    self._dl = min(2.0 * self._dl, nominal)
+                    self._streak = 0
+                if lam == target:
+                    return out
+                continue
+            half = 0.5 * (lam - lam_c)
+            if cutback <= 0 or half < dl_min * (1.0 - 1e-9):
+                if cutback > 0:  # with cutback = 0, U stays where newton! left it (Solver.jl:463-467)
+                    self.rollback()
+                err = RuntimeError(f"load step to lambda={target:.6f} failed at lambda={lam:.6f} "
+                                   f"(increment {lam - lam_c:.3e}, cut-back limit {dl_min:.3e})")
+                err.records = out
+                raise err
+            self.rollback()
+            self._dl = half
+            self._streak = 0
+
+    def solve(self, nsteps=10, tol=1e-8, maxiter=25, verbose=False, callback=None, steps=None, cutback=0,
+              grow_after=2):
+        """solve! (Solver.jl:419-475): reset, freeze BCs, ramp lambda = n/nsteps, commit after
+        each converged load step.  cutback > 0 bisects failed increments (advance); with the
+        default 0 the first failed load step ends the solve, as in the reference."""
         self.reset()
         self._freeze()
         out = SolveResult(converged=True)
+        out.increments = []
+        out.cutbacks = 0
         for n in (range(1, nsteps + 1) if steps is None else steps):
             lam = n / nsteps
-            conv, hist, kry, res = self.newton(lam, tol=tol, maxiter=maxiter)
-            out.iters.append(len(hist))
-            out.residuals.append(hist)
-            out.cg_iters.append(kry)
-            out.solver_used.append([res.solver_used[i] for i in range(len(kry))])
-            out.times.append({"assemble": res.t_assemble, "bc": res.t_bc, "setup": res.t_setup,
-                              "solve": res.t_solve, "total": res.t_total})
+            try:
+                recs = self.advance(lam, nsteps, tol=tol, maxiter=maxiter, cutback=cutback, grow_after=grow_after)
+                ok = True
+            except RuntimeError as e:
+                recs, ok = getattr(e, "records", []), False
+                if cutback <= 0:
+                    warnings.warn(f"load step {n} did not converge in {maxiter} iterations")
+            out.increments += recs
+            out.cutbacks += sum(1 for r in recs if not r["converged"])
+            last = [r for r in recs if r["converged"]]
+            for r in last:
+                out.iters.append(r["iters"])
+                out.residuals.append(r["residuals"])
+                out.cg_iters.append(r["krylov"])
+                out.solver_used.append(r["solver_used"])
+                out.times.append(r["times"])
             if verbose:
-                print(f"load step {n}/{nsteps} (lambda={lam:.4f}) newton={len(hist)} krylov={kry} "
-                      f"|R|={hist[-1]:.3e} t={res.t_total:.3f}s")
-            if not conv:
+                print(f"load step {n}/{nsteps} (lambda={lam:.4f}) increments={len(recs)} "
+                      f"newton={[r['iters'] for r in recs]}")
+            if not ok:
                 out.converged = False
-                warnings.warn(f"load step {n} did not converge in {maxiter} iterations")
                 break
-            self.commit()
             if callback is not None:
                 callback(n, self)
         return out

@@ -1,0 +1,131 @@
+"""The load-step bisection rule (lagrangian/docs/SCALING.md:268-274) on the CPU: the host
+driver of the device path (Model.advance / Model.solve) and the oracle's solve take the
+same increments for the same failures.  Newton itself is replaced by a stub that fails
+whenever the increment exceeds a limit, so the attempted load factors can be written down
+by hand:
+
+  nsteps = 4 (nominal increment 0.25), limit 0.1, grow after 2 successes:
+  0.25 x  0.125 x  0.0625 ok  0.125 ok (grow: 0.125)  0.25 x ... """
+import numpy as np
+import pytest
+
+import oracle as O
+import plasticity_amd.model as PM
+
+
+def expected(nsteps, limit, cutback, grow_after, nload):
+    """Hand-rolled restatement of the rule, independent of both implementations."""
+    nominal = 1.0 / nsteps
+    lam_c, dl, streak, out = 0.0, nominal, 0, []
+    for n in range(1, nload + 1):
+        target = n / nsteps
+        while lam_c < target:
+            lam = min(lam_c + dl, target)
+            if abs(lam - target) <= 1e-12 * nominal:
+                lam = target
+            ok = lam - lam_c <= limit + 1e-15
+            out.append((round(lam, 12), ok))
+            if ok:
+                lam_c, streak = lam, streak + 1
+                if streak >= grow_after and dl < nominal:
+                    dl, streak = min(2 * dl, nominal), 0
+            else:
+                if (lam - lam_c) / 2 < nominal / 2 ** cutback * (1 - 1e-9):
+                    return out, False
+                dl, streak = (lam - lam_c) / 2, 0
+    return out, True
+
+
+class _StubModel(PM.Model):
+    """Model.advance / solve against a fake newton!: no device involved."""
+
+    def __init__(self, limit):
+        self.limit, self.lam_c_dev, self.calls = limit, 0.0, []
+        self._lam_c, self._dl, self._streak = 0.0, None, 0
+        self.dir_dofs, self.neu_dofs = [], []
+
+    def reset(self):
+        self._lam_c, self._dl, self._streak = 0.0, None, 0
+        self.lam_c_dev = 0.0
+        return self
+
+    def _freeze(self):
+        pass
+
+    def newton(self, lam, tol=1e-8, maxiter=25):
+        ok = lam - self.lam_c_dev <= self.limit + 1e-15
+        self.calls.append((round(lam, 12), ok))
+
+        class R:
+            solves, stopped = 2, 0
+            solver_used = [0, 0]
+            t_assemble = t_bc = t_setup = t_solve = t_total = 0.0
+        if not ok and len(self.calls) % 2:
+            raise PM.ElementInversionError(0, 0, -1.0)  # inversions count as failures too
+        return ok, [1.0, 0.0] if ok else [1.0, 2.0], [3, 4], R()
+
+    def commit(self):
+        self.lam_c_dev = self.calls[-1][0]
+
+    def rollback(self):
+        pass
+
+
+@pytest.mark.parametrize("nsteps,limit,cutback,nload", [(4, 0.1, 4, 4), (20, 0.0126, 6, 3), (10, 0.1, 2, 10),
+                                                        (8, 0.01, 2, 2)])
+def test_host_driver_follows_the_rule(nsteps, limit, cutback, nload):
+    exp, ok = expected(nsteps, limit, cutback, 2, nload)
+    m = _StubModel(limit)
+    res = m.solve(nsteps=nsteps, steps=range(1, nload + 1), cutback=cutback)
+    assert m.calls == exp
+    assert res.converged == ok
+    assert res.cutbacks == sum(1 for _, c in exp if not c)
+
+
+def test_without_cutback_the_first_failure_ends_the_solve():
+    m = _StubModel(0.1)
+    with pytest.warns(UserWarning):
+        res = m.solve(nsteps=4, cutback=0)
+    assert m.calls == [(0.25, False)] and not res.converged
+
+
+@pytest.mark.parametrize("nsteps,limit,cutback,nload", [(4, 0.1, 4, 4), (20, 0.0126, 6, 3), (8, 0.01, 2, 2)])
+def test_oracle_follows_the_rule(nsteps, limit, cutback, nload, monkeypatch):
+    nodes, conn = O.box_mesh(1, 1, 1, 1, 1, 1)
+    M = O.OracleModel(nodes, conn, O.Material(200e9, 0.3, 250e6, 1e9))
+    state = {"lam_c": 0.0}
+
+    def newton(lam, Fext, bcs, tol=1e-8, maxiter=25, solver=None, diverge=0.0):
+        ok = lam - state["lam_c"] <= limit + 1e-15
+        if ok:
+            state["lam_c"] = lam
+        return ok, [1.0], []
+
+    monkeypatch.setattr(M, "newton", newton)
+    monkeypatch.setattr(M, "commit", lambda: None)
+    recs = []
+    conv, *_ = M.solve(nsteps=nsteps, cutback=cutback, records=recs, steps=range(1, nload + 1))
+    exp, ok = expected(nsteps, limit, cutback, 2, nload)
+    assert [(round(lam, 12), c) for lam, c, _ in recs] == exp
+    assert conv == ok
+
+
+def test_oracle_gauss_stress_homogeneous_closed_form():
+    """gauss_stress on a jittered mesh under a homogeneous deformation U = (F - I) X: every
+    Gauss point has the same un-barred F, so sigma = tau / det F exactly where det F > 0,
+    and tau itself where det F <= 0 (Model.jl:203-206); the small-strain kind copies tau."""
+    nodes, conn = O.box_mesh(1, 1, 1, 3, 2, 2, jitter=0.2, seed=9)
+    M = O.OracleModel(nodes, conn, O.Material(200e9, 0.3, 250e6, 1e9))
+    rng = np.random.default_rng(1)
+    tau = rng.standard_normal((8 * M.ne, 6))
+    M.st_c["sig"] = tau
+    for F in (np.array([[1.02, 0.01, 0.0], [0.003, 0.97, 0.02], [0.0, -0.01, 1.05]]),
+              np.diag([-0.5, 1.0, 1.0])):
+        M.U = ((nodes @ F.T) - nodes).ravel()
+        sig = M.gauss_stress()
+        J = np.linalg.det(F)
+        np.testing.assert_allclose(sig, tau / J if J > 0 else tau, rtol=1e-12, atol=0)
+    Ms = O.OracleModel(nodes, conn, O.Material(200e9, 0.3, 250e6, 1e9), element="small")
+    Ms.st_c["sig"] = tau
+    Ms.U = rng.standard_normal(Ms.ndof)
+    assert np.array_equal(Ms.gauss_stress(), tau)

@@ -1,6 +1,6 @@
 """Model set-up on the device (build.hip): the box mesh generator is bit-identical to the
 host and oracle generators; the device-built pattern, scatter slots and colours equal the
-host-built ones (pfem_options.reserved[6] = 1), so K and R assembled through either are
+host-built ones (pfem_options.host_build = 1), so K and R assembled through either are
 bit-identical -- for box numbering (device parity colours) and for a permuted element
 order (device pattern, host greedy colours)."""
 import numpy as np
@@ -27,7 +27,7 @@ def test_device_box_mesh_is_bit_identical(dims, jit, seed, need_gpu):
 
 def _pair(mesh, element="finite_fbar"):
     oh = pa.default_options()
-    oh.reserved[6] = 1
+    oh.host_build = 1
     md = pa.Model(mesh, pa.J2Material(**MAT), element=element)
     mh = pa.Model(mesh, pa.J2Material(**MAT), element=element, options=oh)
     return md, mh

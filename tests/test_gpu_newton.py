@@ -220,7 +220,7 @@ def test_kinematic_hardening_newton_matches_oracle(element, need_gpu):
 @pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
 def test_vcycle_graph_replay_equals_eager_launches(precision, need_gpu):
     """The V-cycle replayed as a HIP graph (default) against the same kernels launched one
-    by one (reserved[6] bit 1): four solves on two tangents -- the second and fourth reuse
+    by one (vcycle_eager): four solves on two tangents -- the second and fourth reuse
     the hierarchy, so one captured graph is replayed across solves with assembly, Dirichlet
     imposition and host reads in between -- give bitwise-identical corrections."""
     mesh = pa.box_mesh(1, 1, 1, 6, 6, 6, jitter=0.1, seed=0)
@@ -230,7 +230,7 @@ def test_vcycle_graph_replay_equals_eager_launches(precision, need_gpu):
     for graphs in (True, False):
         o = pa.default_options(precision=precision, coarse_max=60)
         if not graphs:
-            o.reserved[6] |= 2
+            o.vcycle_eager = 1
         m = pa.Model(mesh, pa.J2Material(**MAT), options=o)
         m.constrain(d, v, r)
         m._freeze()
@@ -254,7 +254,7 @@ def test_vcycle_graph_replay_equals_eager_launches(precision, need_gpu):
 @pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
 def test_streamed_prolongation_equals_thread_per_row(precision, need_gpu):
     """The fine-level prolongation over chunks of consecutive rows (default) against one
-    thread per row (reserved[6] bit 8).  The two sum each row's block products in a
+    thread per row (prolong_rowwise).  The two sum each row's block products in a
     different association, so the V-cycles agree to rounding: three solves on three
     tangents to rtol 1e-10 take the same iterations (+-1) and give corrections within
     1e-9.  At 6^3 the fine prolongator (343 rows, 1-8 blocks each) spans several chunks."""

@@ -73,7 +73,11 @@ def test_entry_points_reject_bad_arguments_before_touching_a_device():
     assert lib.pfem_reset(None) == BAD
     assert lib.pfem_commit(None, C.byref(f)) == BAD
     ni, nu_, rr = C.c_int(), C.c_int(), C.c_double()
-    assert lib.pfem_linear_solve(None, 1e-8, 1, C.byref(ni), C.byref(nu_), C.byref(rr)) == BAD
+    assert lib.pfem_linear_solve(None, None, 1e-8, 1, C.byref(ni), C.byref(nu_), C.byref(rr), None) == BAD
+    assert lib.pfem_impose_dirichlet_set(None, 0, 1.0, None, None) == BAD
+    assert lib.pfem_rollback(None) == BAD
+    assert lib.pfem_get_trial_stress(None, None) == BAD
+    assert lib.pfem_gauss_stress(None, None) == BAD
     assert lib.pfem_set_dirichlet(None, 0, None, None, None) == BAD
     assert lib.pfem_attach_net(None, None, 0, 1) == BAD
     # stand-alone entry points
@@ -96,3 +100,35 @@ def _batch_kind(kind):
     f = L.Fail()
     return L.lib().pfem_element_batch(kind, C.byref(m), 1, L.ptr(z), L.ptr(z), L.ptr(z), L.ptr(z), L.ptr(z),
                                       L.ptr(z), 0, L.ptr(z), L.ptr(z), L.ptr(z), C.byref(f))
+
+
+def test_element_repeating_a_node_is_rejected():
+    """pfem_create refuses an element whose 8 node ids are not distinct (the scatter gives
+    every node pair of an element its own K slot and first-touch bit); the check runs
+    before any device call."""
+    lib = L.lib()
+    nodes = np.zeros(3 * 9)
+    conn = np.array([0, 1, 2, 3, 4, 5, 6, 6], dtype=np.int64)
+    mat = L.Material(**VALID)
+    md = L.MeshDesc(nnodes=9, nelem=1, nodes=L.ptr(nodes), conn=L.ptr(conn), material_id=None, nmaterials=1,
+                    materials=C.cast(C.pointer(mat), C.c_void_p), element_kind=L.HEX8_FINITE_FBAR)
+    out = C.c_void_p()
+    assert lib.pfem_create(C.byref(md), None, C.byref(out)) == L.PFEM_ERR_BAD_ARG
+    assert "repeats node 6" in lib.pfem_status_string(L.PFEM_ERR_BAD_ARG).decode()
+    conn[7] = 7
+    assert lib.pfem_create(C.byref(md), None, C.byref(out)) in (L.PFEM_ERR_NO_DEVICE, L.PFEM_OK)
+    if out.value:
+        lib.pfem_destroy(out)
+
+
+def test_named_options_and_their_defaults():
+    o = pa.default_options()
+    assert o.amg_rebuild == L.AMG_KEEP_P_UNTIL_FALLBACK and o.amg_wcycle == 0 and o.krylov_fallback == 0
+    assert o.line_search == 0 and o.newton_diverge == 0.0 and o.fp32_krylov == 0 and o.host_build == 0
+    assert list(o.reserved) == [0] * 8
+    o = pa.default_options(amg_rebuild="per_load_step", wcycle=True, gcr_fallback=False, rho_iters=(30, 5),
+                           stream_prolongation=False, line_search=3, newton_diverge=10.0)
+    assert (o.amg_rebuild, o.amg_wcycle, o.krylov_fallback, o.rho_iters_cold, o.rho_iters_warm) == (0, 1, 1, 30, 5)
+    assert o.prolong_rowwise == 1 and o.line_search == 3 and o.newton_diverge == 10.0
+    with pytest.raises(TypeError):
+        pa.default_options(no_such_option=1)
