@@ -129,11 +129,32 @@ def copy_calibration(nbytes=1 << 30, reps=10):
     log(f"[bench] copy calibration: {reps} x {nbytes} B device-to-device")
 
 
-def time_variant(pa, wl, precision, learned, warmup, steps):
+class PathTotals:
+    """Sums over the load increments of timed load steps (advance records)."""
+
+    def __init__(self):
+        self.newton = self.accepted = self.krylov = self.increments = self.cutbacks = 0
+        self.t_asm = self.t_setup = self.t_solve = 0.0
+
+    def add(self, recs):
+        for r in recs:
+            self.increments += 1
+            self.cutbacks += 0 if r["converged"] else 1
+            self.newton += r["iters"]
+            self.accepted += r["iters"] if r["converged"] else 0
+            self.krylov += sum(r["krylov"])
+            t = r.get("times") or {}
+            self.t_asm += t.get("assemble", 0.0)
+            self.t_setup += t.get("setup", 0.0)
+            self.t_solve += t.get("solve", 0.0)
+
+
+def time_variant(pa, wl, precision, learned, warmup, steps, args):
     """The same load steps of the workload on a fresh model with another linear solver:
     precision "fp32_ir", learned = path of U-net weights (GCR with the network, AMG
     fallback) or None.  Returns s/Newton step over the timed load steps and the counts."""
-    opts = pa.default_options(eta_min=wl.krylov_rtol, eta_max=wl.krylov_rtol, precision=precision, amg_rebuild=6)
+    opts = pa.default_options(eta_min=wl.krylov_rtol, eta_max=wl.krylov_rtol, precision=precision,
+                              amg_rebuild=args.amg_rebuild, newton_diverge=args.diverge)
     mats = [pa.J2Material(**d) for d in wl.materials]
     m = pa.Model(wl.mesh, mats if len(mats) > 1 else mats[0], options=opts, material_id=wl.material_id)
     m.constrain(wl.dir_dofs, wl.dir_vals, wl.dir_ramp)
@@ -143,28 +164,23 @@ def time_variant(pa, wl, precision, learned, warmup, steps):
         from plasticity_amd.learned import UNet, grid_of
         net = UNet(grid_of(wl.mesh)).load(learned)
         m.attach_net(net, "precondition")
-    newton = krylov = 0
-    t_asm = t_setup = t_solve = 0.0
+    tot = PathTotals()
     t0 = None
     for k in range(1, warmup + steps + 1):
         if k == warmup + 1:
             t0 = time.perf_counter()
-        conv, hist, kry, res = m.newton(k / wl.nsteps, tol=wl.newton_tol, maxiter=25)
-        if not conv:
-            raise RuntimeError(f"{precision} learned={bool(learned)}: load step {k} did not converge")
-        m.commit()
+        recs = m.advance(k / wl.nsteps, wl.nsteps, tol=wl.newton_tol, maxiter=args.maxiter, cutback=args.cutback,
+                         grow_iters=args.grow_iters)
         if k > warmup:
-            newton += res.iters
-            krylov += sum(res.krylov_iters[:res.iters])
-            t_asm += res.t_assemble
-            t_setup += res.t_setup
-            t_solve += res.t_solve
+            tot.add(recs)
     elapsed = time.perf_counter() - t0
+    U, abar = m.U, m.get_state()["abar"]
     if learned:
         m.attach_net(None, "off")
     m.close()
-    return {"value": elapsed / newton, "newton_steps": newton, "krylov_its": krylov, "seconds": elapsed,
-            "t_assemble_s": t_asm, "t_amg_setup_s": t_setup, "t_solve_s": t_solve}
+    return {"value": elapsed / tot.newton, "newton_steps": tot.newton, "krylov_its": tot.krylov,
+            "increments": tot.increments, "cutbacks": tot.cutbacks, "seconds": elapsed,
+            "t_assemble_s": tot.t_asm, "t_amg_setup_s": tot.t_setup, "t_solve_s": tot.t_solve}, U, abar
 
 
 def main():
@@ -205,6 +221,17 @@ def main():
     ap.add_argument("--cycle", default="v", choices=["v", "w"], help="AMG cycle")
     ap.add_argument("--fallback", default="gcr", choices=["bicgstab", "gcr"],
                     help="Krylov method after a CG breakdown")
+    ap.add_argument("--cutback", type=int, default=6,
+                    help="load-step bisection: a failed load increment is undone and halved, at most this many "
+                         "times below the nominal increment (0: stop at the first failed load step, the "
+                         "reference's solve!)")
+    ap.add_argument("--diverge", type=float, default=0.0,
+                    help="end a diverging Newton loop early when ||R|| exceeds this multiple of its best value "
+                         "(0: run maxiter iterations, the reference)")
+    ap.add_argument("--maxiter", type=int, default=25, help="Newton iterations per load increment")
+    ap.add_argument("--grow-iters", type=int, default=6,
+                    help="a cut increment doubles again only after two converged increments that each took at "
+                         "most this many Newton iterations")
     ap.add_argument("--verbose", type=int, default=0)
     ap.add_argument("--fp16-fine", action="store_true",
                     help="FP32-IR: fine-level smoother products with an FP16 copy of the scaled K")
@@ -226,8 +253,8 @@ def main():
 
     t0 = time.perf_counter()
     wl = workload(args.workload, n=args.n, nsteps=args.nsteps, device=True)
-    if args.steps is None:
-        args.steps = max(1, wl.nsteps - args.warmup) if args.workload == 1 else 3
+    if args.steps is None:  # the rest of the load path after the warm-up
+        args.steps = max(1, wl.nsteps - args.warmup)
     log(f"[bench] workload generated in {time.perf_counter() - t0:.2f}s (mesh on the device)")
     if args.mode == "spmv":
         m = pa.Model(wl.mesh, pa.J2Material(**wl.materials[0]), options=pa.default_options())
@@ -243,7 +270,7 @@ def main():
                               line_search=args.line_search, rho_iters=args.rho_iters, smoother=args.smoother,
                               smoother_degree=args.smoother_degree,
                               wcycle=args.cycle == "w",
-                              gcr_fallback=args.fallback == "gcr")
+                              gcr_fallback=args.fallback == "gcr", newton_diverge=args.diverge)
     t0 = time.perf_counter()
     mats = [pa.J2Material(**d) for d in wl.materials]
     m = pa.Model(wl.mesh, mats if len(mats) > 1 else mats[0], options=opts, material_id=wl.material_id)
@@ -261,31 +288,30 @@ def main():
         f"/{'colours' if st0.built_on_device & 2 else 'host'})")
 
     def load_step(k):
-        lam = k / wl.nsteps
-        conv, hist, kry, res = m.newton(lam, tol=wl.newton_tol, maxiter=25)
-        if not conv:
-            raise RuntimeError(f"load step {k} did not converge: {hist}")
-        m.commit()
-        return res
+        # newton! + commit, with the failed increments bisected (Model.advance)
+        return m.advance(k / wl.nsteps, wl.nsteps, tol=wl.newton_tol, maxiter=args.maxiter, cutback=args.cutback,
+                         grow_iters=args.grow_iters)
+
+    def show(k, recs, t):
+        inc = ", ".join(f"{r['lam'] * wl.nsteps:.4g}:{r['iters']}{'' if r['converged'] else 'x'}" for r in recs)
+        kry = [r["krylov"] for r in recs if r["converged"]]
+        sol = [r["solver_used"] for r in recs if r["converged"]]
+        return f"load step {k}: increments [{inc}] krylov {kry} solver {sol} t={t:.3f}s"
 
     for k in range(1, args.warmup + 1):
-        res = load_step(k)
-        log(f"[bench] warmup load step {k}: newton {res.iters} krylov {list(res.krylov_iters[:res.iters])} "
-            f"t={res.t_total:.3f}s")
+        t1 = time.perf_counter()
+        recs = load_step(k)
+        log("[bench] warmup " + show(k, recs, time.perf_counter() - t1))
     s0 = m.stats()
-    newton = krylov = 0
-    t_asm = t_setup = t_solve = 0.0
+    tot = PathTotals()
     t0 = time.perf_counter()
     for k in range(args.warmup + 1, args.warmup + args.steps + 1):
-        res = load_step(k)
-        newton += res.iters
-        krylov += sum(res.krylov_iters[:res.iters])
-        t_asm += res.t_assemble
-        t_setup += res.t_setup
-        t_solve += res.t_solve
-        log(f"[bench] load step {k}: newton {res.iters} krylov {list(res.krylov_iters[:res.iters])} "
-            f"solver {list(res.solver_used[:res.iters - 1])} t={res.t_total:.3f}s")
+        t1 = time.perf_counter()
+        recs = load_step(k)
+        tot.add(recs)
+        log("[bench] " + show(k, recs, time.perf_counter() - t1))
     elapsed = time.perf_counter() - t0
+    newton, krylov, t_asm, t_setup, t_solve = tot.newton, tot.krylov, tot.t_asm, tot.t_setup, tot.t_solve
     s1 = m.stats()
     per_newton = elapsed / newton
     nnzb, nb, ndof = s1.nnzb, s1.nnodes, s1.ndof
@@ -310,10 +336,11 @@ def main():
         m.attach_net(None, "off") if net is not None else None
         m.close()
         o64 = pa.default_options(eta_min=wl.krylov_rtol, eta_max=wl.krylov_rtol, amg_rebuild=args.amg_rebuild,
-                                 line_search=args.line_search)
+                                 line_search=args.line_search, newton_diverge=args.diverge)
         m64 = pa.Model(wl.mesh, mats if len(mats) > 1 else mats[0], options=o64, material_id=wl.material_id)
         m64.constrain(wl.dir_dofs, wl.dir_vals, wl.dir_ramp)
-        r64 = m64.solve(nsteps=wl.nsteps, tol=wl.newton_tol, steps=range(1, args.warmup + args.steps + 1))
+        r64 = m64.solve(nsteps=wl.nsteps, tol=wl.newton_tol, steps=range(1, args.warmup + args.steps + 1),
+                        maxiter=args.maxiter, cutback=args.cutback, grow_iters=args.grow_iters)
         if not r64.converged:
             raise RuntimeError("FP64 comparison run did not converge")
         U64, a64 = m64.U, m64.get_state()["abar"]
@@ -335,7 +362,7 @@ def main():
             if k == "fp32_ir_learned" and learned is None:
                 continue
             try:
-                v = time_variant(pa, wl, "fp32_ir", learned, args.warmup, args.steps)
+                v, _, _ = time_variant(pa, wl, "fp32_ir", learned, args.warmup, args.steps, args)
             except Exception as e:  # a failed variant is reported, the FP64 line still printed
                 variants[k] = {"error": repr(e)}
                 log(f"[bench] {k} failed: {e!r}")
@@ -369,7 +396,10 @@ def main():
                                   "every Newton iteration", 6: "prolongators kept until a solve falls back, the rest "
                                   "rebuilt every Newton iteration"}.get(args.amg_rebuild, "?") + ")",
                    "amg_cycle": args.cycle.upper(),
-                   "newton_steps_timed": newton, "krylov_its_timed": krylov,
+                   "newton_steps_timed": newton, "newton_steps_accepted": tot.accepted,
+                   "krylov_its_timed": krylov,
+                   "load_increments_timed": tot.increments, "cutbacks_timed": tot.cutbacks,
+                   "cutback_limit": args.cutback, "newton_maxiter": args.maxiter,
                    "t_assemble_s": t_asm, "t_amg_setup_s": t_setup, "t_solve_s": t_solve,
                    "amg_levels": list(s1.amg_ndof[:s1.amg_levels]),
                    **({"vs_fp64_path": compare} if compare else {})},

@@ -122,8 +122,8 @@ typedef struct pfem_options {
                                    refinement sweep); 0 = FP64 flexible CG whose preconditioner is the FP32
                                    V-cycle (default) */
     double newton_diverge;      /* pfem_newton stops a load step early (not converged) when, from the third
-                                   iteration on, ||R_k|| > newton_diverge * ||R_1||; 0 = never (the reference
-                                   runs maxiter iterations; default).  Used with load-step cut-back. */
+                                   iteration on, ||R_k|| > newton_diverge * min_{j<k} ||R_j||; 0 = never (the
+                                   reference runs maxiter iterations; default).  For load-step cut-back. */
     int32_t reserved[8];        /* zero */
 } pfem_options;
 

@@ -474,8 +474,9 @@ class OracleModel:
 
     def newton(self, lam, Fext, bcs, tol=1e-8, maxiter=25, solver=None, diverge=0.0):
         """newton! (Solver.jl:317-389).  `solver(K, b, eta)` -> (dU, niter).  diverge > 0 stops
-        the step (not converged) when, from the third iteration on, ||R|| > diverge * ||R_1||
-        (the device driver's optional early stop; the reference runs maxiter iterations)."""
+        the step (not converged) when, from the third iteration on, ||R|| > diverge times the
+        smallest earlier ||R|| (the device driver's optional early stop; the reference runs
+        maxiter iterations)."""
         import scipy.sparse.linalg as spla
         if solver is None:
             solver = lambda K, b, eta: (spla.spsolve(K.tocsc(), b), 0)  # noqa: E731
@@ -494,11 +495,10 @@ class OracleModel:
             hist.append(rnorm)
             if it == 1:
                 ref = max(rnorm, fext_norm, 1.0)
-                r1 = rnorm
             if rnorm <= tol * ref:
                 conv = True
                 break
-            if diverge > 0 and it >= 3 and not rnorm <= diverge * r1:
+            if diverge > 0 and it >= 3 and not rnorm <= diverge * min(hist[:-1]):
                 break
             eta = forcing_term(it, rnorm, rkm1, eta_prev)
             dU, n = solver(K, -R, eta)                                # :378-379 (also at maxiter)
@@ -522,15 +522,15 @@ class OracleModel:
         return out
 
     def solve(self, nsteps=10, tol=1e-8, maxiter=25, solver=None, callback=None, cutback=0, grow_after=2,
-              diverge=0.0, steps=None, records=None):
+              grow_iters=6, diverge=0.0, steps=None, records=None):
         """solve! (Solver.jl:419-475) with the direct linear solve.
 
         cutback > 0: load-step bisection (lagrangian/docs/SCALING.md:268-274).  A failed
         increment (Newton not converged, or an inverted element) is undone (U back to the
         committed U; the committed history is untouched) and halved, at most `cutback`
-        times below 1/nsteps; after `grow_after` converged increments in a row it doubles
-        again, never above 1/nsteps, and an increment that would pass the load step's end
-        n/nsteps is cut to end there.  `records` (a list) receives one
+        times below 1/nsteps; after `grow_after` converged increments in a row that each took
+        at most `grow_iters` Newton iterations it doubles again, never above 1/nsteps, and an
+        increment that would pass the load step's end n/nsteps is cut to end there.  `records` (a list) receives one
         (lam, converged, newton iterations) tuple per attempt."""
         self.U[:] = 0.0
         self.U_c = self.U.copy()
@@ -564,7 +564,7 @@ class OracleModel:
                     cgs.append(cg)
                     self.commit()
                     lam_c = lam
-                    streak += 1
+                    streak = streak + 1 if len(hist) <= grow_iters else 0
                     if streak >= grow_after and dl < nominal:
                         dl, streak = min(2.0 * dl, nominal), 0
                     if lam == target:

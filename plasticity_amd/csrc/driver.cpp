@@ -696,7 +696,7 @@ int pfem_newton(pfem_ctx *c, double lambda, double tol, int maxiter, pfem_newton
         hipStream_t s = c->stream;
         const double T0 = now();
         const double fext_norm = std::fabs(lambda) * c->fext0_norm;  // Solver.jl:336
-        double ref = 1.0, rkm1 = 0.0, eta_prev = c->opt.eta_max, r1 = 0.0;
+        double ref = 1.0, rkm1 = 0.0, eta_prev = c->opt.eta_max, rbest = 0.0;
         bool first = true;
         // K, R and ||R|| at the current U (assemble, R -= lambda Fext, impose_dirichlet! for
         // the ramped set, then for the fixed set)
@@ -722,16 +722,16 @@ int pfem_newton(pfem_ctx *c, double lambda, double tol, int maxiter, pfem_newton
         for (int it = 1; it <= maxiter; ++it) {
             if (!have_residual) {
                 const int rc0 = evaluate(&rnorm);
-                if (rc0) return rc0;
+                if (rc0) {
+                    res->t_total = now() - T0;  // the iterations before an inversion stay reported
+                    return rc0;
+                }
             }
             have_residual = false;
             int rc = PFEM_OK;
             res->residuals[it - 1] = rnorm;
             res->iters = it;
-            if (it == 1) {
-                ref = std::max(std::max(rnorm, fext_norm), 1.0);  // :360-362
-                r1 = rnorm;
-            }
+            if (it == 1) ref = std::max(std::max(rnorm, fext_norm), 1.0);  // :360-362
             if (c->opt.verbose) fprintf(stderr, "    iter %d  |R| = %.6e\n", it, rnorm);
             if (rnorm <= tol * ref) {  // :364
                 res->converged = 1;
@@ -739,11 +739,12 @@ int pfem_newton(pfem_ctx *c, double lambda, double tol, int maxiter, pfem_newton
             }
             // optional early stop of a diverging load step (not in the reference, which runs
             // maxiter iterations): from the third iteration on, ||R|| above newton_diverge
-            // times the first residual
-            if (c->opt.newton_diverge > 0 && it >= 3 && !(rnorm <= c->opt.newton_diverge * r1)) {
+            // times the smallest residual so far
+            if (c->opt.newton_diverge > 0 && it >= 3 && !(rnorm <= c->opt.newton_diverge * rbest)) {
                 res->stopped = 1;
                 break;
             }
+            rbest = it == 1 ? rnorm : std::min(rbest, rnorm);
             const double eta = forcing_term(c->opt, it, rnorm, rkm1, eta_prev);  // :371
             res->eta[it - 1] = eta;
             // :374 rebuilds once per load step; the other policies (pfem_amg_rebuild) rebuild

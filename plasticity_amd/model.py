@@ -357,7 +357,9 @@ class Model:
         f = L.Fail()
         rc = L.lib().pfem_newton(self._h, float(lam), float(tol), int(maxiter), C.byref(res), C.byref(f))
         if rc == L.PFEM_ERR_INVERTED:
-            raise ElementInversionError(f.e, f.g, f.J)
+            err = ElementInversionError(f.e, f.g, f.J)
+            err.result = res  # the iterations done before the inversion
+            raise err
         L.check(rc, "newton")
         n = res.iters
         hist = list(res.residuals[:n])
@@ -376,15 +378,17 @@ class Model:
         trial assembly)."""
         L.check(L.lib().pfem_rollback(self._h), "rollback")
 
-    def advance(self, target, nsteps, tol=1e-8, maxiter=25, cutback=0, grow_after=2, verbose=False):
+    def advance(self, target, nsteps, tol=1e-8, maxiter=25, cutback=0, grow_after=2, grow_iters=6,
+                verbose=False):
         """Advance the committed load factor to `target` (a point n/nsteps of the load path)
         with newton! + commit, bisecting the increment on failure (lagrangian/docs/SCALING.md
         :268-274; the reference's solve! only warns and stops, Solver.jl:463-467).
 
         An increment whose Newton loop does not converge, or inverts an element, is undone
         (pfem_rollback) and halved -- at most `cutback` halvings below the nominal increment
-        1/nsteps.  After `grow_after` converged increments in a row the increment doubles again,
-        never above 1/nsteps; an increment that would pass `target` is cut to end on it.
+        1/nsteps.  After `grow_after` converged increments in a row that each needed at most
+        `grow_iters` Newton iterations the increment doubles again, never above 1/nsteps; an
+        increment that would pass `target` is cut to end on it.
         Returns the list of attempted increments, each a dict (lam, converged, inverted,
         newton iterations, residuals, krylov, times); raises RuntimeError when the increment
         would have to fall below its limit (or at the first failure when cutback = 0)."""
@@ -404,8 +408,13 @@ class Model:
                            times={"assemble": res.t_assemble, "bc": res.t_bc, "setup": res.t_setup,
                                   "solve": res.t_solve, "total": res.t_total})
             except ElementInversionError as e:
-                rec.update(inverted=True, error=str(e), iters=0, residuals=[], krylov=[], solver_used=[],
-                           times={})
+                r = getattr(e, "result", None)
+                n = r.iters if r is not None else 0
+                rec.update(inverted=True, error=str(e), iters=n,
+                           residuals=list(r.residuals[:n]) if r is not None else [],
+                           krylov=list(r.krylov_iters[:r.solves]) if r is not None else [], solver_used=[],
+                           times={"assemble": r.t_assemble, "bc": r.t_bc, "setup": r.t_setup,
+                                  "solve": r.t_solve, "total": r.t_total} if r is not None else {})
             out.append(rec)
             if verbose:
                 print(f"  lambda {lam:.6f} (+{lam - lam_c:.3e}): {'ok' if rec['converged'] else 'failed'} "
@@ -413,7 +422,7 @@ class Model:
             if rec["converged"]:
                 self.commit()
                 self._lam_c = lam
-                self._streak += 1
+                self._streak = self._streak + 1 if rec["iters"] <= grow_iters else 0
                 if self._streak >= grow_after and self._dl < nominal:
                     self._dl = min(2.0 * self._dl, nominal)
                     self._streak = 0
@@ -433,7 +442,7 @@ class Model:
             self._streak = 0
 
     def solve(self, nsteps=10, tol=1e-8, maxiter=25, verbose=False, callback=None, steps=None, cutback=0,
-              grow_after=2):
+              grow_after=2, grow_iters=6):
         """solve! (Solver.jl:419-475): reset, freeze BCs, ramp lambda = n/nsteps, commit after
         each converged load step.  cutback > 0 bisects failed increments (advance); with the
         default 0 the first failed load step ends the solve, as in the reference."""
@@ -445,7 +454,8 @@ class Model:
         for n in (range(1, nsteps + 1) if steps is None else steps):
             lam = n / nsteps
             try:
-                recs = self.advance(lam, nsteps, tol=tol, maxiter=maxiter, cutback=cutback, grow_after=grow_after)
+                recs = self.advance(lam, nsteps, tol=tol, maxiter=maxiter, cutback=cutback, grow_after=grow_after,
+                                    grow_iters=grow_iters)
                 ok = True
             except RuntimeError as e:
                 recs, ok = getattr(e, "records", []), False

@@ -13,8 +13,9 @@ import oracle as O
 import plasticity_amd.model as PM
 
 
-def expected(nsteps, limit, cutback, grow_after, nload):
-    """Hand-rolled restatement of the rule, independent of both implementations."""
+def expected(nsteps, limit, cutback, grow_after, nload, iters=2, grow_iters=6):
+    """Hand-rolled restatement of the rule, independent of both implementations (every
+    converged increment takes `iters` Newton iterations)."""
     nominal = 1.0 / nsteps
     lam_c, dl, streak, out = 0.0, nominal, 0, []
     for n in range(1, nload + 1):
@@ -26,7 +27,7 @@ def expected(nsteps, limit, cutback, grow_after, nload):
             ok = lam - lam_c <= limit + 1e-15
             out.append((round(lam, 12), ok))
             if ok:
-                lam_c, streak = lam, streak + 1
+                lam_c, streak = lam, (streak + 1 if iters <= grow_iters else 0)
                 if streak >= grow_after and dl < nominal:
                     dl, streak = min(2 * dl, nominal), 0
             else:
@@ -39,8 +40,8 @@ def expected(nsteps, limit, cutback, grow_after, nload):
 class _StubModel(PM.Model):
     """Model.advance / solve against a fake newton!: no device involved."""
 
-    def __init__(self, limit):
-        self.limit, self.lam_c_dev, self.calls = limit, 0.0, []
+    def __init__(self, limit, iters=2):
+        self.limit, self.lam_c_dev, self.calls, self.iters = limit, 0.0, [], iters
         self._lam_c, self._dl, self._streak = 0.0, None, 0
         self.dir_dofs, self.neu_dofs = [], []
 
@@ -62,7 +63,7 @@ class _StubModel(PM.Model):
             t_assemble = t_bc = t_setup = t_solve = t_total = 0.0
         if not ok and len(self.calls) % 2:
             raise PM.ElementInversionError(0, 0, -1.0)  # inversions count as failures too
-        return ok, [1.0, 0.0] if ok else [1.0, 2.0], [3, 4], R()
+        return ok, [1.0] * (self.iters - 1) + [0.0] if ok else [1.0, 2.0], [3, 4], R()
 
     def commit(self):
         self.lam_c_dev = self.calls[-1][0]
@@ -82,6 +83,18 @@ def test_host_driver_follows_the_rule(nsteps, limit, cutback, nload):
     assert res.cutbacks == sum(1 for _, c in exp if not c)
 
 
+@pytest.mark.parametrize("iters", [6, 7])
+def test_slow_increments_do_not_grow(iters):
+    """Converged increments that needed more than grow_iters (6) Newton iterations never
+    grow the increment again: no failed attempt after the first cut."""
+    exp, ok = expected(4, 0.1, 4, 2, 2, iters=iters)
+    m = _StubModel(0.1, iters=iters)
+    m.solve(nsteps=4, steps=range(1, 3), cutback=4)
+    assert m.calls == exp
+    fails = sum(1 for _, c in exp if not c)
+    assert (fails == 2) if iters == 7 else (fails > 2)  # 0.25 x, 0.125 x, then 0.0625 for good
+
+
 def test_without_cutback_the_first_failure_ends_the_solve():
     m = _StubModel(0.1)
     with pytest.warns(UserWarning):
@@ -99,7 +112,7 @@ def test_oracle_follows_the_rule(nsteps, limit, cutback, nload, monkeypatch):
         ok = lam - state["lam_c"] <= limit + 1e-15
         if ok:
             state["lam_c"] = lam
-        return ok, [1.0], []
+        return ok, [1.0, 0.0], []
 
     monkeypatch.setattr(M, "newton", newton)
     monkeypatch.setattr(M, "commit", lambda: None)
