@@ -11,7 +11,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(HERE, "libpfem.so")
+# PFEM_LIB selects another build of the same library (e.g. a check build)
+LIBPATH = os.environ.get("PFEM_LIB") or os.path.join(HERE, "libpfem.so")
 
 PFEM_OK = 0
 PFEM_ERR_INVERTED = 1

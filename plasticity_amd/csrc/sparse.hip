@@ -219,6 +219,13 @@ __global__ __launch_bounds__(256) void fcg_dots_kernel(int64_t n, const double *
 }
 template <class T>
 __global__ void axpby_kernel(int64_t n, double a, const T *__restrict__ x, double b, T *__restrict__ y) {
+    // b = 0 overwrites y without reading it: y may hold anything (a fresh buffer), and
+    // 0 * NaN would be NaN
+    if (b == 0.0) {
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+            y[i] = (T)(a * (double)x[i]);
+        return;
+    }
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         y[i] = (T)(a * (double)x[i] + b * (double)y[i]);
 }
