@@ -305,6 +305,21 @@ int pfem_element_batch(int32_t element_kind, const pfem_material *mat, int64_t n
                        double *cpinv, int commit, double *Fe, double *Ke, double *sig,
                        pfem_fail *fail);
 
+/* pfem_element_batch without any allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * material-point loops, harnesses that forbid hipMalloc).  Same arguments, kernels and
+ * results as pfem_element_batch, limited to nelem <= PFEM_BATCH_NOALLOC_MAX
+ * (PFEM_ERR_BAD_ARG above).  The device copies of inputs and outputs and the
+ * per-Gauss-point packets live in one file-scope device array of about 32 MB that each
+ * GPU the library runs on holds until the process ends; the call runs on the null
+ * stream and is synchronous.  Calls that overlap on one GPU (several host threads) share
+ * the array and race. */
+#define PFEM_BATCH_NOALLOC_MAX 2048
+int pfem_element_batch_noalloc(int32_t element_kind, const pfem_material *mat, int64_t nelem,
+                               const double *Xe, const double *ue, double *epsp, double *beta,
+                               double *abar, double *cpinv, int commit, double *Fe, double *Ke,
+                               double *sig, pfem_fail *fail);
+
 typedef struct pfem_stats {
     int64_t nnodes, nelem, ndof, nnzb, ncolors;
     int32_t amg_levels;

@@ -22,6 +22,7 @@ PFEM_ERR_UNSUPPORTED = 4
 PFEM_ERR_HIP = 5
 PFEM_ERR_BAD_ARG = 6
 PFEM_ERR_NO_DEVICE = 7
+BATCH_NOALLOC_MAX = 2048
 
 HEX8_FINITE = 0
 HEX8_FINITE_FBAR = 1
@@ -135,6 +136,8 @@ SIGNATURES = {
     "pfem_commit": (C.c_int, [_vp, C.POINTER(Fail)]),
     "pfem_element_batch": (C.c_int, [C.c_int32, C.POINTER(Material), C.c_int64, _vp, _vp, _vp, _vp, _vp, _vp,
                                      C.c_int, _vp, _vp, _vp, C.POINTER(Fail)]),
+    "pfem_element_batch_noalloc": (C.c_int, [C.c_int32, C.POINTER(Material), C.c_int64, _vp, _vp, _vp, _vp, _vp,
+                                             _vp, C.c_int, _vp, _vp, _vp, C.POINTER(Fail)]),
     "pfem_get_stats": (C.c_int, [_vp, C.POINTER(Stats)]),
     "pfem_bench_spmv": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
     "pfem_dense_inverse": (C.c_int, [C.c_int64, _vp, _vp]),

@@ -18,7 +18,7 @@ OUT = os.path.join(HERE, "libpfem.so")
 BUILD = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["element.hip", "sparse.hip", "amg.hip", "unet.hip", "build.hip", "peaks.hip", "context.cpp", "driver.cpp", "amg.cpp", "unet.cpp"]
+SOURCES = ["element.hip", "sparse.hip", "amg.hip", "unet.hip", "build.hip", "peaks.hip", "batch_noalloc.hip", "context.cpp", "driver.cpp", "amg.cpp", "unet.cpp"]
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
     # never let the compiler form packed FP32 ops
