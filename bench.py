@@ -47,54 +47,69 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(newton_per_step, krylov_per_newton, n_sample=50):
-    """Oracle (C/OpenMP restatement of the reference algorithm: colour-ordered FP64
-    assembly, impose_dirichlet!, PCG preconditioned by one SA-AMG V-cycle with rigid-body
-    near-null space and Jacobi(2/3) smoothing) timed on the host cores on a bounded
-    sample: the same workload on an n_sample^3 mesh.  Every piece is O(N), so the
-    per-Newton-step time is scaled by the DOF ratio to the 100^3 workload.  Krylov
-    iterations per Newton step are taken from the GPU run of the same algorithm and the
-    AMG set-up is amortised once per load step (the reference's rebuild policy)."""
+def host_cpu():
+    """Host cores this process may use, the machine's count and the CPU model."""
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return threads, os.cpu_count(), model
+
+
+def cpu_baseline(n=100, maxiter=12):
+    """The oracle -- the C/OpenMP restatement of the reference algorithm: colour-ordered
+    FP64 assembly, impose_dirichlet!, PCG preconditioned by one SA-AMG V-cycle (rigid-body
+    near-null space, damped Jacobi smoothing) -- run on all host cores for the first load
+    step of the same workload at full size (100^3, 3,090,903 DOF, Krylov rtol 1e-8 as the
+    GPU run), with its own Newton and PCG iteration counts.  The hierarchy is built from
+    every Newton iteration's tangent: kept from the first iteration, as the reference does
+    (Solver.jl:374), it makes PCG break down on the plastic F-bar tangent (where the
+    reference's F-bar path would use UMFPACK).  s/Newton step = wall time of the load step
+    / Newton iterations (residual evaluations), as on the GPU."""
     import numpy as np
 
     import oracle as O
     from plasticity_amd.workloads import workload
-    nthreads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    nthreads = max(1, min(nthreads, 16))
-    wl = workload(1, n=n_sample)
-    M = O.OracleModel(wl.mesh.nodes, wl.mesh.elements, O.Material(**wl.materials[0]), nthreads=nthreads)
+    threads, nproc, model = host_cpu()
+    wl = workload(1, n=n)
+    M = O.OracleModel(wl.mesh.nodes, wl.mesh.elements, O.Material(**wl.materials[0]), nthreads=threads)
     M.constrain(wl.dir_dofs, wl.dir_vals, wl.dir_ramp)
-    (rd, rv), (fd, fv) = M.freeze_bcs()
+    bcs = M.freeze_bcs()
+    (rd, _), (fd, _) = bcs
     isbc = np.zeros(M.ndof, np.uint8)
     isbc[rd] = 1
     isbc[fd] = 1
-    U = np.zeros(M.ndof)
-    t1 = time.perf_counter()
-    K, R = M.assemble(U, M._new_state(8 * M.ne))
-    t_asm = time.perf_counter() - t1
-    t2 = time.perf_counter()
-    O.impose_dirichlet(K, R, rd, rv, True, 1 / wl.nsteps, U)
-    O.impose_dirichlet(K, R, fd, fv, False, 1 / wl.nsteps, U)
-    float(np.sqrt((R ** 2).sum()))
-    t_bc = time.perf_counter() - t2
-    t3 = time.perf_counter()
-    amg = O.OracleAMG(K, M.nodes, isbc, coarse_max=1000, nthreads=nthreads)
-    t_setup = time.perf_counter() - t3
-    b = -R
-    reps = 5
-    t4 = time.perf_counter()
-    for _ in range(reps):
-        z = amg.vcycle(b)
-        amg.apply_A(z)
-    t_iter = (time.perf_counter() - t4) / reps
-    per_newton_sample = t_asm + t_bc + t_setup / newton_per_step + krylov_per_newton * t_iter
-    scale = 3090903 / M.ndof
+    st = {"amg": None, "t_setup": 0.0, "t_solve": 0.0, "its": []}
+
+    def solver(K, b, eta):
+        t = time.perf_counter()
+        st["amg"] = None  # the previous hierarchy is freed before the next is built
+        st["amg"] = O.OracleAMG(K, M.nodes, isbc, coarse_max=1000, nthreads=threads)
+        st["t_setup"] += time.perf_counter() - t
+        t1 = time.perf_counter()
+        x, it = st["amg"].solve(b, wl.krylov_rtol, itmax=200)
+        st["t_solve"] += time.perf_counter() - t1
+        st["its"].append(it)
+        return x, abs(it)
+
+    t0 = time.perf_counter()
+    conv, hist, _ = M.newton(1 / wl.nsteps, np.zeros(M.ndof), bcs, tol=wl.newton_tol, maxiter=maxiter,
+                             solver=solver)
+    elapsed = time.perf_counter() - t0
     return {
-        "value": per_newton_sample * scale, "unit": "s/Newton step", "cores": nthreads, "kind": "port",
-        "sample": (f"oracle (C/OpenMP, {nthreads} threads) on the same workload at {n_sample}^3 ({M.ndof} DOF), "
-                   f"scaled x{scale:.2f} by DOF to 100^3: assembly {t_asm:.2f}s + BC/norm {t_bc:.2f}s + "
-                   f"AMG set-up {t_setup:.2f}s / {newton_per_step:.1f} Newton steps per load step + "
-                   f"{krylov_per_newton:.1f} PCG its x {t_iter * 1e3:.1f} ms (V-cycle + SpMV)"),
+        "value": elapsed / len(hist), "unit": "s/Newton step", "cores": threads, "kind": "port",
+        "nproc": nproc, "cpu_model": model, "threads": threads, "converged": bool(conv),
+        "newton_steps": len(hist), "pcg_its": st["its"],
+        "sample": (f"oracle (C/OpenMP, {threads} threads of {nproc} logical CPUs, {model}): load step 1 of the "
+                   f"same workload at {n}^3 ({M.ndof} DOF{', full size' if n == 100 else ''}), {len(hist)} Newton steps "
+                   f"({'converged' if conv else 'not converged'}), PCG its {st['its']} (negative: breakdown), "
+                   f"{elapsed:.1f}s wall of which AMG set-up {st['t_setup']:.1f}s and PCG {st['t_solve']:.1f}s"),
     }
 
 
@@ -154,7 +169,8 @@ def time_variant(pa, wl, precision, learned, warmup, steps, args):
     precision "fp32_ir", learned = path of U-net weights (GCR with the network, AMG
     fallback) or None.  Returns s/Newton step over the timed load steps and the counts."""
     opts = pa.default_options(eta_min=wl.krylov_rtol, eta_max=wl.krylov_rtol, precision=precision,
-                              amg_rebuild=args.amg_rebuild, newton_diverge=args.diverge)
+                              amg_rebuild=args.amg_rebuild, newton_diverge=args.diverge,
+                              fp32_krylov=args.fp32_krylov)
     mats = [pa.J2Material(**d) for d in wl.materials]
     m = pa.Model(wl.mesh, mats if len(mats) > 1 else mats[0], options=opts, material_id=wl.material_id)
     m.constrain(wl.dir_dofs, wl.dir_vals, wl.dir_ramp)
@@ -207,6 +223,8 @@ def main():
                     help="precondition with the learned U-net (weights .npz) in flexible GCR, AMG fallback")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_ir"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=100,
+                    help="elements per edge of the CPU baseline's run (default: the workload's full 100^3)")
     ap.add_argument("--no-variants", action="store_true",
                     help="FP64 run of workload 1: skip timing the FP32-IR and FP32-IR + learned solvers "
                          "on the same load steps")
@@ -233,6 +251,9 @@ def main():
                     help="a cut increment doubles again only after two converged increments that each took at "
                          "most this many Newton iterations")
     ap.add_argument("--verbose", type=int, default=0)
+    ap.add_argument("--fp32-krylov", type=int, default=0, choices=[0, 1],
+                    help="FP32-IR: 1 = the Krylov iteration itself in FP32 inside FP64 iterative refinement, "
+                         "0 = FP64 flexible CG with the FP32 V-cycle")
     ap.add_argument("--fp16-fine", action="store_true",
                     help="FP32-IR: fine-level smoother products with an FP16 copy of the scaled K")
     ap.add_argument("--mode", default="newton", choices=["newton", "spmv", "copy"],
@@ -270,7 +291,8 @@ def main():
                               line_search=args.line_search, rho_iters=args.rho_iters, smoother=args.smoother,
                               smoother_degree=args.smoother_degree,
                               wcycle=args.cycle == "w",
-                              gcr_fallback=args.fallback == "gcr", newton_diverge=args.diverge)
+                              gcr_fallback=args.fallback == "gcr", newton_diverge=args.diverge,
+                              fp32_krylov=args.fp32_krylov)
     t0 = time.perf_counter()
     mats = [pa.J2Material(**d) for d in wl.materials]
     m = pa.Model(wl.mesh, mats if len(mats) > 1 else mats[0], options=opts, material_id=wl.material_id)
@@ -330,6 +352,13 @@ def main():
         f"assemble {t_asm:.3f}s (gp kernel {gp_ms:.2f} ms + scatter {sc_ms:.2f} ms per assembly), "
         f"AMG set-up {t_setup:.3f}s, solve {t_solve:.3f}s; fine SpMV {t_sp * 1e3:.3f} ms = {achieved} GB/s; "
         f"AMG levels {list(s1.amg_ndof[:s1.amg_levels])} OC {s1.amg_operator_complexity:.3f}")
+    n32 = s1.n_krylov_spmv32 - s0.n_krylov_spmv32
+    spmv32 = None
+    if n32:
+        t32 = (s1.t_krylov_spmv32 - s0.t_krylov_spmv32) / n32
+        b32 = spmv_bytes(nnzb, nb, ndof, 4)
+        spmv32 = {"avg_launch_ms": t32 * 1e3, "GBps": b32 / t32 / 1e9, "bytes_per_launch": b32, "launches": n32}
+        log(f"[bench] FP32 Krylov SpMV {t32 * 1e3:.3f} ms = {b32 / t32 / 1e9:.0f} GB/s ({n32} launches)")
     compare = None
     if args.compare:
         U_run, a_run = m.U, m.get_state()["abar"]
@@ -377,7 +406,7 @@ def main():
     cpu = None
     if not args.no_cpu and args.workload == 1:
         try:
-            cpu = cpu_baseline(newton / args.steps, krylov / max(newton, 1))
+            cpu = cpu_baseline(n=args.cpu_n)
             log(f"[bench] cpu baseline {cpu}")
         except Exception as e:  # the baseline never blocks the GPU line
             log(f"[bench] cpu baseline failed: {e!r}")
@@ -402,6 +431,7 @@ def main():
                    "cutback_limit": args.cutback, "newton_maxiter": args.maxiter,
                    "t_assemble_s": t_asm, "t_amg_setup_s": t_setup, "t_solve_s": t_solve,
                    "amg_levels": list(s1.amg_ndof[:s1.amg_levels]),
+                   **({"fp32_krylov_spmv": spmv32} if spmv32 else {}),
                    **({"vs_fp64_path": compare} if compare else {})},
         "roofline": {"kernel": "fine BSR(3x3) SpMV (sell_lds_kernel<double,double,2>, FP64) in PCG",
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -65,9 +65,10 @@ typedef struct pfem_mesh_desc {
 
 typedef enum pfem_precision {
     PFEM_PREC_FP64 = 0,         /* plain FP64 PCG + SA-AMG (the drop-in) */
-    PFEM_PREC_FP32_IR = 1       /* FP64 flexible PCG refined by FP32 V-cycle corrections (FP32 copies of
-                                   every AMG level and of the fine K); FP32 Krylov inside FP64 iterative
-                                   refinement after a breakdown; the FP64 ladder last */
+    PFEM_PREC_FP32_IR = 1       /* FP32 copies of every AMG level and of the fine K; with fp32_krylov the
+                                   Krylov iteration runs in FP32 inside FP64 iterative refinement, else
+                                   FP64 flexible PCG refined by FP32 V-cycle corrections; flexible GCR
+                                   with the FP32 cycle after a breakdown; the FP64 ladder last */
 } pfem_precision;
 
 typedef enum pfem_smoother {
@@ -240,7 +241,8 @@ int pfem_residual_norm(pfem_ctx *ctx, double *out);
  * ladder after FP32-IR failed), 3 FP32-IR flexible PCG, 4 FP32-IR with BiCGStab,
  * 5 flexible GCR with the learned preconditioner, 6 the same finished by AMG,
  * 7 FP32-IR finished by flexible GCR(16) with the FP32 AMG cycle after a CG breakdown,
- * 8 FP64 finished by flexible GCR(16) with the AMG cycle after PCG failed.  The GCR
+ * 8 FP64 finished by flexible GCR(16) with the AMG cycle after PCG failed, 9 FP32-IR with
+ * the Krylov iteration in FP32 inside FP64 refinement (fp32_krylov).  The GCR
  * fallbacks (7, 8) are the default; krylov_fallback = 1 selects BiCGStab (2, 4).
  * true_relres = ||b - K dU|| / ||b||.
  * A solve that ends with ||b - K dU|| > ||b|| returns dU = 0 (not applied); its status
@@ -335,6 +337,8 @@ typedef struct pfem_stats {
     int64_t device_bytes;                    /* device memory held by the context */
     double t_create_pattern;                 /* pfem_create: pattern, scatter slots and colours (s) */
     int32_t built_on_device;                 /* bit 0: pattern + slots built on the device, bit 1: colours */
+    double t_krylov_spmv32;                  /* accumulated device time of the FP32 Krylov SpMV (fp32_krylov) */
+    int64_t n_krylov_spmv32;                 /* number of those launches */
 } pfem_stats;
 
 int pfem_get_stats(pfem_ctx *ctx, pfem_stats *st);

@@ -321,7 +321,7 @@ def forcing_term(it, rk, rkm1, eta_prev, eta_min=1e-8, eta_max=0.1, gamma=0.9, a
 
 
 class OracleAMG:
-    """PCG + smoothed-aggregation AMG (rigid-body near-null space, Jacobi(2/3) smoother)
+    """PCG + smoothed-aggregation AMG (rigid-body near-null space, damped Jacobi smoother)
     on the oracle's CSC K -- the reference's CG+AMG linear solve (Solver.jl:235-290)."""
 
     def __init__(self, K, nodes, isbc, coarse_max=1000, nthreads=0):

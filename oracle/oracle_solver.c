@@ -151,6 +151,7 @@ typedef struct olevel {
     csr A, P, R;
     double *dinv;      /* 1 / diag(A), 0 where the diagonal vanishes */
     double *x, *b, *r; /* V-cycle work */
+    double omega;      /* Jacobi damping of this level's smoother, 4 / (3 rho(D^-1 A)) */
 } olevel;
 
 typedef struct or_amg {
@@ -159,7 +160,6 @@ typedef struct or_amg {
     int64_t nc;        /* coarsest size */
     double *lu;        /* dense LU of the coarsest operator (row-major), with pivots */
     int64_t *piv;
-    double omega_jac;  /* smoother damping (2/3) */
 } or_amg;
 
 static double *diag_inverse(const csr *A) {
@@ -328,7 +328,6 @@ or_amg *or_amg_setup(int64_t n, const int64_t *colptr, const int64_t *rowval, co
     (void)nthreads;
 #endif
     or_amg *amg = calloc(1, sizeof(or_amg));
-    amg->omega_jac = 2.0 / 3.0;
     csr Kc = {n, n, (int64_t *)colptr, (int64_t *)rowval, (double *)nzval};
     csr A = csr_transpose(&Kc); /* CSR of K */
     int64_t nb = n / 3;
@@ -383,6 +382,10 @@ or_amg *or_amg_setup(int64_t n, const int64_t *colptr, const int64_t *rowval, co
         free(agg);
         double rho = spectral_radius_dinvA(&A, L->dinv, 10);
         double w = (4.0 / 3.0) / rho;
+        /* the smoother takes the same damping: a fixed 2/3 exceeds 2 / rho(D^-1 A) for the
+           elastic tangent (rho ~ 3-4 with a scalar diagonal), which makes the sweep divergent
+           and the V-cycle indefinite (PCG broke down at its second iteration) */
+        L->omega = w;
         csr AP = csr_matmul(&A, &Pt);
         /* P = Pt - w D^-1 A Pt: merge rows of Pt into the (superset) pattern of AP */
         for (int64_t i = 0; i < AP.nr; ++i) {
@@ -454,7 +457,7 @@ static void vcycle(const or_amg *amg, int l, const double *b, double *x) {
         dense_lu_solve(n, amg->lu, amg->piv, x);
         return;
     }
-    const double w = amg->omega_jac;
+    const double w = L->omega;
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; ++i) x[i] = w * L->dinv[i] * b[i]; /* pre-sweep from x = 0 */
     csr_spmv(&L->A, x, L->r);

@@ -90,7 +90,7 @@ class Stats(C.Structure):
                 ("t_setup_rho", C.c_double), ("t_setup_galerkin", C.c_double), ("t_setup_coarse", C.c_double),
                 ("n_assemble", C.c_int64), ("n_setup", C.c_int64), ("n_solve", C.c_int64),
                 ("n_krylov", C.c_int64), ("device_bytes", C.c_int64), ("t_create_pattern", C.c_double),
-                ("built_on_device", C.c_int32)]
+                ("built_on_device", C.c_int32), ("t_krylov_spmv32", C.c_double), ("n_krylov_spmv32", C.c_int64)]
 
 
 class NetDesc(C.Structure):

@@ -511,6 +511,94 @@ static int ir_solve(pfem_ctx *c, const double *b, double *x, double rtol, int it
     return rn <= rtol * bn ? 0 : 1;
 }
 
+// FP32-IR as iterative refinement: the Krylov iteration itself runs in FP32 -- flexible
+// CG (FCG(1), as fpcg) with the FP32 copy of K, FP32 vectors and the FP32 V-cycle -- while
+// the solution accumulates in FP64 and every refinement sweep starts from the FP64
+// residual b - K x.  A sweep stops when its preconditioned residual norm has fallen by
+// IR_TAU (an FP32 recursion tracks the true residual over about five decades) or when it
+// meets the target; a target inside that range is accepted without another sweep.  Stop
+// test as pcg(): sqrt(r'z) <= rtol sqrt(r0'z0) with r0 = b.  0 solved, 1 itmax, 2 breakdown.
+static int ir32_fcg(pfem_ctx *c, const double *b, double *x, double rtol, int itmax, int *iters) {
+    constexpr double IR_TAU = 1e-5;
+    hipStream_t s = c->stream;
+    const int64_t n = c->ndof;
+    float *r = c->f_r.p, *z = c->f_z.p, *p = c->f_p.p, *q = c->f_q.p;
+    double *r64 = c->w_r.p, *sc = c->scal.p;
+    if (c->mpartial.n < multi_dot_partials()) c->mpartial.alloc(multi_dot_partials());
+    fill<double>(n, 0.0, x, s);
+    *iters = 0;
+    double eps = 0.0;
+    for (int sweep = 0; sweep < 16; ++sweep) {
+        if (sweep == 0) {
+            convert<double, float>(n, b, r, s);
+        } else {  // FP64 refinement residual
+            residual_fine64(c->K, x, b, r64, s);
+            convert<double, float>(n, r64, r, s);
+        }
+        amg_apply32(c, c->amg.get(), r, z);
+        dot_dev<float, float>(n, r, z, c->partial.p, sc + 0, s);
+        const double rz = c->sync_read(sc + 0);
+        if (!(rz >= 0)) return 2;
+        const double rn = std::sqrt(rz);
+        if (sweep == 0) {
+            if (rn == 0.0) return 0;
+            eps = rtol * rn;
+        } else if (rn <= eps) {
+            return 0;
+        }
+        const double stop = std::max(eps, IR_TAU * rn);
+        PF_HIP(hipMemcpyAsync(p, z, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+        bool reached = false;
+        for (int k = 0;; ++k) {
+            if (*iters >= itmax) return 1;
+            ++*iters;
+            PF_HIP(hipEventRecord(c->ev0, s));
+            spmv_fine32_dot(c->K, p, q, c->partial.p, sc + 1, s);  // q = K p, sc[1] = p'q
+            PF_HIP(hipEventRecord(c->ev1, s));
+            cg_update_mixed(n, sc + 0, sc + 1, p, q, x, r, s);     // x += a p (FP64), r -= a q
+            amg_apply32(c, c->amg.get(), r, z);
+            fcg_dots32(n, r, z, q, c->mpartial.p, sc + 2, s);      // sc[2] = r'z, sc[3] = r'r, sc[4] = z'q
+            PF_HIP(hipMemcpyAsync(c->h_scal, sc, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
+            c->sync();
+            float ms = 0.f;
+            PF_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            c->st.t_krylov_spmv32 += 1e-3 * ms;
+            c->st.n_krylov_spmv32++;
+            const double pq = c->h_scal[1], rzn = c->h_scal[2];
+            if (!(pq > 0)) {
+                if (c->opt.verbose > 1) fprintf(stderr, "      ir32 p'Kp = %.3e at sweep it %d\n", pq, k + 1);
+                if (k == 0 || !std::isfinite(pq) || pq == 0.0) return 2;  // undone with the whole solve
+                // a sweep that had made progress: undo this iteration's x += a p, a = r'z / p'q,
+                // and restart from the FP64 residual
+                axpy_f32_to_f64(n, -c->h_scal[0] / pq, p, x, s);
+                break;
+            }
+            if (!(rzn >= 0)) {
+                // the FP32 recursion has lost definiteness (near its rounding level, or an
+                // indefinite cycle): x holds this iteration's update, so a sweep that had made
+                // progress ends here and the next one restarts from the FP64 residual
+                if (c->opt.verbose > 1) fprintf(stderr, "      ir32 r'z = %.3e at sweep it %d\n", rzn, k + 1);
+                if (k == 0) return 2;
+                break;
+            }
+            const double nrm = std::sqrt(rzn);
+            if (c->opt.verbose > 2) fprintf(stderr, "      ir32 sweep %d it %3d  %.3e  (stop %.3e)\n", sweep, *iters, nrm, stop);
+            if (nrm <= stop) {
+                reached = nrm <= eps;
+                break;
+            }
+            const double beta = -c->h_scal[4] / pq;  // p = z + beta p
+            axpby<float>(n, 1.0, z, beta, p, s);
+            PF_HIP(hipMemcpyAsync(sc + 0, sc + 2, sizeof(double), hipMemcpyDeviceToDevice, s));
+        }
+        if (c->opt.verbose > 1)
+            fprintf(stderr, "      ir32 sweep %d: start %.3e, %d its so far, target %s\n", sweep, rn, *iters,
+                    reached ? "reached" : "not reached");
+        if (reached && eps >= IR_TAU * rn) return 0;
+    }
+    return 1;
+}
+
 // FP32 workspaces and the FP32 copy of the imposed fine K.
 static void prepare_fp32(pfem_ctx *c) {
     if (!c->f_r.p)
@@ -606,7 +694,10 @@ int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *us
     } else if (c->opt.precision == PFEM_PREC_FP32_IR) {
         prepare_fp32(c);
         which = 3;
-        if (c->net && c->net_mode == 1)
+        if (c->opt.fp32_krylov && !(c->net && c->net_mode == 1)) {
+            which = 9;
+            rc = ir32_fcg(c, c->rhs.p, c->dU.p, rtol, itmax, &it);
+        } else if (c->net && c->net_mode == 1)
             rc = fpcg(c, c->rhs.p, c->dU.p, rtol, itmax, true, 0, &it, LoggingPc<Amg32Pc>{c, Amg32Pc{c}});
         else
             rc = fpcg(c, c->rhs.p, c->dU.p, rtol, itmax, true, 0, &it, Amg32Pc{c});

@@ -45,12 +45,17 @@ void cg_update_r32(int64_t n, const double *num, const double *den, const double
 template <class ZT>
 void fcg_dots(int64_t n, const double *r, const ZT *z, double *zout, const double *q, double *partial, double *out,
               hipStream_t s);
+// out[0] = r'z, out[1] = r'r, out[2] = z'q of FP32 vectors, FP64 accumulation
+void fcg_dots32(int64_t n, const float *r, const float *z, const float *q, double *partial, double *out,
+                hipStream_t s);
 template <class T>
 void axpby(int64_t n, double a, const T *x, double b, T *y, hipStream_t s);
 void axpy_safe_ratio(int64_t n, const double *num, const double *den, double sign, const double *x, double *y,
                      hipStream_t s);
 void cg_update_mixed(int64_t n, const double *num, const double *den, const float *p, const float *q, double *x,
                      float *r, hipStream_t s);
+// y += a x, FP32 x into FP64 y
+void axpy_f32_to_f64(int64_t n, double a, const float *x, double *y, hipStream_t s);
 template <class S, class D>
 void convert(int64_t n, const S *src, D *dst, hipStream_t s);
 template <class T>

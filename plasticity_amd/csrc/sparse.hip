@@ -464,6 +464,33 @@ template void fcg_dots<double>(int64_t, const double *, const double *, double *
                                double *, hipStream_t);
 template void fcg_dots<float>(int64_t, const double *, const float *, double *, const double *, double *, double *,
                               hipStream_t);
+// r'z, r'r and z'q of FP32 vectors in one pass, accumulated in FP64 (the grid and the
+// per-thread order of dot_partial_kernel)
+__global__ __launch_bounds__(256) void fcg_dots32_kernel(int64_t n, const float *__restrict__ r,
+                                                         const float *__restrict__ z, const float *__restrict__ q,
+                                                         double *__restrict__ partial) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const double ri = r[i], zi = z[i];
+        s0 += ri * zi;
+        s1 += ri * ri;
+        s2 += zi * (double)q[i];
+    }
+    s0 = block_sum(s0);
+    s1 = block_sum(s1);
+    s2 = block_sum(s2);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = s0;
+        partial[RED_BLOCKS + blockIdx.x] = s1;
+        partial[2 * RED_BLOCKS + blockIdx.x] = s2;
+    }
+}
+void fcg_dots32(int64_t n, const float *r, const float *z, const float *q, double *partial, double *out,
+                hipStream_t s) {
+    hipLaunchKernelGGL(fcg_dots32_kernel, dim3(RED_BLOCKS), dim3(256), 0, s, n, r, z, q, partial);
+    hipLaunchKernelGGL(multi_finish_kernel, dim3(3), dim3(1024), 0, s, RED_BLOCKS, partial, out);
+    PF_HIP(hipGetLastError());
+}
 template <class T>
 void cg_update(int64_t n, const double *num, const double *den, const T *p, const T *q, T *x, T *r,
                hipStream_t s) {
@@ -524,6 +551,14 @@ void residual_fine_half(const FineMatrix &K, const float *x, const float *b, flo
 void axpy_safe_ratio(int64_t n, const double *num, const double *den, double sign, const double *x, double *y,
                      hipStream_t s) {
     hipLaunchKernelGGL(axpy_safe_ratio_kernel, dim3(grid_strided(n)), dim3(256), 0, s, n, num, den, sign, x, y);
+    PF_HIP(hipGetLastError());
+}
+__global__ void axpy_f32_to_f64_kernel(int64_t n, double a, const float *__restrict__ x, double *__restrict__ y) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] += a * (double)x[i];
+}
+void axpy_f32_to_f64(int64_t n, double a, const float *x, double *y, hipStream_t s) {
+    hipLaunchKernelGGL(axpy_f32_to_f64_kernel, dim3(grid_strided(n)), dim3(256), 0, s, n, a, x, y);
     PF_HIP(hipGetLastError());
 }
 void cg_update_mixed(int64_t n, const double *num, const double *den, const float *p, const float *q, double *x,

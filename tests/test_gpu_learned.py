@@ -2,13 +2,13 @@
 FP32 reference (oracle/unet_ref.py), Adam training, and flexible CG with the network
 and AMG fallback inside the Newton solver.
 
-Tolerances: the forward pass rounds weights, input and every activation to BF16 exactly
-where the reference does, so the outputs differ only by FP32 accumulation order and
-the rare rounding flip it causes (relative L2 error <= 2e-2, typically ~1e-3).  The
-backward pass also feeds BF16-rounded gradient tensors to the MFMA units, which the
-FP32 reference does not, so each layer's weight gradient is compared with relative L2
-error <= 8e-2 and cosine similarity >= 0.99 -- a transposed, flipped or misplaced tap
-gives errors of order 1."""
+Tolerances, about 10x the largest error observed: the forward pass rounds weights, input
+and every activation to BF16 exactly where the reference does, so the outputs differ only
+by FP32 accumulation order and the rare rounding flip it causes (observed relative L2
+error 4e-7 and 6e-5; bound 6e-4).  The backward pass also feeds BF16-rounded gradient
+tensors to the MFMA units, which the FP32 reference does not (observed per-layer relative
+L2 error 1.7e-3 to 3.8e-3, cosine >= 0.99999; bounds 4e-2 and 0.999) -- a transposed,
+flipped or misplaced tap gives errors of order 1."""
 import numpy as np
 import pytest
 import torch
@@ -46,7 +46,7 @@ def test_forward_matches_torch_reference(need_gpu, grid):
     e = rel(y, ref)
     print(f"  relative L2 error {e:.2e}", flush=True)
     assert np.isfinite(y).all() and np.linalg.norm(ref) > 0
-    assert e <= 2e-2
+    assert e <= 6e-4
 
 
 def test_loss_gradient_matches_torch_reference(need_gpu):
@@ -71,7 +71,7 @@ def test_loss_gradient_matches_torch_reference(need_gpu):
         e = rel(a, b)
         cs = float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b)))
         print(f"  layer {k:2d}: rel err {e:.2e} cos {cs:.5f}", flush=True)
-        assert e <= 8e-2 and cs >= 0.99
+        assert e <= 4e-2 and cs >= 0.999
 
 
 def test_padding_weights_stay_zero_and_training_lowers_the_loss(need_gpu):
@@ -162,3 +162,39 @@ def test_trained_network_gains_over_the_method_without_it(need_gpu):
             assert used == 5 and relres <= 1e-2 * 1.0001
     assert its["trained"] < its["jacobi"]
     assert its["trained"] < its["untrained"]
+
+
+def _plastic_system(n, precision):
+    """Held-out sample block of seed 100 after two committed AMG load steps; the caller
+    solves the first Newton system of load step 3 (the first plastic one)."""
+    m, wl = sample_model(100, n, 0.15, options=pa.default_options(eta_min=1e-8, eta_max=1e-8,
+                                                                  precision=precision))
+    for k in (1, 2):
+        conv, *_ = m.newton(k / wl.nsteps, tol=wl.newton_tol, maxiter=25)
+        assert conv
+        m.commit()
+    return m, 3 / wl.nsteps
+
+
+def test_config3_held_out_seed_100_on_the_128_node_grid(need_gpu):
+    """The shipped network (plasticity_amd/weights, trained on samples logged from 64^3-node
+    blocks of seeds 2-20) on the configuration's own 128^3-node grid (127^3 elements,
+    6.2M DOF) of held-out seed 100: the first plastic Newton system solved to rtol 1e-8 by
+    flexible GCR with the network (AMG takes over when GCR slows down), against FP32-IR AMG
+    and the same GCR with block-Jacobi alone."""
+    import os
+    n = 127
+    weights = os.path.join(os.path.dirname(pa.__file__), "weights", "unet_16_32_64_64.npz")
+    net = UNet((n + 1,) * 3, CH).load(weights)
+    out = {}
+    for label, mode, nn, prec in [("learned GCR + AMG", "precondition", net, "fp64"),
+                                  ("block-Jacobi GCR + AMG", "jacobi", None, "fp64"),
+                                  ("FP32-IR AMG", "amg", None, "fp32_ir")]:
+        m, lam = _plastic_system(n, prec)
+        out[label] = _solve_its(m, lam, 1e-8, mode, nn)
+        m.close()
+        it, used, relres = out[label]
+        print(f"[config 3] {label:24s}: {it:4d} Krylov its (solver {used}), true relres {relres:.2e}", flush=True)
+    it, used, relres = out["learned GCR + AMG"]
+    assert used in (5, 6) and relres <= 1e-7
+    assert out["FP32-IR AMG"][2] <= 1e-7

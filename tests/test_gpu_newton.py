@@ -120,6 +120,26 @@ def test_fp32_ir_matches_fp64_oracle(policy, coarse_max, need_gpu):
         assert eu < 1e-8
 
 
+@pytest.mark.parametrize("policy", [0, 6])
+def test_fp32_krylov_ir_matches_fp64_oracle(policy, need_gpu):
+    """FP32-IR with the Krylov iteration itself in FP32 (FP32 K p, FP32 vectors, FP32
+    V-cycle; FP64 solution and refinement residual): the 80-step jittered compression
+    within 1e-8 of the FP64 oracle, and the solves were done by that solver (9)."""
+    g = np.load(os.path.join(HERE, "golden", "jitter6.npz"))
+    print(f"[newton fp32-krylov IR] 6^3 jittered compression, AMG policy {policy}", flush=True)
+    m, res, out = run_gpu(jitter6(), {20, 80}, precision="fp32_ir", fp32_krylov=1, amg_rebuild=policy,
+                          coarse_max=60)
+    assert res.converged
+    used = {u for step in res.solver_used for u in step}
+    print(f"  solvers used {sorted(used)}; krylov per load step {[sum(k) for k in res.cg_iters[:5]]}...", flush=True)
+    assert 9 in used
+    for n in (20, 80):
+        eu = rel(out[f"U_{n}"], g[f"U_{n}"])
+        ea = np.abs(out[f"abar_{n}"] - g[f"abar_{n}"]).max() / np.abs(g[f"abar_{n}"]).max()
+        print(f"  step {n}: |dU|/|U| = {eu:.2e}  abar {ea:.2e}", flush=True)
+        assert eu < 1e-8 and ea < 1e-8
+
+
 @pytest.mark.parametrize("element", ["finite", "finite_fbar"])
 def test_uniaxial_closed_form_on_gpu(element, need_gpu):
     E, sy0, H = 210e3, 250.0, 1000.0
