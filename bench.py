@@ -247,6 +247,8 @@ def main():
                     help="end a diverging Newton loop early when ||R|| exceeds this multiple of its best value "
                          "(0: run maxiter iterations, the reference)")
     ap.add_argument("--maxiter", type=int, default=25, help="Newton iterations per load increment")
+    ap.add_argument("--newton-tol", type=float, default=None,
+                    help="Newton tolerance on the reference-scaled residual (default: the workload's, 1e-8)")
     ap.add_argument("--grow-iters", type=int, default=6,
                     help="a cut increment doubles again only after two converged increments that each took at "
                          "most this many Newton iterations")
@@ -274,6 +276,8 @@ def main():
 
     t0 = time.perf_counter()
     wl = workload(args.workload, n=args.n, nsteps=args.nsteps, device=True)
+    if args.newton_tol is not None:
+        wl.newton_tol = args.newton_tol
     if args.steps is None:  # the rest of the load path after the warm-up
         args.steps = max(1, wl.nsteps - args.warmup)
     log(f"[bench] workload generated in {time.perf_counter() - t0:.2f}s (mesh on the device)")
@@ -320,9 +324,12 @@ def main():
         sol = [r["solver_used"] for r in recs if r["converged"]]
         return f"load step {k}: increments [{inc}] krylov {kry} solver {sol} t={t:.3f}s"
 
+    path = []  # load factors of the committed increments, replayed by --compare
+
     for k in range(1, args.warmup + 1):
         t1 = time.perf_counter()
         recs = load_step(k)
+        path += [r["lam"] for r in recs if r["converged"]]
         log("[bench] warmup " + show(k, recs, time.perf_counter() - t1))
     s0 = m.stats()
     tot = PathTotals()
@@ -331,6 +338,7 @@ def main():
         t1 = time.perf_counter()
         recs = load_step(k)
         tot.add(recs)
+        path += [r["lam"] for r in recs if r["converged"]]
         log("[bench] " + show(k, recs, time.perf_counter() - t1))
     elapsed = time.perf_counter() - t0
     newton, krylov, t_asm, t_setup, t_solve = tot.newton, tot.krylov, tot.t_asm, tot.t_setup, tot.t_solve
@@ -368,12 +376,18 @@ def main():
                                  line_search=args.line_search, newton_diverge=args.diverge)
         m64 = pa.Model(wl.mesh, mats if len(mats) > 1 else mats[0], options=o64, material_id=wl.material_id)
         m64.constrain(wl.dir_dofs, wl.dir_vals, wl.dir_ramp)
-        r64 = m64.solve(nsteps=wl.nsteps, tol=wl.newton_tol, steps=range(1, args.warmup + args.steps + 1),
-                        maxiter=args.maxiter, cutback=args.cutback, grow_iters=args.grow_iters)
-        if not r64.converged:
-            raise RuntimeError("FP64 comparison run did not converge")
+        m64.reset()
+        m64._freeze()
+        # the same committed load increments as the run above (plasticity is path dependent):
+        # the difference is the precision's, not the increments'
+        for lam in path:
+            conv, hist, _, _ = m64.newton(lam, tol=wl.newton_tol, maxiter=args.maxiter)
+            if not conv:
+                raise RuntimeError(f"FP64 comparison run did not converge at lambda={lam}")
+            m64.commit()
         U64, a64 = m64.U, m64.get_state()["abar"]
-        compare = {"reference": "plain FP64 GPU path (PCG + SA-AMG), same load steps",
+        compare = {"reference": "plain FP64 GPU path (PCG + SA-AMG), the same load increments",
+                   "increments": len(path), "newton_tol": wl.newton_tol,
                    "u_rel_err": float(np.linalg.norm(U_run - U64) / np.linalg.norm(U64)),
                    "abar_rel_err": float(np.linalg.norm(a_run - a64) / max(np.linalg.norm(a64), 1e-300))}
         log(f"[bench] vs FP64 path: displacement rel err {compare['u_rel_err']:.2e}, "
