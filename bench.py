@@ -27,9 +27,13 @@ sys.path.insert(0, ROOT)
 
 METRIC = "s/Newton step @3M DOF, rtol 1e-8: FP64 CG+AMG vs FP32-IR+learned PC"
 HBM_PEAK_GBS = 8000.0
-# FP64 operations per Hex8 F-bar element with all 8 Gauss points plastic: instrumented
-# count of the oracle's element routine (oracle/flopcount.cpp, tests/test_flopcount.py)
-ELEMENT_FLOPS = 160073
+# FP64 operations per Hex8 F-bar element with all 8 Gauss points plastic, counted with a
+# counting-double build (oracle/flopcount.cpp, tests/test_flopcount.py): ELEMENT_FLOPS for
+# the structure-exploiting form of the element (the device kernels' algorithm without
+# their redundant work; same Fe and Ke as the oracle to 1e-15), ELEMENT_FLOPS_DENSE for
+# the oracle's routine as written (dense 9x9 A times the 9x24 Geff, unit-dF columns).
+ELEMENT_FLOPS = 83913
+ELEMENT_FLOPS_DENSE = 160073
 # HBM bytes per fine-SpMV launch at 100^3 (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors
 # from a 1 GiB device copy), measured with rocprofv3 --pmc on `--mode spmv --n 100`:
 # 1,034,757.0 KiB fetched + 24,148.0 KiB written per dispatch.
@@ -380,11 +384,15 @@ def main():
         m64._freeze()
         # the same committed load increments as the run above (plasticity is path dependent):
         # the difference is the precision's, not the increments'
-        for lam in path:
+        t_rep = time.perf_counter()
+        for i, lam in enumerate(path):
             conv, hist, _, _ = m64.newton(lam, tol=wl.newton_tol, maxiter=args.maxiter)
             if not conv:
                 raise RuntimeError(f"FP64 comparison run did not converge at lambda={lam}")
             m64.commit()
+            if (i + 1) % 10 == 0 or i + 1 == len(path):
+                log(f"[bench] FP64 replay: {i + 1} of {len(path)} increments, "
+                    f"{time.perf_counter() - t_rep:.1f}s")
         U64, a64 = m64.U, m64.get_state()["abar"]
         compare = {"reference": "plain FP64 GPU path (PCG + SA-AMG), the same load increments",
                    "increments": len(path), "newton_tol": wl.newton_tol,
@@ -461,6 +469,8 @@ def main():
                              "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                              "frac": ELEMENT_FLOPS * wl.mesh.nelem / ((gp_ms + sc_ms) * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
                              "flops_per_element": ELEMENT_FLOPS,
+                             "flops_per_element_dense": ELEMENT_FLOPS_DENSE,
+                             "achieved_dense_count": ELEMENT_FLOPS_DENSE * wl.mesh.nelem / ((gp_ms + sc_ms) * 1e-3) / 1e12,
                              "elements_per_s": wl.mesh.nelem / ((gp_ms + sc_ms) * 1e-3),
                              "gp_kernel_ms": gp_ms, "scatter_ms": sc_ms},
     }
