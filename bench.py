@@ -174,7 +174,7 @@ def time_variant(pa, wl, precision, learned, warmup, steps, args):
     fallback) or None.  Returns s/Newton step over the timed load steps and the counts."""
     opts = pa.default_options(eta_min=wl.krylov_rtol, eta_max=wl.krylov_rtol, precision=precision,
                               amg_rebuild=args.amg_rebuild, newton_diverge=args.diverge,
-                              fp32_krylov=args.fp32_krylov)
+                              fp32_krylov=args.fp32_krylov, assembly_chunk=args.assembly_chunk)
     mats = [pa.J2Material(**d) for d in wl.materials]
     m = pa.Model(wl.mesh, mats if len(mats) > 1 else mats[0], options=opts, material_id=wl.material_id)
     m.constrain(wl.dir_dofs, wl.dir_vals, wl.dir_ramp)
@@ -257,6 +257,10 @@ def main():
                     help="a cut increment doubles again only after two converged increments that each took at "
                          "most this many Newton iterations")
     ap.add_argument("--verbose", type=int, default=0)
+    ap.add_argument("--assembly-chunk", type=int, default=0,
+                    help="> 0: pipelined assembly, the Gauss-point kernel of one chunk of this many elements "
+                         "beside the scatter of the previous chunk (0: whole-mesh Gauss-point kernel, then "
+                         "the scatter)")
     ap.add_argument("--fp32-krylov", type=int, default=0, choices=[0, 1],
                     help="FP32-IR: 1 = the Krylov iteration itself in FP32 inside FP64 iterative refinement, "
                          "0 = FP64 flexible CG with the FP32 V-cycle")
@@ -300,7 +304,7 @@ def main():
                               smoother_degree=args.smoother_degree,
                               wcycle=args.cycle == "w",
                               gcr_fallback=args.fallback == "gcr", newton_diverge=args.diverge,
-                              fp32_krylov=args.fp32_krylov)
+                              fp32_krylov=args.fp32_krylov, assembly_chunk=args.assembly_chunk)
     t0 = time.perf_counter()
     mats = [pa.J2Material(**d) for d in wl.materials]
     m = pa.Model(wl.mesh, mats if len(mats) > 1 else mats[0], options=opts, material_id=wl.material_id)
@@ -377,7 +381,8 @@ def main():
         m.attach_net(None, "off") if net is not None else None
         m.close()
         o64 = pa.default_options(eta_min=wl.krylov_rtol, eta_max=wl.krylov_rtol, amg_rebuild=args.amg_rebuild,
-                                 line_search=args.line_search, newton_diverge=args.diverge)
+                                 line_search=args.line_search, newton_diverge=args.diverge,
+                                 assembly_chunk=args.assembly_chunk)
         m64 = pa.Model(wl.mesh, mats if len(mats) > 1 else mats[0], options=o64, material_id=wl.material_id)
         m64.constrain(wl.dir_dofs, wl.dir_vals, wl.dir_ramp)
         m64.reset()
@@ -472,7 +477,8 @@ def main():
                              "flops_per_element_dense": ELEMENT_FLOPS_DENSE,
                              "achieved_dense_count": ELEMENT_FLOPS_DENSE * wl.mesh.nelem / ((gp_ms + sc_ms) * 1e-3) / 1e12,
                              "elements_per_s": wl.mesh.nelem / ((gp_ms + sc_ms) * 1e-3),
-                             "gp_kernel_ms": gp_ms, "scatter_ms": sc_ms},
+                             "gp_kernel_ms": gp_ms, "scatter_ms": sc_ms,
+                             "assembly_chunk": args.assembly_chunk},
     }
     print(json.dumps(out), flush=True)
 

@@ -125,7 +125,12 @@ typedef struct pfem_options {
     double newton_diverge;      /* pfem_newton stops a load step early (not converged) when, from the third
                                    iteration on, ||R_k|| > newton_diverge * min_{j<k} ||R_j||; 0 = never (the
                                    reference runs maxiter iterations; default).  For load-step cut-back. */
-    int32_t reserved[8];        /* zero */
+    int32_t assembly_chunk;     /* > 0: pipelined assembly -- the elements of each colour in chunks of at most
+                                   this many; the Gauss-point kernel of chunk k+1 runs beside the scatter of
+                                   chunk k on a second stream, and the packets live in two chunk-sized
+                                   buffers instead of one per Gauss point of the mesh.  0: the Gauss-point
+                                   kernel over the whole mesh, then the scatter colour by colour. */
+    int32_t reserved[7];        /* zero */
 } pfem_options;
 
 typedef struct pfem_fail {

@@ -110,17 +110,21 @@ struct ElementArgs {
     int commit;
     double *epsp, *abar, *cpinv, *sig;  // Gauss state (6 x ngp / ngp)
     double *beta;           // back stress (6 x ngp); reference configuration for finite strain
-    double *packet;         // per GP packet (PK doubles)
+    double *packet;         // per GP packet (PK doubles), 8 (i - e_begin) + g for list position i
     double *g0;             // per element 24
     unsigned long long *fail_key;  // min (8e+g) of inverted GPs, ~0ull if none
+    const int64_t *elist;   // element of list position i (null: element i)
 };
 constexpr int PK = 132;     // doubles per Gauss-point packet
 
 void launch_gp_kernel(const ElementArgs &a, int64_t e_begin, int64_t e_count, hipStream_t s);
 // first: per element, bit 8a+b set where block (a, b) is the first contribution to its
 // slot in colour order (stored, not added: K needs no clearing before the scatter)
+// local: the packets of elem_list[li] are at a.packet + 8 li PK (a chunk's buffer) instead
+// of a.packet + 8 e PK (one packet per Gauss point of the mesh)
 void launch_scatter_kernel(const ElementArgs &a, const int64_t *elem_list, int64_t count,
-                           const int32_t *slots, const uint64_t *first, double *Kv, double *R, hipStream_t s);
+                           const int32_t *slots, const uint64_t *first, double *Kv, double *R, hipStream_t s,
+                           bool local = false);
 void build_first_touch(int64_t nslots, int ncolors, const int64_t *order, const int64_t *color_ptr,
                        const int32_t *slots, uint64_t *mask, hipStream_t s);
 void launch_dense_ke_kernel(const ElementArgs &a, int64_t nelem, double *Fe, double *Ke,

@@ -157,6 +157,10 @@ int pfem_create(const pfem_mesh_desc *md, const pfem_options *opts, pfem_ctx **o
         PF_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         PF_HIP(hipEventCreate(&c->ev0));
         PF_HIP(hipEventCreate(&c->ev1));
+        if (c->opt.assembly_chunk > 0) {
+            PF_HIP(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+            for (auto &ev : c->ev_pipe) PF_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        }
         PF_HIP(hipHostMalloc((void **)&c->h_scal, 64 * sizeof(double), hipHostMallocDefault));
         hipStream_t s = c->stream;
         const int64_t nn = md->nnodes, ne = md->nelem;
@@ -297,7 +301,14 @@ int pfem_create(const pfem_mesh_desc *md, const pfem_options *opts, pfem_ctx **o
         c->epsp.alloc(6 * ngp); c->abar.alloc(ngp); c->cpinv.alloc(6 * ngp); c->sig.alloc(6 * ngp);
         c->sig_c.alloc(6 * ngp);
         if (c->kin || c->kind == PFEM_HEX8_SMALL) c->beta.alloc(6 * ngp);
-        c->packet.alloc((size_t)ngp * PK);
+        // packets: one per Gauss point of the mesh, or two chunk-sized buffers when the
+        // assembly is pipelined (the buffer is also pfem_gauss_stress's 6 x ngp scratch)
+        int64_t max_colour = 0;
+        for (int col = 0; col < c->ncolors; ++col)
+            max_colour = std::max<int64_t>(max_colour, c->color_ptr[col + 1] - c->color_ptr[col]);
+        c->packet_elems = c->opt.assembly_chunk > 0 ? std::min<int64_t>(c->opt.assembly_chunk, max_colour) : 0;
+        c->packet.alloc(c->packet_elems ? std::max<size_t>((size_t)2 * c->packet_elems * 8 * PK, (size_t)6 * ngp)
+                                        : (size_t)ngp * PK);
         c->g0.alloc(24 * ne);
         c->fail_key.alloc(1);
         const int64_t nd = c->ndof;
@@ -326,13 +337,17 @@ int pfem_create(const pfem_mesh_desc *md, const pfem_options *opts, pfem_ctx **o
 void pfem_destroy(pfem_ctx *c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     c->amg.reset();
     if (c->h_scal) (void)hipHostFree(c->h_scal);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
-    hipStream_t s = c->stream;
+    for (auto ev : c->ev_pipe)
+        if (ev) (void)hipEventDestroy(ev);
+    hipStream_t s = c->stream, s2 = c->stream2;
     delete c;  // buffers freed by their destructors
     if (s) (void)hipStreamDestroy(s);
+    if (s2) (void)hipStreamDestroy(s2);
 }
 
 int pfem_reset(pfem_ctx *c) {

@@ -82,6 +82,10 @@ struct pfem_ctx {
     // statistics
     pfem_stats st{};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // pipelined assembly (assembly_chunk > 0): the scatter's stream and the events that
+    // order it against the Gauss-point kernels (start, gp[2], scatter[2], join)
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_pipe[6] = {};
 
     double sync_read(const double *dptr);  // D2H of one device scalar
     void sync() { PF_HIP(hipStreamSynchronize(stream)); }

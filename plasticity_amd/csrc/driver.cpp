@@ -62,16 +62,50 @@ int assemble_impl(pfem_ctx *c, int commit, pfem_fail *fail, bool state_only) {
     // Assembly.jl:361-362 zeroes K and R: R here, K through the scatter's first-touch
     // stores
     if (!state_only) c->R.zero(s);
+    const bool pipelined = c->packet_elems > 0 && !state_only;
     PF_HIP(hipEventRecord(c->ev0, s));
-    launch_gp_kernel(a, 0, c->nelem, s);
+    if (pipelined) {
+        // Chunks of at most packet_elems elements of one colour, in colour order.  Chunk k:
+        // Gauss-point kernel on `s` into buffer k % 2 (after the scatter of chunk k - 2 has
+        // read it), then its scatter on `stream2` (after that Gauss-point kernel).  The
+        // scatters run in colour order on one stream, so first-touch stores still precede
+        // the adds into each block.
+        hipStream_t s2 = c->stream2;
+        hipEvent_t ev_start = c->ev_pipe[0], ev_join = c->ev_pipe[5];
+        hipEvent_t ev_gp[2] = {c->ev_pipe[1], c->ev_pipe[2]}, ev_sc[2] = {c->ev_pipe[3], c->ev_pipe[4]};
+        PF_HIP(hipEventRecord(ev_start, s));  // everything before (R cleared, solves reading K)
+        PF_HIP(hipStreamWaitEvent(s2, ev_start, 0));
+        const int64_t chunk = c->packet_elems;
+        a.elist = c->color_order.p;
+        int64_t k = 0;
+        for (int col = 0; col < c->ncolors; ++col)
+            for (int64_t b = c->color_ptr[col]; b < c->color_ptr[col + 1]; b += chunk, ++k) {
+                const int64_t n = std::min(chunk, c->color_ptr[col + 1] - b);
+                const int buf = (int)(k & 1);
+                if (k >= 2) PF_HIP(hipStreamWaitEvent(s, ev_sc[buf], 0));
+                a.packet = c->packet.p + (size_t)buf * chunk * 8 * PK;
+                launch_gp_kernel(a, b, n, s);
+                PF_HIP(hipEventRecord(ev_gp[buf], s));
+                PF_HIP(hipStreamWaitEvent(s2, ev_gp[buf], 0));
+                launch_scatter_kernel(a, c->color_order.p + b, n, c->slots.p, c->first_touch.p, c->K.v64.p, c->R.p,
+                                      s2, true);
+                PF_HIP(hipEventRecord(ev_sc[buf], s2));
+            }
+        PF_HIP(hipEventRecord(ev_join, s2));
+        PF_HIP(hipStreamWaitEvent(s, ev_join, 0));
+    } else {
+        launch_gp_kernel(a, 0, c->nelem, s);
+    }
     PF_HIP(hipEventRecord(c->ev1, s));
     unsigned long long key = 0;
     PF_HIP(hipMemcpyAsync(&key, c->fail_key.p, sizeof key, hipMemcpyDeviceToHost, s));
     c->sync();
     float ms_gp = 0.f;
     PF_HIP(hipEventElapsedTime(&ms_gp, c->ev0, c->ev1));
-    if (!state_only) c->st.t_gp_kernel += 1e-3 * ms_gp;  // per full assembly (n_assemble)
+    // per full assembly (n_assemble); pipelined: the span of both kernels together
+    if (!state_only) c->st.t_gp_kernel += 1e-3 * ms_gp;
     if (key != ~0ull) {  // ElementInversionError(e, g, J)
+        if (pipelined) c->R.zero(s);  // the scatters ran: R is returned cleared as without the pipeline
         launch_detF_kernel(a, (int64_t)key, c->scal.p, s);
         double J = c->sync_read(c->scal.p);
         if (fail) { fail->e = (int64_t)(key / 8); fail->g = (int32_t)(key % 8); fail->J = J; }
@@ -86,15 +120,18 @@ int assemble_impl(pfem_ctx *c, int commit, pfem_fail *fail, bool state_only) {
         c->sync();
         return PFEM_OK;
     }
-    PF_HIP(hipEventRecord(c->ev0, s));
-    for (int col = 0; col < c->ncolors; ++col)
-        launch_scatter_kernel(a, c->color_order.p + c->color_ptr[col], c->color_ptr[col + 1] - c->color_ptr[col],
-                              c->slots.p, c->first_touch.p, c->K.v64.p, c->R.p, s);
-    PF_HIP(hipEventRecord(c->ev1, s));
-    c->sync();
-    float ms_sc = 0.f;
-    PF_HIP(hipEventElapsedTime(&ms_sc, c->ev0, c->ev1));
-    c->st.t_scatter_kernel += 1e-3 * ms_sc;
+    if (!pipelined) {
+        PF_HIP(hipEventRecord(c->ev0, s));
+        for (int col = 0; col < c->ncolors; ++col)
+            launch_scatter_kernel(a, c->color_order.p + c->color_ptr[col],
+                                  c->color_ptr[col + 1] - c->color_ptr[col], c->slots.p, c->first_touch.p,
+                                  c->K.v64.p, c->R.p, s);
+        PF_HIP(hipEventRecord(c->ev1, s));
+        c->sync();
+        float ms_sc = 0.f;
+        PF_HIP(hipEventElapsedTime(&ms_sc, c->ev0, c->ev1));
+        c->st.t_scatter_kernel += 1e-3 * ms_sc;
+    }
     c->st.t_assemble += now() - t0;
     c->st.n_assemble++;
     c->K_valid = true;

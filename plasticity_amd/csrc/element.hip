@@ -175,8 +175,9 @@ __global__ __launch_bounds__(256) void gp_kernel(ElementArgs a, int64_t e_begin,
     __shared__ double sX[32][24];
     __shared__ double sU[32][24];
     const int le = threadIdx.x >> 3, g = threadIdx.x & 7;
-    const int64_t e = e_begin + (int64_t)blockIdx.x * 32 + le;
-    const bool valid = e < e_end;
+    const int64_t li = e_begin + (int64_t)blockIdx.x * 32 + le;  // list position
+    const bool valid = li < e_end;
+    const int64_t e = valid ? (a.elist ? a.elist[li] : li) : 0;
     if (valid) {
         const int64_t node = a.conn[8 * e + g];
 #pragma unroll
@@ -455,7 +456,7 @@ __global__ __launch_bounds__(256) void gp_kernel(ElementArgs a, int64_t e_begin,
         kc = (2 * G * s32 / (3 * G + Hs) * (Y / (nrm * nrm))) / nrm;
     }
 
-    double *pk = a.packet + (size_t)gp * PK;
+    double *pk = a.packet + (size_t)(8 * (li - e_begin) + g) * PK;
     const double sw = s * w;
 
     // --- A = dP/dFbar, one column per unit perturbation dF = e_p e_q^T
@@ -596,8 +597,9 @@ __global__ __launch_bounds__(256) void gp_small_kernel(ElementArgs a, int64_t e_
     __shared__ double sX[32][24];
     __shared__ double sU[32][24];
     const int le = threadIdx.x >> 3, g = threadIdx.x & 7;
-    const int64_t e = e_begin + (int64_t)blockIdx.x * 32 + le;
-    const bool valid = e < e_end;
+    const int64_t li = e_begin + (int64_t)blockIdx.x * 32 + le;  // list position
+    const bool valid = li < e_end;
+    const int64_t e = valid ? (a.elist ? a.elist[li] : li) : 0;
     if (valid) {
         const int64_t node = a.conn[8 * e + g];
 #pragma unroll
@@ -695,7 +697,7 @@ __global__ __launch_bounds__(256) void gp_small_kernel(ElementArgs a, int64_t e_
             const double Idev = (v == u ? (v < 3 ? 1.0 : 0.5) : 0.0) - ((v < 3 && u < 3) ? 1.0 / 3.0 : 0.0);
             Dm[v][u] = w * (Cvu - cdev * Idev + cnn * nh[v] * nh[u]);
         }
-    double *pk = a.packet + (size_t)gp * PK;
+    double *pk = a.packet + (size_t)(8 * (li - e_begin) + g) * PK;
     const int VO[9] = {0, 3, 5, 3, 1, 4, 5, 4, 2};  // Voigt index of (i, j), i + 3j
 #pragma unroll
     for (int c = 0; c < 9; ++c)
@@ -784,7 +786,8 @@ __global__ __launch_bounds__(64) void scatter_kernel(const double *__restrict__ 
                                                       const int64_t *__restrict__ conn,
                                                       const int32_t *__restrict__ slots,
                                                       const uint64_t *__restrict__ first,
-                                                      double *__restrict__ Kv, double *__restrict__ R, int fbar) {
+                                                      double *__restrict__ Kv, double *__restrict__ R, int fbar,
+                                                      int local) {
     constexpr int NC = PK / 2;  // 16-byte chunks per Gauss-point packet (66)
     __shared__ __attribute__((aligned(16))) double sp[2][PK];
     __shared__ __attribute__((aligned(16))) double T[8 * 27];
@@ -792,7 +795,7 @@ __global__ __launch_bounds__(64) void scatter_kernel(const double *__restrict__ 
     if (li >= count) return;
     const int64_t e = elist[li];
     const int lane = threadIdx.x, a = lane >> 3, b = lane & 7;
-    const double2 *src = reinterpret_cast<const double2 *>(packet + (size_t)e * 8 * PK);
+    const double2 *src = reinterpret_cast<const double2 *>(packet + (size_t)(local ? li : e) * 8 * PK);
     // lanes 0 and 1 also fetch the two chunks beyond 64; the other lanes re-read their own
     // chunk into r1 and never store it
     double2 r0 = src[lane], r1 = src[lane < NC - 64 ? 64 + lane : lane];
@@ -983,10 +986,11 @@ void launch_gp_kernel(const ElementArgs &a, int64_t e_begin, int64_t e_count, hi
 }
 
 void launch_scatter_kernel(const ElementArgs &a, const int64_t *elem_list, int64_t count,
-                           const int32_t *slots, const uint64_t *first, double *Kv, double *R, hipStream_t s) {
+                           const int32_t *slots, const uint64_t *first, double *Kv, double *R, hipStream_t s,
+                           bool local) {
     if (count <= 0) return;
     hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)count), dim3(64), 0, s, a.packet, a.g0, elem_list, count,
-                       a.conn, slots, first, Kv, R, a.fbar);
+                       a.conn, slots, first, Kv, R, a.fbar, (int)local);
     PF_HIP(hipGetLastError());
 }
 
