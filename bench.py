@@ -257,6 +257,8 @@ def main():
                     help="a cut increment doubles again only after two converged increments that each took at "
                          "most this many Newton iterations")
     ap.add_argument("--verbose", type=int, default=0)
+    ap.add_argument("--show-residuals", action="store_true",
+                    help="print each increment's Newton residual history in the per-load-step lines")
     ap.add_argument("--assembly-chunk", type=int, default=0,
                     help="> 0: pipelined assembly, the Gauss-point kernel of one chunk of this many elements "
                          "beside the scatter of the previous chunk (0: whole-mesh Gauss-point kernel, then "
@@ -330,7 +332,10 @@ def main():
         inc = ", ".join(f"{r['lam'] * wl.nsteps:.4g}:{r['iters']}{'' if r['converged'] else 'x'}" for r in recs)
         kry = [r["krylov"] for r in recs if r["converged"]]
         sol = [r["solver_used"] for r in recs if r["converged"]]
-        return f"load step {k}: increments [{inc}] krylov {kry} solver {sol} t={t:.3f}s"
+        res = ""
+        if args.show_residuals:
+            res = " residuals " + str([[float(f"{x:.3e}") for x in r["residuals"]] for r in recs])
+        return f"load step {k}: increments [{inc}] krylov {kry} solver {sol}{res} t={t:.3f}s"
 
     path = []  # load factors of the committed increments, replayed by --compare
 

@@ -130,7 +130,9 @@ typedef struct pfem_options {
                                    chunk k on a second stream, and the packets live in two chunk-sized
                                    buffers instead of one per Gauss point of the mesh.  0: the Gauss-point
                                    kernel over the whole mesh, then the scatter colour by colour. */
-    int32_t reserved[7];        /* zero */
+    int32_t smoother_unfused;   /* 1 = the fine-level Chebyshev steps as a product launch followed by a vector
+                                   launch; 0 = one launch that does both (default; same results bit for bit) */
+    int32_t reserved[6];        /* zero */
 } pfem_options;
 
 typedef struct pfem_fail {

@@ -64,7 +64,8 @@ class Options(C.Structure):
                 ("fp16_fine", C.c_int32), ("line_search", C.c_int32), ("rho_iters_cold", C.c_int32),
                 ("rho_iters_warm", C.c_int32), ("host_build", C.c_int32), ("vcycle_eager", C.c_int32),
                 ("prolong_rowwise", C.c_int32), ("fp32_krylov", C.c_int32), ("newton_diverge", C.c_double),
-                ("assembly_chunk", C.c_int32), ("reserved", C.c_int32 * 7)]
+                ("assembly_chunk", C.c_int32), ("smoother_unfused", C.c_int32),
+                ("reserved", C.c_int32 * 6)]
 
 
 class Fail(C.Structure):

@@ -546,6 +546,23 @@ void level_residual(pfem_ctx *c, AmgLevel &L, int l, const T *x, const T *b, T *
         bsr_residual<6, T, T>(L.nb, L.rowptr.p, L.colind.p, Vec<T>::A(L), x, b, r, c->stream);
     }
 }
+// One Chebyshev step after the first: t = A d, r -= t, d = c1 d + c2 Dinv r, x += d.  On
+// the fine level one fused launch (sell_cheb) unless the FP16 copy serves the smoother's
+// products or smoother_unfused is set; `last` (neither r nor d read again before the
+// next smoother overwrites them) lets the fused launch skip their stores.
+template <class T>
+void cheb_update(pfem_ctx *c, AmgLevel &L, int l, T *&d, T *&t, T *r, T *x, double c1, double c2, bool last) {
+    bool fused = l == 0 && !c->opt.smoother_unfused;
+    if constexpr (std::is_same<T, float>::value) fused = fused && !c->half_fine;
+    if (fused) {
+        sell_cheb<T, T>(c->K, Vec<T>::K(c), d, Vec<T>::Dinv(L), r, last ? nullptr : t, x, c1, c2, c->stream);
+        if (!last) std::swap(d, t);  // the new direction is in t; the old one is free
+        return;
+    }
+    level_matvec<T>(c, L, l, d, t);
+    if (L.bs == 3) cheb_step<3, T, T>(L.nb, Vec<T>::Dinv(L), t, r, d, x, c1, c2, c->stream);
+    else cheb_step<6, T, T>(L.nb, Vec<T>::Dinv(L), t, r, d, x, c1, c2, c->stream);
+}
 template <class T>
 void diag_apply(AmgLevel &L, const T *r, T *z, double scale, hipStream_t s) {
     if (L.bs == 3) block_diag_apply<3, T, T>(L.nb, Vec<T>::Dinv(L), r, z, scale, s);
@@ -586,10 +603,8 @@ void smooth(pfem_ctx *c, Amg *a, int l, const T *b, T *x, bool zero_init) {
         if (!zero_init) level_residual<T>(c, L, l, x, b, r);
         diag_first_step<T>(L, zero_init ? b : r, zero_init ? r : nullptr, d, x, 4.0 / (3.0 * lam), zero_init, s);
         for (int i = 1; i < a->degree; ++i) {
-            level_matvec<T>(c, L, l, d, t);
             const double c1 = (2.0 * i - 1.0) / (2.0 * i + 3.0), c2 = (8.0 * i + 4.0) / ((2.0 * i + 3.0) * lam);
-            if (L.bs == 3) cheb_step<3, T, T>(L.nb, Vec<T>::Dinv(L), t, r, d, x, c1, c2, s);
-            else cheb_step<6, T, T>(L.nb, Vec<T>::Dinv(L), t, r, d, x, c1, c2, s);
+            cheb_update<T>(c, L, l, d, t, r, x, c1, c2, i + 1 == a->degree);
         }
         return;
     }
@@ -600,11 +615,9 @@ void smooth(pfem_ctx *c, Amg *a, int l, const T *b, T *x, bool zero_init) {
     if (!zero_init) level_residual<T>(c, L, l, x, b, r);
     diag_first_step<T>(L, zero_init ? b : r, zero_init ? r : nullptr, d, x, 1.0 / theta, zero_init, s);
     for (int k = 2; k <= a->degree; ++k) {
-        level_matvec<T>(c, L, l, d, t);
         const double rho_new = 1.0 / (2.0 * sigma - rho_old);
         const double c1 = rho_new * rho_old, c2 = 2.0 * rho_new / delta;
-        if (L.bs == 3) cheb_step<3, T, T>(L.nb, Vec<T>::Dinv(L), t, r, d, x, c1, c2, s);
-        else cheb_step<6, T, T>(L.nb, Vec<T>::Dinv(L), t, r, d, x, c1, c2, s);
+        cheb_update<T>(c, L, l, d, t, r, x, c1, c2, k == a->degree);
         rho_old = rho_new;
     }
 }

@@ -122,6 +122,11 @@ template <int BS, class VT, class XT>
 void sell_residual(const FineMatrix &K, const VT *vals, const XT *x, const XT *b, XT *r, hipStream_t s);
 template <int BS, class VT, class XT>
 void sell_spmv(const FineMatrix &K, const VT *vals, const XT *x, XT *y, hipStream_t s);
+// fine-level Chebyshev step with its product: t = K d; r -= t; d' = c1 d + c2 Dinv r;
+// x += d' (d' into d_out; d_out == null: the last step, r and d' not stored)
+template <class VT, class XT>
+void sell_cheb(const FineMatrix &K, const VT *vals, const XT *d, const VT *dinv, XT *r, XT *d_out, XT *x, double c1,
+               double c2, hipStream_t s);
 template <int BS, class DT, class XT>
 void block_diag_apply(int64_t nb, const DT *dinv, const XT *r, XT *z, double scale, hipStream_t s);
 // d = scale Dinv v; zero_init: r = v, x = d; else x += d

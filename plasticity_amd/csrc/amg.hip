@@ -510,9 +510,11 @@ __global__ void cheb_step_kernel(int64_t nb, const DT *__restrict__ dinv, const 
     for (int rr = 0; rr < BS; ++rr) {
         DT s = 0;
 #pragma unroll
-        for (int c = 0; c < BS; ++c) s += m[BS * rr + c] * rv[c];
+        for (int c = 0; c < BS; ++c) s = fma(m[BS * rr + c], rv[c], s);
         const int64_t o = (int64_t)BS * i + rr;
-        const double dn = c1 * (double)d[o] + c2 * (double)s;
+        // explicit fused multiply-adds: the fine level's fused product + step
+        // (sell_cheb_kernel) rounds the same way
+        const double dn = fma(c1, (double)d[o], c2 * (double)s);
         d[o] = (XT)dn;
         x[o] = (XT)((double)x[o] + dn);
     }
@@ -902,6 +904,13 @@ void sell_spmv(const FineMatrix &K, const VT *vals, const XT *x, XT *y, hipStrea
                        K.slice_ptr.p, K.colind.p, vals, x, (const XT *)nullptr, y, nullptr);
     PF_HIP(hipGetLastError());
 }
+template <class VT, class XT>
+void sell_cheb(const FineMatrix &K, const VT *vals, const XT *d, const VT *dinv, XT *r, XT *d_out, XT *x, double c1,
+               double c2, hipStream_t s) {
+    hipLaunchKernelGGL((sell_cheb_kernel<VT, XT>), dim3((unsigned)K.nslices), dim3(SELL_C), 0, s, K.nb,
+                       K.slice_ptr.p, K.colind.p, vals, d, dinv, r, d_out, x, c1, c2);
+    PF_HIP(hipGetLastError());
+}
 template <int BS, class DT, class XT>
 void block_diag_apply(int64_t nb, const DT *dinv, const XT *r, XT *z, double scale, hipStream_t s) {
     hipLaunchKernelGGL((block_diag_apply_kernel<BS, DT, XT>), dim3(grid_for(nb * BS, 256)), dim3(256), 0, s, nb,
@@ -985,6 +994,8 @@ template void amg_transpose_P<6>(int64_t, const int32_t *, const double *, doubl
     template void sell_residual<3, VT, XT>(const FineMatrix &, const VT *, const XT *, const XT *, XT *,           \
                                            hipStream_t);                                                           \
     template void sell_spmv<3, VT, XT>(const FineMatrix &, const VT *, const XT *, XT *, hipStream_t);             \
+    template void sell_cheb<VT, XT>(const FineMatrix &, const VT *, const XT *, const VT *, XT *, XT *, XT *,      \
+                                    double, double, hipStream_t);                                                  \
     template void block_diag_apply<3, VT, XT>(int64_t, const VT *, const XT *, XT *, double, hipStream_t);         \
     template void block_diag_apply<6, VT, XT>(int64_t, const VT *, const XT *, XT *, double, hipStream_t);         \
     template void cheb_step<3, VT, XT>(int64_t, const VT *, const XT *, XT *, XT *, XT *, double, double,          \

@@ -17,8 +17,8 @@ namespace {
 
 constexpr int64_t NA_MAX = PFEM_BATCH_NOALLOC_MAX;
 // doubles per element: X 24, U 24, conn 8 (int64), epsp 48, beta 48, abar 8, cpinv 48,
-// sig 48, packets 8 * PK, g0 24, Fe 24, Ke 576
-constexpr int64_t NA_PER_ELEM = 24 + 24 + 8 + 48 + 48 + 8 + 48 + 48 + 8 * PK + 24 + 24 + 576;
+// sig 48, packets 8 * PK, reference gradients 8 * 24, g0 24, Fe 24, Ke 576
+constexpr int64_t NA_PER_ELEM = 24 + 24 + 8 + 48 + 48 + 8 + 48 + 48 + 8 * PK + 8 * 24 + 24 + 24 + 576;
 constexpr size_t NA_HEAD = 768;  // material, inversion key, det F of the reported point
 constexpr size_t NA_BYTES = NA_HEAD + (size_t)NA_MAX * NA_PER_ELEM * sizeof(double);
 
@@ -80,7 +80,8 @@ extern "C" int pfem_element_batch_noalloc(int32_t kind, const pfem_material *mat
         double *X = take(24 * ne), *U = take(24 * ne);
         int64_t *cn = reinterpret_cast<int64_t *>(take(8 * ne));
         double *ep = take(6 * ngp), *bt = take(6 * ngp), *ab = take(ngp), *cp = take(6 * ngp), *sg = take(6 * ngp);
-        double *pk = take(ngp * PK), *g0 = take(24 * ne), *fe = take(24 * ne), *ke = take(576 * ne);
+        double *pk = take(ngp * PK), *gr = take(ngp * 24), *g0 = take(24 * ne), *fe = take(24 * ne),
+               *ke = take(576 * ne);
         const size_t d = sizeof(double);
         PF_HIP(hipMemcpyAsync(md, &dm, sizeof dm, hipMemcpyHostToDevice, s));
         PF_HIP(hipMemcpyAsync(X, Xe, 24 * ne * d, hipMemcpyHostToDevice, s));
@@ -103,6 +104,7 @@ extern "C" int pfem_element_batch_noalloc(int32_t kind, const pfem_material *mat
         a.kind = kind; a.fbar = kind == PFEM_HEX8_FINITE_FBAR; a.kin = dm.Hkin > 0; a.commit = commit;
         a.epsp = ep; a.beta = bt; a.abar = ab; a.cpinv = cp;
         a.sig = sg; a.packet = pk; a.g0 = g0; a.fail_key = key;
+        a.geo = gr; a.geo_write = 1;
         launch_gp_kernel(a, 0, ne, s);
         unsigned long long hk = 0;
         PF_HIP(hipMemcpyAsync(&hk, key, sizeof hk, hipMemcpyDeviceToHost, s));

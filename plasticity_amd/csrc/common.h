@@ -110,12 +110,33 @@ struct ElementArgs {
     int commit;
     double *epsp, *abar, *cpinv, *sig;  // Gauss state (6 x ngp / ngp)
     double *beta;           // back stress (6 x ngp); reference configuration for finite strain
-    double *packet;         // per GP packet (PK doubles), 8 (i - e_begin) + g for list position i
+    double *packet;         // per GP packet (pk_len doubles), 8 (i - e_begin) + g for list position i
     double *g0;             // per element 24
     unsigned long long *fail_key;  // min (8e+g) of inverted GPs, ~0ull if none
     const int64_t *elist;   // element of list position i (null: element i)
+    double *geo;            // per GP 24: reference gradients dN/dX (written when geo_write)
+    int geo_write;
 };
-constexpr int PK = 132;     // doubles per Gauss-point packet
+// Gauss-point packet, written by the Gauss-point kernels and read by the contraction:
+//   [0, na)           s w A: A = dP/dFbar (s the F-bar factor, w the weight; the
+//                     small-strain kernel's A holds w D).  Where A has major symmetry
+//                     (every case but the objective back stress of finite-strain
+//                     kinematic hardening) its upper triangle, packed by columns:
+//                     (r, c), r <= c, at c (c + 1) / 2 + r (na = 45); else all 81, r + 9c.
+//   [na, na + 9)      (s w / 3) A vec(F)   (F-bar)
+//   [na + 9, na + 18) F^-1                 (F-bar)
+//   [na + 18, na + 27) w P
+// The reference gradients of the Gauss point (24) are in the geometry array `geo`, written
+// once per context.
+constexpr int PK_SYM = 72, PK_FULL = 108;
+constexpr int PK = PK_FULL;  // the larger layout, for buffer sizes
+__host__ __device__ constexpr int pk_len(bool sym) { return sym ? PK_SYM : PK_FULL; }
+__host__ __device__ constexpr int pk_na(bool sym) { return sym ? 45 : 81; }
+__host__ __device__ constexpr int pk_sym_index(int r, int c) {  // packed upper triangle
+    return r <= c ? c * (c + 1) / 2 + r : r * (r + 1) / 2 + c;
+}
+// packets of this launch's kind and materials use the packed symmetric layout
+inline bool packet_sym(const ElementArgs &a) { return a.kind == 2 || !a.kin; }
 
 void launch_gp_kernel(const ElementArgs &a, int64_t e_begin, int64_t e_count, hipStream_t s);
 // first: per element, bit 8a+b set where block (a, b) is the first contribution to its

@@ -307,8 +307,10 @@ int pfem_create(const pfem_mesh_desc *md, const pfem_options *opts, pfem_ctx **o
         for (int col = 0; col < c->ncolors; ++col)
             max_colour = std::max<int64_t>(max_colour, c->color_ptr[col + 1] - c->color_ptr[col]);
         c->packet_elems = c->opt.assembly_chunk > 0 ? std::min<int64_t>(c->opt.assembly_chunk, max_colour) : 0;
-        c->packet.alloc(c->packet_elems ? std::max<size_t>((size_t)2 * c->packet_elems * 8 * PK, (size_t)6 * ngp)
-                                        : (size_t)ngp * PK);
+        const size_t plen = pk_len(c->kind == PFEM_HEX8_SMALL || !c->kin);
+        c->packet.alloc(std::max<size_t>(c->packet_elems ? (size_t)2 * c->packet_elems * 8 * plen : (size_t)ngp * plen,
+                                         (size_t)6 * ngp));
+        c->geo.alloc((size_t)24 * ngp);
         c->g0.alloc(24 * ne);
         c->fail_key.alloc(1);
         const int64_t nd = c->ndof;

@@ -43,7 +43,9 @@ struct pfem_ctx {
     pfem::dbuf<double> epsp, abar, cpinv, sig, sig_c;
     pfem::dbuf<double> beta;   // back stress; allocated for small strain or Hkin > 0 only
     pfem::dbuf<double> packet, g0;
-    int64_t packet_elems = 0;  // elements per packet chunk
+    int64_t packet_elems = 0;  // elements per packet chunk (pipelined assembly), else 0
+    pfem::dbuf<double> geo;    // per Gauss point: reference gradients dN/dX (24)
+    bool geo_ready = false;    // geo written (by the first full assembly)
     pfem::dbuf<unsigned long long> fail_key;
 
     // vectors (U_c: U at the last commit, restored by pfem_rollback)

@@ -47,6 +47,7 @@ static ElementArgs element_args(pfem_ctx *c, int commit) {
     a.packet = c->packet.p;
     a.g0 = c->g0.p;
     a.fail_key = c->fail_key.p;
+    a.geo = c->geo.p;
     return a;
 }
 
@@ -58,6 +59,7 @@ int assemble_impl(pfem_ctx *c, int commit, pfem_fail *fail, bool state_only) {
     if (commit) a.sig = c->sig_c.p;
     if (state_only) a.packet = nullptr;  // stress and history only: no tangent, no K or R
     else c->K_valid = false;             // until the scatter has run
+    a.geo_write = !state_only && !c->geo_ready;  // the reference gradients, once per context
     PF_HIP(hipMemsetAsync(c->fail_key.p, 0xff, sizeof(unsigned long long), s));
     // Assembly.jl:361-362 zeroes K and R: R here, K through the scatter's first-touch
     // stores
@@ -83,7 +85,7 @@ int assemble_impl(pfem_ctx *c, int commit, pfem_fail *fail, bool state_only) {
                 const int64_t n = std::min(chunk, c->color_ptr[col + 1] - b);
                 const int buf = (int)(k & 1);
                 if (k >= 2) PF_HIP(hipStreamWaitEvent(s, ev_sc[buf], 0));
-                a.packet = c->packet.p + (size_t)buf * chunk * 8 * PK;
+                a.packet = c->packet.p + (size_t)buf * chunk * 8 * pk_len(packet_sym(a));
                 launch_gp_kernel(a, b, n, s);
                 PF_HIP(hipEventRecord(ev_gp[buf], s));
                 PF_HIP(hipStreamWaitEvent(s2, ev_gp[buf], 0));
@@ -100,6 +102,7 @@ int assemble_impl(pfem_ctx *c, int commit, pfem_fail *fail, bool state_only) {
     unsigned long long key = 0;
     PF_HIP(hipMemcpyAsync(&key, c->fail_key.p, sizeof key, hipMemcpyDeviceToHost, s));
     c->sync();
+    if (a.geo_write) c->geo_ready = true;  // every Gauss point writes it, inverted or not
     float ms_gp = 0.f;
     PF_HIP(hipEventElapsedTime(&ms_gp, c->ev0, c->ev1));
     // per full assembly (n_assemble); pipelined: the span of both kernels together
@@ -990,7 +993,7 @@ int pfem_element_batch(int32_t kind, const pfem_material *mat, int64_t nelem, co
         struct SG { hipStream_t s; ~SG() { (void)hipStreamDestroy(s); } } sg{s};
         const int64_t ngp = 8 * nelem;
         dbuf<double> X(24 * nelem), U(24 * nelem), ep(6 * ngp), bt(6 * ngp), ab(ngp), cp(6 * ngp), sg6(6 * ngp),
-            pk((size_t)ngp * PK), g0(24 * nelem), fe(24 * nelem), ke(576 * nelem);
+            pk((size_t)ngp * PK), geo((size_t)ngp * 24), g0(24 * nelem), fe(24 * nelem), ke(576 * nelem);
         dbuf<int64_t> cn(8 * nelem);
         dbuf<DevMat> md(1);
         dbuf<unsigned long long> fk(1);
@@ -1008,6 +1011,7 @@ int pfem_element_batch(int32_t kind, const pfem_material *mat, int64_t nelem, co
         a.kind = kind; a.fbar = kind == PFEM_HEX8_FINITE_FBAR; a.kin = dm.Hkin > 0; a.commit = commit;
         a.epsp = ep.p; a.beta = bt.p; a.abar = ab.p; a.cpinv = cp.p;
         a.sig = sg6.p; a.packet = pk.p; a.g0 = g0.p; a.fail_key = fk.p;
+        a.geo = geo.p; a.geo_write = 1;
         launch_gp_kernel(a, 0, nelem, s);
         unsigned long long key = 0;
         PF_HIP(hipMemcpyAsync(&key, fk.p, sizeof key, hipMemcpyDeviceToHost, s));

@@ -6,7 +6,9 @@
 //                  centroid F0/J0 (F-bar), F, Fbar = (J0/J)^(1/3) F, trial be and its
 //                  eigen-decomposition, J2 radial return, P = tau Fbar^-T and the
 //                  9x9 two-point tangent A = dP/dFbar, written (scaled by the
-//                  quadrature weight) to a per-GP packet.  Restates
+//                  quadrature weight) to a per-GP packet -- its upper triangle where A
+//                  has major symmetry -- and, at the first assembly of a context, the
+//                  reference gradients to the geometry array.  Restates
 //                  element_force_tangent_finite! Elements.jl:405-494 with
 //                  finite_kinematics FiniteStrain.jl:148-170, finite_stress_update
 //                  :292-346, dPdF :503-547 and return_map Materials.jl:159-250.
@@ -199,6 +201,9 @@ __global__ __launch_bounds__(256) void gp_kernel(ElementArgs a, int64_t e_begin,
     double dNdX[24], detJ;
     ref_grads(Xe, x0, x1, x2, dNdX, detJ);
     const double w = detJ;  // GAUSS_WTS == 1
+    if (a.geo_write)
+#pragma unroll
+        for (int k = 0; k < 24; ++k) a.geo[24 * gp + k] = dNdX[k];
 
     // --- F-bar centroid quantities (Elements.jl:421-424, :517-519)
     double J0 = 1.0, F0i[9];
@@ -456,7 +461,9 @@ __global__ __launch_bounds__(256) void gp_kernel(ElementArgs a, int64_t e_begin,
         kc = (2 * G * s32 / (3 * G + Hs) * (Y / (nrm * nrm))) / nrm;
     }
 
-    double *pk = a.packet + (size_t)(8 * (li - e_begin) + g) * PK;
+    constexpr bool SYM = !KIN;  // A has major symmetry without the back-stress rotation term
+    constexpr int NA = pk_na(SYM);
+    double *pk = a.packet + (size_t)(8 * (li - e_begin) + g) * pk_len(SYM);
     const double sw = s * w;
 
     // --- A = dP/dFbar, one column per unit perturbation dF = e_p e_q^T
@@ -566,7 +573,11 @@ __global__ __launch_bounds__(256) void gp_kernel(ElementArgs a, int64_t e_begin,
             for (int c = 0; c < 3; ++c) {
                 double dP = dT[r] * Fit[0 + 3 * c] + dT[r + 3] * Fit[1 + 3 * c] + dT[r + 6] * Fit[2 + 3 * c] -
                             wq[r] * Fit[pp + 3 * c];
-                pk[(r + 3 * c) + 9 * col] = sw * dP;
+                if (SYM) {
+                    if (r + 3 * c <= col) pk[col * (col + 1) / 2 + r + 3 * c] = sw * dP;
+                } else {
+                    pk[(r + 3 * c) + 9 * col] = sw * dP;
+                }
                 qv[r + 3 * c] += dP * fcol;
             }
     }
@@ -575,14 +586,12 @@ __global__ __launch_bounds__(256) void gp_kernel(ElementArgs a, int64_t e_begin,
         inv3(F, Fi);
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
-            pk[81 + k] = (sw / 3) * qv[k];
-            pk[90 + k] = Fi[k];
+            pk[NA + k] = (sw / 3) * qv[k];
+            pk[NA + 9 + k] = Fi[k];
         }
     }
 #pragma unroll
-    for (int k = 0; k < 9; ++k) pk[99 + k] = w * P[k];
-#pragma unroll
-    for (int k = 0; k < 24; ++k) pk[108 + k] = dNdX[k];
+    for (int k = 0; k < 9; ++k) pk[NA + 18 + k] = w * P[k];
 
 }
 
@@ -618,6 +627,9 @@ __global__ __launch_bounds__(256) void gp_small_kernel(ElementArgs a, int64_t e_
     const double x0 = (((g + 1) & 2) ? gq : -gq), x1 = ((g & 2) ? gq : -gq), x2 = ((g & 4) ? gq : -gq);
     double dNdX[24], w;
     ref_grads(Xe, x0, x1, x2, dNdX, w);  // w = detJ (GAUSS_WTS == 1)
+    if (a.geo_write)
+#pragma unroll
+        for (int k = 0; k < 24; ++k) a.geo[24 * gp + k] = dNdX[k];
     double H[9];
     defgrad(ue, dNdX, H);
     H[0] -= 1.0; H[4] -= 1.0; H[8] -= 1.0;
@@ -697,23 +709,30 @@ __global__ __launch_bounds__(256) void gp_small_kernel(ElementArgs a, int64_t e_
             const double Idev = (v == u ? (v < 3 ? 1.0 : 0.5) : 0.0) - ((v < 3 && u < 3) ? 1.0 / 3.0 : 0.0);
             Dm[v][u] = w * (Cvu - cdev * Idev + cnn * nh[v] * nh[u]);
         }
-    double *pk = a.packet + (size_t)(8 * (li - e_begin) + g) * PK;
+    double *pk = a.packet + (size_t)(8 * (li - e_begin) + g) * PK_SYM;  // D symmetric: packed layout
     const int VO[9] = {0, 3, 5, 3, 1, 4, 5, 4, 2};  // Voigt index of (i, j), i + 3j
 #pragma unroll
     for (int c = 0; c < 9; ++c)
 #pragma unroll
-        for (int r = 0; r < 9; ++r) pk[r + 9 * c] = Dm[VO[r]][VO[c]];
+        for (int r = 0; r <= c; ++r) pk[c * (c + 1) / 2 + r] = Dm[VO[r]][VO[c]];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) pk[99 + k] = w * sg[VO[k]];
-#pragma unroll
-    for (int k = 0; k < 24; ++k) pk[108 + k] = dNdX[k];
+    for (int k = 0; k < 9; ++k) pk[pk_na(true) + 18 + k] = w * sg[VO[k]];
+}
+
+// Entry (p, q) = (i + 3j, k + 3l) of s w A in a Gauss-point packet of either layout.
+template <bool SYM>
+__device__ __forceinline__ int pk_a_index(int p, int q) {
+    return SYM ? pk_sym_index(p, q) : p + 9 * q;
 }
 
 // Contraction of the 8 GP packets of an element into 3x3 blocks Ke(a, b),
-// b in [4*half, 4*half+4), plus the residual part of node a (half == 0).
-__device__ __forceinline__ void contract_blocks(const double *__restrict__ pe,
-                                                const double *__restrict__ g0e, int a, int half,
-                                                int fbar, double acc[4][9], double fe[3]) {
+// b in [4*half, 4*half+4), plus the residual part of node a (half == 0).  ge: the
+// element's 8 x 24 reference gradients.
+template <bool SYM>
+__device__ __forceinline__ void contract_blocks(const double *__restrict__ pe, const double *__restrict__ ge,
+                                                const double *__restrict__ g0e, int a, int half, int fbar,
+                                                double acc[4][9], double fe[3]) {
+    constexpr int LEN = pk_len(SYM), NA = pk_na(SYM);
 #pragma unroll
     for (int bb = 0; bb < 4; ++bb)
 #pragma unroll
@@ -726,25 +745,27 @@ __device__ __forceinline__ void contract_blocks(const double *__restrict__ pe,
         for (int k = 0; k < 3; ++k) g0b[bb][k] = fbar ? g0e[3 * (4 * half + bb) + k] : 0.0;
 #pragma unroll 1
     for (int g = 0; g < 8; ++g) {
-        const double *pk = pe + g * PK;
-        const double dNa0 = pk[108 + 3 * a], dNa1 = pk[109 + 3 * a], dNa2 = pk[110 + 3 * a];
+        const double *pk = pe + g * LEN;
+        const double *dn = ge + 24 * g;
+        const double dNa0 = dn[3 * a], dNa1 = dn[3 * a + 1], dNa2 = dn[3 * a + 2];
         double Ua[3][9];  // Ua[i][(k,l)] = sum_j dNa_j SA[(i,j),(k,l)]
 #pragma unroll
         for (int c = 0; c < 9; ++c)
 #pragma unroll
             for (int i = 0; i < 3; ++i)
-                Ua[i][c] = dNa0 * pk[i + 9 * c] + dNa1 * pk[i + 3 + 9 * c] + dNa2 * pk[i + 6 + 9 * c];
+                Ua[i][c] = dNa0 * pk[pk_a_index<SYM>(i, c)] + dNa1 * pk[pk_a_index<SYM>(i + 3, c)] +
+                           dNa2 * pk[pk_a_index<SYM>(i + 6, c)];
         double h[3] = {0, 0, 0}, Fi[9];
         if (fbar) {
 #pragma unroll
-            for (int i = 0; i < 3; ++i) h[i] = dNa0 * pk[81 + i] + dNa1 * pk[84 + i] + dNa2 * pk[87 + i];
+            for (int i = 0; i < 3; ++i) h[i] = dNa0 * pk[NA + i] + dNa1 * pk[NA + 3 + i] + dNa2 * pk[NA + 6 + i];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) Fi[k] = pk[90 + k];
+            for (int k = 0; k < 9; ++k) Fi[k] = pk[NA + 9 + k];
         }
 #pragma unroll
         for (int bb = 0; bb < 4; ++bb) {
             const int bn = 4 * half + bb;
-            const double d0 = pk[108 + 3 * bn], d1 = pk[109 + 3 * bn], d2 = pk[110 + 3 * bn];
+            const double d0 = dn[3 * bn], d1 = dn[3 * bn + 1], d2 = dn[3 * bn + 2];
             double gd[3] = {0, 0, 0};
             if (fbar) {
 #pragma unroll
@@ -760,7 +781,7 @@ __device__ __forceinline__ void contract_blocks(const double *__restrict__ pe,
         if (half == 0) {
 #pragma unroll
             for (int i = 0; i < 3; ++i)
-                fe[i] += dNa0 * pk[99 + i] + dNa1 * pk[102 + i] + dNa2 * pk[105 + i];
+                fe[i] += dNa0 * pk[NA + 18 + i] + dNa1 * pk[NA + 21 + i] + dNa2 * pk[NA + 24 + i];
         }
     }
 }
@@ -775,12 +796,14 @@ __device__ __forceinline__ void contract_blocks(const double *__restrict__ pe,
 // for every node a, the 27 partial sums T_a[i][k + 3l] = sum_j dNa_j SA_(i+3j),(k+3l)
 // (216 values, at most four per lane, into LDS), then lane (a, b) finishes its block as
 // sum_l T_a[i][k + 3l] dNb_l: 12 + 27 multiply-adds per lane instead of 81 + 9 (96
-// instead of 143 VGPRs).  The packets are staged one Gauss point at a time in a double
-// buffer (the next point's 16-byte loads are in flight during the contraction), so a
-// workgroup needs 3.8 KB of LDS instead of 10 KB and occupancy is set by registers.
-// 6.6 ms per 100^3 assembly (8.9 ms with the one-stage contraction and all 8 packets
-// in LDS, 7.5 ms with the two stages and all 8 packets).
+// instead of 143 VGPRs).  Each Gauss point's packet and reference gradients are staged
+// in a double buffer (the next point's 16-byte loads are in flight during the
+// contraction), so a workgroup needs at most 4.1 KB of LDS and occupancy is set by
+// registers.  With the packed symmetric layout a Gauss point reads 96 doubles (72 of
+// packet, 24 of gradients) instead of 132.
+template <bool SYM>
 __global__ __launch_bounds__(64) void scatter_kernel(const double *__restrict__ packet,
+                                                      const double *__restrict__ geo,
                                                       const double *__restrict__ g0,
                                                       const int64_t *__restrict__ elist, int64_t count,
                                                       const int64_t *__restrict__ conn,
@@ -788,24 +811,30 @@ __global__ __launch_bounds__(64) void scatter_kernel(const double *__restrict__ 
                                                       const uint64_t *__restrict__ first,
                                                       double *__restrict__ Kv, double *__restrict__ R, int fbar,
                                                       int local) {
-    constexpr int NC = PK / 2;  // 16-byte chunks per Gauss-point packet (66)
-    __shared__ __attribute__((aligned(16))) double sp[2][PK];
+    constexpr int LEN = pk_len(SYM), NA = pk_na(SYM);
+    constexpr int NCP = LEN / 2, NC = NCP + 12;  // 16-byte chunks: packet, then the 24 gradients
+    constexpr int ST = LEN + 24;                 // doubles staged per Gauss point
+    __shared__ __attribute__((aligned(16))) double sp[2][ST];
     __shared__ __attribute__((aligned(16))) double T[8 * 27];
     const int64_t li = blockIdx.x;
     if (li >= count) return;
     const int64_t e = elist[li];
     const int lane = threadIdx.x, a = lane >> 3, b = lane & 7;
-    const double2 *src = reinterpret_cast<const double2 *>(packet + (size_t)(local ? li : e) * 8 * PK);
-    // lanes 0 and 1 also fetch the two chunks beyond 64; the other lanes re-read their own
-    // chunk into r1 and never store it
-    double2 r0 = src[lane], r1 = src[lane < NC - 64 ? 64 + lane : lane];
-    int s1_dn[4], s1_sa[4];
+    const double2 *src = reinterpret_cast<const double2 *>(packet + (size_t)(local ? li : e) * 8 * LEN);
+    const double2 *gsrc = reinterpret_cast<const double2 *>(geo + (size_t)e * 8 * 24);
+    // chunk t of Gauss point g: the packet's for t < NCP, the gradients' after
+    auto chunk = [&](int g, int t) { return t < NCP ? src[g * NCP + t] : gsrc[g * 12 + (t - NCP)]; };
+    const bool two = NC > 64 && lane < NC - 64;  // lanes that also take chunk 64 + lane
+    double2 r0 = lane < NC ? chunk(0, lane) : double2{0.0, 0.0};
+    double2 r1 = two ? chunk(0, 64 + lane) : double2{0.0, 0.0};
+    int s1_dn[4], s1_sa[4][3];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const int v = lane + 64 * q;
+        const int v = lane + 64 * q < 216 ? lane + 64 * q : 0;
         const int ap = v / 27, rem = v - 27 * ap, i = rem / 9, kl = rem - 9 * i;
-        s1_dn[q] = 108 + 3 * ap;
-        s1_sa[q] = i + 9 * kl;
+        s1_dn[q] = LEN + 3 * ap;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) s1_sa[q][j] = pk_a_index<SYM>(i + 3 * j, kl);
     }
     double acc[9];
 #pragma unroll
@@ -815,29 +844,29 @@ __global__ __launch_bounds__(64) void scatter_kernel(const double *__restrict__ 
     if (fbar)
 #pragma unroll
         for (int k = 0; k < 3; ++k) g0b[k] = g0[24 * e + 3 * b + k];
-    reinterpret_cast<double2 *>(sp[0])[lane] = r0;
-    if (lane < NC - 64) reinterpret_cast<double2 *>(sp[0])[64 + lane] = r1;
+    if (lane < NC) reinterpret_cast<double2 *>(sp[0])[lane] = r0;
+    if (two) reinterpret_cast<double2 *>(sp[0])[64 + lane] = r1;
     __syncthreads();
 #pragma unroll 1
     for (int g = 0; g < 8; ++g) {
         if (g < 7) {  // the next Gauss point's packet, written to the other buffer at the end
-            r0 = src[(g + 1) * NC + lane];
-            r1 = src[(g + 1) * NC + (lane < NC - 64 ? 64 + lane : lane)];
+            if (lane < NC) r0 = chunk(g + 1, lane);
+            if (two) r1 = chunk(g + 1, 64 + lane);
         }
         const double *pk = sp[g & 1];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             if (lane + 64 * q < 216) {
-                const double *dn = pk + s1_dn[q], *sa = pk + s1_sa[q];
-                T[lane + 64 * q] = dn[0] * sa[0] + dn[1] * sa[3] + dn[2] * sa[6];
+                const double *dn = pk + s1_dn[q];
+                T[lane + 64 * q] = dn[0] * pk[s1_sa[q][0]] + dn[1] * pk[s1_sa[q][1]] + dn[2] * pk[s1_sa[q][2]];
             }
         }
         __syncthreads();
         double dNa[3], dNb[3];
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            dNa[j] = pk[108 + 3 * a + j];
-            dNb[j] = pk[108 + 3 * b + j];
+            dNa[j] = pk[LEN + 3 * a + j];
+            dNb[j] = pk[LEN + 3 * b + j];
         }
         const double *ta = T + 27 * a;
 #pragma unroll
@@ -848,10 +877,12 @@ __global__ __launch_bounds__(64) void scatter_kernel(const double *__restrict__ 
         if (fbar) {
             double h[3], gd[3];
 #pragma unroll
-            for (int i = 0; i < 3; ++i) h[i] = dNa[0] * pk[81 + i] + dNa[1] * pk[84 + i] + dNa[2] * pk[87 + i];
+            for (int i = 0; i < 3; ++i)
+                h[i] = dNa[0] * pk[NA + i] + dNa[1] * pk[NA + 3 + i] + dNa[2] * pk[NA + 6 + i];
 #pragma unroll
             for (int k = 0; k < 3; ++k)
-                gd[k] = g0b[k] - (dNb[0] * pk[90 + 3 * k] + dNb[1] * pk[91 + 3 * k] + dNb[2] * pk[92 + 3 * k]);
+                gd[k] = g0b[k] - (dNb[0] * pk[NA + 9 + 3 * k] + dNb[1] * pk[NA + 10 + 3 * k] +
+                                  dNb[2] * pk[NA + 11 + 3 * k]);
 #pragma unroll
             for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -859,10 +890,11 @@ __global__ __launch_bounds__(64) void scatter_kernel(const double *__restrict__ 
         }
         if (b == 0)
 #pragma unroll
-            for (int i = 0; i < 3; ++i) fe[i] += dNa[0] * pk[99 + i] + dNa[1] * pk[102 + i] + dNa[2] * pk[105 + i];
+            for (int i = 0; i < 3; ++i)
+                fe[i] += dNa[0] * pk[NA + 18 + i] + dNa[1] * pk[NA + 21 + i] + dNa[2] * pk[NA + 24 + i];
         if (g < 7) {
-            reinterpret_cast<double2 *>(sp[(g + 1) & 1])[lane] = r0;
-            if (lane < NC - 64) reinterpret_cast<double2 *>(sp[(g + 1) & 1])[64 + lane] = r1;
+            if (lane < NC) reinterpret_cast<double2 *>(sp[(g + 1) & 1])[lane] = r0;
+            if (two) reinterpret_cast<double2 *>(sp[(g + 1) & 1])[64 + lane] = r1;
         }
         __syncthreads();
     }
@@ -883,7 +915,9 @@ __global__ __launch_bounds__(64) void scatter_kernel(const double *__restrict__ 
     }
 }
 
+template <bool SYM>
 __global__ __launch_bounds__(256) void dense_ke_kernel(const double *__restrict__ packet,
+                                                       const double *__restrict__ geo,
                                                        const double *__restrict__ g0, int64_t nelem,
                                                        double *__restrict__ Fe, double *__restrict__ Ke,
                                                        int fbar) {
@@ -892,7 +926,8 @@ __global__ __launch_bounds__(256) void dense_ke_kernel(const double *__restrict_
     if (e >= nelem) return;
     const int sub = (int)(t & 15), a = sub >> 1, half = sub & 1;
     double acc[4][9], fe[3];
-    contract_blocks(packet + (size_t)e * 8 * PK, g0 + 24 * e, a, half, fbar, acc, fe);
+    contract_blocks<SYM>(packet + (size_t)e * 8 * pk_len(SYM), geo + (size_t)e * 8 * 24, g0 + 24 * e, a, half, fbar,
+                         acc, fe);
 #pragma unroll
     for (int bb = 0; bb < 4; ++bb) {
         const int bn = 4 * half + bb;
@@ -989,8 +1024,12 @@ void launch_scatter_kernel(const ElementArgs &a, const int64_t *elem_list, int64
                            const int32_t *slots, const uint64_t *first, double *Kv, double *R, hipStream_t s,
                            bool local) {
     if (count <= 0) return;
-    hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)count), dim3(64), 0, s, a.packet, a.g0, elem_list, count,
-                       a.conn, slots, first, Kv, R, a.fbar, (int)local);
+    if (packet_sym(a))
+        hipLaunchKernelGGL(scatter_kernel<true>, dim3((unsigned)count), dim3(64), 0, s, a.packet, a.geo, a.g0,
+                           elem_list, count, a.conn, slots, first, Kv, R, a.fbar, (int)local);
+    else
+        hipLaunchKernelGGL(scatter_kernel<false>, dim3((unsigned)count), dim3(64), 0, s, a.packet, a.geo, a.g0,
+                           elem_list, count, a.conn, slots, first, Kv, R, a.fbar, (int)local);
     PF_HIP(hipGetLastError());
 }
 
@@ -1035,8 +1074,12 @@ void build_first_touch(int64_t nslots, int ncolors, const int64_t *order, const 
 void launch_dense_ke_kernel(const ElementArgs &a, int64_t nelem, double *Fe, double *Ke, hipStream_t s) {
     if (nelem <= 0) return;
     const int64_t blocks = (16 * nelem + 255) / 256;
-    hipLaunchKernelGGL(dense_ke_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a.packet, a.g0, nelem, Fe,
-                       Ke, a.fbar);
+    if (packet_sym(a))
+        hipLaunchKernelGGL(dense_ke_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, s, a.packet, a.geo, a.g0,
+                           nelem, Fe, Ke, a.fbar);
+    else
+        hipLaunchKernelGGL(dense_ke_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, s, a.packet, a.geo, a.g0,
+                           nelem, Fe, Ke, a.fbar);
     PF_HIP(hipGetLastError());
 }
 

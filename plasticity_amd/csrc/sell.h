@@ -31,20 +31,16 @@ __device__ __forceinline__ int64_t xcd_contiguous(int64_t b, int64_t G) {
     return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
 }
 
-template <class VT, class XT, int MODE>
-__global__ __launch_bounds__(64) void sell_lds_kernel(int64_t nb, const int64_t *__restrict__ slice_ptr,
-                                                      const int32_t *__restrict__ colind,
-                                                      const VT *__restrict__ vals, const XT *__restrict__ x,
-                                                      const XT *__restrict__ b, XT *__restrict__ y,
-                                                      double *__restrict__ partial) {
+// Row block i = s * SELL_C + lane of slice s: y = sum over the slice columns of
+// block(k) x(col(k)), the blocks staged in `tile` (SELL_C * 9 values of LDS).
+template <class VT, class XT>
+__device__ __forceinline__ void sell_slice_product(int64_t s, int lane, const int64_t *__restrict__ slice_ptr,
+                                                   const int32_t *__restrict__ colind, const VT *__restrict__ vals,
+                                                   const XT *__restrict__ x, VT *tile, VT &y0, VT &y1, VT &y2) {
     constexpr int NV = SELL_C * 9 * (int)sizeof(VT) / 16;  // 16-byte chunks per slice column
-    __shared__ __attribute__((aligned(16))) VT tile[SELL_C * 9];
-    const int lane = threadIdx.x;
-    const int64_t s = xcd_contiguous(blockIdx.x, gridDim.x);
-    const int64_t i = s * SELL_C + lane;
     const int64_t base = slice_ptr[s];
     const int w = (int)((slice_ptr[s + 1] - base) / SELL_C);
-    VT y0 = 0, y1 = 0, y2 = 0;
+    y0 = 0, y1 = 0, y2 = 0;
     for (int k = 0; k < w; ++k) {
         const int64_t col0 = base + (int64_t)k * SELL_C;
 #pragma unroll
@@ -58,6 +54,59 @@ __global__ __launch_bounds__(64) void sell_lds_kernel(int64_t nb, const int64_t 
         y2 += v[6] * (VT)x0 + v[7] * (VT)x1 + v[8] * (VT)x2;
         __syncthreads();
     }
+}
+
+// One Chebyshev step of the fine-level smoother with its product in the same pass:
+// t = K d for the lane's block row, then r -= t, d' = c1 d + c2 Dinv r, x += d' -- the
+// arithmetic of the separate product + step launches, in the same order (bitwise-equal
+// results), without writing and re-reading t.  d is the product's input vector, read by
+// every workgroup, so d' goes to d_out; d_out == null (the smoother's last step, after
+// which neither r nor d is read again) skips the r and d' stores.
+template <class VT, class XT>
+__global__ __launch_bounds__(64) void sell_cheb_kernel(int64_t nb, const int64_t *__restrict__ slice_ptr,
+                                                       const int32_t *__restrict__ colind,
+                                                       const VT *__restrict__ vals, const XT *__restrict__ d,
+                                                       const VT *__restrict__ dinv, XT *__restrict__ r,
+                                                       XT *__restrict__ d_out, XT *__restrict__ x, double c1,
+                                                       double c2) {
+    __shared__ __attribute__((aligned(16))) VT tile[SELL_C * 9];
+    const int lane = threadIdx.x;
+    const int64_t s = xcd_contiguous(blockIdx.x, gridDim.x);
+    const int64_t i = s * SELL_C + lane;
+    VT y[3];
+    sell_slice_product<VT, XT>(s, lane, slice_ptr, colind, vals, d, tile, y[0], y[1], y[2]);
+    if (i >= nb) return;
+    VT rv[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        rv[c] = (VT)r[3 * i + c] - (VT)(XT)y[c];
+        if (d_out) r[3 * i + c] = (XT)rv[c];
+    }
+    const VT *m = dinv + 9 * i;
+#pragma unroll
+    for (int rr = 0; rr < 3; ++rr) {
+        VT sm = 0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) sm = fma(m[3 * rr + c], rv[c], sm);
+        const int64_t o = 3 * i + rr;
+        const double dn = fma(c1, (double)d[o], c2 * (double)sm);
+        if (d_out) d_out[o] = (XT)dn;
+        x[o] = (XT)((double)x[o] + dn);
+    }
+}
+
+template <class VT, class XT, int MODE>
+__global__ __launch_bounds__(64) void sell_lds_kernel(int64_t nb, const int64_t *__restrict__ slice_ptr,
+                                                      const int32_t *__restrict__ colind,
+                                                      const VT *__restrict__ vals, const XT *__restrict__ x,
+                                                      const XT *__restrict__ b, XT *__restrict__ y,
+                                                      double *__restrict__ partial) {
+    __shared__ __attribute__((aligned(16))) VT tile[SELL_C * 9];
+    const int lane = threadIdx.x;
+    const int64_t s = xcd_contiguous(blockIdx.x, gridDim.x);
+    const int64_t i = s * SELL_C + lane;
+    VT y0, y1, y2;
+    sell_slice_product<VT, XT>(s, lane, slice_ptr, colind, vals, x, tile, y0, y1, y2);
     double d = 0.0;
     if (i < nb) {
         if (MODE == SELL_RES) {

@@ -217,6 +217,47 @@ def evaluate(net_weights, seeds, n, load_steps=(1, 2, 3), precision="fp64", log=
     return out
 
 
+def layer_flops(grid, channels=CHANNELS):
+    """MFMA operations (2 per multiply-add, padded channel counts) of each layer of one
+    forward pass on a node grid: (name, taps, cin, cout, output voxels, flops).  Level
+    l + 1 has ceil(n / 2) nodes per direction; the layer table is unet.cpp's."""
+    c0, c1, c2, c3 = channels
+    lv = [tuple(int(g) for g in grid)]
+    for _ in range(3):
+        lv.append(tuple((n + 1) // 2 for n in lv[-1]))
+    vox = [int(np.prod(g)) for g in lv]
+    table = [("in", 27, 16, c0, 0), ("e0", 27, c0, c0, 0), ("d1", 8, c0, c1, 1), ("e1", 27, c1, c1, 1),
+             ("d2", 8, c1, c2, 2), ("e2", 27, c2, c2, 2), ("d3", 8, c2, c3, 3), ("e3", 27, c3, c3, 3),
+             ("u2", 27, c3, c2, 2), ("u1", 27, c2, c1, 1), ("u0", 27, c1, c0, 0), ("out", 1, c0, 16, 0)]
+    return [(n, t, ci, co, vox[l], 2 * vox[l] * t * ci * co) for n, t, ci, co, l in table]
+
+
+def bench_net(n=100, reps=20, train_steps=10, log=print):
+    """Times the forward pass (host copies of r and N(r) included) and Adam steps on a
+    grid of (n + 1)^3 nodes with random weights and random logged pairs; the kernels'
+    own times come from a kernel trace of this command."""
+    grid = (n + 1,) * 3
+    net = UNet(grid, CHANNELS, capacity=8, seed=0)
+    rng = np.random.default_rng(0)
+    x = rng.normal(size=3 * net.nvox).astype(np.float32)
+    for _ in range(4):
+        net.add_pair(rng.normal(size=3 * net.nvox), rng.normal(size=3 * net.nvox))
+    net.forward(x)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        net.forward(x)
+    t_fwd = (time.perf_counter() - t0) / reps
+    net.train(1, batch=2, lr=1e-4, seed=0)
+    t0 = time.perf_counter()
+    net.train(train_steps, batch=2, lr=1e-4, seed=1)
+    t_step = (time.perf_counter() - t0) / train_steps
+    fl = sum(r[-1] for r in layer_flops(grid))
+    log(f"[net] {n + 1}^3 nodes: forward {t_fwd * 1e3:.3f} ms ({fl / 1e9:.1f} GFLOP, {fl / t_fwd / 1e12:.1f} TFLOP/s "
+        f"with host copies); Adam step (batch 2) {t_step * 1e3:.1f} ms")
+    net.close()
+    return {"forward_ms": t_fwd * 1e3, "forward_gflop": fl / 1e9, "adam_step_ms": t_step * 1e3}
+
+
 def _seeds(spec):
     a, b = spec.split("-") if "-" in spec else (spec, spec)
     return list(range(int(a), int(b) + 1))
@@ -226,7 +267,7 @@ def main(argv=None):
     import argparse
     import json
     ap = argparse.ArgumentParser(description="train / evaluate the learned preconditioner")
-    ap.add_argument("cmd", choices=["train", "eval"])
+    ap.add_argument("cmd", choices=["train", "eval", "bench"])
     ap.add_argument("--n", type=int, default=63, help="elements per side of the sample blocks")
     ap.add_argument("--seeds", default="2-20")
     ap.add_argument("--load-steps", default="1-5")
@@ -254,6 +295,8 @@ def main(argv=None):
             log(f"[learned] adam step {done}/{a.steps}: mean loss {loss:.4f} ({time.perf_counter() - t0:.1f}s)")
         net.save(a.out or a.weights)
         log(f"[learned] saved {a.out or a.weights}")
+    elif a.cmd == "bench":
+        print(json.dumps(bench_net(a.n, log=log)))
     else:
         with np.load(a.weights, allow_pickle=False) as f:
             w = f["w"].copy()

@@ -1,0 +1,48 @@
+"""Fine-level Chebyshev steps fused with their product (sell_cheb_kernel) against the
+separate product + step launches (option smoother_unfused = 1): the same arithmetic in
+the same order, so the preconditioned solves agree bit for bit -- for both precisions,
+the first- and fourth-kind Chebyshev smoothers, and degree 2 (one step per smoothing:
+the fused launch skips the dead r and d stores) as well as degrees 3 and 4 (the new
+direction goes to the spare buffer and the two are swapped)."""
+import numpy as np
+import pytest
+
+import plasticity_amd as pa
+from plasticity_amd.workloads import uniaxial_constraints
+
+pytestmark = pytest.mark.gpu
+MAT = dict(E=200e9, nu=0.3, sy0=250e6, Hiso=1e9)
+
+
+def _solve(precision, smoother, degree, unfused):
+    mesh = pa.box_mesh(1, 1, 1, 10, 10, 10, jitter=0.1, seed=2)
+    opts = pa.default_options(precision=precision, smoother=smoother, smoother_degree=degree,
+                              smoother_unfused=unfused, coarse_max=60)
+    m = pa.Model(mesh, pa.J2Material(**MAT), options=opts)
+    d, v, r = uniaxial_constraints(mesh, -0.02)
+    m.constrain(d, v, r)
+    m._freeze()
+    U = (np.array([-0.004, 0.0015, 0.0015]) * mesh.nodes).ravel()
+    m.assemble(U, commit=True)
+    m.U = 1.5 * U
+    m.assemble(1.5 * U)
+    m.impose_dirichlet(1.0)
+    dU, it, used, relres = m.linear_solve(1e-10)
+    m.reset()  # a load step of the compression from the undeformed state
+    conv, hist, kry, _ = m.newton(1.0, tol=1e-8, maxiter=25)
+    out = (dU.copy(), it, used, relres, conv, hist, kry, m.U.copy())
+    m.close()
+    return out
+
+
+@pytest.mark.parametrize("precision,smoother,degree", [
+    ("fp64", "chebyshev", 2), ("fp32_ir", "chebyshev", 2), ("fp64", "chebyshev", 3),
+    ("fp32_ir", "chebyshev", 4), ("fp64", "chebyshev4", 3), ("fp32_ir", "chebyshev4", 2)])
+def test_fused_chebyshev_step_is_bitwise_equal(precision, smoother, degree, need_gpu):
+    print(f"[smoother] {precision} {smoother} degree {degree}: fused vs separate launches", flush=True)
+    a = _solve(precision, smoother, degree, 0)
+    b = _solve(precision, smoother, degree, 1)
+    print(f"  krylov {a[1]} / {b[1]}, relres {a[3]:.2e}, newton {len(a[5])} its", flush=True)
+    assert a[3] < 1e-9 and a[4]
+    assert np.array_equal(a[0], b[0]) and a[1:4] == b[1:4]
+    assert a[5] == b[5] and a[6] == b[6] and np.array_equal(a[7], b[7])
