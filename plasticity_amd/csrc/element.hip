@@ -162,6 +162,12 @@ __device__ __forceinline__ double yield_slope(const DevMat &m, double a) {
 // Divided difference of (1/2) ln at (bA, bB) with the reference's degenerate
 // switch (FiniteStrain.jl:467-476).  Off the switch, 0.5*log1p(d/bB)/d is the
 // same quantity as (0.5 ln bA - 0.5 ln bB)/(bA - bB) without cancellation.
+// Element k of three, by selects: the tangent's column loop indexes small arrays with
+// its runtime column (p, q); selects keep those arrays in registers instead of scratch.
+__device__ __forceinline__ double pick3(int k, double a0, double a1, double a2) {
+    return k == 0 ? a0 : (k == 1 ? a1 : a2);
+}
+
 __device__ __forceinline__ double halflog_dd(double bA, double bB) {
     if (fabs(bA - bB) < EIG_TOL * (fabs(bA) + fabs(bB) + 1.0)) return 1.0 / (bA + bB);
     double d = bA - bB;
@@ -478,12 +484,21 @@ __global__ __launch_bounds__(256) void gp_kernel(ElementArgs a, int64_t e_begin,
 #pragma unroll 1
     for (int col = 0; col < 9; ++col) {
         const int pp = col % 3, qq = col / 3;
+        // column qq of M and of Fb^-T, row pp of Fb^-T, F[col]
+        double mq[3], fq[3], fp[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            mq[i] = pick3(qq, M[i], M[i + 3], M[i + 6]);
+            fq[i] = pick3(qq, Fit[i], Fit[i + 3], Fit[i + 6]);
+            fp[i] = pick3(pp, Fit[3 * i], Fit[3 * i + 1], Fit[3 * i + 2]);
+        }
+        const double fcol = pick3(qq, pick3(pp, F[0], F[1], F[2]), pick3(pp, F[3], F[4], F[5]),
+                                  pick3(pp, F[6], F[7], F[8]));
         double u[3], v[3];
 #pragma unroll
         for (int A_ = 0; A_ < 3; ++A_) {
-            u[A_] = Q[pp + 3 * A_];
-            v[A_] = Q[0 + 3 * A_] * M[0 + 3 * qq] + Q[1 + 3 * A_] * M[1 + 3 * qq] +
-                    Q[2 + 3 * A_] * M[2 + 3 * qq];
+            u[A_] = pick3(pp, Q[3 * A_], Q[1 + 3 * A_], Q[2 + 3 * A_]);
+            v[A_] = Q[0 + 3 * A_] * mq[0] + Q[1 + 3 * A_] * mq[1] + Q[2 + 3 * A_] * mq[2];
         }
         double Y[3][3];  // gamma o (Q^T dbe Q)
 #pragma unroll
@@ -521,8 +536,8 @@ __global__ __launch_bounds__(256) void gp_kernel(ElementArgs a, int64_t e_begin,
             double x[3], y[3];
 #pragma unroll
             for (int A_ = 0; A_ < 3; ++A_) {
-                x[A_] = RV[pp + 3 * A_];
-                y[A_] = Vp[qq + 3 * A_];
+                x[A_] = pick3(pp, RV[3 * A_], RV[1 + 3 * A_], RV[2 + 3 * A_]);
+                y[A_] = pick3(qq, Vp[3 * A_], Vp[1 + 3 * A_], Vp[2 + 3 * A_]);
             }
             double Om[3][3];
 #pragma unroll
@@ -564,15 +579,13 @@ __global__ __launch_bounds__(256) void gp_kernel(ElementArgs a, int64_t e_begin,
         // dP = dtau Fb^-T - (tau Fb^-T[:,q]) (Fb^-T[p,:])   (FiniteStrain.jl:539-540)
         double wq[3];
 #pragma unroll
-        for (int r = 0; r < 3; ++r)
-            wq[r] = T[r] * Fit[0 + 3 * qq] + T[r + 3] * Fit[1 + 3 * qq] + T[r + 6] * Fit[2 + 3 * qq];
-        const double fcol = F[col];
+        for (int r = 0; r < 3; ++r) wq[r] = T[r] * fq[0] + T[r + 3] * fq[1] + T[r + 6] * fq[2];
 #pragma unroll
         for (int r = 0; r < 3; ++r)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 double dP = dT[r] * Fit[0 + 3 * c] + dT[r + 3] * Fit[1 + 3 * c] + dT[r + 6] * Fit[2 + 3 * c] -
-                            wq[r] * Fit[pp + 3 * c];
+                            wq[r] * fp[c];
                 if (SYM) {
                     if (r + 3 * c <= col) pk[col * (col + 1) / 2 + r + 3 * c] = sw * dP;
                 } else {
