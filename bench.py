@@ -148,6 +148,21 @@ def copy_calibration(nbytes=1 << 30, reps=10):
     log(f"[bench] copy calibration: {reps} x {nbytes} B device-to-device")
 
 
+def vcycle_figures(m, prec, reps=20):
+    """Live HIP-event timing of the AMG cycle and of one fused fine-level Chebyshev step
+    (SELL product with the smoother update as its epilogue) on the model's current
+    hierarchy, against their algorithmic bytes (every launch's operands read once and
+    outputs written once)."""
+    r = m.bench_vcycle(prec=prec, reps=reps)
+    cyc_gbs = r["cycle_bytes"] / r["cycle_s"] / 1e9
+    st_gbs = r["step_bytes"] / r["step_s"] / 1e9
+    return {"precision": "fp32" if prec else "fp64",
+            "cycle_ms": r["cycle_s"] * 1e3, "cycle_bytes": r["cycle_bytes"], "cycle_GBps": cyc_gbs,
+            "cycle_frac": cyc_gbs / HBM_PEAK_GBS,
+            "smoother_step_ms": r["step_s"] * 1e3, "smoother_step_bytes": r["step_bytes"],
+            "smoother_step_GBps": st_gbs, "smoother_step_frac": st_gbs / HBM_PEAK_GBS}
+
+
 class PathTotals:
     """Sums over the load increments of timed load steps (advance records)."""
 
@@ -196,12 +211,19 @@ def time_variant(pa, wl, precision, learned, warmup, steps, args):
             tot.add(recs)
     elapsed = time.perf_counter() - t0
     U, abar = m.U, m.get_state()["abar"]
+    vc = None
+    if not learned:
+        try:
+            vc = vcycle_figures(m, 1 if precision == "fp32_ir" else 0)
+        except Exception as e:
+            log(f"[bench] V-cycle timing failed: {e!r}")
     if learned:
         m.attach_net(None, "off")
     m.close()
     return {"value": elapsed / tot.newton, "newton_steps": tot.newton, "krylov_its": tot.krylov,
             "increments": tot.increments, "cutbacks": tot.cutbacks, "seconds": elapsed,
-            "t_assemble_s": tot.t_asm, "t_amg_setup_s": tot.t_setup, "t_solve_s": tot.t_solve}, U, abar
+            "t_assemble_s": tot.t_asm, "t_amg_setup_s": tot.t_setup, "t_solve_s": tot.t_solve,
+            **({"vcycle": vc} if vc else {})}, U, abar
 
 
 def main():
@@ -384,6 +406,14 @@ def main():
         b32 = spmv_bytes(nnzb, nb, ndof, 4)
         spmv32 = {"avg_launch_ms": t32 * 1e3, "GBps": b32 / t32 / 1e9, "bytes_per_launch": b32, "launches": n32}
         log(f"[bench] FP32 Krylov SpMV {t32 * 1e3:.3f} ms = {b32 / t32 / 1e9:.0f} GB/s ({n32} launches)")
+    vcyc = None
+    if net is None:
+        try:
+            vcyc = vcycle_figures(m, 1 if args.precision == "fp32_ir" else 0)
+            log(f"[bench] AMG cycle ({vcyc['precision']}): {vcyc['cycle_ms']:.3f} ms = {vcyc['cycle_GBps']:.0f} GB/s; "
+                f"fine smoother step {vcyc['smoother_step_ms']:.3f} ms = {vcyc['smoother_step_GBps']:.0f} GB/s")
+        except Exception as e:  # the measurement never blocks the line
+            log(f"[bench] V-cycle timing failed: {e!r}")
     compare = None
     if args.compare:
         U_run, a_run = m.U, m.get_state()["abar"]
@@ -475,6 +505,7 @@ def main():
                      "traffic": SPMV_TRAFFIC_100 if (args.workload == 1 and ndof == 3090903) else None,
                      "bytes_per_launch": bytes_sp, "avg_launch_ms": t_sp * 1e3},
         "cpu_baseline": cpu,
+        "vcycle_roofline": vcyc,
         "solver_variants": variants,
         "measured_peaks": peaks,
         "element_roofline": {"kernel": "gp_kernel + scatter_kernel (Hex8 F-bar integration and tangent "

@@ -367,6 +367,16 @@ int pfem_device_peaks(int64_t copy_bytes, int reps, double *copy_gbs, double *re
  * context's stream with events; returns mean seconds per SpMV. */
 int pfem_bench_spmv(pfem_ctx *ctx, int prec, int reps, double *sec_per_spmv);
 
+/* Time `reps` applications of the current AMG cycle (prec 0: FP64, 1: FP32, which needs
+ * PFEM_PREC_FP32_IR) and `reps` fused fine-level Chebyshev smoother steps (product
+ * t = K d with the update r -= t, d' = c1 d + c2 D^-1 r, x += d') on the context's
+ * stream with events, after one untimed application.  Returns seconds per cycle and per
+ * step and their algorithmic bytes (every launch's operands read once, outputs written
+ * once).  Needs a built preconditioner (a pfem_linear_solve or pfem_newton before);
+ * overwrites the cycle's work vectors.  No counterpart in the reference (measurement). */
+int pfem_bench_vcycle(pfem_ctx *ctx, int prec, int reps, double *sec_per_cycle, double *bytes_per_cycle,
+                      double *sec_per_smoother_step, double *bytes_per_smoother_step);
+
 /* ---------------------------------------------------------------------------
  * Learned preconditioner.  No counterpart in the reference: the network and
  * its training are additions beside the drop-in path.  A bias-free 3D

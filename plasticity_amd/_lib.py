@@ -141,6 +141,7 @@ SIGNATURES = {
                                              _vp, C.c_int, _vp, _vp, _vp, C.POINTER(Fail)]),
     "pfem_get_stats": (C.c_int, [_vp, C.POINTER(Stats)]),
     "pfem_bench_spmv": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
+    "pfem_bench_vcycle": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp, _dp, _dp]),
     "pfem_dense_inverse": (C.c_int, [C.c_int64, _vp, _vp]),
     "pfem_device_peaks": (C.c_int, [C.c_int64, C.c_int, _dp, _dp, _dp]),
     "pfem_status_string": (C.c_char_p, [C.c_int]),

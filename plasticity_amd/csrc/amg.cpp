@@ -690,4 +690,82 @@ void apply_vcycle(pfem_ctx *c, Amg *a, int k, const T *r, T *z) {
 void amg_apply64(pfem_ctx *c, Amg *a, const double *r, double *z) { apply_vcycle<double>(c, a, 0, r, z); }
 void amg_apply32(pfem_ctx *c, Amg *a, const float *r, float *z) { apply_vcycle<float>(c, a, 1, r, z); }
 
+// ---------------------------------------------------------------- measurement
+// Algorithmic bytes of the cycle's launches, each counted with its operands read once
+// and its outputs written once (s = bytes per value; integer indices 4 bytes).  The
+// FP16 smoother copy of the fine K (fp16_fine) is not counted.
+namespace {
+double op_bytes(pfem_ctx *c, const AmgLevel &L, int l, double s) {
+    return l == 0 ? (double)c->K.nnzb * (9.0 * s + 4.0) + 4.0 * (L.nb + 1)
+                  : (double)L.nnzb * (36.0 * s + 4.0) + 4.0 * (L.nb + 1);
+}
+double smooth_bytes(pfem_ctx *c, Amg *a, int l, double s, bool zero_init) {
+    const AmgLevel &L = a->L[l];
+    const double A = op_bytes(c, L, l, s), v = (double)L.n * s, D = (double)L.nb * L.bs * L.bs * s;
+    const bool fused = l == 0 && !c->opt.smoother_unfused && !(s == 4.0 && c->half_fine);
+    if (a->smoother == PFEM_SMOOTH_JACOBI) {
+        double b = 0;
+        for (int it = 0; it < a->degree; ++it)
+            b += (zero_init && it == 0) ? 2 * v + D : (A + 3 * v) + (4 * v + D);
+        return b;
+    }
+    double b = zero_init ? 4 * v + D : (A + 3 * v) + (4 * v + D);  // first step
+    for (int k = 2; k <= a->degree; ++k)
+        b += fused ? A + (k == a->degree ? 4 : 6) * v + D : (A + 2 * v) + (7 * v + D);
+    return b;
+}
+double cycle_bytes(pfem_ctx *c, Amg *a, int l, double s) {
+    const AmgLevel &L = a->L[l];
+    if ((size_t)l + 1 == a->L.size()) return (double)a->nc * a->nc * 8.0 + 2.0 * a->nc * s;
+    const AmgLevel &N = a->L[l + 1];
+    const double v = (double)L.n * s, vN = (double)N.n * s;
+    double b = smooth_bytes(c, a, l, s, true) + op_bytes(c, L, l, s) + 3 * v;        // pre-smooth, residual
+    b += (double)L.nR * (6.0 * L.bs * s + 4.0) + 4.0 * (L.nagg + 1) + v + vN;          // restriction
+    b += cycle_bytes(c, a, l + 1, s);
+    if (a->wcycle && (size_t)l + 2 < a->L.size())
+        b += op_bytes(c, N, l + 1, s) + 3 * vN + cycle_bytes(c, a, l + 1, s) + 3 * vN;
+    b += (double)L.nP * (6.0 * L.bs * s + 4.0) + 4.0 * (L.nb + 1) + vN + 2 * v;        // prolongation
+    return b + smooth_bytes(c, a, l, s, false);                                         // post-smooth
+}
+}  // namespace
+
+void amg_bench(pfem_ctx *c, Amg *a, int prec, int reps, double *sec_cycle, double *bytes_cycle, double *sec_step,
+               double *bytes_step) {
+    hipStream_t s = c->stream;
+    AmgLevel &L = a->L[0];
+    const int64_t n = L.n;
+    const double vs = prec == 1 ? 4.0 : 8.0;
+    float ms_cycle = 0, ms_step = 0;
+    // the cycle maps the level-0 W-cycle vectors (used on coarser levels only) to each other
+    auto run = [&](auto *in, auto *out, auto *d, auto *r, auto *x, auto *t, const auto *Kv, const auto *dinv) {
+        using T = std::remove_const_t<std::remove_pointer_t<decltype(in)>>;
+        fill<T>(n, T(1), in, s);
+        fill<T>(n, T(1), d, s);
+        fill<T>(n, T(1), r, s);
+        fill<T>(n, T(0), x, s);
+        apply_vcycle<T>(c, a, prec == 1 ? 1 : 0, in, out);  // capture (graph) and warm caches
+        PF_HIP(hipEventRecord(c->ev0, s));
+        for (int i = 0; i < reps; ++i) apply_vcycle<T>(c, a, prec == 1 ? 1 : 0, in, out);
+        PF_HIP(hipEventRecord(c->ev1, s));
+        c->sync();
+        PF_HIP(hipEventElapsedTime(&ms_cycle, c->ev0, c->ev1));
+        // one fused fine-level Chebyshev step (product + update, r and d stored)
+        sell_cheb<T, T>(c->K, Kv, d, dinv, r, t, x, 0.5, 1e-3, s);
+        PF_HIP(hipEventRecord(c->ev0, s));
+        for (int i = 0; i < reps; ++i) sell_cheb<T, T>(c->K, Kv, d, dinv, r, t, x, 0.5, 1e-3, s);
+        PF_HIP(hipEventRecord(c->ev1, s));
+        c->sync();
+        PF_HIP(hipEventElapsedTime(&ms_step, c->ev0, c->ev1));
+    };
+    if (prec == 1)
+        run(L.wr32.p, L.we32.p, L.d32.p, L.r32.p, L.x32.p, L.t32.p, (const float *)c->K.v32.p,
+            (const float *)L.dinv32.p);
+    else
+        run(L.wr.p, L.we.p, L.d.p, L.r.p, L.x.p, L.t.p, (const double *)c->K.v64.p, (const double *)L.dinv.p);
+    *sec_cycle = 1e-3 * ms_cycle / reps;
+    *sec_step = 1e-3 * ms_step / reps;
+    *bytes_cycle = cycle_bytes(c, a, 0, vs);
+    *bytes_step = op_bytes(c, L, 0, vs) + 6.0 * n * vs + (double)L.nb * 9 * vs;
+}
+
 }  // namespace pfem

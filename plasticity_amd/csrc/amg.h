@@ -86,6 +86,10 @@ void amg_numeric_setup(pfem_ctx *c, Amg *a, bool keep_P = false);
 void amg_refresh_fine(pfem_ctx *c, Amg *a);  // level-0 smoother from the current K only
 void amg_apply64(pfem_ctx *c, Amg *a, const double *r, double *z);  // z = V-cycle(r)
 void amg_apply32(pfem_ctx *c, Amg *a, const float *r, float *z);
+// Times `reps` cycles (prec 0 FP64, 1 FP32) and `reps` fused fine-level Chebyshev steps
+// on the current hierarchy; returns seconds per cycle / step and their algorithmic bytes.
+void amg_bench(pfem_ctx *c, Amg *a, int prec, int reps, double *sec_cycle, double *bytes_cycle, double *sec_step,
+               double *bytes_step);
 
 // amg.hip launchers
 template <int BS>

@@ -1098,4 +1098,17 @@ int pfem_bench_spmv(pfem_ctx *c, int prec, int reps, double *sec) {
     }
 }
 
+int pfem_bench_vcycle(pfem_ctx *c, int prec, int reps, double *sec_cycle, double *bytes_cycle, double *sec_step,
+                      double *bytes_step) {
+    if (!c || !sec_cycle || !bytes_cycle || !sec_step || !bytes_step || reps < 1 || (prec != 0 && prec != 1))
+        return PFEM_ERR_BAD_ARG;
+    if (!c->amg || !c->pc_ready || (prec == 1 && !c->amg->have32)) return PFEM_ERR_BAD_ARG;
+    try {
+        amg_bench(c, c->amg.get(), prec, reps, sec_cycle, bytes_cycle, sec_step, bytes_step);
+        return PFEM_OK;
+    } catch (const hip_failure &e) {
+        return PFEM_ERR_HIP;
+    }
+}
+
 }  // extern "C"

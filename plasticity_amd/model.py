@@ -501,6 +501,13 @@ class Model:
         L.check(L.lib().pfem_bench_spmv(self._h, int(prec), int(reps), C.byref(sec)), "bench_spmv")
         return sec.value
 
+    def bench_vcycle(self, prec=0, reps=20):
+        """Seconds and algorithmic bytes of one AMG cycle and of one fused fine-level
+        Chebyshev step (prec 0 FP64, 1 FP32) on the current hierarchy."""
+        v = [C.c_double() for _ in range(4)]
+        L.check(L.lib().pfem_bench_vcycle(self._h, int(prec), int(reps), *[C.byref(x) for x in v]), "bench_vcycle")
+        return {"cycle_s": v[0].value, "cycle_bytes": v[1].value, "step_s": v[2].value, "step_bytes": v[3].value}
+
 
 def element_batch(kind, material: J2Material, Xe, ue, epsp=None, abar=None, cpinv=None, commit=False,
                   beta=None):

@@ -46,3 +46,36 @@ def test_fused_chebyshev_step_is_bitwise_equal(precision, smoother, degree, need
     assert a[3] < 1e-9 and a[4]
     assert np.array_equal(a[0], b[0]) and a[1:4] == b[1:4]
     assert a[5] == b[5] and a[6] == b[6] and np.array_equal(a[7], b[7])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+def test_vcycle_timing_entry_point(precision, need_gpu):
+    """pfem_bench_vcycle times the cycle and the fused fine smoother step on the current
+    hierarchy; its byte count of the step is the SELL operator (values + column index per
+    block, row pointers), the block-diagonal inverse and six vectors (d, r twice, x twice,
+    d'); the cycle holds at least the four fine-level products of a degree-2 V-cycle.  A
+    solve after the timing is unchanged (the timing only uses the cycle's own work
+    vectors)."""
+    print(f"[vcycle timing] {precision}", flush=True)
+    mesh = pa.box_mesh(1, 1, 1, 10, 10, 10, jitter=0.1, seed=2)
+    m = pa.Model(mesh, pa.J2Material(**MAT), options=pa.default_options(precision=precision, coarse_max=60))
+    d, v, r = uniaxial_constraints(mesh, -0.02)
+    m.constrain(d, v, r)
+    m._freeze()
+    U = (np.array([-0.004, 0.0015, 0.0015]) * mesh.nodes).ravel()
+    m.assemble(U)
+    m.impose_dirichlet(1.0)
+    dU0 = m.linear_solve(1e-10, rebuild_pc=True)[0].copy()
+    st = m.stats()
+    for prec in ([0, 1] if precision == "fp32_ir" else [0]):
+        s = 8 if prec == 0 else 4
+        f = m.bench_vcycle(prec=prec, reps=5)
+        step = st.nnzb * (9 * s + 4) + 4 * (st.nnodes + 1) + 6 * st.ndof * s + 9 * st.nnodes * s
+        print(f"  prec {prec}: cycle {f['cycle_s'] * 1e6:.1f} us {f['cycle_bytes']:.0f} B, "
+              f"step {f['step_s'] * 1e6:.1f} us {f['step_bytes']:.0f} B", flush=True)
+        assert f["cycle_s"] > 0 and f["step_s"] > 0
+        assert f["step_bytes"] == step
+        assert f["cycle_bytes"] > 4 * (st.nnzb * (9 * s + 4))
+    dU1 = m.linear_solve(1e-10, rebuild_pc=False)[0]
+    assert np.array_equal(dU0, dU1)
+    m.close()
