@@ -293,8 +293,9 @@ def main():
                          "0 = FP64 flexible CG with the FP32 V-cycle")
     ap.add_argument("--fp16-fine", action="store_true",
                     help="FP32-IR: fine-level smoother products with an FP16 copy of the scaled K")
-    ap.add_argument("--mode", default="newton", choices=["newton", "spmv", "copy"],
-                    help="newton: the metric; spmv: only repeated fine SpMVs (for counter runs); "
+    ap.add_argument("--mode", default="newton", choices=["newton", "spmv", "assemble", "copy"],
+                    help="newton: the metric; spmv: only repeated fine SpMVs (for counter runs); assemble: only "
+                         "repeated tangent assemblies at a plastic state (for counter runs); "
                          "copy: a plain 1 GiB device-to-device copy, the calibration of byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -323,6 +324,21 @@ def main():
         st = m.stats()
         b = spmv_bytes(st.nnzb, st.nnodes, st.ndof, 8 if args.precision == "fp64" else 4)
         log(f"[bench] fine SpMV x20: {sec * 1e3:.3f} ms each, {b / sec / 1e9:.0f} GB/s algorithmic ({b} B)")
+        return
+    if args.mode == "assemble":
+        m = pa.Model(wl.mesh, pa.J2Material(**wl.materials[0]), options=pa.default_options())
+        # homogeneous 1% compression (every Gauss point plastic: yield strain 0.125%)
+        U = (np.array([-0.01, 0.003, 0.003]) * wl.mesh.nodes).ravel()
+        m.assemble(U)
+        s0 = m.stats()
+        reps = 5
+        for _ in range(reps):
+            m.assemble(U)
+        s1 = m.stats()
+        gp = (s1.t_gp_kernel - s0.t_gp_kernel) / reps
+        sc = (s1.t_scatter_kernel - s0.t_scatter_kernel) / reps
+        log(f"[bench] assembly x{reps} at 1% compression: gp kernel {gp * 1e3:.3f} ms + scatter {sc * 1e3:.3f} ms "
+            f"per assembly ({wl.mesh.nelem} elements)")
         return
     log(f"[bench] workload: {wl.name}, nodes {wl.mesh.nnodes}, dof {3 * wl.mesh.nnodes}")
     opts = pa.default_options(eta_min=wl.krylov_rtol, eta_max=wl.krylov_rtol, precision=args.precision,

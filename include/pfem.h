@@ -403,6 +403,9 @@ typedef struct pfem_net_desc {
 int pfem_net_create(const pfem_net_desc *desc, pfem_net **out);
 void pfem_net_destroy(pfem_net *net);
 int64_t pfem_net_num_params(const pfem_net *net);
+/* Kernel of the 3x3x3 layers: 1 (default) LDS-tiled implicit GEMM on the 16x16x32 BF16
+ * MFMA, 0 one wave per 16 voxels on the 16x16x16 MFMA (kept for comparison). */
+int pfem_net_set_conv_kernel(pfem_net *net, int kind);
 /* FP32 master weights, layer by layer, each [tap][c_in][c_out]. */
 int pfem_net_get_params(pfem_net *net, float *w);
 int pfem_net_set_params(pfem_net *net, const float *w);   /* also resets Adam */

@@ -80,6 +80,8 @@ void net_pack(pfem_net *n, hipStream_t s) {
         const float *W = n->W.p + l.woff;
         launch_pack(l.ntaps, l.cin, l.cout, W, l.pk_f.p, false, false, s);
         if (k > 0) launch_pack(l.ntaps, l.cin, l.cout, W, l.pk_b.p, true, l.mode == NET_C3 || l.mode == NET_UPC3, s);
+        if (l.pk32_f.p) launch_pack32(l.cin, l.cout, W, l.pk32_f.p, false, false, s);
+        if (l.pk32_b.p) launch_pack32(l.cin, l.cout, W, l.pk32_b.p, true, true, s);
     }
     n->packed = true;
 }
@@ -96,6 +98,7 @@ void net_forward(pfem_net *n, hipStream_t s) {
         a.dout = n->dims[l.lout];
         a.in = layer_input(n, k);
         a.w = l.pk_f.p;
+        a.w32 = n->conv_kind == 1 ? l.pk32_f.p : nullptr;
         a.skip = l.skip >= 0 ? (const void *)n->act[l.skip].p : nullptr;
         const bool last = k == NET_LAYERS - 1;
         a.out = last ? (void *)n->out16.p : (void *)n->act[k].p;
@@ -132,6 +135,7 @@ void net_backward(pfem_net *n, bool accumulate, float scale, hipStream_t s) {
         a.relu = 0;
         a.in = g;
         a.w = l.pk_b.p;
+        a.w32 = n->conv_kind == 1 ? l.pk32_b.p : nullptr;
         a.skip = nullptr;
         float *gin = n->gr[k - 1].p;
         switch (l.mode) {
@@ -238,6 +242,10 @@ int pfem_net_create(const pfem_net_desc *d, pfem_net **out) {
             off += l.nw;
             l.pk_f.alloc(l.nw);
             if (k > 0) l.pk_b.alloc(l.nw);
+            if (l.mode == NET_C3 || l.mode == NET_UPC3) {
+                l.pk32_f.alloc(pack32_elems(l.cin, l.cout));
+                if (k > 0) l.pk32_b.alloc(pack32_elems(l.cout, l.cin));
+            }
             const int64_t ne = n->nvox[l.lout];
             const int tiles = (l.cin / 16) * (l.cout / 16);
             int64_t target = std::max<int64_t>(1, 2048 / ((int64_t)l.ntaps * tiles));
@@ -291,6 +299,12 @@ void pfem_net_destroy(pfem_net *n) {
 }
 
 int64_t pfem_net_num_params(const pfem_net *n) { return n ? n->nparams : 0; }
+
+int pfem_net_set_conv_kernel(pfem_net *n, int kind) {
+    if (!n || (kind != 0 && kind != 1)) return PFEM_ERR_BAD_ARG;
+    n->conv_kind = kind;
+    return PFEM_OK;
+}
 
 int pfem_net_get_params(pfem_net *n, float *w) {
     if (!n || !w) return PFEM_ERR_BAD_ARG;

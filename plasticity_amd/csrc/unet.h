@@ -17,7 +17,8 @@ struct ConvArgs {
     int mode, cin, cout, relu;
     NetDims din, dout;
     const void *in;    // input tensor (BF16 or FP32), cin channels on din
-    const void *w;     // packed BF16 weights
+    const void *w;     // packed BF16 weights (16x16x16 B fragments, conv_kernel)
+    const void *w32;   // packed BF16 weights (16x16x32 B fragments, conv3_tile_kernel) or null
     const void *skip;  // BF16, cout channels on dout, added before the ReLU (or null)
     void *out;         // cout channels on dout
 };
@@ -37,6 +38,10 @@ void launch_wreduce(int64_t nw, int nparts, const float *partial, float *dw, boo
                     hipStream_t s);
 void launch_pack(int ntaps, int cin, int cout, const float *W, uint16_t *pk, bool transpose, bool flip,
                  hipStream_t s);
+// BF16 elements of the 16x16x32 B-fragment packing of a 3x3x3 layer whose GEMM has
+// cinp input and coutp output channels (K = 27 cinp in steps of 32, N = coutp)
+int64_t pack32_elems(int cinp, int coutp);
+void launch_pack32(int cin, int cout, const float *W, uint16_t *pk, bool transpose, bool flip, hipStream_t s);
 void launch_relu_back(int64_t n, float *d, const float *extra, const uint16_t *y, hipStream_t s);
 void launch_pool_sum(NetDims fine, NetDims coarse, int C, const float *df, float *dc, hipStream_t s);
 void launch_absmax(int64_t n, const double *v, const uint8_t *mask, double *partial, double *out, hipStream_t s);
@@ -58,6 +63,7 @@ struct NetLayer {
     int ntaps;
     int64_t woff, nw;                            // slice of the master weight vector
     dbuf<uint16_t> pk_f, pk_b;                   // packed BF16 weights: forward, backward-data
+    dbuf<uint16_t> pk32_f, pk32_b;               // the same for the LDS-tiled 3x3x3 kernel (C3, UPC3)
     int nchunks = 1;
     int64_t chunk = 64;
 };
@@ -83,6 +89,7 @@ struct pfem_net {
     pfem::dbuf<double> red, scal;                // reductions; scal[0..2] dots, [3] loss sum, [4..5] scales
     int adam_step = 0;
     bool packed = false;
+    int conv_kind = 1;  // 3x3x3 layers: 1 LDS-tiled 16x16x32 kernel, 0 per-voxel-wave 16x16x16 kernel
     // (residual, correction) sample store: capacity pairs of 2 x 3*nvox[0] BF16
     int64_t capacity = 0, npairs = 0, next = 0;
     pfem::dbuf<uint16_t> pairs;

@@ -3,8 +3,18 @@
 //
 // Tensors are channels-last over a z-major voxel grid: t[((z*ny + y)*nx + x)*C + c].
 // Activations are BF16 (stored as uint16), gradients FP32.  Every convolution, forward
-// or backward, is an implicit GEMM on v_mfma_f32_16x16x16_bf16:
+// or backward, is an implicit GEMM on BF16 MFMA with FP32 accumulation:
 //
+//   conv3_tile_kernel<CIN, COUT, MODE>  the 3x3x3 layers (C3, UPC3; forward and
+//                             backward data): one workgroup = an output tile of
+//                             16 x 4 x 4 voxels whose 18 x 6 x 6 input neighbourhood
+//                             (for UPC3 the nearest-neighbour upsample of the coarse
+//                             input) is staged in LDS once and read by all 27 taps;
+//                             each wave computes one z-plane of the tile, 4 M-tiles of
+//                             16 voxels x COUT/16 N-tiles on v_mfma_f32_16x16x32_bf16,
+//                             K = 27 taps x CIN in steps of 32, so every B fragment
+//                             (weights, L2-resident) feeds 4 MFMAs and every A fragment
+//                             (one 16-byte LDS read) COUT/16.
 //   conv_kernel<COUT, MODE>   one wave = 16 consecutive voxels of the enumeration grid
 //                             x all COUT output channels (COUT/16 accumulator tiles);
 //                             the K loop runs over taps x input channels in steps of
@@ -18,6 +28,8 @@
 //
 // MFMA 16x16x16 lane layout (wave64): A[i][k] in lane i + 16*(k/4), item k%4;
 // B[k][j] in lane j + 16*(k/4), item k%4; D[i][j] in lane j + 16*(i/4), item i%4.
+// MFMA 16x16x32 (gfx950): A[i][k] in lane i + 16*(k/8), item k%8; B[k][j] in lane
+// j + 16*(k/8), item k%8; D as above (checked on the device with integer operands).
 //
 // Modes (input voxel u for enumeration voxel v and tap t):
 //   C3    3x3x3, stride 1, zero padding            u = v + d(t), same grid
@@ -36,6 +48,8 @@ namespace {
 
 typedef short s4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
     uint32_t b = __float_as_uint(f);
@@ -145,6 +159,132 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs a) {
             if (a.relu) v = fmaxf(v, 0.f);
             store_out(out + idx, v);
         }
+    }
+}
+
+
+// ---------------------------------------------------------------- LDS-tiled 3x3x3
+constexpr int T3X = 16, T3Y = 4, T3Z = 4;                      // output tile
+constexpr int H3X = T3X + 2, H3Y = T3Y + 2, H3Z = T3Z + 2;     // staged neighbourhood
+constexpr int H3VOX = H3X * H3Y * H3Z;
+
+// 8 consecutive channels of one voxel as 8 BF16 (16 bytes)
+__device__ __forceinline__ u32x4 load_bf8(const uint16_t *p) { return *reinterpret_cast<const u32x4 *>(p); }
+__device__ __forceinline__ u32x4 load_bf8(const float *p) {
+    const float4 lo = *reinterpret_cast<const float4 *>(p), hi = *reinterpret_cast<const float4 *>(p + 4);
+    u32x4 r;
+    r[0] = (uint32_t)f2bf(lo.x) | ((uint32_t)f2bf(lo.y) << 16);
+    r[1] = (uint32_t)f2bf(lo.z) | ((uint32_t)f2bf(lo.w) << 16);
+    r[2] = (uint32_t)f2bf(hi.x) | ((uint32_t)f2bf(hi.y) << 16);
+    r[3] = (uint32_t)f2bf(hi.z) | ((uint32_t)f2bf(hi.w) << 16);
+    return r;
+}
+
+// Output tile (blockIdx.x*16.., blockIdx.y*4.., blockIdx.z*4..) of a 3x3x3 convolution
+// on the output grid a.dout.  Tile voxel (lx, ly, lz) of the LDS neighbourhood is grid
+// voxel w = origin + (lx, ly, lz) - 1: for C3 the input at w, for UPC3 the coarse input
+// at w >> 1; zero where w lies outside the output grid (the zero padding of the
+// upsampled tensor).  The voxel stride is CIN + 8 BF16 (16 bytes of padding), so the
+// 16 lanes reading 16 consecutive voxels with 16-byte LDS loads touch disjoint banks.
+template <int CIN, int COUT, int MODE, class TI, class TO>
+__global__ __launch_bounds__(256) void conv3_tile_kernel(ConvArgs a) {
+    constexpr int NT = COUT / 16;
+    constexpr int VS = CIN + 8;
+    constexpr int NS = (27 * CIN + 31) / 32;  // K steps
+    constexpr int CPV = CIN / 8;              // 16-byte chunks per voxel
+    __shared__ __attribute__((aligned(16))) uint16_t tile[H3VOX * VS];
+    const int x0 = blockIdx.x * T3X, y0 = blockIdx.y * T3Y, z0 = blockIdx.z * T3Z;
+    const NetDims dout = a.dout, din = a.din;
+    const TI *in = static_cast<const TI *>(a.in);
+    for (int q = threadIdx.x; q < H3VOX * CPV; q += 256) {
+        const int v = q / CPV, part = q - v * CPV;
+        const int lx = v % H3X, ly = (v / H3X) % H3Y, lz = v / (H3X * H3Y);
+        const int wx = x0 + lx - 1, wy = y0 + ly - 1, wz = z0 + lz - 1;
+        u32x4 val = {0u, 0u, 0u, 0u};
+        if (wx >= 0 && wy >= 0 && wz >= 0 && wx < dout.nx && wy < dout.ny && wz < dout.nz) {
+            const int64_t u = MODE == NET_UPC3 ? ((int64_t)(wz >> 1) * din.ny + (wy >> 1)) * din.nx + (wx >> 1)
+                                               : ((int64_t)wz * din.ny + wy) * din.nx + wx;
+            val = load_bf8(in + u * CIN + part * 8);
+        }
+        *reinterpret_cast<u32x4 *>(&tile[v * VS + part * 8]) = val;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = lane & 15, g = lane >> 4;
+    const u32x4 *wp = reinterpret_cast<const u32x4 *>(a.w32) + lane;
+    f4_t acc[T3Y][NT];
+#pragma unroll
+    for (int my = 0; my < T3Y; ++my)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[my][nt] = f4_t{0.f, 0.f, 0.f, 0.f};
+    for (int st = 0; st < NS; ++st) {
+        const int k = st * 32 + g * 8;
+        const int t = k / CIN, c0 = k % CIN;  // this lane's tap and first channel
+        u32x4 bw[NT];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) bw[nt] = wp[(st * NT + nt) * 64];
+        const int tt = t < 27 ? t : 0;
+        const int dx = tt % 3, dy = (tt / 3) % 3, dz = tt / 9;
+        const int base = ((wave + dz) * H3Y + dy) * H3X + dx + m;
+#pragma unroll
+        for (int my = 0; my < T3Y; ++my) {
+            u32x4 av = *reinterpret_cast<const u32x4 *>(&tile[(base + my * H3X) * VS + c0]);
+            if (t >= 27) av = u32x4{0u, 0u, 0u, 0u};  // K padding of the last step
+            const bf8_t af = __builtin_bit_cast(bf8_t, av);
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                acc[my][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, __builtin_bit_cast(bf8_t, bw[nt]),
+                                                                      acc[my][nt], 0, 0, 0);
+        }
+    }
+    const uint16_t *skip = static_cast<const uint16_t *>(a.skip);
+    TO *out = static_cast<TO *>(a.out);
+    const int z = z0 + wave;
+    if (z >= dout.nz) return;
+#pragma unroll
+    for (int my = 0; my < T3Y; ++my) {
+        const int y = y0 + my;
+        if (y >= dout.ny) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int x = x0 + 4 * g + j;
+            if (x >= dout.nx) continue;
+            const int64_t o = ((int64_t)z * dout.ny + y) * dout.nx + x;
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                float v = acc[my][nt][j];
+                const int64_t idx = o * COUT + nt * 16 + m;
+                if (skip) v += bf2f(skip[idx]);
+                if (a.relu) v = fmaxf(v, 0.f);
+                store_out(out + idx, v);
+            }
+        }
+    }
+}
+
+// W[t][ci][co] (FP32, the layer's own cin x cout) -> B fragments of the 16x16x32 MFMA for
+// the GEMM K = (tap, ci') (27 taps x cinp, 32 per step, zero beyond), N = co':
+//   pk[((s NT + nt) 64 + l) 8 + q] = W'(k = 32 s + 8 (l / 16) + q, n = 16 nt + l % 16),
+// transpose: ci' = co, co' = ci (backward data); flip: tap t reads tap 26 - t.
+__global__ void pack32_kernel(int cin, int cout, const float *__restrict__ W, uint16_t *__restrict__ pk,
+                              int transpose, int flip) {
+    const int cinp = transpose ? cout : cin, coutp = transpose ? cin : cout;
+    const int NT = coutp / 16, K = 27 * cinp, NS = (K + 31) / 32;
+    const int64_t total = (int64_t)NS * NT * 512;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int qi = (int)(q & 7), l = (int)((q >> 3) & 63);
+        const int64_t r = q >> 9;
+        const int nt = (int)(r % NT), st = (int)(r / NT);
+        const int k = 32 * st + 8 * (l >> 4) + qi, n = 16 * nt + (l & 15);
+        float v = 0.f;
+        if (k < K) {
+            const int t = k / cinp, kp = k % cinp;
+            const int ts = flip ? 26 - t : t;
+            const int ci = transpose ? n : kp, co = transpose ? kp : n;
+            v = W[((int64_t)ts * cin + ci) * cout + co];
+        }
+        pk[q] = f2bf(v);
     }
 }
 
@@ -408,6 +548,13 @@ void launch_conv_t(const ConvArgs &a, hipStream_t s) {
     hipLaunchKernelGGL((conv_kernel<COUT, MODE, TI, TO>), grid, dim3(256), 0, s, a);
 }
 
+template <int CIN, int COUT, int MODE, class TI, class TO>
+void launch_conv3_tile_t(const ConvArgs &a, hipStream_t s) {
+    dim3 grid((unsigned)((a.dout.nx + T3X - 1) / T3X), (unsigned)((a.dout.ny + T3Y - 1) / T3Y),
+              (unsigned)((a.dout.nz + T3Z - 1) / T3Z));
+    hipLaunchKernelGGL((conv3_tile_kernel<CIN, COUT, MODE, TI, TO>), grid, dim3(256), 0, s, a);
+}
+
 template <int MODE, class TI, class TG>
 void launch_wgrad_t(const WgradArgs &a, int nchunks, hipStream_t s) {
     dim3 grid((unsigned)nchunks, mode_taps_host(MODE), (unsigned)((a.cin / 16) * (a.cout / 16)));
@@ -421,6 +568,32 @@ int mode_taps_host(int mode) { return (mode == NET_C3 || mode == NET_UPC3) ? 27 
 // Dispatch over (COUT, MODE, input type, output type).  in_f32: input tensor FP32
 // (gradients) else BF16; out_f32: output FP32 else BF16.
 void launch_conv(const ConvArgs &a, bool in_f32, bool out_f32, hipStream_t s) {
+    if (a.w32 && (a.mode == NET_C3 || a.mode == NET_UPC3) && in_f32 == out_f32) {
+#define PF_T3(CI, CO, TI, TO)                                                                         \
+    if (a.mode == NET_C3) launch_conv3_tile_t<CI, CO, NET_C3, TI, TO>(a, s);                          \
+    else launch_conv3_tile_t<CI, CO, NET_UPC3, TI, TO>(a, s);
+#define PF_T3_CO(CI, TI, TO)                                                                          \
+    switch (a.cout) {                                                                                 \
+        case 16: PF_T3(CI, 16, TI, TO) break;                                                         \
+        case 32: PF_T3(CI, 32, TI, TO) break;                                                         \
+        case 64: PF_T3(CI, 64, TI, TO) break;                                                         \
+        default: throw status_failure(PFEM_ERR_BAD_ARG, "conv cout");                                 \
+    }
+#define PF_T3_CI(TI, TO)                                                                              \
+    switch (a.cin) {                                                                                  \
+        case 16: PF_T3_CO(16, TI, TO) break;                                                          \
+        case 32: PF_T3_CO(32, TI, TO) break;                                                          \
+        case 64: PF_T3_CO(64, TI, TO) break;                                                          \
+        default: throw status_failure(PFEM_ERR_BAD_ARG, "conv cin");                                  \
+    }
+        if (in_f32) { PF_T3_CI(float, float) }
+        else { PF_T3_CI(uint16_t, uint16_t) }
+#undef PF_T3_CI
+#undef PF_T3_CO
+#undef PF_T3
+        PF_HIP(hipGetLastError());
+        return;
+    }
 #define PF_CONV_MODES(CO, TI, TO)                                                  \
     switch (a.mode) {                                                              \
         case NET_C3: launch_conv_t<CO, NET_C3, TI, TO>(a, s); break;               \
@@ -475,6 +648,15 @@ void launch_pack(int ntaps, int cin, int cout, const float *W, uint16_t *pk, boo
                  hipStream_t s) {
     const int64_t total = (int64_t)ntaps * cin * cout;
     hipLaunchKernelGGL(pack_kernel, dim3(grid1(total)), dim3(256), 0, s, ntaps, cin, cout, W, pk, transpose ? 1 : 0,
+                       flip ? 1 : 0);
+    PF_HIP(hipGetLastError());
+}
+
+int64_t pack32_elems(int cinp, int coutp) { return (int64_t)((27 * cinp + 31) / 32) * (coutp / 16) * 512; }
+
+void launch_pack32(int cin, int cout, const float *W, uint16_t *pk, bool transpose, bool flip, hipStream_t s) {
+    const int64_t total = pack32_elems(transpose ? cout : cin, transpose ? cin : cout);
+    hipLaunchKernelGGL(pack32_kernel, dim3(grid1(total)), dim3(256), 0, s, cin, cout, W, pk, transpose ? 1 : 0,
                        flip ? 1 : 0);
     PF_HIP(hipGetLastError());
 }

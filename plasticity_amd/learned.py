@@ -61,6 +61,12 @@ class UNet:
         except Exception:
             pass
 
+    def set_conv_kernel(self, kind):
+        """3x3x3 layers on the LDS-tiled 16x16x32 MFMA kernel (1, default) or the
+        one-wave-per-16-voxels 16x16x16 kernel (0)."""
+        L.check(L.lib().pfem_net_set_conv_kernel(self._h, int(kind)), "net_set_conv_kernel")
+        return self
+
     def get_params(self):
         w = np.empty(self.nparams, dtype=np.float32)
         L.check(L.lib().pfem_net_get_params(self._h, L.ptr(w)), "net_get_params")
