@@ -249,7 +249,8 @@ int pfem_residual_norm(pfem_ctx *ctx, double *out);
  * 5 flexible GCR with the learned preconditioner, 6 the same finished by AMG,
  * 7 FP32-IR finished by flexible GCR(16) with the FP32 AMG cycle after a CG breakdown,
  * 8 FP64 finished by flexible GCR(16) with the AMG cycle after PCG failed, 9 FP32-IR with
- * the Krylov iteration in FP32 inside FP64 refinement (fp32_krylov).  The GCR
+ * the Krylov iteration in FP32 inside FP64 refinement (fp32_krylov), 10 FP32-IR flexible
+ * PCG with the network's correction of the FP32 V-cycle (pfem_attach_net mode 5).  The GCR
  * fallbacks (7, 8) are the default; krylov_fallback = 1 selects BiCGStab (2, 4).
  * true_relres = ||b - K dU|| / ||b||.
  * A solve that ends with ||b - K dU|| > ||b|| returns dU = 0 (not applied); its status
@@ -425,9 +426,17 @@ int pfem_net_train(pfem_net *net, int steps, int batch, double lr, uint64_t seed
  * net's store; 2: precondition pfem_linear_solve with the network (flexible
  * CG, AMG fallback); 3: the same preconditioner without its network
  * (block-Jacobi flexible CG, AMG fallback; net may be NULL), the baseline that
- * shows what the network adds.  The net's grid must hold exactly the
- * context's nodes. */
+ * shows what the network adds; 4 (FP32-IR only): log error samples from every
+ * `stride`-th FP32 V-cycle application of the FP32-IR flexible PCG -- input the
+ * cycle's output z_k = V(r_k), target the error it left, x* - x_k - z_k, both divided
+ * by max |z_k|, kept only from solves that converge; 5 (FP32-IR only): precondition the
+ * FP32-IR flexible PCG with the V-cycle and the network's correction of its output,
+ * z = V(r) + c N(V(r) / c), c = max |V(r)| (the plain cycle finishes a solve the
+ * corrected one does not).  The net's grid must hold exactly the context's nodes. */
 int pfem_attach_net(pfem_ctx *ctx, pfem_net *net, int mode, int stride);
+/* Training loss: 0 (default) 1 - cos(N(x), t) (scale-invariant, for mode 1/2 samples),
+ * 1 ||N(x) - t||^2 / ||t||^2 (for the error samples of mode 4, whose scale matters). */
+int pfem_net_set_loss(pfem_net *net, int kind);
 
 /* Human-readable text for a status. */
 const char *pfem_status_string(int status);

@@ -148,6 +148,7 @@ SIGNATURES = {
     "pfem_net_create": (C.c_int, [C.POINTER(NetDesc), C.POINTER(_vp)]),
     "pfem_net_destroy": (None, [_vp]),
     "pfem_net_set_conv_kernel": (C.c_int, [_vp, C.c_int]),
+    "pfem_net_set_loss": (C.c_int, [_vp, C.c_int]),
     "pfem_net_num_params": (C.c_int64, [_vp]),
     "pfem_net_get_params": (C.c_int, [_vp, _vp]),
     "pfem_net_set_params": (C.c_int, [_vp, _vp]),

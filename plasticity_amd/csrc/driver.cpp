@@ -463,6 +463,27 @@ struct Amg64Pc {
     pfem_ctx *c;
     void operator()(const double *r, double *z) const { amg_apply64(c, c->amg.get(), r, z); }
 };
+// The FP32 V-cycle followed by the network's correction of its output (attach mode 5):
+// z = V(r), then z += c N(z / c) with c = max |z| -- the network sees the V-cycle's
+// approximation of the error and adds what it predicts the cycle left out.
+struct Amg32NetPc {
+    pfem_ctx *c;
+    void operator()(const double *r, double *z) const {
+        Amg32Pc{c}(r, z);
+        net_correct(c->net, z, c->bcflag.p, c->stream);
+    }
+};
+// The FP32 V-cycle, with every `net_stride`-th application reserving an error sample
+// (attach mode 4): input z_k = V(r_k) at the iterate x_k; the target x* - x_k - z_k is
+// written when the solve has converged (net_log_error_finish).
+struct Amg32ErrLogPc {
+    pfem_ctx *c;
+    const double *x;
+    void operator()(const double *r, double *z) const {
+        Amg32Pc{c}(r, z);
+        if (c->net_calls++ % c->net_stride == 0) net_log_error_begin(c->net, z, x, c->bcflag.p, c->stream);
+    }
+};
 // The learned preconditioner, a two-level method whose coarse correction is the
 // network: n = N(r) (constrained DOFs 0), scaled by g = (Kn)'r / (Kn)'(Kn) (0 if
 // Kn = 0), then one block-Jacobi sweep on what is left:
@@ -702,7 +723,7 @@ static int learned_solve(pfem_ctx *c, double rtol, int itmax, int *total, int *w
 // keeping the prolongators of the last full set-up
 int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *used, double *relres,
                       bool *discarded) {
-    const bool learned = c->net_mode >= 2;
+    const bool learned = c->net_mode == 2 || c->net_mode == 3;
     if (learned) {
         if (c->opt.precision == PFEM_PREC_FP32_IR) prepare_fp32(c);
         // the AMG hierarchy is rebuilt only if it is needed (a full rebuild request wins
@@ -734,13 +755,25 @@ int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *us
     } else if (c->opt.precision == PFEM_PREC_FP32_IR) {
         prepare_fp32(c);
         which = 3;
-        if (c->opt.fp32_krylov && !(c->net && c->net_mode == 1)) {
+        if (c->opt.fp32_krylov && !c->net) {
             which = 9;
             rc = ir32_fcg(c, c->rhs.p, c->dU.p, rtol, itmax, &it);
-        } else if (c->net && c->net_mode == 1)
+        } else if (c->net && c->net_mode == 1) {
             rc = fpcg(c, c->rhs.p, c->dU.p, rtol, itmax, true, 0, &it, LoggingPc<Amg32Pc>{c, Amg32Pc{c}});
-        else
+        } else if (c->net && c->net_mode == 4) {
+            rc = fpcg(c, c->rhs.p, c->dU.p, rtol, itmax, true, 0, &it, Amg32ErrLogPc{c, c->dU.p});
+            net_log_error_finish(c->net, c->dU.p, c->bcflag.p, rc == 0, s);
+        } else if (c->net && c->net_mode == 5) {
+            which = 10;
+            rc = fpcg(c, c->rhs.p, c->dU.p, rtol, itmax, true, 0, &it, Amg32NetPc{c});
+            if (rc != 0) {  // the corrected cycle failed: the plain FP32 cycle from scratch
+                total += it;
+                which = 3;
+                rc = fpcg(c, c->rhs.p, c->dU.p, rtol, itmax, true, 0, &it, Amg32Pc{c});
+            }
+        } else {
             rc = fpcg(c, c->rhs.p, c->dU.p, rtol, itmax, true, 0, &it, Amg32Pc{c});
+        }
         total += it;
         // after a breakdown or a stall: flexible GCR(16) with the same FP32 cycle (it needs
         // no symmetry, minimises ||r|| and costs one product and one cycle per

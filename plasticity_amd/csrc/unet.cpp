@@ -184,13 +184,53 @@ void net_log_pair(pfem_net *n, const double *r, const double *z, const uint8_t *
     n->npairs = std::min(n->npairs + 1, n->capacity);
 }
 
-// One sample: forward from the BF16 3-channel input, loss 1 - cos(out, target), and
-// the backward pass into G.
+void net_correct(pfem_net *n, double *z, const uint8_t *mask, hipStream_t s) {
+    if (!n->packed) net_pack(n, s);
+    launch_absmax(3 * n->nvox[0], z, mask, n->red.p, n->scal.p + 4, s);
+    launch_input_from_dofs(n->nvox[0], z, mask, n->scal.p + 4, n->x16.p, s);
+    net_forward(n, s);
+    launch_output_add_scaled(n->nvox[0], n->out16.p, mask, n->scal.p + 4, z, s);
+}
+
+void net_log_error_begin(pfem_net *n, const double *z, const double *x, const uint8_t *mask, hipStream_t s) {
+    if (n->capacity <= 0 || n->npend >= pfem_net::MAX_PEND || n->npend >= n->capacity) return;
+    const int64_t m = 3 * n->nvox[0];
+    if (!n->pend_u.p) {
+        n->pend_u.alloc((size_t)pfem_net::MAX_PEND * m);
+        n->pend_scale.alloc(pfem_net::MAX_PEND);
+    }
+    const int k = n->npend++;
+    const int64_t slot = (n->next + k) % n->capacity;
+    n->pend_slot[k] = slot;
+    double *u = n->pend_u.p + (size_t)k * m, *sc = n->pend_scale.p + k;
+    launch_absmax(m, z, mask, n->red.p, sc, s);
+    launch_dofs_to_pair(m, z, mask, sc, n->pairs.p + slot * 2 * m, s);
+    PF_HIP(hipMemcpyAsync(u, z, m * sizeof(double), hipMemcpyDeviceToDevice, s));
+    axpby<double>(m, 1.0, x, 1.0, u, s);  // u = x_k + z_k
+}
+
+void net_log_error_finish(pfem_net *n, const double *xstar, const uint8_t *mask, bool keep, hipStream_t s) {
+    const int64_t m = 3 * n->nvox[0];
+    if (keep) {
+        for (int k = 0; k < n->npend; ++k)
+            launch_diff_to_pair(m, xstar, n->pend_u.p + (size_t)k * m, mask, n->pend_scale.p + k,
+                                n->pairs.p + n->pend_slot[k] * 2 * m + m, s);
+        n->next = (n->next + n->npend) % n->capacity;
+        n->npairs = std::min<int64_t>(n->npairs + n->npend, n->capacity);
+    }
+    n->npend = 0;
+}
+
+// One sample: forward from the BF16 3-channel input, the loss (1 - cos(out, target) or
+// ||out - target||^2 / ||target||^2) and the backward pass into G.
 static void net_sample(pfem_net *n, const uint16_t *in3, const uint16_t *tg3, bool accumulate, float scale,
                        hipStream_t s) {
     launch_input_from_pairs(n->nvox[0], in3, n->x16.p, s);
     net_forward(n, s);
-    launch_cos_loss(n->nvox[0], n->out16.p, tg3, n->red.p, n->scal.p, n->scal.p + 3, n->gr[NET_LAYERS - 1].p, s);
+    if (n->loss_kind == 1)
+        launch_mse_loss(n->nvox[0], n->out16.p, tg3, n->red.p, n->scal.p, n->scal.p + 3, n->gr[NET_LAYERS - 1].p, s);
+    else
+        launch_cos_loss(n->nvox[0], n->out16.p, tg3, n->red.p, n->scal.p, n->scal.p + 3, n->gr[NET_LAYERS - 1].p, s);
     net_backward(n, accumulate, scale, s);
 }
 
@@ -299,6 +339,12 @@ void pfem_net_destroy(pfem_net *n) {
 }
 
 int64_t pfem_net_num_params(const pfem_net *n) { return n ? n->nparams : 0; }
+
+int pfem_net_set_loss(pfem_net *n, int kind) {
+    if (!n || (kind != 0 && kind != 1)) return PFEM_ERR_BAD_ARG;
+    n->loss_kind = kind;
+    return PFEM_OK;
+}
 
 int pfem_net_set_conv_kernel(pfem_net *n, int kind) {
     if (!n || (kind != 0 && kind != 1)) return PFEM_ERR_BAD_ARG;
@@ -416,12 +462,15 @@ int pfem_net_train(pfem_net *n, int steps, int batch, double lr, uint64_t seed, 
 }
 
 int pfem_attach_net(pfem_ctx *c, pfem_net *n, int mode, int stride) {
-    if (!c || mode < 0 || mode > 3 || stride < 1) return PFEM_ERR_BAD_ARG;
-    if ((mode == 1 || mode == 2) && (!n || n->nvox[0] != c->nnodes || n->device != c->device))
-        return PFEM_ERR_BAD_ARG;
-    if (mode == 3) n = nullptr;
+    if (!c || mode < 0 || mode > 5 || stride < 1) return PFEM_ERR_BAD_ARG;
+    const bool uses_net = mode != 0 && mode != 3;
+    if (uses_net && (!n || n->nvox[0] != c->nnodes || n->device != c->device)) return PFEM_ERR_BAD_ARG;
+    if ((mode == 4 || mode == 5) && c->opt.precision != PFEM_PREC_FP32_IR) return PFEM_ERR_BAD_ARG;
+    if (mode == 4 && n->capacity <= 0) return PFEM_ERR_BAD_ARG;
     if (hipDeviceSynchronize() != hipSuccess) return PFEM_ERR_HIP;
-    c->net = (mode == 1 || mode == 2) ? n : nullptr;
+    if (!uses_net) n = nullptr;
+    if (n) n->npend = 0;
+    c->net = n;
     c->net_mode = mode;
     c->net_stride = stride;
     c->net_calls = 0;

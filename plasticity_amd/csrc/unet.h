@@ -54,6 +54,12 @@ void launch_dofs_to_pair(int64_t n, const double *v, const uint8_t *mask, const 
                          hipStream_t s);
 void launch_cos_loss(int64_t nvox, const float *o16, const uint16_t *z3, double *partial, double *sums,
                      double *loss_acc, float *d16, hipStream_t s);
+void launch_output_add_scaled(int64_t nvox, const float *o16, const uint8_t *mask, const double *scale, double *z,
+                              hipStream_t s);
+void launch_diff_to_pair(int64_t n, const double *a, const double *b, const uint8_t *mask, const double *scale,
+                         uint16_t *p3, hipStream_t s);
+void launch_mse_loss(int64_t nvox, const float *o16, const uint16_t *t3, double *partial, double *sums,
+                     double *loss_acc, float *d16, hipStream_t s);
 void launch_adam(int64_t n, float *w, const float *g, float *m, float *v, float lr, float b1, float b2, float eps,
                  int step, hipStream_t s);
 
@@ -90,6 +96,13 @@ struct pfem_net {
     int adam_step = 0;
     bool packed = false;
     int conv_kind = 1;  // 3x3x3 layers: 1 LDS-tiled 16x16x32 kernel, 0 per-voxel-wave 16x16x16 kernel
+    int loss_kind = 0;  // training loss: 0 = 1 - cos(N(x), t), 1 = ||N(x) - t||^2 / ||t||^2
+    // samples of an error-logging solve awaiting its solution: pair-store slot, x_k + z_k
+    // (FP64, pend_u) and the input's scale (pend_scale)
+    static constexpr int MAX_PEND = 16;
+    int npend = 0;
+    int64_t pend_slot[MAX_PEND] = {};
+    pfem::dbuf<double> pend_u, pend_scale;
     // (residual, correction) sample store: capacity pairs of 2 x 3*nvox[0] BF16
     int64_t capacity = 0, npairs = 0, next = 0;
     pfem::dbuf<uint16_t> pairs;
@@ -104,4 +117,12 @@ void net_pack(pfem_net *n, hipStream_t s);
 void net_apply(pfem_net *n, const double *r, const uint8_t *mask, double *z, hipStream_t s);
 // Append the sample (r, z) (constrained DOFs zeroed, each scaled by its max |.|).
 void net_log_pair(pfem_net *n, const double *r, const double *z, const uint8_t *mask, hipStream_t s);
+// The correction of a V-cycle output z: z += c N(z / c), c = max |z| over the free DOFs.
+void net_correct(pfem_net *n, double *z, const uint8_t *mask, hipStream_t s);
+// Error samples of an iterative solve: at iterate x_k with V-cycle output z_k, reserve a
+// sample with input z_k / c_k (c_k = max |z_k|); once the solve has converged to x*,
+// its target (x* - x_k - z_k) / c_k -- the error the V-cycle left, on the input's scale --
+// is written and the samples are kept (keep) or dropped.
+void net_log_error_begin(pfem_net *n, const double *z, const double *x, const uint8_t *mask, hipStream_t s);
+void net_log_error_finish(pfem_net *n, const double *xstar, const uint8_t *mask, bool keep, hipStream_t s);
 }  // namespace pfem

@@ -519,6 +519,81 @@ __global__ void cos_grad_kernel(int64_t nvox, const float *__restrict__ o16, con
     }
 }
 
+
+// z += scale * o on the free DOFs (constrained DOFs untouched): the network's correction
+// of a V-cycle output, rescaled by the factor its input was divided by.
+__global__ void output_add_scaled_kernel(int64_t nvox, const float *__restrict__ o16, const uint8_t *__restrict__ mask,
+                                         const double *__restrict__ scale, double *__restrict__ z) {
+    const double sc = *scale;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nvox * 3;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        if (mask && mask[q]) continue;
+        z[q] += sc * (double)o16[(q / 3) * 16 + q % 3];
+    }
+}
+// (a - b) / scale -> BF16 3-channel sample (constrained DOFs 0)
+__global__ void diff_to_pair_kernel(int64_t n, const double *__restrict__ a, const double *__restrict__ b,
+                                    const uint8_t *__restrict__ mask, const double *__restrict__ scale,
+                                    uint16_t *__restrict__ p3) {
+    const double s = *scale > 0 ? 1.0 / *scale : 0.0;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        p3[q] = f2bf((mask && mask[q]) ? 0.f : (float)((a[q] - b[q]) * s));
+}
+
+// Loss ||o - t||^2 / ||t||^2 over the 3 used channels: per-block partial sums of
+// ((o - t).(o - t), t.t).
+__global__ __launch_bounds__(256) void mse_partial_kernel(int64_t nvox, const float *__restrict__ o16,
+                                                          const uint16_t *__restrict__ t3, double *__restrict__ partial) {
+    __shared__ double sh[2][256];
+    double dd = 0, tt = 0;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nvox * 3;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const double t = bf2f(t3[q]), d = o16[(q / 3) * 16 + q % 3] - t;
+        dd += d * d;
+        tt += t * t;
+    }
+    sh[0][threadIdx.x] = dd;
+    sh[1][threadIdx.x] = tt;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int k = 0; k < 2; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 2; ++k) partial[2 * blockIdx.x + k] = sh[k][0];
+}
+__global__ __launch_bounds__(256) void mse_finish_kernel(int nb, const double *__restrict__ partial,
+                                                         double *__restrict__ sums, double *__restrict__ loss_acc) {
+    __shared__ double sh[2][256];
+    double a[2] = {0, 0};
+    for (int q = threadIdx.x; q < nb; q += 256)
+        for (int k = 0; k < 2; ++k) a[k] += partial[2 * q + k];
+    for (int k = 0; k < 2; ++k) sh[k][threadIdx.x] = a[k];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int k = 0; k < 2; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        sums[0] = sh[0][0];
+        sums[1] = sh[1][0];
+        if (loss_acc) *loss_acc += sh[1][0] > 0 ? sh[0][0] / sh[1][0] : 0.0;
+    }
+}
+// dL/do = 2 (o - t) / ||t||^2, channels >= 3 get 0
+__global__ void mse_grad_kernel(int64_t nvox, const float *__restrict__ o16, const uint16_t *__restrict__ t3,
+                                const double *__restrict__ sums, float *__restrict__ d16) {
+    const double a = sums[1] > 0 ? 2.0 / sums[1] : 0.0;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nvox * 16;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(q & 15);
+        const int64_t v = q >> 4;
+        d16[q] = c < 3 ? (float)(a * (o16[q] - bf2f(t3[3 * v + c]))) : 0.f;
+    }
+}
+
 // Adam (Kingma & Ba 2015) on FP32 master weights.  Padding weights (input channels 3..15
 // of the first layer, output channels 3..15 of the last) start at zero and receive
 // exactly zero gradient, so they stay zero.
@@ -703,6 +778,24 @@ void launch_cos_loss(int64_t nvox, const float *o16, const uint16_t *z3, double 
     hipLaunchKernelGGL(cos_partial_kernel, dim3(NET_RED_BLOCKS), dim3(256), 0, s, nvox, o16, z3, partial);
     hipLaunchKernelGGL(cos_finish_kernel, dim3(1), dim3(256), 0, s, NET_RED_BLOCKS, partial, sums, loss_acc);
     if (d16) hipLaunchKernelGGL(cos_grad_kernel, dim3(grid1(nvox * 16)), dim3(256), 0, s, nvox, o16, z3, sums, d16);
+    PF_HIP(hipGetLastError());
+}
+
+void launch_output_add_scaled(int64_t nvox, const float *o16, const uint8_t *mask, const double *scale, double *z,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(output_add_scaled_kernel, dim3(grid1(nvox * 3)), dim3(256), 0, s, nvox, o16, mask, scale, z);
+    PF_HIP(hipGetLastError());
+}
+void launch_diff_to_pair(int64_t n, const double *a, const double *b, const uint8_t *mask, const double *scale,
+                         uint16_t *p3, hipStream_t s) {
+    hipLaunchKernelGGL(diff_to_pair_kernel, dim3(grid1(n)), dim3(256), 0, s, n, a, b, mask, scale, p3);
+    PF_HIP(hipGetLastError());
+}
+void launch_mse_loss(int64_t nvox, const float *o16, const uint16_t *t3, double *partial, double *sums,
+                     double *loss_acc, float *d16, hipStream_t s) {
+    hipLaunchKernelGGL(mse_partial_kernel, dim3(NET_RED_BLOCKS), dim3(256), 0, s, nvox, o16, t3, partial);
+    hipLaunchKernelGGL(mse_finish_kernel, dim3(1), dim3(256), 0, s, NET_RED_BLOCKS, partial, sums, loss_acc);
+    if (d16) hipLaunchKernelGGL(mse_grad_kernel, dim3(grid1(nvox * 16)), dim3(256), 0, s, nvox, o16, t3, sums, d16);
     PF_HIP(hipGetLastError());
 }
 

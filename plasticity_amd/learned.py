@@ -25,7 +25,7 @@ import numpy as np
 from . import _lib as L
 
 CHANNELS = (16, 32, 64, 64)
-MODES = {"off": 0, "log": 1, "precondition": 2, "jacobi": 3}
+MODES = {"off": 0, "log": 1, "precondition": 2, "jacobi": 3, "log_error": 4, "correct": 5}
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "unet_16_32_64_64.npz")
 
 
@@ -60,6 +60,11 @@ class UNet:
             self.close()
         except Exception:
             pass
+
+    def set_loss(self, kind):
+        """Training loss: 0 = 1 - cos(N(x), t), 1 = ||N(x) - t||^2 / ||t||^2."""
+        L.check(L.lib().pfem_net_set_loss(self._h, int(kind)), "net_set_loss")
+        return self
 
     def set_conv_kernel(self, kind):
         """3x3x3 layers on the LDS-tiled 16x16x32 MFMA kernel (1, default) or the
@@ -178,6 +183,76 @@ def collect_samples(net, seeds, n, load_steps=(1, 2, 3), stride=2, log=print):
     return net.num_pairs
 
 
+def collect_error_samples(net, seeds, n, load_steps=(1, 2, 3), stride=1, log=print):
+    """FP32-IR Newton load steps (linear rtol 1e-8) on the sample meshes; every
+    `stride`-th FP32 V-cycle application of their flexible PCG solves logs an error
+    sample (attach mode 'log_error': input V(r_k), target x* - x_k - V(r_k), the error
+    the cycle left, both divided by max |V(r_k)|)."""
+    from .model import ElementInversionError, default_options
+    t0 = time.perf_counter()
+    for s in seeds:
+        m, wl = sample_model(s, n, inclusion_fraction(s),
+                             options=default_options(precision="fp32_ir", eta_min=1e-8, eta_max=1e-8))
+        attach(m, net, "log_error", stride)
+        for k in load_steps:
+            try:
+                conv, hist, kry, _ = m.newton(k / wl.nsteps, tol=wl.newton_tol, maxiter=25)
+            except ElementInversionError as e:
+                log(f"[learned] seed {s} load step {k}: {e}")
+                break
+            if not conv:
+                log(f"[learned] seed {s} load step {k}: Newton did not converge")
+                break
+            m.commit()
+        attach(m, None, "off")
+        log(f"[learned] seed {s} inclusions {inclusion_fraction(s):.2f}: error samples {net.num_pairs} "
+            f"({time.perf_counter() - t0:.1f}s)")
+        m.close()
+    return net.num_pairs
+
+
+def evaluate_correction(net_weights, seeds, n, load_steps=(1, 2, 3), log=print):
+    """Newton load steps on held-out sample meshes under FP32-IR at linear rtol 1e-8, with
+    the plain FP32 V-cycle and with the V-cycle corrected by the network (attach mode
+    'correct').  Returns per-seed Krylov iterations, solve seconds and fallbacks."""
+    from .model import default_options
+    out = []
+    for s in seeds:
+        row = {"seed": s, "inclusion": inclusion_fraction(s)}
+        for label in ("fp32_ir", "correct"):
+            m, wl = sample_model(s, n, inclusion_fraction(s),
+                                 options=default_options(precision="fp32_ir", eta_min=1e-8, eta_max=1e-8))
+            net = None
+            if label == "correct":
+                net = UNet(grid_of(wl.mesh), CHANNELS)
+                net.set_params(net_weights)
+                attach(m, net, "correct")
+            kry = newton = 0
+            t_solve = 0.0
+            used = []
+            for k in load_steps:
+                conv, hist, kr, res = m.newton(k / wl.nsteps, tol=wl.newton_tol, maxiter=25)
+                if not conv:
+                    raise RuntimeError(f"seed {s} load step {k} ({label}) did not converge")
+                m.commit()
+                kry += sum(kr)
+                newton += len(hist)
+                t_solve += res.t_solve
+                used += [res.solver_used[i] for i in range(len(kr))]
+            row[label] = {"krylov": kry, "newton": newton, "t_solve": t_solve, "solves": len(used),
+                          "plain_finish": sum(1 for u in used if u != 10) if label == "correct" else 0}
+            attach(m, None, "off")
+            if net is not None:
+                net.close()
+            m.close()
+        log(f"[learned] correction eval seed {s}: FP32-IR {row['fp32_ir']['krylov']} its "
+            f"{row['fp32_ir']['t_solve']:.3f}s | corrected {row['correct']['krylov']} its "
+            f"{row['correct']['t_solve']:.3f}s (plain cycle finished {row['correct']['plain_finish']} of "
+            f"{row['correct']['solves']} solves)")
+        out.append(row)
+    return out
+
+
 def evaluate(net_weights, seeds, n, load_steps=(1, 2, 3), precision="fp64", log=print):
     """Newton load steps on held-out sample meshes at linear rtol 1e-8, once with the AMG
     solver and once with the learned preconditioner in flexible GCR (AMG fallback), and
@@ -238,12 +313,13 @@ def layer_flops(grid, channels=CHANNELS):
     return [(n, t, ci, co, vox[l], 2 * vox[l] * t * ci * co) for n, t, ci, co, l in table]
 
 
-def bench_net(n=100, reps=20, train_steps=10, log=print):
+def bench_net(n=100, reps=20, train_steps=10, conv_kernel=1, log=print):
     """Times the forward pass (host copies of r and N(r) included) and Adam steps on a
     grid of (n + 1)^3 nodes with random weights and random logged pairs; the kernels'
-    own times come from a kernel trace of this command."""
+    own times come from a kernel trace of this command.  conv_kernel: 1 LDS-tiled
+    16x16x32 kernel for the 3x3x3 layers, 0 the one-wave-per-16-voxels kernel."""
     grid = (n + 1,) * 3
-    net = UNet(grid, CHANNELS, capacity=8, seed=0)
+    net = UNet(grid, CHANNELS, capacity=8, seed=0).set_conv_kernel(conv_kernel)
     rng = np.random.default_rng(0)
     x = rng.normal(size=3 * net.nvox).astype(np.float32)
     for _ in range(4):
@@ -273,7 +349,7 @@ def main(argv=None):
     import argparse
     import json
     ap = argparse.ArgumentParser(description="train / evaluate the learned preconditioner")
-    ap.add_argument("cmd", choices=["train", "eval", "bench"])
+    ap.add_argument("cmd", choices=["train", "eval", "bench", "correct"])
     ap.add_argument("--n", type=int, default=63, help="elements per side of the sample blocks")
     ap.add_argument("--seeds", default="2-20")
     ap.add_argument("--load-steps", default="1-5")
@@ -285,6 +361,9 @@ def main(argv=None):
     ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--out", default=None)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_ir"])
+    ap.add_argument("--eval-seeds", default="100-100", help="correct: held-out seeds")
+    ap.add_argument("--conv-kernel", type=int, default=1, choices=[0, 1],
+                    help="bench: 3x3x3 layers on the LDS-tiled 16x16x32 kernel (1) or the 16x16x16 one (0)")
     a = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)  # noqa: E731
     if a.cmd == "train":
@@ -301,8 +380,28 @@ def main(argv=None):
             log(f"[learned] adam step {done}/{a.steps}: mean loss {loss:.4f} ({time.perf_counter() - t0:.1f}s)")
         net.save(a.out or a.weights)
         log(f"[learned] saved {a.out or a.weights}")
+    elif a.cmd == "correct":
+        # error samples from FP32-IR solves, relative-MSE training, held-out evaluation
+        net = UNet((a.n + 1,) * 3, CHANNELS, capacity=a.capacity, seed=0).set_loss(1)
+        collect_error_samples(net, _seeds(a.seeds), a.n, load_steps=_seeds(a.load_steps), stride=a.stride, log=log)
+        t0 = time.perf_counter()
+        chunk = max(1, a.steps // 10)
+        done = 0
+        while done < a.steps:
+            k = min(chunk, a.steps - done)
+            lr = a.lr if done < 0.8 * a.steps else 0.1 * a.lr
+            loss = net.train(k, batch=a.batch, lr=lr, seed=done)
+            done += k
+            log(f"[learned] adam step {done}/{a.steps}: mean relative squared error {loss:.4f} "
+                f"({time.perf_counter() - t0:.1f}s)")
+        w = net.get_params()
+        if a.out:
+            net.save(a.out)
+        net.close()
+        res = evaluate_correction(w, _seeds(a.eval_seeds), a.n, load_steps=_seeds(a.load_steps), log=log)
+        print(json.dumps(res))
     elif a.cmd == "bench":
-        print(json.dumps(bench_net(a.n, log=log)))
+        print(json.dumps(bench_net(a.n, conv_kernel=a.conv_kernel, log=log)))
     else:
         with np.load(a.weights, allow_pickle=False) as f:
             w = f["w"].copy()
