@@ -205,10 +205,14 @@ def time_variant(pa, wl, precision, learned, warmup, steps, args):
     for k in range(1, warmup + steps + 1):
         if k == warmup + 1:
             t0 = time.perf_counter()
+        t1 = time.perf_counter()
         recs = m.advance(k / wl.nsteps, wl.nsteps, tol=wl.newton_tol, maxiter=args.maxiter, cutback=args.cutback,
                          grow_iters=args.grow_iters)
         if k > warmup:
             tot.add(recs)
+        log(f"[bench] {precision}{' + learned' if learned else ''} load step {k}: "
+            f"{sum(r['iters'] for r in recs)} Newton, {sum(sum(r['krylov']) for r in recs)} Krylov, "
+            f"{time.perf_counter() - t1:.3f}s")
     elapsed = time.perf_counter() - t0
     U, abar = m.U, m.get_state()["abar"]
     vc = None

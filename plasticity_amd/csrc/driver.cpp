@@ -688,7 +688,12 @@ static int learned_solve(pfem_ctx *c, double rtol, int itmax, int *total, int *w
     if (c->opt.verbose > 1) fprintf(stderr, "      learned gcr: rc %d after %d its\n", rc, it);
     if (rc == 0) return 0;
     *which = 6;
-    if (c->pc_stale || !c->pc_ready) {
+    if (c->pc_stale == 2 && c->pc_ready) {  // the fine-level smoother only
+        const double t0 = now();
+        amg_refresh_fine(c, c->amg.get());
+        c->st.t_setup += now() - t0;
+        c->pc_stale = 0;
+    } else if (c->pc_stale || !c->pc_ready) {
         pc_setup(c, c->pc_stale == 3);
         c->pc_stale = 0;
     }
