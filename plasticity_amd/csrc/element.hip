@@ -12,9 +12,9 @@
 //                  element_force_tangent_finite! Elements.jl:405-494 with
 //                  finite_kinematics FiniteStrain.jl:148-170, finite_stress_update
 //                  :292-346, dPdF :503-547 and return_map Materials.jl:159-250.
-//   scatter_kernel one 64-lane wave per element of one colour: lane (a, b) contracts
-//                  the 8 packets (in two stages, through per-node partial sums in LDS)
-//                  into the 3x3 block Ke(a, b) and adds it into the fine
+//   scatter_kernel one 64-lane wave per four elements of one colour: lane (el, a, half)
+//                  contracts the 8 packets of element el into the 3x3 blocks Ke(a, b),
+//                  b in one half of the nodes, and adds them into the fine
 //                  BSR K through a precomputed slot map (no search, no atomics:
 //                  same-colour elements share no node, Assembly.jl:370-385).
 //                  Ke = sum_g w G^T A Geff with Geff = s G + (s/3) vec(F)(g0 - gJ)^T
@@ -799,21 +799,19 @@ __device__ __forceinline__ void contract_blocks(const double *__restrict__ pe, c
     }
 }
 
-// One 64-lane wave per element of one colour; lane (a, b) = (lane / 8, lane % 8)
-// accumulates the 3x3 block Ke(a, b) over the 8 GP packets and adds it into the fine
-// BSR K through the precomputed slot map (same-colour elements share no node, so no
-// atomics; colours run serially, Assembly.jl:370-385).
-//   Ke(a,b)_ik = sum_g [ sum_jl dNa_j dNb_l SA_(i+3j),(k+3l) + h_i (g0_b - gJ_b)_k ]
+// One 64-lane wave per four elements of one colour; lane (el, a, half) =
+// (lane / 16, lane % 16 / 2, lane % 2) accumulates the four 3x3 blocks Ke(a, b),
+// b = 4 half .. 4 half + 3, of element el over the 8 GP packets and adds them into the
+// fine BSR K through the precomputed slot map (same-colour elements share no node, so
+// no atomics; colours run serially, Assembly.jl:370-385).
+//   Ke(a,b)_ik = sum_g [ sum_jl dNa_j dNb_l SA_(i+3j),(k+3l) - h_i gJ_b,k ] + g0_b,k sum_g h_i
 //   h_i = sum_j dNa_j q_(i+3j),  gJ_b,k = sum_j dNb_j (F^-1)_jk        (Elements.jl:470-478)
-// The main term is contracted in two stages per Gauss point: the 64 lanes first form,
-// for every node a, the 27 partial sums T_a[i][k + 3l] = sum_j dNa_j SA_(i+3j),(k+3l)
-// (216 values, at most four per lane, into LDS), then lane (a, b) finishes its block as
-// sum_l T_a[i][k + 3l] dNb_l: 12 + 27 multiply-adds per lane instead of 81 + 9 (96
-// instead of 143 VGPRs).  Each Gauss point's packet and reference gradients are staged
-// in a double buffer (the next point's 16-byte loads are in flight during the
-// contraction), so a workgroup needs at most 4.1 KB of LDS and occupancy is set by
-// registers.  With the packed symmetric layout a Gauss point reads 96 doubles (72 of
-// packet, 24 of gradients) instead of 132.
+// Per Gauss point the lane runs over the 9 columns c = k + 3l of SA: it forms
+// U_i = sum_j dNa_j SA_(i+3j),c and adds U_i dNb_l to its four blocks, so every SA entry
+// is read from LDS once per lane and the 16 lanes of an element read 4x less than one
+// lane per block did (LDS bandwidth was the limit).  Each Gauss point's packets and
+// reference gradients (four elements) are staged in a double buffer, the next point's
+// 16-byte loads in flight during the contraction.
 template <bool SYM>
 __global__ __launch_bounds__(64) void scatter_kernel(const double *__restrict__ packet,
                                                       const double *__restrict__ geo,
@@ -825,103 +823,128 @@ __global__ __launch_bounds__(64) void scatter_kernel(const double *__restrict__ 
                                                       double *__restrict__ Kv, double *__restrict__ R, int fbar,
                                                       int local) {
     constexpr int LEN = pk_len(SYM), NA = pk_na(SYM);
-    constexpr int NCP = LEN / 2, NC = NCP + 12;  // 16-byte chunks: packet, then the 24 gradients
-    constexpr int ST = LEN + 24;                 // doubles staged per Gauss point
-    __shared__ __attribute__((aligned(16))) double sp[2][ST];
-    __shared__ __attribute__((aligned(16))) double T[8 * 27];
-    const int64_t li = blockIdx.x;
-    if (li >= count) return;
-    const int64_t e = elist[li];
-    const int lane = threadIdx.x, a = lane >> 3, b = lane & 7;
-    const double2 *src = reinterpret_cast<const double2 *>(packet + (size_t)(local ? li : e) * 8 * LEN);
-    const double2 *gsrc = reinterpret_cast<const double2 *>(geo + (size_t)e * 8 * 24);
-    // chunk t of Gauss point g: the packet's for t < NCP, the gradients' after
-    auto chunk = [&](int g, int t) { return t < NCP ? src[g * NCP + t] : gsrc[g * 12 + (t - NCP)]; };
-    const bool two = NC > 64 && lane < NC - 64;  // lanes that also take chunk 64 + lane
-    double2 r0 = lane < NC ? chunk(0, lane) : double2{0.0, 0.0};
-    double2 r1 = two ? chunk(0, 64 + lane) : double2{0.0, 0.0};
-    int s1_dn[4], s1_sa[4][3];
+    constexpr int ST = LEN + 24;               // doubles staged per element and Gauss point
+    constexpr int NCP = LEN / 2, NC = ST / 2;  // 16-byte chunks: packet, then the 24 gradients
+    constexpr int NCH = (4 * NC + 63) / 64;    // chunks per lane and Gauss point
+    __shared__ __attribute__((aligned(16))) double sp[2][4 * ST];
+    const int lane = threadIdx.x, el = lane >> 4, a = (lane >> 1) & 7, half = lane & 1;
+    const int64_t li0 = (int64_t)blockIdx.x * 4;
+    const bool act = li0 + el < count;
+    const int64_t e = act ? elist[li0 + el] : 0;
+    // the lane's chunks: source of Gauss point 0 and the stride per Gauss point (double2)
+    const double2 *csrc[NCH];
+    int cstride[NCH];
+    bool cok[NCH];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int v = lane + 64 * q < 216 ? lane + 64 * q : 0;
-        const int ap = v / 27, rem = v - 27 * ap, i = rem / 9, kl = rem - 9 * i;
-        s1_dn[q] = LEN + 3 * ap;
-#pragma unroll
-        for (int j = 0; j < 3; ++j) s1_sa[q][j] = pk_a_index<SYM>(i + 3 * j, kl);
+    for (int k = 0; k < NCH; ++k) {
+        const int q = lane + 64 * k, ce = q / NC, t = q - ce * NC;
+        cok[k] = q < 4 * NC && li0 + ce < count;
+        const int64_t cli = cok[k] ? li0 + ce : 0;
+        const int64_t ee = cok[k] ? elist[cli] : 0;
+        if (t < NCP) {
+            csrc[k] = reinterpret_cast<const double2 *>(packet + (size_t)(local ? cli : ee) * 8 * LEN) + t;
+            cstride[k] = NCP;
+        } else {
+            csrc[k] = reinterpret_cast<const double2 *>(geo + (size_t)ee * 8 * 24) + (t - NCP);
+            cstride[k] = 12;
+        }
     }
-    double acc[9];
+    double2 rb[NCH];
 #pragma unroll
-    for (int t = 0; t < 9; ++t) acc[t] = 0.0;
-    double fe[3] = {0.0, 0.0, 0.0};
-    double g0b[3] = {0.0, 0.0, 0.0};
-    if (fbar)
+    for (int k = 0; k < NCH; ++k) rb[k] = cok[k] ? csrc[k][0] : double2{0.0, 0.0};
 #pragma unroll
-        for (int k = 0; k < 3; ++k) g0b[k] = g0[24 * e + 3 * b + k];
-    if (lane < NC) reinterpret_cast<double2 *>(sp[0])[lane] = r0;
-    if (two) reinterpret_cast<double2 *>(sp[0])[64 + lane] = r1;
+    for (int k = 0; k < NCH; ++k)
+        if (lane + 64 * k < 4 * NC) reinterpret_cast<double2 *>(sp[0])[lane + 64 * k] = rb[k];
     __syncthreads();
+    double acc[4][9];
+#pragma unroll
+    for (int bb = 0; bb < 4; ++bb)
+#pragma unroll
+        for (int t = 0; t < 9; ++t) acc[bb][t] = 0.0;
+    double fe[3] = {0.0, 0.0, 0.0}, H[3] = {0.0, 0.0, 0.0};
 #pragma unroll 1
     for (int g = 0; g < 8; ++g) {
-        if (g < 7) {  // the next Gauss point's packet, written to the other buffer at the end
-            if (lane < NC) r0 = chunk(g + 1, lane);
-            if (two) r1 = chunk(g + 1, 64 + lane);
+        if (g < 7) {  // the next Gauss point, written to the other buffer at the end
+#pragma unroll
+            for (int k = 0; k < NCH; ++k)
+                if (cok[k]) rb[k] = csrc[k][(g + 1) * cstride[k]];
         }
-        const double *pk = sp[g & 1];
+        const double *pk = sp[g & 1] + el * ST;
+        double dNa[3], dNb[4][3];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            if (lane + 64 * q < 216) {
-                const double *dn = pk + s1_dn[q];
-                T[lane + 64 * q] = dn[0] * pk[s1_sa[q][0]] + dn[1] * pk[s1_sa[q][1]] + dn[2] * pk[s1_sa[q][2]];
-            }
+        for (int j = 0; j < 3; ++j) dNa[j] = pk[LEN + 3 * a + j];
+#pragma unroll
+        for (int bb = 0; bb < 4; ++bb)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) dNb[bb][j] = pk[LEN + 3 * (4 * half + bb) + j];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            const int k = c % 3, l = c / 3;
+            double u[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                u[i] = dNa[0] * pk[pk_a_index<SYM>(i, c)] + dNa[1] * pk[pk_a_index<SYM>(i + 3, c)] +
+                       dNa[2] * pk[pk_a_index<SYM>(i + 6, c)];
+#pragma unroll
+            for (int bb = 0; bb < 4; ++bb)
+#pragma unroll
+                for (int i = 0; i < 3; ++i) acc[bb][3 * i + k] += u[i] * dNb[bb][l];
         }
-        __syncthreads();
-        double dNa[3], dNb[3];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            dNa[j] = pk[LEN + 3 * a + j];
-            dNb[j] = pk[LEN + 3 * b + j];
-        }
-        const double *ta = T + 27 * a;
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-                acc[3 * i + k] += ta[9 * i + k] * dNb[0] + ta[9 * i + k + 3] * dNb[1] + ta[9 * i + k + 6] * dNb[2];
         if (fbar) {
-            double h[3], gd[3];
+            double h[3], Fi[9];
 #pragma unroll
-            for (int i = 0; i < 3; ++i)
+            for (int i = 0; i < 3; ++i) {
                 h[i] = dNa[0] * pk[NA + i] + dNa[1] * pk[NA + 3 + i] + dNa[2] * pk[NA + 6 + i];
+                H[i] += h[i];
+            }
 #pragma unroll
-            for (int k = 0; k < 3; ++k)
-                gd[k] = g0b[k] - (dNb[0] * pk[NA + 9 + 3 * k] + dNb[1] * pk[NA + 10 + 3 * k] +
-                                  dNb[2] * pk[NA + 11 + 3 * k]);
+            for (int t = 0; t < 9; ++t) Fi[t] = pk[NA + 9 + t];
 #pragma unroll
-            for (int i = 0; i < 3; ++i)
+            for (int bb = 0; bb < 4; ++bb)
 #pragma unroll
-                for (int k = 0; k < 3; ++k) acc[3 * i + k] += h[i] * gd[k];
+                for (int k = 0; k < 3; ++k) {
+                    const double gJ = dNb[bb][0] * Fi[3 * k] + dNb[bb][1] * Fi[3 * k + 1] + dNb[bb][2] * Fi[3 * k + 2];
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) acc[bb][3 * i + k] -= h[i] * gJ;
+                }
         }
-        if (b == 0)
+        if (half == 0)
 #pragma unroll
             for (int i = 0; i < 3; ++i)
                 fe[i] += dNa[0] * pk[NA + 18 + i] + dNa[1] * pk[NA + 21 + i] + dNa[2] * pk[NA + 24 + i];
         if (g < 7) {
-            if (lane < NC) reinterpret_cast<double2 *>(sp[(g + 1) & 1])[lane] = r0;
-            if (two) reinterpret_cast<double2 *>(sp[(g + 1) & 1])[64 + lane] = r1;
+#pragma unroll
+            for (int k = 0; k < NCH; ++k)
+                if (lane + 64 * k < 4 * NC) reinterpret_cast<double2 *>(sp[(g + 1) & 1])[lane + 64 * k] = rb[k];
         }
         __syncthreads();
     }
+    if (!act) return;
+    if (fbar)
+#pragma unroll
+        for (int bb = 0; bb < 4; ++bb)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double g0k = g0[24 * e + 3 * (4 * half + bb) + k];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) acc[bb][3 * i + k] += g0k * H[i];
+            }
     // the block's first contribution in colour order is stored (K is not cleared
     // before the assembly), every later one added
-    double *kb = Kv + 9 * (int64_t)slots[64 * e + 8 * a + b];
-    if ((first[e] >> (8 * a + b)) & 1) {
+    const uint64_t fm = first[e];
 #pragma unroll
-        for (int t = 0; t < 9; ++t) kb[t] = acc[t];
-    } else {
+    for (int bb = 0; bb < 4; ++bb) {
+        const int b = 4 * half + bb;
+        double *kb = Kv + 9 * (int64_t)slots[64 * e + 8 * a + b];
+        if ((fm >> (8 * a + b)) & 1) {
 #pragma unroll
-        for (int t = 0; t < 9; ++t) kb[t] += acc[t];
+            for (int t = 0; t < 9; ++t) kb[t] = acc[bb][t];
+        } else {
+#pragma unroll
+            for (int t = 0; t < 9; ++t) kb[t] += acc[bb][t];
+        }
     }
-    if (b == 0) {
+    if (half == 0) {
         const int64_t n = conn[8 * e + a];
 #pragma unroll
         for (int i = 0; i < 3; ++i) R[3 * n + i] += fe[i];
@@ -1038,11 +1061,11 @@ void launch_scatter_kernel(const ElementArgs &a, const int64_t *elem_list, int64
                            bool local) {
     if (count <= 0) return;
     if (packet_sym(a))
-        hipLaunchKernelGGL(scatter_kernel<true>, dim3((unsigned)count), dim3(64), 0, s, a.packet, a.geo, a.g0,
-                           elem_list, count, a.conn, slots, first, Kv, R, a.fbar, (int)local);
+        hipLaunchKernelGGL(scatter_kernel<true>, dim3((unsigned)((count + 3) / 4)), dim3(64), 0, s, a.packet, a.geo,
+                           a.g0, elem_list, count, a.conn, slots, first, Kv, R, a.fbar, (int)local);
     else
-        hipLaunchKernelGGL(scatter_kernel<false>, dim3((unsigned)count), dim3(64), 0, s, a.packet, a.geo, a.g0,
-                           elem_list, count, a.conn, slots, first, Kv, R, a.fbar, (int)local);
+        hipLaunchKernelGGL(scatter_kernel<false>, dim3((unsigned)((count + 3) / 4)), dim3(64), 0, s, a.packet, a.geo,
+                           a.g0, elem_list, count, a.conn, slots, first, Kv, R, a.fbar, (int)local);
     PF_HIP(hipGetLastError());
 }
 
