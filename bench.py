@@ -163,6 +163,48 @@ def vcycle_figures(m, prec, reps=20):
             "smoother_step_GBps": st_gbs, "smoother_step_frac": st_gbs / HBM_PEAK_GBS}
 
 
+def fp32_operator_error(pa, wl, args, report=(1, 2, 5, 10, 20, 40)):
+    """For the first Newton system of selected load steps of the workload's path: the
+    correction dU (FP64 solve to rtol 1e-10) and e = ||(fl32(K) - K) dU|| / ||K dU||, the
+    relative residual that an exact solve with the FP32-rounded K leaves -- the contraction
+    per refinement sweep of a Krylov iteration on the FP32 K.  A random vector is shown
+    beside it: rounding K to FP32 moves K p by ~1e-7 relative for rough p, far more for
+    the smooth corrections of an ill-conditioned tangent."""
+    import scipy.sparse as sps
+    opts = pa.default_options(eta_min=wl.krylov_rtol, eta_max=wl.krylov_rtol)
+    mats = [pa.J2Material(**d) for d in wl.materials]
+    m = pa.Model(wl.mesh, mats if len(mats) > 1 else mats[0], options=opts, material_id=wl.material_id)
+    m.constrain(wl.dir_dofs, wl.dir_vals, wl.dir_ramp)
+    m.reset()
+    m._freeze()
+    rng = np.random.default_rng(0)
+    rows = []
+    done = 0
+    for k in sorted(r for r in report if r <= wl.nsteps):
+        for j in range(done + 1, k):
+            m.advance(j / wl.nsteps, wl.nsteps, tol=wl.newton_tol, maxiter=args.maxiter, cutback=args.cutback)
+        done = k - 1
+        lam = k / wl.nsteps
+        m.assemble()
+        m.subtract_external(lam)
+        m.impose_dirichlet(lam)
+        b = -m.residual()
+        dU = m.linear_solve(1e-10)[0]
+        rp, ci, v = m.get_K_bsr()
+        K = sps.bsr_matrix((v, ci, rp), shape=(m.ndof, m.ndof))
+        K32 = sps.bsr_matrix((v.astype(np.float32).astype(np.float64), ci, rp), shape=(m.ndof, m.ndof))
+        y = K @ dU
+        p = rng.standard_normal(m.ndof)
+        row = {"load_step": k, "solve_relres": float(np.linalg.norm(b - y) / np.linalg.norm(b)),
+               "fp32_K_error_on_dU": float(np.linalg.norm(K32 @ dU - y) / np.linalg.norm(y)),
+               "fp32_K_error_on_random": float(np.linalg.norm(K32 @ p - K @ p) / np.linalg.norm(K @ p))}
+        del K, K32, v
+        rows.append(row)
+        log(f"[bench] fp32 operator, load step {k}: {row}")
+    print(json.dumps({"workload": wl.name, "ndof": m.ndof, "rows": rows}), flush=True)
+    m.close()
+
+
 class PathTotals:
     """Sums over the load increments of timed load steps (advance records)."""
 
@@ -297,10 +339,12 @@ def main():
                          "0 = FP64 flexible CG with the FP32 V-cycle")
     ap.add_argument("--fp16-fine", action="store_true",
                     help="FP32-IR: fine-level smoother products with an FP16 copy of the scaled K")
-    ap.add_argument("--mode", default="newton", choices=["newton", "spmv", "assemble", "copy"],
+    ap.add_argument("--mode", default="newton", choices=["newton", "spmv", "assemble", "copy", "fp32-operator"],
                     help="newton: the metric; spmv: only repeated fine SpMVs (for counter runs); assemble: only "
                          "repeated tangent assemblies at a plastic state (for counter runs); "
-                         "copy: a plain 1 GiB device-to-device copy, the calibration of byte counters")
+                         "copy: a plain 1 GiB device-to-device copy, the calibration of byte counters; "
+                         "fp32-operator: how far the FP32-rounded tangent moves K dU for the Newton corrections of "
+                         "the load path (the limit of a Krylov iteration on the FP32 K)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this implementation runs the Newton-step path on one GPU only; --gpus must be 1 "
@@ -344,6 +388,8 @@ def main():
         log(f"[bench] assembly x{reps} at 1% compression: gp kernel {gp * 1e3:.3f} ms + scatter {sc * 1e3:.3f} ms "
             f"per assembly ({wl.mesh.nelem} elements)")
         return
+    if args.mode == "fp32-operator":
+        return fp32_operator_error(pa, wl, args)
     log(f"[bench] workload: {wl.name}, nodes {wl.mesh.nnodes}, dof {3 * wl.mesh.nnodes}")
     opts = pa.default_options(eta_min=wl.krylov_rtol, eta_max=wl.krylov_rtol, precision=args.precision,
                               verbose=args.verbose, amg_rebuild=args.amg_rebuild, fp16_fine=args.fp16_fine,

@@ -250,7 +250,9 @@ int pfem_residual_norm(pfem_ctx *ctx, double *out);
  * 7 FP32-IR finished by flexible GCR(16) with the FP32 AMG cycle after a CG breakdown,
  * 8 FP64 finished by flexible GCR(16) with the AMG cycle after PCG failed, 9 FP32-IR with
  * the Krylov iteration in FP32 inside FP64 refinement (fp32_krylov), 10 FP32-IR flexible
- * PCG with the network's correction of the FP32 V-cycle (pfem_attach_net mode 5).  The GCR
+ * PCG with the network's correction of the FP32 V-cycle (pfem_attach_net mode 5), 11 the
+ * FP32 Krylov iteration (9) finished by FP64 flexible PCG with the FP32 cycle after its
+ * recursion broke down at its rounding level.  The GCR
  * fallbacks (7, 8) are the default; krylov_fallback = 1 selects BiCGStab (2, 4).
  * true_relres = ||b - K dU|| / ||b||.
  * A solve that ends with ||b - K dU|| > ||b|| returns dU = 0 (not applied); its status

@@ -576,11 +576,19 @@ static int ir_solve(pfem_ctx *c, const double *b, double *x, double rtol, int it
 // CG (FCG(1), as fpcg) with the FP32 copy of K, FP32 vectors and the FP32 V-cycle -- while
 // the solution accumulates in FP64 and every refinement sweep starts from the FP64
 // residual b - K x.  A sweep stops when its preconditioned residual norm has fallen by
-// IR_TAU (an FP32 recursion tracks the true residual over about five decades) or when it
-// meets the target; a target inside that range is accepted without another sweep.  Stop
-// test as pcg(): sqrt(r'z) <= rtol sqrt(r0'z0) with r0 = b.  0 solved, 1 itmax, 2 breakdown.
-static int ir32_fcg(pfem_ctx *c, const double *b, double *x, double rtol, int itmax, int *iters) {
+// IR_TAU (an FP32 recursion tracks the true residual over about five decades), when it
+// meets the target, or when the recursion loses definiteness near its rounding level
+// (p'Kp <= 0 or r'z < 0: the iteration's update is undone or kept, and the next sweep
+// restarts from the FP64 residual).  Convergence is only ever accepted at the start of a
+// sweep, on the FP64 residual: sqrt(r'z) <= rtol sqrt(r0'z0) with r0 = b, the test of
+// pcg().  0 solved; 1 itmax; 2 breakdown before any progress (x = 0); 4 stopped short
+// with progress in x (a breakdown at the first iteration of a later sweep, or the sweep
+// cap): *rn_left = sqrt(r'z) of the FP64 residual at x and *eps the absolute target, so
+// a caller can finish the solve from x.
+static int ir32_fcg(pfem_ctx *c, const double *b, double *x, double rtol, int itmax, int *iters, double *rn_left,
+                    double *eps_out) {
     constexpr double IR_TAU = 1e-5;
+    constexpr int MAX_SWEEPS = 16;
     hipStream_t s = c->stream;
     const int64_t n = c->ndof;
     float *r = c->f_r.p, *z = c->f_z.p, *p = c->f_p.p, *q = c->f_q.p;
@@ -589,7 +597,7 @@ static int ir32_fcg(pfem_ctx *c, const double *b, double *x, double rtol, int it
     fill<double>(n, 0.0, x, s);
     *iters = 0;
     double eps = 0.0;
-    for (int sweep = 0; sweep < 16; ++sweep) {
+    for (int sweep = 0;; ++sweep) {
         if (sweep == 0) {
             convert<double, float>(n, b, r, s);
         } else {  // FP64 refinement residual
@@ -599,17 +607,19 @@ static int ir32_fcg(pfem_ctx *c, const double *b, double *x, double rtol, int it
         amg_apply32(c, c->amg.get(), r, z);
         dot_dev<float, float>(n, r, z, c->partial.p, sc + 0, s);
         const double rz = c->sync_read(sc + 0);
-        if (!(rz >= 0)) return 2;
+        if (!(rz >= 0)) return sweep == 0 ? 2 : 4;
         const double rn = std::sqrt(rz);
+        *rn_left = rn;
         if (sweep == 0) {
             if (rn == 0.0) return 0;
             eps = rtol * rn;
+            *eps_out = eps;
         } else if (rn <= eps) {
             return 0;
         }
+        if (sweep == MAX_SWEEPS) return 4;
         const double stop = std::max(eps, IR_TAU * rn);
         PF_HIP(hipMemcpyAsync(p, z, n * sizeof(float), hipMemcpyDeviceToDevice, s));
-        bool reached = false;
         for (int k = 0;; ++k) {
             if (*iters >= itmax) return 1;
             ++*iters;
@@ -628,36 +638,28 @@ static int ir32_fcg(pfem_ctx *c, const double *b, double *x, double rtol, int it
             const double pq = c->h_scal[1], rzn = c->h_scal[2];
             if (!(pq > 0)) {
                 if (c->opt.verbose > 1) fprintf(stderr, "      ir32 p'Kp = %.3e at sweep it %d\n", pq, k + 1);
-                if (k == 0 || !std::isfinite(pq) || pq == 0.0) return 2;  // undone with the whole solve
-                // a sweep that had made progress: undo this iteration's x += a p, a = r'z / p'q,
-                // and restart from the FP64 residual
-                axpy_f32_to_f64(n, -c->h_scal[0] / pq, p, x, s);
-                break;
+                // undo this iteration's x += a p, a = r'z / p'q (when a is finite)
+                if (std::isfinite(pq) && pq != 0.0) axpy_f32_to_f64(n, -c->h_scal[0] / pq, p, x, s);
+                if (k == 0) return sweep == 0 ? 2 : 4;
+                break;  // the next sweep restarts from the FP64 residual
             }
             if (!(rzn >= 0)) {
                 // the FP32 recursion has lost definiteness (near its rounding level, or an
-                // indefinite cycle): x holds this iteration's update, so a sweep that had made
+                // indefinite cycle): x keeps this iteration's update, a sweep that had made
                 // progress ends here and the next one restarts from the FP64 residual
                 if (c->opt.verbose > 1) fprintf(stderr, "      ir32 r'z = %.3e at sweep it %d\n", rzn, k + 1);
-                if (k == 0) return 2;
+                if (k == 0) return sweep == 0 ? 2 : 4;
                 break;
             }
             const double nrm = std::sqrt(rzn);
             if (c->opt.verbose > 2) fprintf(stderr, "      ir32 sweep %d it %3d  %.3e  (stop %.3e)\n", sweep, *iters, nrm, stop);
-            if (nrm <= stop) {
-                reached = nrm <= eps;
-                break;
-            }
+            if (nrm <= stop) break;  // checked on the FP64 residual at the next sweep's start
             const double beta = -c->h_scal[4] / pq;  // p = z + beta p
             axpby<float>(n, 1.0, z, beta, p, s);
             PF_HIP(hipMemcpyAsync(sc + 0, sc + 2, sizeof(double), hipMemcpyDeviceToDevice, s));
         }
-        if (c->opt.verbose > 1)
-            fprintf(stderr, "      ir32 sweep %d: start %.3e, %d its so far, target %s\n", sweep, rn, *iters,
-                    reached ? "reached" : "not reached");
-        if (reached && eps >= IR_TAU * rn) return 0;
+        if (c->opt.verbose > 1) fprintf(stderr, "      ir32 sweep %d: start %.3e, %d its so far\n", sweep, rn, *iters);
     }
-    return 1;
 }
 
 // FP32 workspaces and the FP32 copy of the imposed fine K.
@@ -762,7 +764,28 @@ int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *us
         which = 3;
         if (c->opt.fp32_krylov && !c->net) {
             which = 9;
-            rc = ir32_fcg(c, c->rhs.p, c->dU.p, rtol, itmax, &it);
+            double rn_left = 0.0, eps_abs = 0.0;
+            rc = ir32_fcg(c, c->rhs.p, c->dU.p, rtol, itmax, &it, &rn_left, &eps_abs);
+            if (rc == 2) {
+                // the FP32 recursion broke down before any progress: the FP64 flexible CG
+                // with the FP32 cycle solves from scratch (a breakdown of the FP32
+                // recursion says nothing about the hierarchy, so policy 6 keeps it)
+                total += it;
+                which = 3;
+                rc = fpcg(c, c->rhs.p, c->dU.p, rtol, itmax, true, 0, &it, Amg32Pc{c});
+            } else if (rc == 4) {
+                // the FP32 recursion stopped short of the target (a breakdown at its rounding
+                // level): FP64 flexible CG with the FP32 cycle solves for the remaining
+                // residual, to the same absolute target, and its correction is added
+                total += it;
+                which = 11;
+                if (c->fcg_P.n < (size_t)2 * c->ndof) c->fcg_P.alloc((size_t)2 * c->ndof);
+                double *b2 = c->fcg_P.p, *x2 = c->fcg_P.p + c->ndof;
+                residual_fine64(c->K, c->dU.p, c->rhs.p, b2, s);
+                const double rt = std::min(0.5, eps_abs / rn_left);
+                rc = fpcg(c, b2, x2, rt, itmax, true, 0, &it, Amg32Pc{c});
+                axpby<double>(c->ndof, 1.0, x2, 1.0, c->dU.p, s);
+            }
         } else if (c->net && c->net_mode == 1) {
             rc = fpcg(c, c->rhs.p, c->dU.p, rtol, itmax, true, 0, &it, LoggingPc<Amg32Pc>{c, Amg32Pc{c}});
         } else if (c->net && c->net_mode == 4) {
@@ -944,7 +967,10 @@ int pfem_newton(pfem_ctx *c, double lambda, double tol, int maxiter, pfem_newton
             res->solves++;
             // policy 6: a solve that needed a fallback (PCG retried after a refresh, the
             // BiCGStab ladder, or BiCGStab refinement) rebuilds the prolongators next time
-            if (policy == 6) c->refresh_P = (used == 1 || used == 2 || used == 4 || used == 7 || used == 8);
+            // (not under fp32_krylov: its fallbacks follow breakdowns of the FP32 recursion,
+            // and prolongators rebuilt from a plastic tangent then failed the next load step)
+            if (policy == 6)
+                c->refresh_P = !c->opt.fp32_krylov && (used == 1 || used == 2 || used == 4 || used == 7 || used == 8);
             if (c->opt.verbose)
                 fprintf(stderr, "    iter %d  eta = %.2e  krylov = %d (solver %d, true relres %.2e)%s\n", it, eta,
                         niter, used, rel, dropped ? " discarded" : "");
