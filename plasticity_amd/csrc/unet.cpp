@@ -197,14 +197,13 @@ void net_log_error_begin(pfem_net *n, const double *z, const double *x, const ui
     const int64_t m = 3 * n->nvox[0];
     if (!n->pend_u.p) {
         n->pend_u.alloc((size_t)pfem_net::MAX_PEND * m);
+        n->pend_in.alloc((size_t)pfem_net::MAX_PEND * m);
         n->pend_scale.alloc(pfem_net::MAX_PEND);
     }
     const int k = n->npend++;
-    const int64_t slot = (n->next + k) % n->capacity;
-    n->pend_slot[k] = slot;
     double *u = n->pend_u.p + (size_t)k * m, *sc = n->pend_scale.p + k;
     launch_absmax(m, z, mask, n->red.p, sc, s);
-    launch_dofs_to_pair(m, z, mask, sc, n->pairs.p + slot * 2 * m, s);
+    launch_dofs_to_pair(m, z, mask, sc, n->pend_in.p + (size_t)k * m, s);
     PF_HIP(hipMemcpyAsync(u, z, m * sizeof(double), hipMemcpyDeviceToDevice, s));
     axpby<double>(m, 1.0, x, 1.0, u, s);  // u = x_k + z_k
 }
@@ -212,9 +211,12 @@ void net_log_error_begin(pfem_net *n, const double *z, const double *x, const ui
 void net_log_error_finish(pfem_net *n, const double *xstar, const uint8_t *mask, bool keep, hipStream_t s) {
     const int64_t m = 3 * n->nvox[0];
     if (keep) {
-        for (int k = 0; k < n->npend; ++k)
-            launch_diff_to_pair(m, xstar, n->pend_u.p + (size_t)k * m, mask, n->pend_scale.p + k,
-                                n->pairs.p + n->pend_slot[k] * 2 * m + m, s);
+        for (int k = 0; k < n->npend; ++k) {
+            uint16_t *slot = n->pairs.p + ((n->next + k) % n->capacity) * 2 * m;
+            PF_HIP(hipMemcpyAsync(slot, n->pend_in.p + (size_t)k * m, m * sizeof(uint16_t), hipMemcpyDeviceToDevice,
+                                  s));
+            launch_diff_to_pair(m, xstar, n->pend_u.p + (size_t)k * m, mask, n->pend_scale.p + k, slot + m, s);
+        }
         n->next = (n->next + n->npend) % n->capacity;
         n->npairs = std::min<int64_t>(n->npairs + n->npend, n->capacity);
     }

@@ -97,11 +97,12 @@ struct pfem_net {
     bool packed = false;
     int conv_kind = 1;  // 3x3x3 layers: 1 LDS-tiled 16x16x32 kernel, 0 per-voxel-wave 16x16x16 kernel
     int loss_kind = 0;  // training loss: 0 = 1 - cos(N(x), t), 1 = ||N(x) - t||^2 / ||t||^2
-    // samples of an error-logging solve awaiting its solution: pair-store slot, x_k + z_k
-    // (FP64, pend_u) and the input's scale (pend_scale)
+    // samples of an error-logging solve awaiting its solution: the BF16 input (pend_in),
+    // x_k + z_k (FP64, pend_u) and the input's scale (pend_scale); they enter the pair
+    // store only when the solve has converged
     static constexpr int MAX_PEND = 16;
     int npend = 0;
-    int64_t pend_slot[MAX_PEND] = {};
+    pfem::dbuf<uint16_t> pend_in;
     pfem::dbuf<double> pend_u, pend_scale;
     // (residual, correction) sample store: capacity pairs of 2 x 3*nvox[0] BF16
     int64_t capacity = 0, npairs = 0, next = 0;
