@@ -412,13 +412,10 @@ void amg_numeric_setup(pfem_ctx *c, Amg *a, bool keep_P) {
         c->sync();
         const double tr = now();
         // power iteration converges from below, slowly where the top of the spectrum is
-        // crowded (12 steps gave 0.91 rho on a notched bar): 25 steps cold, 4 warm.  A
-        // rebuild of the prolongators (after a solve that needed a fallback) takes the cold
-        // count too: a low rho there makes both the smoothed P and the Chebyshev smoother
-        // amplify the top of the spectrum, and every later solve inherits that P
+        // crowded (12 steps gave 0.91 rho on a notched bar): 25 steps cold, 4 warm
         const int cold = c->opt.rho_iters_cold > 0 ? c->opt.rho_iters_cold : 25;
         const int warm = c->opt.rho_iters_warm > 0 ? c->opt.rho_iters_warm : 4;
-        L.rho = spectral_radius(c, a, (int)l, (L.pw_warm && keep_P) ? warm : cold);
+        L.rho = spectral_radius(c, a, (int)l, L.pw_warm ? warm : cold);
         t_rho += now() - tr;
         const double omega = (4.0 / 3.0) / L.rho;
         AmgLevel &N = a->L[l + 1];
