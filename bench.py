@@ -232,7 +232,7 @@ def time_variant(pa, wl, precision, learned, warmup, steps, args):
     opts = pa.default_options(eta_min=wl.krylov_rtol, eta_max=wl.krylov_rtol, precision=precision,
                               amg_rebuild=args.amg_rebuild, newton_diverge=args.diverge,
                               fp32_krylov=args.fp32_krylov, assembly_chunk=args.assembly_chunk,
-                              smoother_unfused=args.smoother_unfused)
+                              smoother_unfused=args.smoother_unfused, coarse_degree=args.coarse_degree)
     mats = [pa.J2Material(**d) for d in wl.materials]
     m = pa.Model(wl.mesh, mats if len(mats) > 1 else mats[0], options=opts, material_id=wl.material_id)
     m.constrain(wl.dir_dofs, wl.dir_vals, wl.dir_ramp)
@@ -310,6 +310,8 @@ def main():
     ap.add_argument("--smoother-degree", type=int, default=0,
                     help="Chebyshev degree / Jacobi sweeps (0: library default, 2 / 1)")
     ap.add_argument("--cycle", default="v", choices=["v", "w"], help="AMG cycle")
+    ap.add_argument("--coarse-degree", type=int, default=0,
+                    help="Chebyshev degree on the levels below the finest (0: the fine level's)")
     ap.add_argument("--fallback", default="gcr", choices=["bicgstab", "gcr"],
                     help="Krylov method after a CG breakdown")
     ap.add_argument("--cutback", type=int, default=6,
@@ -398,7 +400,7 @@ def main():
                               wcycle=args.cycle == "w",
                               gcr_fallback=args.fallback == "gcr", newton_diverge=args.diverge,
                               fp32_krylov=args.fp32_krylov, assembly_chunk=args.assembly_chunk,
-                              smoother_unfused=args.smoother_unfused)
+                              smoother_unfused=args.smoother_unfused, coarse_degree=args.coarse_degree)
     t0 = time.perf_counter()
     mats = [pa.J2Material(**d) for d in wl.materials]
     m = pa.Model(wl.mesh, mats if len(mats) > 1 else mats[0], options=opts, material_id=wl.material_id)
@@ -556,7 +558,7 @@ def main():
                                   "rebuilt every Newton iteration", 5: "prolongators built once, the rest rebuilt "
                                   "every Newton iteration", 6: "prolongators kept until a solve falls back, the rest "
                                   "rebuilt every Newton iteration"}.get(args.amg_rebuild, "?") + ")",
-                   "amg_cycle": args.cycle.upper(),
+                   "amg_cycle": args.cycle.upper(), "coarse_degree": args.coarse_degree,
                    "newton_steps_timed": newton, "newton_steps_accepted": tot.accepted,
                    "krylov_its_timed": krylov,
                    "load_increments_timed": tot.increments, "cutbacks_timed": tot.cutbacks,

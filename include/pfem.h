@@ -132,7 +132,9 @@ typedef struct pfem_options {
                                    kernel over the whole mesh, then the scatter colour by colour. */
     int32_t smoother_unfused;   /* 1 = the fine-level Chebyshev steps as a product launch followed by a vector
                                    launch; 0 = one launch that does both (default; same results bit for bit) */
-    int32_t reserved[6];        /* zero */
+    int32_t coarse_degree;      /* Chebyshev degree / Jacobi sweeps on the levels below the finest (<= 0: the
+                                   fine level's) */
+    int32_t reserved[5];        /* zero */
 } pfem_options;
 
 typedef struct pfem_fail {

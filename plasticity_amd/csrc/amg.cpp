@@ -164,6 +164,7 @@ Amg *amg_create(pfem_ctx *c) {
     // iteration, but with them kept (rebuild policies 4-6) the V-cycle is faster overall.
     a->wcycle = c->opt.amg_wcycle != 0;
     a->degree = c->opt.smoother_degree > 0 ? c->opt.smoother_degree : (a->smoother == PFEM_SMOOTH_JACOBI ? 1 : 2);
+    a->coarse_degree = c->opt.coarse_degree > 0 ? c->opt.coarse_degree : a->degree;
     const int64_t coarse_max = c->opt.coarse_max > 0 ? c->opt.coarse_max : 1500;
     const int max_levels = c->opt.max_levels > 1 ? c->opt.max_levels : 10;
 
@@ -583,10 +584,11 @@ template <class T>
 void smooth(pfem_ctx *c, Amg *a, int l, const T *b, T *x, bool zero_init) {
     AmgLevel &L = a->L[l];
     hipStream_t s = c->stream;
+    const int deg = a->degree_at(l);
     T *r = Vec<T>::r(L), *d = Vec<T>::d(L), *t = Vec<T>::t(L);
     if (a->smoother == PFEM_SMOOTH_JACOBI) {
         const double w = (4.0 / 3.0) / L.rho;
-        for (int it = 0; it < a->degree; ++it) {
+        for (int it = 0; it < deg; ++it) {
             if (zero_init && it == 0) {
                 diag_apply<T>(L, b, x, w, s);
                 continue;
@@ -602,9 +604,9 @@ void smooth(pfem_ctx *c, Amg *a, int l, const T *b, T *x, bool zero_init) {
         const double lam = 1.1 * L.rho;
         if (!zero_init) level_residual<T>(c, L, l, x, b, r);
         diag_first_step<T>(L, zero_init ? b : r, zero_init ? r : nullptr, d, x, 4.0 / (3.0 * lam), zero_init, s);
-        for (int i = 1; i < a->degree; ++i) {
+        for (int i = 1; i < deg; ++i) {
             const double c1 = (2.0 * i - 1.0) / (2.0 * i + 3.0), c2 = (8.0 * i + 4.0) / ((2.0 * i + 3.0) * lam);
-            cheb_update<T>(c, L, l, d, t, r, x, c1, c2, i + 1 == a->degree);
+            cheb_update<T>(c, L, l, d, t, r, x, c1, c2, i + 1 == deg);
         }
         return;
     }
@@ -614,10 +616,10 @@ void smooth(pfem_ctx *c, Amg *a, int l, const T *b, T *x, bool zero_init) {
     double rho_old = 1.0 / sigma;
     if (!zero_init) level_residual<T>(c, L, l, x, b, r);
     diag_first_step<T>(L, zero_init ? b : r, zero_init ? r : nullptr, d, x, 1.0 / theta, zero_init, s);
-    for (int k = 2; k <= a->degree; ++k) {
+    for (int k = 2; k <= deg; ++k) {
         const double rho_new = 1.0 / (2.0 * sigma - rho_old);
         const double c1 = rho_new * rho_old, c2 = 2.0 * rho_new / delta;
-        cheb_update<T>(c, L, l, d, t, r, x, c1, c2, k == a->degree);
+        cheb_update<T>(c, L, l, d, t, r, x, c1, c2, k == deg);
         rho_old = rho_new;
     }
 }
@@ -703,15 +705,16 @@ double smooth_bytes(pfem_ctx *c, Amg *a, int l, double s, bool zero_init) {
     const AmgLevel &L = a->L[l];
     const double A = op_bytes(c, L, l, s), v = (double)L.n * s, D = (double)L.nb * L.bs * L.bs * s;
     const bool fused = l == 0 && !c->opt.smoother_unfused && !(s == 4.0 && c->half_fine);
+    const int deg = a->degree_at(l);
     if (a->smoother == PFEM_SMOOTH_JACOBI) {
         double b = 0;
-        for (int it = 0; it < a->degree; ++it)
+        for (int it = 0; it < deg; ++it)
             b += (zero_init && it == 0) ? 2 * v + D : (A + 3 * v) + (4 * v + D);
         return b;
     }
     double b = zero_init ? 4 * v + D : (A + 3 * v) + (4 * v + D);  // first step
-    for (int k = 2; k <= a->degree; ++k)
-        b += fused ? A + (k == a->degree ? 4 : 6) * v + D : (A + 2 * v) + (7 * v + D);
+    for (int k = 2; k <= deg; ++k)
+        b += fused ? A + (k == deg ? 4 : 6) * v + D : (A + 2 * v) + (7 * v + D);
     return b;
 }
 double cycle_bytes(pfem_ctx *c, Amg *a, int l, double s) {

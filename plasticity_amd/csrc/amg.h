@@ -71,7 +71,9 @@ struct Amg {
     dbuf<double> gj_work, gj_prow, gj_fcol, gj_dw;  // Gauss-Jordan scratch
     dbuf<int> gj_iw;
     int smoother = PFEM_SMOOTH_CHEBYSHEV;
-    int degree = 2;
+    int degree = 2;         // fine level
+    int coarse_degree = 2;  // levels below the finest
+    int degree_at(int l) const { return l == 0 ? degree : coarse_degree; }
     bool wcycle = false;
     bool have32 = false;
     double op_complexity = 1.0;
