@@ -346,33 +346,50 @@ void graph_reset(Amg *a) {
 void amg_destroy(Amg *a) {
     if (!a) return;
     graph_reset(a);
+    for (hipEvent_t e : a->ev_level)
+        if (e) (void)hipEventDestroy(e);
+    if (a->ev_rho) (void)hipEventDestroy(a->ev_rho);
+    if (a->s_rho) (void)hipStreamDestroy(a->s_rho);
     delete a;
 }
 
 // ---------------------------------------------------------------- numeric set-up
 namespace {
 
-// rho(D^-1 A) by power iteration.  The iterate is kept between set-ups, so later
-// rebuilds (a slowly changing tangent) start from the previous dominant direction and
-// need few iterations; normalisation happens on the device, one host read at the end.
-double spectral_radius(pfem_ctx *c, Amg *a, int l, int iters) {
+// rho(D^-1 A) by power iteration, launched on stream s with the reduction scratch
+// `partial` and ||D^-1 A x||^2 left in *nrm (device).  The iterate is kept between
+// set-ups, so later rebuilds (a slowly changing tangent) start from the previous
+// dominant direction and need few iterations; normalisation happens on the device.
+void power_iteration(pfem_ctx *c, Amg *a, int l, int iters, hipStream_t s, double *partial, double *nrm) {
     AmgLevel &L = a->L[l];
-    hipStream_t s = c->stream;
     double *x = L.pw.p, *y = L.r.p;
-    double *nrm = c->scal.p + 10;
-    dot_dev<double, double>(L.n, x, x, c->partial.p, nrm, s);
+    dot_dev<double, double>(L.n, x, x, partial, nrm, s);
     for (int it = 0; it < iters; ++it) {
         scale_by_rnorm(L.n, nrm, x, s);  // ||x|| = 1
         if (l == 0) sell_spmv<3, double, double>(c->K, c->K.v64.p, x, L.t.p, s);
         else bsr_spmv<6, 6, double, double>(L.nb, L.rowptr.p, L.colind.p, L.A.p, x, L.t.p, 0.0, s);
         if (L.bs == 3) block_diag_apply<3, double, double>(L.nb, L.dinv.p, L.t.p, y, 1.0, s);
         else block_diag_apply<6, double, double>(L.nb, L.dinv.p, L.t.p, y, 1.0, s);
-        dot_dev<double, double>(L.n, y, y, c->partial.p, nrm, s);  // ||D^-1 A x||^2
+        dot_dev<double, double>(L.n, y, y, partial, nrm, s);  // ||D^-1 A x||^2
         PF_HIP(hipMemcpyAsync(x, y, L.n * sizeof(double), hipMemcpyDeviceToDevice, s));
     }
-    const double lam = std::sqrt(c->sync_read(nrm));
     L.pw_warm = true;
+}
+double rho_of(double nrm2) {
+    const double lam = std::sqrt(nrm2);
     return lam > 0 ? lam : 1.0;
+}
+double spectral_radius(pfem_ctx *c, Amg *a, int l, int iters) {
+    double *nrm = c->scal.p + 10;
+    power_iteration(c, a, l, iters, c->stream, c->partial.p, nrm);
+    return rho_of(c->sync_read(nrm));
+}
+int rho_iters(pfem_ctx *c, const AmgLevel &L) {
+    // power iteration converges from below, slowly where the top of the spectrum is
+    // crowded (12 steps gave 0.91 rho on a notched bar): 25 steps cold, 4 warm
+    const int cold = c->opt.rho_iters_cold > 0 ? c->opt.rho_iters_cold : 25;
+    const int warm = c->opt.rho_iters_warm > 0 ? c->opt.rho_iters_warm : 4;
+    return L.pw_warm ? warm : cold;
 }
 
 // Dense inverse of the coarsest operator, computed on the device.
@@ -401,27 +418,44 @@ void amg_numeric_setup(pfem_ctx *c, Amg *a, bool keep_P) {
     graph_reset(a);  // smoother coefficients change with rho: captured V-cycles are stale
     hipStream_t s = c->stream;
     const bool want32 = c->opt.precision == PFEM_PREC_FP32_IR;
+    const size_t nlev = a->L.size();
+    // With the prolongators kept, rho only feeds the smoothers: each level's power
+    // iteration runs on s_rho as soon as its operator and block-diagonal inverse exist,
+    // beside the Galerkin products of the next level on the context's stream (the same
+    // kernels in the same order per stream, so the hierarchy is bit for bit the one the
+    // sequential set-up builds).  A rebuilt P needs rho for its smoothing weight first.
+    const bool overlap = keep_P && nlev > 1;
+    if (overlap && !a->s_rho) {
+        PF_HIP(hipStreamCreateWithFlags(&a->s_rho, hipStreamNonBlocking));
+        PF_HIP(hipEventCreateWithFlags(&a->ev_rho, hipEventDisableTiming));
+        a->ev_level.assign(nlev, nullptr);
+        for (auto &e : a->ev_level) PF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        a->rho_partial.alloc(c->partial.n);
+        a->rho_scal.alloc(nlev);
+    }
     double t_rho = 0.0;
     const double t_begin = now();
-    for (size_t l = 0; l < a->L.size(); ++l) {
+    for (size_t l = 0; l < nlev; ++l) {
         AmgLevel &L = a->L[l];
         const bool fine = (l == 0);
         const double *Av = fine ? c->K.v64.p : L.A.p;
         if (L.bs == 3) amg_block_inverse<3>(L.nb, fine ? c->K.diagpos.p : L.diagpos.p, Av, L.dinv.p, s);
         else amg_block_inverse<6>(L.nb, L.diagpos.p, Av, L.dinv.p, s);
-        if (l + 1 == a->L.size()) break;  // coarsest: dense solve
-        c->sync();
-        const double tr = now();
-        // power iteration converges from below, slowly where the top of the spectrum is
-        // crowded (12 steps gave 0.91 rho on a notched bar): 25 steps cold, 4 warm
-        const int cold = c->opt.rho_iters_cold > 0 ? c->opt.rho_iters_cold : 25;
-        const int warm = c->opt.rho_iters_warm > 0 ? c->opt.rho_iters_warm : 4;
-        L.rho = spectral_radius(c, a, (int)l, L.pw_warm ? warm : cold);
-        t_rho += now() - tr;
-        const double omega = (4.0 / 3.0) / L.rho;
+        if (l + 1 == nlev) break;  // coarsest: dense solve
+        if (overlap) {
+            PF_HIP(hipEventRecord(a->ev_level[l], s));
+            PF_HIP(hipStreamWaitEvent(a->s_rho, a->ev_level[l], 0));
+            power_iteration(c, a, (int)l, rho_iters(c, L), a->s_rho, a->rho_partial.p, a->rho_scal.p + l);
+        } else {
+            c->sync();
+            const double tr = now();
+            L.rho = spectral_radius(c, a, (int)l, rho_iters(c, L));
+            t_rho += now() - tr;
+        }
         AmgLevel &N = a->L[l + 1];
         if (L.bs == 3) {
             if (!keep_P) {
+                const double omega = (4.0 / 3.0) / L.rho;
                 amg_smooth_P<3, true>(L.nP, L.P_row.p, L.P_colind.p, c->K.slice_ptr.p, c->K.rowlen.p,
                                       nullptr, c->K.colind.p, Av, L.agg.p, L.Pt.p, L.dinv.p, omega, L.P.p, s);
                 amg_transpose_P<3>(L.nR, L.R_src.p, L.P.p, L.R.p, s);
@@ -432,6 +466,7 @@ void amg_numeric_setup(pfem_ctx *c, Amg *a, bool keep_P) {
                        L.Q.p, N.A.p, s);
         } else {
             if (!keep_P) {
+                const double omega = (4.0 / 3.0) / L.rho;
                 amg_smooth_P<6, false>(L.nP, L.P_row.p, L.P_colind.p, nullptr, nullptr, L.rowptr.p,
                                        L.colind.p, Av, L.agg.p, L.Pt.p, L.dinv.p, omega, L.P.p, s);
                 amg_transpose_P<6>(L.nR, L.R_src.p, L.P.p, L.R.p, s);
@@ -447,8 +482,19 @@ void amg_numeric_setup(pfem_ctx *c, Amg *a, bool keep_P) {
     coarse_inverse(c, a);
     c->sync();
     c->st.t_setup_coarse += now() - t_coarse0;
+    if (overlap) {  // the power iterations ran beside; collect their results
+        const double tr = now();
+        PF_HIP(hipEventRecord(a->ev_rho, a->s_rho));
+        PF_HIP(hipStreamWaitEvent(s, a->ev_rho, 0));
+        std::vector<double> nrm(nlev, 0.0);
+        PF_HIP(hipMemcpyAsync(nrm.data(), a->rho_scal.p, (nlev - 1) * sizeof(double), hipMemcpyDeviceToHost,
+                              a->s_rho));
+        PF_HIP(hipStreamSynchronize(a->s_rho));
+        for (size_t l = 0; l + 1 < nlev; ++l) a->L[l].rho = rho_of(nrm[l]);
+        t_rho += now() - tr;
+    }
     c->st.t_setup_rho += t_rho;
-    c->st.t_setup_galerkin += t_coarse0 - t_begin - t_rho;
+    c->st.t_setup_galerkin += t_coarse0 - t_begin - (overlap ? 0.0 : t_rho);
     if (want32) {
         for (size_t l = 0; l < a->L.size(); ++l) {
             AmgLevel &L = a->L[l];

@@ -70,6 +70,12 @@ struct Amg {
     dbuf<double> coarse_inv;
     dbuf<double> gj_work, gj_prow, gj_fcol, gj_dw;  // Gauss-Jordan scratch
     dbuf<int> gj_iw;
+    // concurrent set-up: the rho power iterations run on their own stream (with their own
+    // reduction scratch) beside the Galerkin products when the prolongators are kept
+    hipStream_t s_rho = nullptr;
+    std::vector<hipEvent_t> ev_level;  // level l's operator and block-diagonal inverse ready
+    hipEvent_t ev_rho = nullptr;
+    dbuf<double> rho_partial, rho_scal;
     int smoother = PFEM_SMOOTH_CHEBYSHEV;
     int degree = 2;         // fine level
     int coarse_degree = 2;  // levels below the finest
