@@ -232,7 +232,8 @@ def time_variant(pa, wl, precision, learned, warmup, steps, args):
     opts = pa.default_options(eta_min=wl.krylov_rtol, eta_max=wl.krylov_rtol, precision=precision,
                               amg_rebuild=args.amg_rebuild, newton_diverge=args.diverge,
                               fp32_krylov=args.fp32_krylov, assembly_chunk=args.assembly_chunk,
-                              smoother_unfused=args.smoother_unfused, coarse_degree=args.coarse_degree)
+                              smoother_unfused=args.smoother_unfused, coarse_degree=args.coarse_degree,
+                              krylov_sync=args.krylov_sync)
     mats = [pa.J2Material(**d) for d in wl.materials]
     m = pa.Model(wl.mesh, mats if len(mats) > 1 else mats[0], options=opts, material_id=wl.material_id)
     m.constrain(wl.dir_dofs, wl.dir_vals, wl.dir_ramp)
@@ -330,6 +331,9 @@ def main():
     ap.add_argument("--verbose", type=int, default=0)
     ap.add_argument("--show-residuals", action="store_true",
                     help="print each increment's Newton residual history in the per-load-step lines")
+    ap.add_argument("--krylov-sync", type=int, default=0, choices=[0, 1],
+                    help="1: PCG / flexible CG read each iteration's scalars before launching the next "
+                         "product (0: the next product is queued first; same results)")
     ap.add_argument("--smoother-unfused", type=int, default=0, choices=[0, 1],
                     help="1: fine-level Chebyshev steps as a product launch plus a vector launch")
     ap.add_argument("--assembly-chunk", type=int, default=0,
@@ -400,7 +404,8 @@ def main():
                               wcycle=args.cycle == "w",
                               gcr_fallback=args.fallback == "gcr", newton_diverge=args.diverge,
                               fp32_krylov=args.fp32_krylov, assembly_chunk=args.assembly_chunk,
-                              smoother_unfused=args.smoother_unfused, coarse_degree=args.coarse_degree)
+                              smoother_unfused=args.smoother_unfused, coarse_degree=args.coarse_degree,
+                              krylov_sync=args.krylov_sync)
     t0 = time.perf_counter()
     mats = [pa.J2Material(**d) for d in wl.materials]
     m = pa.Model(wl.mesh, mats if len(mats) > 1 else mats[0], options=opts, material_id=wl.material_id)

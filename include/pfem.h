@@ -134,7 +134,11 @@ typedef struct pfem_options {
                                    launch; 0 = one launch that does both (default; same results bit for bit) */
     int32_t coarse_degree;      /* Chebyshev degree / Jacobi sweeps on the levels below the finest (<= 0: the
                                    fine level's) */
-    int32_t reserved[5];        /* zero */
+    int32_t krylov_sync;        /* 1 = PCG / flexible CG read each iteration's scalars on the host before they
+                                   launch the next iteration; 0 = the next search direction and its product
+                                   are launched first and the stopping test overlaps them (default; the same
+                                   results bit for bit) */
+    int32_t reserved[4];        /* zero */
 } pfem_options;
 
 typedef struct pfem_fail {
@@ -162,7 +166,7 @@ int pfem_box_mesh(double lx, double ly, double lz, int64_t nx, int64_t ny, int64
  * on the device, colours the elements (the parity colours of Assembly.jl:172-188
  * for box numbering, on the device; greedy first-fit in element order otherwise,
  * on the host), and resets the Gauss state (Mesh.jl:132-138).  The pattern is
- * built on the host instead when bit 0 of opts->reserved[6] is set or when a node is shared
+ * built on the host instead when opts->host_build is set or when a node is shared
  * by more than 32 elements. */
 int pfem_create(const pfem_mesh_desc *mesh, const pfem_options *opts, pfem_ctx **out);
 void pfem_destroy(pfem_ctx *ctx);

@@ -65,7 +65,8 @@ class Options(C.Structure):
                 ("rho_iters_warm", C.c_int32), ("host_build", C.c_int32), ("vcycle_eager", C.c_int32),
                 ("prolong_rowwise", C.c_int32), ("fp32_krylov", C.c_int32), ("newton_diverge", C.c_double),
                 ("assembly_chunk", C.c_int32), ("smoother_unfused", C.c_int32),
-                ("coarse_degree", C.c_int32), ("reserved", C.c_int32 * 5)]
+                ("coarse_degree", C.c_int32), ("krylov_sync", C.c_int32),
+                ("reserved", C.c_int32 * 4)]
 
 
 class Fail(C.Structure):

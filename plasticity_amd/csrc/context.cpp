@@ -157,6 +157,8 @@ int pfem_create(const pfem_mesh_desc *md, const pfem_options *opts, pfem_ctx **o
         PF_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         PF_HIP(hipEventCreate(&c->ev0));
         PF_HIP(hipEventCreate(&c->ev1));
+        for (auto &ev : c->ev_kry) PF_HIP(hipEventCreate(&ev));
+        PF_HIP(hipEventCreateWithFlags(&c->ev_scal, hipEventDisableTiming));
         if (c->opt.assembly_chunk > 0) {
             PF_HIP(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
             for (auto &ev : c->ev_pipe) PF_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
@@ -344,6 +346,9 @@ void pfem_destroy(pfem_ctx *c) {
     if (c->h_scal) (void)hipHostFree(c->h_scal);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (hipEvent_t ev : c->ev_kry)
+        if (ev) (void)hipEventDestroy(ev);
+    if (c->ev_scal) (void)hipEventDestroy(c->ev_scal);
     for (auto ev : c->ev_pipe)
         if (ev) (void)hipEventDestroy(ev);
     hipStream_t s = c->stream, s2 = c->stream2;

@@ -84,6 +84,10 @@ struct pfem_ctx {
     // statistics
     pfem_stats st{};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // Krylov loops with the next product launched before the stopping test: two
+    // (start, end) pairs timing the products of consecutive iterations, and the event
+    // after each iteration's scalar copy to the host
+    hipEvent_t ev_kry[4] = {}, ev_scal = nullptr;
     // pipelined assembly (assembly_chunk > 0): the scatter's stream and the events that
     // order it against the Gauss-point kernels (start, gp[2], scatter[2], join)
     hipStream_t stream2 = nullptr;

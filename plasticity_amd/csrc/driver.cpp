@@ -202,10 +202,21 @@ static int pcg(pfem_ctx *c, const T *b, T *x, double rtol, int itmax, int *iters
     if (rn0 == 0.0) return 0;
     const double eps = rtol * rn0;
     PF_HIP(hipMemcpyAsync(p, z, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    // q = K p with sc[1] = p'q for iteration k, timed by the event pair of k's parity
+    auto product = [&](int k) {
+        PF_HIP(hipEventRecord(c->ev_kry[2 * (k & 1)], s));
+        Prec<T>::spmv_dot(c, p, q, sc + 1);
+        PF_HIP(hipEventRecord(c->ev_kry[2 * (k & 1) + 1], s));
+    };
+    // p = z + (rzn/rz) p, then the next product
+    auto next_direction = [&](int k) {
+        xpby_ratio<T>(n, sc + 2, sc + 0, z, p, s);
+        PF_HIP(hipMemcpyAsync(sc + 0, sc + 2, sizeof(double), hipMemcpyDeviceToDevice, s));
+        product(k);
+    };
+    product(1);
+    double nrm_prev = rn0, nrm_prev2 = 0.0;
     for (int it = 1; it <= itmax; ++it) {
-        PF_HIP(hipEventRecord(c->ev0, s));
-        Prec<T>::spmv_dot(c, p, q, sc + 1);  // q = K p, sc[1] = p'q
-        PF_HIP(hipEventRecord(c->ev1, s));
         // p'q is read with r'z, one host round trip per iteration: after a breakdown
         // (p'q <= 0) this iteration's update is discarded with the solve (every caller's
         // next solver restarts from x = 0)
@@ -213,11 +224,22 @@ static int pcg(pfem_ctx *c, const T *b, T *x, double rtol, int itmax, int *iters
         Prec<T>::pc(c, r, z);
         dot_dev<T, T>(n, r, z, c->partial.p, sc + 2, s);
         PF_HIP(hipMemcpyAsync(c->h_scal + 1, sc + 1, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-        c->sync();
+        // Unless this iteration is expected to meet the stopping test (the last rate
+        // carried forward), the next direction and its product are queued before the
+        // host waits for the scalars: x and r stay as they are if it stops here.
+        const bool ahead = !c->opt.krylov_sync && it < itmax &&
+                           !(nrm_prev2 > 0.0 && nrm_prev * (nrm_prev / nrm_prev2) <= eps);
+        if (ahead) {
+            PF_HIP(hipEventRecord(c->ev_scal, s));
+            next_direction(it + 1);
+            PF_HIP(hipEventSynchronize(c->ev_scal));
+        } else {
+            c->sync();
+        }
         const double pq = c->h_scal[1], rzn = c->h_scal[2];
         if (sizeof(T) == 8) {
             float ms = 0.f;
-            PF_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            PF_HIP(hipEventElapsedTime(&ms, c->ev_kry[2 * (it & 1)], c->ev_kry[2 * (it & 1) + 1]));
             c->st.t_pcg_spmv += 1e-3 * ms;
             c->st.n_pcg_spmv++;
         }
@@ -232,8 +254,9 @@ static int pcg(pfem_ctx *c, const T *b, T *x, double rtol, int itmax, int *iters
         }
         if (c->opt.verbose > 2) fprintf(stderr, "      pcg %3d  sqrt(rz) = %.3e  (target %.3e)\n", it, std::sqrt(rzn), eps);
         if (std::sqrt(rzn) <= eps) return 0;
-        xpby_ratio<T>(n, sc + 2, sc + 0, z, p, s);  // p = z + (rzn/rz) p
-        PF_HIP(hipMemcpyAsync(sc + 0, sc + 2, sizeof(double), hipMemcpyDeviceToDevice, s));
+        if (!ahead && it < itmax) next_direction(it + 1);
+        nrm_prev2 = nrm_prev;
+        nrm_prev = std::sqrt(rzn);
         rz = rzn;
     }
     return 1;
@@ -328,11 +351,16 @@ static int fpcg(pfem_ctx *c, const double *b, double *x, double rtol, int itmax,
     double best = std::sqrt(rr);
     int best_it = 0;
     PF_HIP(hipMemcpyAsync(p, z, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    // q = K p with sc[1] = p'q for iteration k, timed by the event pair of k's parity
+    auto product = [&](int k) {
+        PF_HIP(hipEventRecord(c->ev_kry[2 * (k & 1)], s));
+        Prec<double>::spmv_dot(c, p, q, sc + 1);
+        PF_HIP(hipEventRecord(c->ev_kry[2 * (k & 1) + 1], s));
+    };
+    product(1);
+    double nrm_prev = pc_norm ? std::sqrt(rz) : std::sqrt(rr), nrm_prev2 = 0.0;
     for (int it = 1; it <= itmax; ++it) {
         *iters = it;
-        PF_HIP(hipEventRecord(c->ev0, s));
-        Prec<double>::spmv_dot(c, p, q, sc + 1);  // q = K p, sc[1] = p'q
-        PF_HIP(hipEventRecord(c->ev1, s));
         // x += a p, r -= a q; z = M(r); sc[2] = r'z, sc[3] = r'r, sc[4] = z'q
         if constexpr (amg32) {
             cg_update_r32(n, sc + 0, sc + 1, p, q, x, r, c->f_r.p, s);
@@ -344,10 +372,24 @@ static int fpcg(pfem_ctx *c, const double *b, double *x, double rtol, int itmax,
             fcg_dots<double>(n, r, z, nullptr, q, c->mpartial.p, sc + 2, s);
         }
         PF_HIP(hipMemcpyAsync(c->h_scal, sc, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
-        c->sync();
+        // Unless this iteration is expected to meet the stopping test (the last rate
+        // carried forward), the next direction p = z + beta p, beta = -(z'q)/(p'q), and
+        // its product are queued before the host waits for the scalars: x and r stay
+        // as they are if the iteration stops here.
+        const bool ahead = !c->opt.krylov_sync && it < itmax &&
+                           !(nrm_prev2 > 0.0 && nrm_prev * (nrm_prev / nrm_prev2) <= eps);
+        if (ahead) {
+            PF_HIP(hipEventRecord(c->ev_scal, s));
+            axpby_negratio<double>(n, 1.0, z, sc + 4, sc + 1, p, s);
+            PF_HIP(hipMemcpyAsync(sc + 0, sc + 2, sizeof(double), hipMemcpyDeviceToDevice, s));
+            product(it + 1);
+            PF_HIP(hipEventSynchronize(c->ev_scal));
+        } else {
+            c->sync();
+        }
         const double pq = c->h_scal[1], rzn = c->h_scal[2], rrn = c->h_scal[3];
         float ms = 0.f;
-        PF_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        PF_HIP(hipEventElapsedTime(&ms, c->ev_kry[2 * (it & 1)], c->ev_kry[2 * (it & 1) + 1]));
         c->st.t_pcg_spmv += 1e-3 * ms;
         c->st.n_pcg_spmv++;
         if (!(pq > 0) || (pc_norm && !(rzn >= 0))) {
@@ -359,10 +401,15 @@ static int fpcg(pfem_ctx *c, const double *b, double *x, double rtol, int itmax,
         if (nrm <= eps) return 0;
         if (std::sqrt(rrn) < 0.999 * best) { best = std::sqrt(rrn); best_it = it; }
         else if (stall > 0 && it - best_it >= stall) return 3;
-        // p = z + beta p, beta = -(z'q)/(p'q)
-        const double beta = -c->h_scal[4] / pq;
-        axpby<double>(n, 1.0, z, beta, p, s);
-        PF_HIP(hipMemcpyAsync(sc + 0, sc + 2, sizeof(double), hipMemcpyDeviceToDevice, s));
+        if (!ahead && it < itmax) {
+            // p = z + beta p, beta = -(z'q)/(p'q)
+            const double beta = -c->h_scal[4] / pq;
+            axpby<double>(n, 1.0, z, beta, p, s);
+            PF_HIP(hipMemcpyAsync(sc + 0, sc + 2, sizeof(double), hipMemcpyDeviceToDevice, s));
+            product(it + 1);
+        }
+        nrm_prev2 = nrm_prev;
+        nrm_prev = nrm;
         rz = rzn;
     }
     return 1;

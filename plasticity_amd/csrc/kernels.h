@@ -50,6 +50,9 @@ void fcg_dots32(int64_t n, const float *r, const float *z, const float *q, doubl
                 hipStream_t s);
 template <class T>
 void axpby(int64_t n, double a, const T *x, double b, T *y, hipStream_t s);
+// axpby with b = -(*num / *den) taken from device memory (same arithmetic as axpby)
+template <class T>
+void axpby_negratio(int64_t n, double a, const T *x, const double *num, const double *den, T *y, hipStream_t s);
 void axpy_safe_ratio(int64_t n, const double *num, const double *den, double sign, const double *x, double *y,
                      hipStream_t s);
 void cg_update_mixed(int64_t n, const double *num, const double *den, const float *p, const float *q, double *x,

@@ -229,6 +229,21 @@ __global__ void axpby_kernel(int64_t n, double a, const T *__restrict__ x, doubl
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         y[i] = (T)(a * (double)x[i] + b * (double)y[i]);
 }
+// axpby with b = -(num/den) read on the device: the same expression (and the same b = 0
+// overwrite) as axpby_kernel, so a flexible-CG direction update formed here equals the
+// one the host forms from the same scalars
+template <class T>
+__global__ void axpby_negratio_kernel(int64_t n, double a, const T *__restrict__ x, const double *num,
+                                      const double *den, T *__restrict__ y) {
+    const double b = -(*num / *den);
+    if (b == 0.0) {
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+            y[i] = (T)(a * (double)x[i]);
+        return;
+    }
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = (T)(a * (double)x[i] + b * (double)y[i]);
+}
 template <class S, class D>
 __global__ void convert_kernel(int64_t n, const S *__restrict__ s, D *__restrict__ d) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -571,6 +586,11 @@ void axpby(int64_t n, double a, const T *x, double b, T *y, hipStream_t s) {
     hipLaunchKernelGGL((axpby_kernel<T>), dim3(grid_strided(n)), dim3(256), 0, s, n, a, x, b, y);
     PF_HIP(hipGetLastError());
 }
+template <class T>
+void axpby_negratio(int64_t n, double a, const T *x, const double *num, const double *den, T *y, hipStream_t s) {
+    hipLaunchKernelGGL((axpby_negratio_kernel<T>), dim3(grid_strided(n)), dim3(256), 0, s, n, a, x, num, den, y);
+    PF_HIP(hipGetLastError());
+}
 template <class S, class D>
 void convert(int64_t n, const S *src, D *dst, hipStream_t s) {
     hipLaunchKernelGGL((convert_kernel<S, D>), dim3(grid_strided(n)), dim3(256), 0, s, n, src, dst);
@@ -587,6 +607,7 @@ void fill(int64_t n, T v, T *d, hipStream_t s) {
     template void cg_update<T>(int64_t, const double *, const double *, const T *, const T *, T *, T *,       \
                                hipStream_t);                                                              \
     template void axpby<T>(int64_t, double, const T *, double, T *, hipStream_t);                            \
+    template void axpby_negratio<T>(int64_t, double, const T *, const double *, const double *, T *, hipStream_t); \
     template void fill<T>(int64_t, T, T *, hipStream_t);
 PF_INST(double)
 PF_INST(float)

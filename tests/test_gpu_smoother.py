@@ -14,10 +14,10 @@ pytestmark = pytest.mark.gpu
 MAT = dict(E=200e9, nu=0.3, sy0=250e6, Hiso=1e9)
 
 
-def _solve(precision, smoother, degree, unfused):
+def _solve(precision, smoother, degree, unfused, **extra):
     mesh = pa.box_mesh(1, 1, 1, 10, 10, 10, jitter=0.1, seed=2)
     opts = pa.default_options(precision=precision, smoother=smoother, smoother_degree=degree,
-                              smoother_unfused=unfused, coarse_max=60)
+                              smoother_unfused=unfused, coarse_max=60, **extra)
     m = pa.Model(mesh, pa.J2Material(**MAT), options=opts)
     d, v, r = uniaxial_constraints(mesh, -0.02)
     m.constrain(d, v, r)
@@ -43,6 +43,21 @@ def test_fused_chebyshev_step_is_bitwise_equal(precision, smoother, degree, need
     a = _solve(precision, smoother, degree, 0)
     b = _solve(precision, smoother, degree, 1)
     print(f"  krylov {a[1]} / {b[1]}, relres {a[3]:.2e}, newton {len(a[5])} its", flush=True)
+    assert a[3] < 1e-9 and a[4]
+    assert np.array_equal(a[0], b[0]) and a[1:4] == b[1:4]
+    assert a[5] == b[5] and a[6] == b[6] and np.array_equal(a[7], b[7])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+def test_krylov_overlap_is_bitwise_equal(precision, need_gpu):
+    """PCG (FP64) and flexible CG (FP32-IR) queue the next direction and its product
+    before the host reads an iteration's scalars (krylov_sync = 0, the default); with
+    krylov_sync = 1 the host reads them first.  The kernels and their order on the stream
+    are the same, so the solves and the Newton load step agree bit for bit."""
+    print(f"[krylov overlap] {precision}: queued vs host-first", flush=True)
+    a = _solve(precision, "chebyshev", 2, 0, krylov_sync=0)
+    b = _solve(precision, "chebyshev", 2, 0, krylov_sync=1)
+    print(f"  krylov {a[1]} / {b[1]}, newton {len(a[5])} its, krylov per newton {a[6]}", flush=True)
     assert a[3] < 1e-9 and a[4]
     assert np.array_equal(a[0], b[0]) and a[1:4] == b[1:4]
     assert a[5] == b[5] and a[6] == b[6] and np.array_equal(a[7], b[7])
