@@ -233,7 +233,7 @@ def time_variant(pa, wl, precision, learned, warmup, steps, args):
                               amg_rebuild=args.amg_rebuild, newton_diverge=args.diverge,
                               fp32_krylov=args.fp32_krylov, assembly_chunk=args.assembly_chunk,
                               smoother_unfused=args.smoother_unfused, coarse_degree=args.coarse_degree,
-                              krylov_sync=args.krylov_sync)
+                              krylov_sync=args.krylov_sync, split_operator=args.split_operator)
     mats = [pa.J2Material(**d) for d in wl.materials]
     m = pa.Model(wl.mesh, mats if len(mats) > 1 else mats[0], options=opts, material_id=wl.material_id)
     m.constrain(wl.dir_dofs, wl.dir_vals, wl.dir_ramp)
@@ -331,6 +331,9 @@ def main():
     ap.add_argument("--verbose", type=int, default=0)
     ap.add_argument("--show-residuals", action="store_true",
                     help="print each increment's Newton residual history in the per-load-step lines")
+    ap.add_argument("--split-operator", type=int, default=1, choices=[0, 1],
+                    help="FP32-IR: 1 = flexible CG's K p from the FP32 copy of K plus a BF16 correction "
+                         "(default); 0 = from the FP64 K")
     ap.add_argument("--krylov-sync", type=int, default=0, choices=[0, 1],
                     help="1: PCG / flexible CG read each iteration's scalars before launching the next "
                          "product (0: the next product is queued first; same results)")
@@ -374,10 +377,17 @@ def main():
     if args.mode == "spmv":
         m = pa.Model(wl.mesh, pa.J2Material(**wl.materials[0]), options=pa.default_options())
         m.assemble(np.zeros(m.ndof))
-        sec = m.bench_spmv(prec=0 if args.precision == "fp64" else 1, reps=20)
         st = m.stats()
-        b = spmv_bytes(st.nnzb, st.nnodes, st.ndof, 8 if args.precision == "fp64" else 4)
-        log(f"[bench] fine SpMV x20: {sec * 1e3:.3f} ms each, {b / sec / 1e9:.0f} GB/s algorithmic ({b} B)")
+        # FP64, the FP32 copy, and FP64 from the FP32 + BF16 split copy (values 4 + 2 bytes,
+        # vectors 8 bytes)
+        for prec, name in ((0, "FP64"), (1, "FP32"), (2, "split FP32+BF16")):
+            sec = m.bench_spmv(prec=prec, reps=20)
+            if prec == 2:
+                b = st.nnzb * (9 * 6 + 4) + (st.nnodes + 1) * 4 + 2 * st.ndof * 8
+            else:
+                b = spmv_bytes(st.nnzb, st.nnodes, st.ndof, 8 if prec == 0 else 4)
+            log(f"[bench] fine SpMV {name} x20: {sec * 1e3:.3f} ms each, {b / sec / 1e9:.0f} GB/s algorithmic "
+                f"({b} B)")
         return
     if args.mode == "assemble":
         m = pa.Model(wl.mesh, pa.J2Material(**wl.materials[0]), options=pa.default_options())
@@ -405,7 +415,7 @@ def main():
                               gcr_fallback=args.fallback == "gcr", newton_diverge=args.diverge,
                               fp32_krylov=args.fp32_krylov, assembly_chunk=args.assembly_chunk,
                               smoother_unfused=args.smoother_unfused, coarse_degree=args.coarse_degree,
-                              krylov_sync=args.krylov_sync)
+                              krylov_sync=args.krylov_sync, split_operator=args.split_operator)
     t0 = time.perf_counter()
     mats = [pa.J2Material(**d) for d in wl.materials]
     m = pa.Model(wl.mesh, mats if len(mats) > 1 else mats[0], options=opts, material_id=wl.material_id)

@@ -138,7 +138,11 @@ typedef struct pfem_options {
                                    launch the next iteration; 0 = the next search direction and its product
                                    are launched first and the stopping test overlaps them (default; the same
                                    results bit for bit) */
-    int32_t reserved[4];        /* zero */
+    int32_t split_operator;     /* FP32-IR: 1 = flexible CG's product K p reads the FP32 copy of K (the V-cycle's)
+                                   plus a BF16 copy of K - fl32(K): FP64 arithmetic on entries that agree with
+                                   K to 2^-33 relative, 54 instead of 72 bytes of values per block (the
+                                   default of pfem_default_options); 0 = the FP64 K */
+    int32_t reserved[3];        /* zero */
 } pfem_options;
 
 typedef struct pfem_fail {
@@ -372,9 +376,16 @@ int pfem_dense_inverse(int64_t n, const double *A, double *inv);
  * of device memory for the call.  No counterpart in the reference. */
 int pfem_device_peaks(int64_t copy_bytes, int reps, double *copy_gbs, double *read_gbs, double *fp64_tflops);
 
-/* Time `reps` fine-level SpMVs y = K x (FP64 if prec == 0, FP32 if 1) on the
- * context's stream with events; returns mean seconds per SpMV. */
+/* Time `reps` fine-level SpMVs y = K x (FP64 if prec == 0, FP32 if 1, FP64 from the
+ * FP32 + BF16 split copy of K if 2) on the context's stream with events; returns mean
+ * seconds per SpMV.  prec 1 and 2 make their copies from the current K. */
 int pfem_bench_spmv(pfem_ctx *ctx, int prec, int reps, double *sec_per_spmv);
+
+/* y = K x for host vectors of ndof entries with the current device K after
+ * assembly and Dirichlet imposition: prec 0 the FP64 K, 1 its FP32 copy (FP32 x and
+ * products, widened), 2 the FP32 + BF16 split copy (FP64 arithmetic).  prec 1 and 2 make
+ * their copies from the current K.  No counterpart in the reference (operator checks). */
+int pfem_apply_K(pfem_ctx *ctx, int prec, const double *x, double *y);
 
 /* Time `reps` applications of the current AMG cycle (prec 0: FP64, 1: FP32, which needs
  * PFEM_PREC_FP32_IR) and `reps` fused fine-level Chebyshev smoother steps (product

@@ -66,7 +66,7 @@ class Options(C.Structure):
                 ("prolong_rowwise", C.c_int32), ("fp32_krylov", C.c_int32), ("newton_diverge", C.c_double),
                 ("assembly_chunk", C.c_int32), ("smoother_unfused", C.c_int32),
                 ("coarse_degree", C.c_int32), ("krylov_sync", C.c_int32),
-                ("reserved", C.c_int32 * 4)]
+                ("split_operator", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 class Fail(C.Structure):
@@ -142,6 +142,7 @@ SIGNATURES = {
                                              _vp, C.c_int, _vp, _vp, _vp, C.POINTER(Fail)]),
     "pfem_get_stats": (C.c_int, [_vp, C.POINTER(Stats)]),
     "pfem_bench_spmv": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
+    "pfem_apply_K": (C.c_int, [_vp, C.c_int, _dp, _dp]),
     "pfem_bench_vcycle": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp, _dp, _dp]),
     "pfem_dense_inverse": (C.c_int, [C.c_int64, _vp, _vp]),
     "pfem_device_peaks": (C.c_int, [C.c_int64, C.c_int, _dp, _dp, _dp]),

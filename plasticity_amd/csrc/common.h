@@ -90,6 +90,7 @@ struct FineMatrix {
     dbuf<int32_t> diagpos;    // nb: slot of the diagonal block
     dbuf<double> v64;         // 9 * nslots
     dbuf<float> v32;          // 9 * nslots (FP32-IR copy)
+    dbuf<uint16_t> lo16;      // 9 * nslots, BF16 bits of K - v32, 9 planes of 64 per slice column
     dbuf<uint16_t> v16;       // 9 * nslots, FP16 bits of S K S (FP32 V-cycle fine level)
     dbuf<float> sinv;         // 3 * nb, sqrt|K_ii| (S^-1)
     std::vector<int64_t> h_slice_ptr;

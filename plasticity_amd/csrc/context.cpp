@@ -120,6 +120,7 @@ void pfem_default_options(pfem_options *o) {
     o->max_levels = 10;
     o->verbose = 0;
     o->amg_rebuild = PFEM_AMG_KEEP_P_UNTIL_FALLBACK;
+    o->split_operator = 1;  // FP32-IR: K p from the FP32 + BF16 split copy of K
 }
 
 int pfem_create(const pfem_mesh_desc *md, const pfem_options *opts, pfem_ctx **out) {
@@ -594,6 +595,32 @@ int pfem_linear_solve(pfem_ctx *c, const double *b, double rtol, int rebuild_pc,
             c->sync();
         }
         return rc;
+    });
+}
+
+int pfem_apply_K(pfem_ctx *c, int prec, const double *x, double *y) {
+    if (!c || !x || !y || prec < 0 || prec > 2) return PFEM_ERR_BAD_ARG;
+    if (int rc = need_K(c)) return rc;
+    return guarded(nullptr, [&]() -> int {
+        hipStream_t s = c->stream;
+        const int64_t n = c->ndof;
+        c->w_p.upload(x, n, s);
+        if (prec == 0) {
+            spmv_fine64(c->K, c->w_p.p, c->w_q.p, s);
+        } else if (prec == 1) {
+            if (!c->K.v32.p) c->K.v32.alloc(9 * c->K.nslots);
+            convert<double, float>(9 * c->K.nslots, c->K.v64.p, c->K.v32.p, s);
+            if (!c->f_p.p) { c->f_p.alloc(n); c->f_q.alloc(n); }
+            convert<double, float>(n, c->w_p.p, c->f_p.p, s);
+            spmv_fine32(c->K, c->f_p.p, c->f_q.p, s);
+            convert<float, double>(n, c->f_q.p, c->w_q.p, s);
+        } else {
+            split_copy(c->K, s);
+            spmv_fine_split(c->K, c->w_p.p, c->w_q.p, s);
+        }
+        c->w_q.download(y, n, s);
+        c->sync();
+        return PFEM_OK;
     });
 }
 

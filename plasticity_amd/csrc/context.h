@@ -79,6 +79,7 @@ struct pfem_ctx {
     bool refresh_P = false;  // AMG policy 6: the last solve fell back, rebuild the prolongators
     pfem::dbuf<double> net_dinv;  // block-diagonal inverse of K for the learned preconditioner
     bool half_fine = false;       // FP32 V-cycle: fine-level products with the FP16 scaled K
+    bool split_op = false;        // FP32-IR flexible CG: K p from the FP32 + BF16 split copy of K
     pfem::dbuf<double> fcg_P, fcg_Q; // truncated flexible CG: search directions and K times them
 
     // statistics

@@ -352,9 +352,11 @@ static int fpcg(pfem_ctx *c, const double *b, double *x, double rtol, int itmax,
     int best_it = 0;
     PF_HIP(hipMemcpyAsync(p, z, n * sizeof(double), hipMemcpyDeviceToDevice, s));
     // q = K p with sc[1] = p'q for iteration k, timed by the event pair of k's parity
+    // (FP32-IR with split_operator: from the FP32 + BF16 split copy of K)
     auto product = [&](int k) {
         PF_HIP(hipEventRecord(c->ev_kry[2 * (k & 1)], s));
-        Prec<double>::spmv_dot(c, p, q, sc + 1);
+        if (c->split_op) spmv_fine_split_dot(c->K, p, q, c->partial.p, sc + 1, s);
+        else Prec<double>::spmv_dot(c, p, q, sc + 1);
         PF_HIP(hipEventRecord(c->ev_kry[2 * (k & 1) + 1], s));
     };
     product(1);
@@ -714,8 +716,13 @@ static void prepare_fp32(pfem_ctx *c) {
     if (!c->f_r.p)
         for (auto *b : {&c->f_r, &c->f_z, &c->f_p, &c->f_q, &c->f_x, &c->f_b, &c->f_rh, &c->f_v, &c->f_s, &c->f_t})
             b->alloc(c->ndof);
-    if (!c->K.v32.p) c->K.v32.alloc(9 * c->K.nslots);
-    convert<double, float>(9 * c->K.nslots, c->K.v64.p, c->K.v32.p, c->stream);
+    c->split_op = c->opt.split_operator == 1;
+    if (c->split_op) {
+        split_copy(c->K, c->stream);  // writes the FP32 copy too
+    } else {
+        if (!c->K.v32.p) c->K.v32.alloc(9 * c->K.nslots);
+        convert<double, float>(9 * c->K.nslots, c->K.v64.p, c->K.v32.p, c->stream);
+    }
     c->half_fine = c->opt.fp16_fine == 1;  // FP16 scaled fine level inside the V-cycle's smoother
     if (c->half_fine) half_scaled_copy(c->K, c->stream);
 }
@@ -1176,11 +1183,18 @@ int pfem_dense_inverse(int64_t n, const double *A, double *inv) {
 }
 
 int pfem_bench_spmv(pfem_ctx *c, int prec, int reps, double *sec) {
-    if (!c || !sec || reps < 1) return PFEM_ERR_BAD_ARG;
+    if (!c || !sec || reps < 1 || prec < 0 || prec > 2) return PFEM_ERR_BAD_ARG;
     try {
         hipStream_t s = c->stream;
         const int64_t n = c->ndof;
-        if (prec == 1) {
+        if (prec == 2) {
+            split_copy(c->K, s);
+            fill<double>(n, 1.0, c->w_p.p, s);
+            spmv_fine_split(c->K, c->w_p.p, c->w_q.p, s);
+            PF_HIP(hipEventRecord(c->ev0, s));
+            for (int i = 0; i < reps; ++i) spmv_fine_split(c->K, c->w_p.p, c->w_q.p, s);
+            PF_HIP(hipEventRecord(c->ev1, s));
+        } else if (prec == 1) {
             if (!c->K.v32.p) {
                 c->K.v32.alloc(9 * c->K.nslots);
                 convert<double, float>(9 * c->K.nslots, c->K.v64.p, c->K.v32.p, s);

@@ -15,6 +15,12 @@ void residual_fine64(const FineMatrix &K, const double *x, const double *b, doub
 size_t reduce_partials_needed(int64_t nb);
 // FP16 copy of the diagonally scaled fine K (K.v16, K.sinv) and its products (FP32 vectors)
 void half_scaled_copy(FineMatrix &K, hipStream_t s);
+// FP32 copy v32 = fl32(K) and its BF16 correction lo16 = bf16(K - v32) in one pass; the
+// FP64 product (and the one with the p.y partials) from that split copy
+void split_copy(FineMatrix &K, hipStream_t s);
+void spmv_fine_split(const FineMatrix &K, const double *x, double *y, hipStream_t s);
+void spmv_fine_split_dot(const FineMatrix &K, const double *x, double *y, double *partial, double *out,
+                         hipStream_t s);
 void spmv_fine_half(const FineMatrix &K, const float *x, float *y, hipStream_t s);
 void residual_fine_half(const FineMatrix &K, const float *x, const float *b, float *r, hipStream_t s);
 template <class A, class B>

@@ -129,6 +129,70 @@ __global__ __launch_bounds__(64) void sell_lds_kernel(int64_t nb, const int64_t 
     }
 }
 
+// The FP64 product from a split copy of the matrix: the FP32 copy hi = fl32(K) (the
+// one the FP32 V-cycle reads) plus lo = bf16(K - hi), which carries the next 8
+// significant bits.  hi + lo is formed exactly in FP64 (at most 33 significant bits)
+// and agrees with K to 2^-33 relative per entry; products and sums are FP64.  54 bytes
+// of values per block instead of 72.  lo is stored per slice column as 9 planes of 64
+// (plane m holds entry m of the column's 64 blocks), so lane i reads its 9 entries
+// from LDS at i, 64 + i, ... without bank conflicts.
+__device__ __forceinline__ double bf16_bits_to_double(uint16_t h) {
+    return (double)__uint_as_float((uint32_t)h << 16);
+}
+template <int MODE>
+__global__ __launch_bounds__(64) void sell_split_kernel(int64_t nb, const int64_t *__restrict__ slice_ptr,
+                                                        const int32_t *__restrict__ colind,
+                                                        const float *__restrict__ hi,
+                                                        const uint16_t *__restrict__ lo,
+                                                        const double *__restrict__ x, const double *__restrict__ b,
+                                                        double *__restrict__ y, double *__restrict__ partial) {
+    constexpr int NH = SELL_C * 9 * 4 / 16;  // 144 chunks of 16 B per slice column (hi)
+    constexpr int NL = SELL_C * 9 * 2 / 16;  // 72 (lo)
+    __shared__ __attribute__((aligned(16))) float thi[SELL_C * 9];
+    __shared__ __attribute__((aligned(16))) uint16_t tlo[SELL_C * 9];
+    const int lane = threadIdx.x;
+    const int64_t s = xcd_contiguous(blockIdx.x, gridDim.x);
+    const int64_t i = s * SELL_C + lane;
+    const int64_t base = slice_ptr[s];
+    const int w = (int)((slice_ptr[s + 1] - base) / SELL_C);
+    double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+    for (int k = 0; k < w; ++k) {
+        const int64_t col0 = base + (int64_t)k * SELL_C;
+#pragma unroll
+        for (int t = lane; t < NH; t += SELL_C) stream_copy16(thi, hi + 9 * col0, t);
+        for (int t = lane; t < NL; t += SELL_C) stream_copy16(tlo, lo + 9 * col0, t);
+        const int64_t j = __builtin_nontemporal_load(colind + col0 + lane);
+        const double x0 = x[3 * j], x1 = x[3 * j + 1], x2 = x[3 * j + 2];
+        __syncthreads();
+        const float *v = thi + 9 * lane;
+        double a[9];
+#pragma unroll
+        for (int m = 0; m < 9; ++m) a[m] = (double)v[m] + bf16_bits_to_double(tlo[SELL_C * m + lane]);
+        y0 += a[0] * x0 + a[1] * x1 + a[2] * x2;
+        y1 += a[3] * x0 + a[4] * x1 + a[5] * x2;
+        y2 += a[6] * x0 + a[7] * x1 + a[8] * x2;
+        __syncthreads();
+    }
+    double d = 0.0;
+    if (i < nb) {
+        if (MODE == SELL_RES) {
+            y[3 * i] = b[3 * i] - y0;
+            y[3 * i + 1] = b[3 * i + 1] - y1;
+            y[3 * i + 2] = b[3 * i + 2] - y2;
+        } else {
+            y[3 * i] = y0;
+            y[3 * i + 1] = y1;
+            y[3 * i + 2] = y2;
+            if (MODE == SELL_DOT) d = x[3 * i] * y0 + x[3 * i + 1] * y1 + x[3 * i + 2] * y2;
+        }
+    }
+    if (MODE == SELL_DOT) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) d += __shfl_down(d, off, 64);
+        if (lane == 0) partial[s] = d;
+    }
+}
+
 // The same product with an FP16 copy of the diagonally scaled matrix
 // Khat = S K S, S = diag(1/sqrt|K_ii|) (unit diagonal, |Khat_ij| <~ 1, so FP16 holds it
 // without overflow): y = S^-1 Khat S^-1 x, evaluated as sinv_i * sum_j Khat_ij (sinv_j x_j)

@@ -543,6 +543,40 @@ __global__ __launch_bounds__(64) void half_scale_values_kernel(int64_t nb, const
             }
     }
 }
+// v32 = fl32(v64) and lo16 = bf16(v64 - v32) (round to nearest even) in one pass; lo16
+// is written as 9 planes of 64 per slice column (slice columns start at multiples of 64)
+__global__ void split_copy_kernel(int64_t nslots, const double *__restrict__ v64, float *__restrict__ v32,
+                                  uint16_t *__restrict__ lo16) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < 9 * nslots;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const double v = v64[t];
+        const float h = (float)v;
+        const uint32_t u = __float_as_uint((float)(v - (double)h));
+        const uint16_t l = (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+        const int64_t slot = t / 9, m = t - 9 * slot, lane = slot & (SELL_C - 1);
+        v32[t] = h;
+        lo16[9 * (slot - lane) + SELL_C * m + lane] = l;
+    }
+}
+void split_copy(FineMatrix &K, hipStream_t s) {
+    if (!K.v32.p) K.v32.alloc(9 * K.nslots);
+    if (!K.lo16.p) K.lo16.alloc(9 * K.nslots);
+    hipLaunchKernelGGL(split_copy_kernel, dim3(grid_strided(9 * K.nslots)), dim3(256), 0, s, K.nslots, K.v64.p,
+                       K.v32.p, K.lo16.p);
+    PF_HIP(hipGetLastError());
+}
+void spmv_fine_split_dot(const FineMatrix &K, const double *x, double *y, double *partial, double *out,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(sell_split_kernel<SELL_DOT>, dim3((unsigned)K.nslices), dim3(SELL_C), 0, s, K.nb,
+                       K.slice_ptr.p, K.colind.p, K.v32.p, K.lo16.p, x, (const double *)nullptr, y, partial);
+    hipLaunchKernelGGL(finish_sum_kernel, dim3(1), dim3(1024), 0, s, (int)K.nslices, partial, out);
+    PF_HIP(hipGetLastError());
+}
+void spmv_fine_split(const FineMatrix &K, const double *x, double *y, hipStream_t s) {
+    hipLaunchKernelGGL(sell_split_kernel<SELL_Y>, dim3((unsigned)K.nslices), dim3(SELL_C), 0, s, K.nb,
+                       K.slice_ptr.p, K.colind.p, K.v32.p, K.lo16.p, x, (const double *)nullptr, y, nullptr);
+    PF_HIP(hipGetLastError());
+}
 void half_scaled_copy(FineMatrix &K, hipStream_t s) {
     if (!K.sinv.p) K.sinv.alloc(3 * K.nb);
     if (!K.v16.p) K.v16.alloc(9 * K.nslots);

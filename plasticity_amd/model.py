@@ -496,6 +496,17 @@ class Model:
         L.check(L.lib().pfem_get_stats(self._h, C.byref(st)), "get_stats")
         return st
 
+    def apply_K(self, x, prec=0):
+        """y = K x with the current device K: prec 0 FP64, 1 the FP32 copy, 2 the FP32 +
+        BF16 split copy (FP64 arithmetic)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.shape != (self.ndof,):
+            raise ValueError(f"x must have {self.ndof} entries")
+        y = np.empty(self.ndof)
+        L.check(L.lib().pfem_apply_K(self._h, int(prec), x.ctypes.data_as(L._dp), y.ctypes.data_as(L._dp)),
+                "apply_K")
+        return y
+
     def bench_spmv(self, prec=0, reps=20):
         sec = C.c_double()
         L.check(L.lib().pfem_bench_spmv(self._h, int(prec), int(reps), C.byref(sec)), "bench_spmv")

@@ -121,6 +121,23 @@ def test_fp32_ir_matches_fp64_oracle(policy, coarse_max, need_gpu):
 
 
 @pytest.mark.parametrize("policy", [0, 6])
+def test_fp32_ir_fp64_operator_matches_fp64_oracle(policy, need_gpu):
+    """FP32-IR whose flexible CG forms K p from the FP64 K (split_operator = 0; the
+    default forms it from the FP32 copy of K plus its BF16 correction, which the test
+    above covers): the 80-step jittered compression within 1e-8 of the FP64 oracle."""
+    g = np.load(os.path.join(HERE, "golden", "jitter6.npz"))
+    print(f"[newton fp32-IR FP64 operator] 6^3 jittered compression, AMG policy {policy}", flush=True)
+    m, res, out = run_gpu(jitter6(), {20, 80}, precision="fp32_ir", split_operator=0, amg_rebuild=policy,
+                          coarse_max=60)
+    assert res.converged
+    for n in (20, 80):
+        eu = rel(out[f"U_{n}"], g[f"U_{n}"])
+        ea = np.abs(out[f"abar_{n}"] - g[f"abar_{n}"]).max() / np.abs(g[f"abar_{n}"]).max()
+        print(f"  step {n}: |dU|/|U| = {eu:.2e}  abar {ea:.2e}", flush=True)
+        assert eu < 1e-8 and ea < 1e-8
+
+
+@pytest.mark.parametrize("policy", [0, 6])
 def test_fp32_krylov_ir_matches_fp64_oracle(policy, need_gpu):
     """FP32-IR with the Krylov iteration itself in FP32 (FP32 K p, FP32 vectors, FP32
     V-cycle; FP64 solution and refinement residual): the 80-step jittered compression

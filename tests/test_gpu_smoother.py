@@ -63,6 +63,36 @@ def test_krylov_overlap_is_bitwise_equal(precision, need_gpu):
     assert a[5] == b[5] and a[6] == b[6] and np.array_equal(a[7], b[7])
 
 
+def test_split_operator_product(need_gpu):
+    """K x from the FP32 + BF16 split copy of K (pfem_apply_K prec 2) against the FP64
+    product and the exported K: every entry of the split copy carries 33 significant bits
+    (2^-33 = 1.2e-10 relative), so the product is within a few 1e-10 of the FP64 one
+    and far closer than the FP32 copy's (2^-24 per entry).  The FP64 product itself
+    equals the exported K (CSC) times x to rounding."""
+    print("[split operator] 12^3 jittered block, plastic tangent with Dirichlet rows", flush=True)
+    mesh = pa.box_mesh(1, 1, 1, 12, 12, 12, jitter=0.1, seed=3)
+    m = pa.Model(mesh, pa.J2Material(**MAT), options=pa.default_options(precision="fp32_ir", coarse_max=60))
+    d, v, r = uniaxial_constraints(mesh, -0.02)
+    m.constrain(d, v, r)
+    m._freeze()
+    U = (np.array([-0.004, 0.0015, 0.0015]) * mesh.nodes).ravel()
+    m.assemble(U, commit=True)
+    m.assemble(1.5 * U)
+    m.impose_dirichlet(1.0)
+    K = m.get_K_csc()
+    rng = np.random.default_rng(5)
+    for x in (rng.standard_normal(m.ndof), 1.5 * U + 1e-3 * rng.standard_normal(m.ndof)):
+        y0, y1, y2 = (m.apply_K(x, prec=p) for p in (0, 1, 2))
+        ref = K @ x
+        e0 = np.linalg.norm(y0 - ref) / np.linalg.norm(ref)
+        e1 = np.linalg.norm(y1 - y0) / np.linalg.norm(y0)
+        e2 = np.linalg.norm(y2 - y0) / np.linalg.norm(y0)
+        print(f"  |y64 - Kx| {e0:.1e}, FP32 copy {e1:.1e}, split copy {e2:.1e}", flush=True)
+        assert e0 < 1e-13
+        assert e2 < 2e-9 and e2 < 1e-2 * e1
+    m.close()
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
 def test_vcycle_timing_entry_point(precision, need_gpu):
     """pfem_bench_vcycle times the cycle and the fused fine smoother step on the current

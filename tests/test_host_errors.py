@@ -129,8 +129,8 @@ def test_named_options_and_their_defaults():
     o = pa.default_options()
     assert o.amg_rebuild == L.AMG_KEEP_P_UNTIL_FALLBACK and o.amg_wcycle == 0 and o.krylov_fallback == 0
     assert o.line_search == 0 and o.newton_diverge == 0.0 and o.fp32_krylov == 0 and o.host_build == 0
-    assert o.assembly_chunk == 0 and o.smoother_unfused == 0 and list(o.reserved) == [0] * 4 and o.coarse_degree == 0
-    assert o.krylov_sync == 0
+    assert o.assembly_chunk == 0 and o.smoother_unfused == 0 and list(o.reserved) == [0] * 3 and o.coarse_degree == 0
+    assert o.krylov_sync == 0 and o.split_operator == 1
     o = pa.default_options(amg_rebuild="per_load_step", wcycle=True, gcr_fallback=False, rho_iters=(30, 5),
                            stream_prolongation=False, line_search=3, newton_diverge=10.0)
     assert (o.amg_rebuild, o.amg_wcycle, o.krylov_fallback, o.rho_iters_cold, o.rho_iters_warm) == (0, 1, 1, 30, 5)
