@@ -302,11 +302,13 @@ def main():
     ap.add_argument("--no-variants", action="store_true",
                     help="FP64 run of workload 1: skip timing the FP32-IR and FP32-IR + learned solvers "
                          "on the same load steps")
-    ap.add_argument("--amg-rebuild", type=int, default=6,
+    ap.add_argument("--amg-rebuild", type=int, default=7,
                     help="0: once per load step (the reference), 1: every Newton iteration, 4: at the first "
                          "Newton iteration of a load step, then all but the prolongators at every iteration, "
                          "5: the prolongators once, everything else at every Newton iteration, 6: as 5, the "
-                         "prolongators rebuilt after a solve that needed a fallback")
+                         "prolongators rebuilt after a solve that needed a fallback, 7: as 6, each Newton "
+                         "iteration but the second reuses the coarse levels of the same iteration of an earlier "
+                         "load step")
     ap.add_argument("--smoother", default="chebyshev", choices=["chebyshev", "chebyshev4", "jacobi"])
     ap.add_argument("--smoother-degree", type=int, default=0,
                     help="Chebyshev degree / Jacobi sweeps (0: library default, 2 / 1)")
@@ -572,7 +574,9 @@ def main():
                                   3: "fine smoother refreshed", 4: "prolongators kept within a load step, the rest "
                                   "rebuilt every Newton iteration", 5: "prolongators built once, the rest rebuilt "
                                   "every Newton iteration", 6: "prolongators kept until a solve falls back, the rest "
-                                  "rebuilt every Newton iteration"}.get(args.amg_rebuild, "?") + ")",
+                                  "rebuilt every Newton iteration", 7: "as 6, with the coarse levels of the same Newton iteration "
+                                  "of an earlier load step reused except at the second iteration"}.get(args.amg_rebuild, "?")
+                               + ")",
                    "amg_cycle": args.cycle.upper(), "coarse_degree": args.coarse_degree,
                    "newton_steps_timed": newton, "newton_steps_accepted": tot.accepted,
                    "krylov_its_timed": krylov,

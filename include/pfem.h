@@ -87,8 +87,15 @@ typedef enum pfem_amg_rebuild {
     PFEM_AMG_FINE_SMOOTHER = 3,     /* at the first, then only the fine-level smoother from each tangent */
     PFEM_AMG_KEEP_P_IN_STEP = 4,    /* at the first, then all but the prolongators from each tangent */
     PFEM_AMG_KEEP_P = 5,            /* prolongators from the first set-up, the rest from every tangent */
-    PFEM_AMG_KEEP_P_UNTIL_FALLBACK = 6  /* as 5, the prolongators rebuilt after a solve that needed a fallback
+    PFEM_AMG_KEEP_P_UNTIL_FALLBACK = 6, /* as 5, the prolongators rebuilt after a solve that needed a fallback
                                            (the library default) */
+    PFEM_AMG_REUSE_COARSE = 7           /* prolongators from the first set-up (as 5); at every Newton iteration
+                                           except the second the levels below the finest are those built at
+                                           the same iteration of an earlier load step (kept aside when built),
+                                           and only the fine-level smoother comes from the new tangent; a kept
+                                           set-up is redone when a solve on it needs more Krylov iterations
+                                           than the solve right after it was built (by max(1, 1/16)), and all
+                                           are dropped after a solve that needed a fallback */
 } pfem_amg_rebuild;
 
 /* Linear-solve options (LinearSolveState, Solver.jl:86-111) and the switches of the

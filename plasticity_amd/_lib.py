@@ -51,6 +51,7 @@ AMG_FINE_SMOOTHER = 3
 AMG_KEEP_P_IN_STEP = 4
 AMG_KEEP_P = 5
 AMG_KEEP_P_UNTIL_FALLBACK = 6
+AMG_REUSE_COARSE = 7
 BC_RAMPED = 0
 BC_FIXED = 1
 

@@ -416,6 +416,8 @@ void coarse_inverse(pfem_ctx *c, Amg *a) {
 
 void amg_numeric_setup(pfem_ctx *c, Amg *a, bool keep_P) {
     graph_reset(a);  // smoother coefficients change with rho: captured V-cycles are stale
+    if (!keep_P)  // new prolongators: stashed coarse levels no longer fit
+        for (AmgStash &st : a->stash) st.ok = false;
     hipStream_t s = c->stream;
     const bool want32 = c->opt.precision == PFEM_PREC_FP32_IR;
     const size_t nlev = a->L.size();
@@ -517,6 +519,60 @@ void amg_numeric_setup(pfem_ctx *c, Amg *a, bool keep_P) {
         a->have32 = true;
     }
     c->sync();
+}
+
+namespace {
+template <class T>
+void copy_dev(dbuf<T> &dst, const dbuf<T> &src, hipStream_t s) {
+    if (dst.n != src.n) dst.alloc(src.n);
+    if (src.n) PF_HIP(hipMemcpyAsync(dst.p, src.p, src.n * sizeof(T), hipMemcpyDeviceToDevice, s));
+}
+}  // namespace
+
+AmgStash &amg_stash_slot(Amg *a, int slot) {
+    if ((int)a->stash.size() <= slot) a->stash.resize(slot + 1);
+    return a->stash[slot];
+}
+
+void amg_stash_coarse(pfem_ctx *c, Amg *a, int slot) {
+    hipStream_t s = c->stream;
+    AmgStash &st = amg_stash_slot(a, slot);
+    st.lv.resize(a->L.size());
+    for (size_t l = 1; l < a->L.size(); ++l) {
+        AmgLevel &L = a->L[l];
+        AmgStash::Level &S = st.lv[l];
+        copy_dev(S.A, L.A, s);
+        copy_dev(S.dinv, L.dinv, s);
+        if (a->have32) {
+            copy_dev(S.A32, L.A32, s);
+            copy_dev(S.dinv32, L.dinv32, s);
+        }
+        S.rho = L.rho;
+    }
+    copy_dev(st.coarse_inv, a->coarse_inv, s);
+    st.ok = true;
+    st.have32 = a->have32;
+}
+
+bool amg_restore_coarse(pfem_ctx *c, Amg *a, int slot) {
+    if ((int)a->stash.size() <= slot) return false;
+    AmgStash &st = a->stash[slot];
+    if (!st.ok || (a->have32 && !st.have32) || st.lv.size() != a->L.size()) return false;
+    hipStream_t s = c->stream;
+    for (size_t l = 1; l < a->L.size(); ++l) {
+        AmgLevel &L = a->L[l];
+        AmgStash::Level &S = st.lv[l];
+        copy_dev(L.A, S.A, s);
+        copy_dev(L.dinv, S.dinv, s);
+        if (a->have32) {
+            copy_dev(L.A32, S.A32, s);
+            copy_dev(L.dinv32, S.dinv32, s);
+        }
+        L.rho = S.rho;
+    }
+    copy_dev(a->coarse_inv, st.coarse_inv, s);
+    amg_refresh_fine(c, a);  // resets the captured cycles too
+    return true;
 }
 
 // Fine-level refresh: the level-0 smoother follows the current K (block-diagonal inverse,

@@ -59,6 +59,23 @@ struct AmgLevel {
     dbuf<float> wr32, we32;
 };
 
+// Numeric data of the levels below the finest (operators, block-diagonal inverses, rho)
+// and the coarsest inverse, kept aside after the set-up of one Newton iteration index
+// (rebuild policy 7) for the same iteration of later load steps.  Valid while the
+// prolongators it was built with are kept.
+struct AmgStash {
+    struct Level {
+        dbuf<double> A, dinv;
+        dbuf<float> A32, dinv32;
+        double rho = 2.0;
+    };
+    std::vector<Level> lv;       // levels 1 .. nlev-1
+    dbuf<double> coarse_inv;
+    bool ok = false, have32 = false;
+    int iters = 0;               // Krylov iterations of the solve right after it was built
+    bool stale = true;           // a later solve on it needed clearly more: rebuild
+};
+
 struct Amg {
     std::vector<AmgLevel> L;
     // V-cycle replayed as a HIP graph ([0] FP64, [1] FP32), captured at the first
@@ -83,6 +100,7 @@ struct Amg {
     bool wcycle = false;
     bool have32 = false;
     double op_complexity = 1.0;
+    std::vector<AmgStash> stash;  // policy 7: one per Newton iteration index (slot)
 };
 
 Amg *amg_create(pfem_ctx *c);                 // symbolic set-up (host)
@@ -92,6 +110,14 @@ void amg_destroy(Amg *a);
 // coarse inverse) is recomputed from the current one
 void amg_numeric_setup(pfem_ctx *c, Amg *a, bool keep_P = false);
 void amg_refresh_fine(pfem_ctx *c, Amg *a);  // level-0 smoother from the current K only
+// Copy the numeric data of the levels below the finest (operators, block-diagonal
+// inverses, rho) and the coarsest inverse into / out of stash slot `slot`.
+// amg_restore_coarse then refreshes the level-0 smoother from the current K; it returns
+// false (nothing changed) when the slot holds no copy made with the current prolongators
+// and precision.
+AmgStash &amg_stash_slot(Amg *a, int slot);
+void amg_stash_coarse(pfem_ctx *c, Amg *a, int slot);
+bool amg_restore_coarse(pfem_ctx *c, Amg *a, int slot);
 void amg_apply64(pfem_ctx *c, Amg *a, const double *r, double *z);  // z = V-cycle(r)
 void amg_apply32(pfem_ctx *c, Amg *a, const float *r, float *z);
 // Times `reps` cycles (prec 0 FP64, 1 FP32) and `reps` fused fine-level Chebyshev steps

@@ -785,6 +785,7 @@ static int learned_solve(pfem_ctx *c, double rtol, int itmax, int *total, int *w
 int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *used, double *relres,
                       bool *discarded) {
     const bool learned = c->net_mode == 2 || c->net_mode == 3;
+    if (learned && rebuild >= 4) rebuild = 3;  // policy 7's kept coarse levels: not with a network
     if (learned) {
         if (c->opt.precision == PFEM_PREC_FP32_IR) prepare_fp32(c);
         // the AMG hierarchy is rebuilt only if it is needed (a full rebuild request wins
@@ -798,6 +799,17 @@ int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *us
         c->st.t_setup += now() - t0;
     } else if (rebuild == 3) {
         pc_setup(c, true);
+    } else if (rebuild == 4) {  // policy 7: the kept coarse levels, the fine smoother from this K
+        const double t0 = now();
+        const bool restored = amg_restore_coarse(c, c->amg.get(), c->stash_slot);
+        c->st.t_setup += now() - t0;
+        if (!restored) rebuild = 5;
+    }
+    if (rebuild == 5) {  // policy 7: set-up keeping P, and its coarse levels kept aside
+        pc_setup(c, true);
+        const double t0 = now();
+        amg_stash_coarse(c, c->amg.get(), c->stash_slot);
+        c->st.t_setup += now() - t0;
     }
     const double t1 = now();
     int it = 0, total = 0, which = 0, rc;
@@ -913,6 +925,16 @@ int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *us
     }
     if (discarded) *discarded = drop;
     if (niter) *niter = total;
+    // policy 7: the solve right after a kept set-up sets the mark; one on the restored
+    // levels that needs clearly more iterations makes the next first iteration rebuild
+    if (rebuild == 5) {
+        AmgStash &st = amg_stash_slot(c->amg.get(), c->stash_slot);
+        st.iters = total;
+        st.stale = false;
+    } else if (rebuild == 4) {
+        AmgStash &st = amg_stash_slot(c->amg.get(), c->stash_slot);
+        if (total > st.iters + std::max(1, st.iters / 16)) st.stale = true;
+    }
     if (used) *used = which;
     c->st.t_solve += now() - t1;
     c->st.n_solve++;
@@ -925,6 +947,20 @@ int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *us
 }  // namespace pfem
 
 extern "C" {
+
+// AMG policy 7 at Newton iteration `it` (not the second): the coarse levels kept from the
+// same iteration of an earlier load step (4) unless that slot is empty or stale, then a
+// set-up keeping P whose coarse levels are kept (5).  The second iteration, the first on
+// the new increment's plastic tangent, always gets its own set-up: coarse levels kept
+// from another load step's second iteration made the Krylov solve break down.
+static int policy7_code(pfem_ctx *c, int it) {
+    constexpr int SLOTS = 8;
+    c->stash_slot = std::min(it, SLOTS) - 1;
+    Amg *a = c->amg.get();
+    if (!a || (int)a->stash.size() <= c->stash_slot) return 5;
+    const AmgStash &st = a->stash[c->stash_slot];
+    return (st.ok && !st.stale) ? 4 : 5;
+}
 
 // _forcing_term (Solver.jl:159-171); it is 1-based
 static double forcing_term(const pfem_options &o, int it, double rk, double rkm1, double eta_prev) {
@@ -997,8 +1033,10 @@ int pfem_newton(pfem_ctx *c, double lambda, double tol, int maxiter, pfem_newton
             // from later tangents too, keep the prolongators (3 = rebuild all but P), or
             // refresh only the fine-level smoother (2)
             const int policy = c->opt.amg_rebuild;
+            const bool keep_P = c->pc_ready && !c->refresh_P;
             int rebuild = policy == 5                                            ? (c->pc_ready ? 3 : 1)
-                          : policy == 6 ? ((c->pc_ready && !c->refresh_P) ? 3 : 1)
+                          : policy == 6 ? (keep_P ? 3 : 1)
+                          : policy == 7 ? (!c->pc_ready ? 1 : it == 2 ? 3 : policy7_code(c, it))
                           : (first || policy == 1 || (policy == 2 && it == 2)) ? 1
                           : policy == 3                                          ? 2
                           : policy == 4                                          ? 3
@@ -1023,8 +1061,13 @@ int pfem_newton(pfem_ctx *c, double lambda, double tol, int maxiter, pfem_newton
             // BiCGStab ladder, or BiCGStab refinement) rebuilds the prolongators next time
             // (not under fp32_krylov: its fallbacks follow breakdowns of the FP32 recursion,
             // and prolongators rebuilt from a plastic tangent then failed the next load step)
-            if (policy == 6)
-                c->refresh_P = !c->opt.fp32_krylov && (used == 1 || used == 2 || used == 4 || used == 7 || used == 8);
+            const bool fell_back = used == 1 || used == 2 || used == 4 || used == 7 || used == 8;
+            if (policy == 6) c->refresh_P = !c->opt.fp32_krylov && fell_back;
+            // policy 7 keeps the prolongators (a rebuild from a plastic tangent made every
+            // later first-iteration solve several times slower) and drops the kept coarse
+            // levels instead
+            if (policy == 7 && fell_back && c->amg)
+                for (AmgStash &st : c->amg->stash) st.ok = false;
             if (c->opt.verbose)
                 fprintf(stderr, "    iter %d  eta = %.2e  krylov = %d (solver %d, true relres %.2e)%s\n", it, eta,
                         niter, used, rel, dropped ? " discarded" : "");

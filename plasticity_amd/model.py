@@ -108,7 +108,8 @@ def default_options(**kw):
             v = {"jacobi": L.SMOOTH_JACOBI, "chebyshev": L.SMOOTH_CHEBYSHEV, "chebyshev4": L.SMOOTH_CHEBYSHEV4}[v]
         if k == "amg_rebuild" and isinstance(v, str):
             v = {"per_load_step": L.AMG_PER_LOAD_STEP, "every_iteration": L.AMG_EVERY_ITERATION,
-                 "keep_p": L.AMG_KEEP_P, "keep_p_until_fallback": L.AMG_KEEP_P_UNTIL_FALLBACK}[v]
+                 "keep_p": L.AMG_KEEP_P, "keep_p_until_fallback": L.AMG_KEEP_P_UNTIL_FALLBACK,
+                 "reuse_coarse": L.AMG_REUSE_COARSE}[v]
         if isinstance(v, bool):
             v = int(v)
         if not hasattr(o, k):

@@ -87,9 +87,10 @@ def test_config0_matches_oracle_golden(need_gpu):
 
 # (AMG rebuild policy, coarsest size): the library default (one level at 6^3: the dense
 # coarse solve), and a 3-level hierarchy under policy 4 (prolongators kept within a load
-# step), 5 (prolongators built once) and 6 (the benchmark's: as 5, rebuilt after a
-# fallback)
-POLICIES = [(0, 1500), (4, 60), (5, 60), (6, 60)]
+# step), 5 (prolongators built once), 6 (the benchmark's: as 5, rebuilt after a
+# fallback) and 7 (as 6, Newton iterations other than the second on the coarse levels
+# kept from the same iteration of an earlier load step)
+POLICIES = [(0, 1500), (4, 60), (5, 60), (6, 60), (7, 60)]
 
 
 @pytest.mark.parametrize("policy,coarse_max", POLICIES)
