@@ -471,7 +471,15 @@ def main():
     nnzb, nb, ndof = s1.nnzb, s1.nnodes, s1.ndof
     n_sp = s1.n_pcg_spmv - s0.n_pcg_spmv
     t_sp = (s1.t_pcg_spmv - s0.t_pcg_spmv) / max(n_sp, 1)
-    bytes_sp = spmv_bytes(nnzb, nb, ndof, 8)  # the Krylov operator is the FP64 K in both modes
+    # the Krylov operator: the FP64 K, or under FP32-IR with the split operator the FP32
+    # copy of K plus its BF16 correction (values 4 + 2 bytes, vectors FP64)
+    split = args.precision == "fp32_ir" and args.split_operator == 1 and net is None and not args.fp32_krylov
+    if split:
+        bytes_sp = nnzb * (9 * 6 + 4) + (nb + 1) * 4 + 2 * ndof * 8
+        sp_kernel = "fine BSR(3x3) product from the FP32 + BF16 split copy of K (sell_split_kernel<2>) in flexible CG"
+    else:
+        bytes_sp = spmv_bytes(nnzb, nb, ndof, 8)
+        sp_kernel = "fine BSR(3x3) SpMV (sell_lds_kernel<double,double,2>, FP64) in PCG"
     achieved = bytes_sp / t_sp / 1e9 if n_sp else None
     n_asm = s1.n_assemble - s0.n_assemble
     gp_ms = (s1.t_gp_kernel - s0.t_gp_kernel) / max(n_asm, 1) * 1e3
@@ -586,10 +594,10 @@ def main():
                    "amg_levels": list(s1.amg_ndof[:s1.amg_levels]),
                    **({"fp32_krylov_spmv": spmv32} if spmv32 else {}),
                    **({"vs_fp64_path": compare} if compare else {})},
-        "roofline": {"kernel": "fine BSR(3x3) SpMV (sell_lds_kernel<double,double,2>, FP64) in PCG",
+        "roofline": {"kernel": sp_kernel,
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-                     "traffic": SPMV_TRAFFIC_100 if (args.workload == 1 and ndof == 3090903) else None,
+                     "traffic": SPMV_TRAFFIC_100 if (args.workload == 1 and ndof == 3090903 and not split) else None,
                      "bytes_per_launch": bytes_sp, "avg_launch_ms": t_sp * 1e3},
         "cpu_baseline": cpu,
         "vcycle_roofline": vcyc,

@@ -543,26 +543,29 @@ __global__ __launch_bounds__(64) void half_scale_values_kernel(int64_t nb, const
             }
     }
 }
-// v32 = fl32(v64) and lo16 = bf16(v64 - v32) (round to nearest even) in one pass; lo16
-// is written as 9 planes of 64 per slice column (slice columns start at multiples of 64)
-__global__ void split_copy_kernel(int64_t nslots, const double *__restrict__ v64, float *__restrict__ v32,
-                                  uint16_t *__restrict__ lo16) {
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < 9 * nslots;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const double v = v64[t];
+// v32 = fl32(v64) and lo16 = bf16(v64 - v32) (round to nearest even) in one pass, one
+// thread per slot: its 9 values are contiguous in v64 and v32, and lo16 holds 9 planes of
+// 64 per slice column (slice columns start at multiples of 64), so the 64 threads of a
+// slice column write each plane as 128 contiguous bytes
+__global__ __launch_bounds__(256) void split_copy_kernel(int64_t nslots, const double *__restrict__ v64,
+                                                         float *__restrict__ v32, uint16_t *__restrict__ lo16) {
+    const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= nslots) return;
+    const int64_t lane = slot & (SELL_C - 1), base = 9 * (slot - lane) + lane;
+#pragma unroll
+    for (int m = 0; m < 9; ++m) {
+        const double v = v64[9 * slot + m];
         const float h = (float)v;
         const uint32_t u = __float_as_uint((float)(v - (double)h));
-        const uint16_t l = (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
-        const int64_t slot = t / 9, m = t - 9 * slot, lane = slot & (SELL_C - 1);
-        v32[t] = h;
-        lo16[9 * (slot - lane) + SELL_C * m + lane] = l;
+        v32[9 * slot + m] = h;
+        lo16[base + SELL_C * m] = (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
     }
 }
 void split_copy(FineMatrix &K, hipStream_t s) {
     if (!K.v32.p) K.v32.alloc(9 * K.nslots);
     if (!K.lo16.p) K.lo16.alloc(9 * K.nslots);
-    hipLaunchKernelGGL(split_copy_kernel, dim3(grid_strided(9 * K.nslots)), dim3(256), 0, s, K.nslots, K.v64.p,
-                       K.v32.p, K.lo16.p);
+    hipLaunchKernelGGL(split_copy_kernel, dim3((unsigned)((K.nslots + 255) / 256)), dim3(256), 0, s, K.nslots,
+                       K.v64.p, K.v32.p, K.lo16.p);
     PF_HIP(hipGetLastError());
 }
 void spmv_fine_split_dot(const FineMatrix &K, const double *x, double *y, double *partial, double *out,
