@@ -121,6 +121,28 @@ def test_fp32_ir_matches_fp64_oracle(policy, coarse_max, need_gpu):
         assert eu < 1e-8
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+def test_policy7_restores_kept_coarse_levels(precision, need_gpu):
+    """AMG policy 7 over the 80-step jittered compression (3-level hierarchy): most Newton
+    iterations restore coarse levels kept from an earlier load step instead of running
+    the Galerkin set-up (fewer set-ups than policy 6, which sets up at every solve),
+    and the run stays within 1e-8 of the FP64 oracle."""
+    g = np.load(os.path.join(HERE, "golden", "jitter6.npz"))
+    print(f"[newton policy 7] 6^3 jittered compression, {precision}", flush=True)
+    m6, res6, _ = run_gpu(jitter6(), set(), precision=precision, amg_rebuild=6, coarse_max=60)
+    m7, res7, out = run_gpu(jitter6(), {20, 80}, precision=precision, amg_rebuild=7, coarse_max=60)
+    assert res6.converged and res7.converged
+    s6, s7 = m6.stats(), m7.stats()
+    print(f"  set-ups: policy 6 {s6.n_setup} for {s6.n_solve} solves, policy 7 {s7.n_setup} for {s7.n_solve}; "
+          f"Krylov {s6.n_krylov} / {s7.n_krylov}", flush=True)
+    assert s6.n_setup >= s6.n_solve and s7.n_setup < 0.5 * s6.n_setup  # observed 398 and 90
+    for n in (20, 80):
+        eu = rel(out[f"U_{n}"], g[f"U_{n}"])
+        ea = np.abs(out[f"abar_{n}"] - g[f"abar_{n}"]).max() / np.abs(g[f"abar_{n}"]).max()
+        print(f"  step {n}: |dU|/|U| = {eu:.2e}  abar {ea:.2e}", flush=True)
+        assert eu < 1e-8 and ea < 1e-8
+
+
 @pytest.mark.parametrize("policy", [0, 6])
 def test_fp32_ir_fp64_operator_matches_fp64_oracle(policy, need_gpu):
     """FP32-IR whose flexible CG forms K p from the FP64 K (split_operator = 0; the
