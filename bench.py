@@ -233,7 +233,8 @@ def time_variant(pa, wl, precision, learned, warmup, steps, args):
                               amg_rebuild=args.amg_rebuild, newton_diverge=args.diverge,
                               fp32_krylov=args.fp32_krylov, assembly_chunk=args.assembly_chunk,
                               smoother_unfused=args.smoother_unfused, coarse_degree=args.coarse_degree,
-                              krylov_sync=args.krylov_sync, split_operator=args.split_operator)
+                              krylov_sync=args.krylov_sync, split_operator=args.split_operator,
+                              vcycle_eager=args.vcycle_eager)
     mats = [pa.J2Material(**d) for d in wl.materials]
     m = pa.Model(wl.mesh, mats if len(mats) > 1 else mats[0], options=opts, material_id=wl.material_id)
     m.constrain(wl.dir_dofs, wl.dir_vals, wl.dir_ramp)
@@ -333,6 +334,8 @@ def main():
     ap.add_argument("--verbose", type=int, default=0)
     ap.add_argument("--show-residuals", action="store_true",
                     help="print each increment's Newton residual history in the per-load-step lines")
+    ap.add_argument("--vcycle-eager", type=int, default=0, choices=[0, 1],
+                    help="1: launch the AMG cycle's kernels one by one instead of replaying a HIP graph")
     ap.add_argument("--split-operator", type=int, default=1, choices=[0, 1],
                     help="FP32-IR: 1 = flexible CG's K p from the FP32 copy of K plus a BF16 correction "
                          "(default); 0 = from the FP64 K")
@@ -417,7 +420,8 @@ def main():
                               gcr_fallback=args.fallback == "gcr", newton_diverge=args.diverge,
                               fp32_krylov=args.fp32_krylov, assembly_chunk=args.assembly_chunk,
                               smoother_unfused=args.smoother_unfused, coarse_degree=args.coarse_degree,
-                              krylov_sync=args.krylov_sync, split_operator=args.split_operator)
+                              krylov_sync=args.krylov_sync, split_operator=args.split_operator,
+                              vcycle_eager=args.vcycle_eager)
     t0 = time.perf_counter()
     mats = [pa.J2Material(**d) for d in wl.materials]
     m = pa.Model(wl.mesh, mats if len(mats) > 1 else mats[0], options=opts, material_id=wl.material_id)
