@@ -77,8 +77,11 @@ struct pfem_ctx {
     int64_t net_calls = 0;
     int pc_stale = 0;       // AMG rebuild requested while the network preconditions (1 full, 3 keep P)
     bool refresh_P = false;  // AMG policy 6: the last solve fell back, rebuild the prolongators
-    // AMG policy 7: stash slot of the current Newton iteration (-1: none)
+    // AMG policy 7: stash slot of the current Newton iteration (-1: none); with the
+    // learned preconditioner, whether its AMG phase restored (4) or kept (5) coarse
+    // levels and that phase's Krylov iterations
     int stash_slot = -1;
+    int stash_used = 0, amg_phase_iters = 0;
     pfem::dbuf<double> net_dinv;  // block-diagonal inverse of K for the learned preconditioner
     bool half_fine = false;       // FP32 V-cycle: fine-level products with the FP16 scaled K
     bool split_op = false;        // FP32-IR flexible CG: K p from the FP32 + BF16 split copy of K

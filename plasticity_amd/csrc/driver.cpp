@@ -749,6 +749,18 @@ static int learned_solve(pfem_ctx *c, double rtol, int itmax, int *total, int *w
         amg_refresh_fine(c, c->amg.get());
         c->st.t_setup += now() - t0;
         c->pc_stale = 0;
+    } else if ((c->pc_stale == 4 || c->pc_stale == 5) && c->pc_ready) {  // policy 7, as linear_solve_impl
+        double t0 = now();
+        const bool restored = c->pc_stale == 4 && amg_restore_coarse(c, c->amg.get(), c->stash_slot);
+        c->st.t_setup += now() - t0;
+        if (!restored) {
+            pc_setup(c, true);
+            t0 = now();
+            amg_stash_coarse(c, c->amg.get(), c->stash_slot);
+            c->st.t_setup += now() - t0;
+        }
+        c->stash_used = restored ? 4 : 5;
+        c->pc_stale = 0;
     } else if (c->pc_stale || !c->pc_ready) {
         pc_setup(c, c->pc_stale == 3);
         c->pc_stale = 0;
@@ -770,10 +782,12 @@ static int learned_solve(pfem_ctx *c, double rtol, int itmax, int *total, int *w
         rc = pcg<double>(c, b2, x2, rt, itmax, &it);
     }
     *total += it;
+    c->amg_phase_iters = it;
     if (rc != 0) {
         int it2 = 0;
         rc = bicgstab<double>(c, b2, x2, rt, 2 * itmax, &it2);
         *total += it2;
+        c->amg_phase_iters += it2;
     }
     axpby<double>(c->ndof, 1.0, x2, 1.0, c->dU.p, s);
     return rc;
@@ -785,7 +799,7 @@ static int learned_solve(pfem_ctx *c, double rtol, int itmax, int *total, int *w
 int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *used, double *relres,
                       bool *discarded) {
     const bool learned = c->net_mode == 2 || c->net_mode == 3;
-    if (learned && rebuild >= 4) rebuild = 3;  // policy 7's kept coarse levels: not with a network
+    c->stash_used = 0;
     if (learned) {
         if (c->opt.precision == PFEM_PREC_FP32_IR) prepare_fp32(c);
         // the AMG hierarchy is rebuilt only if it is needed (a full rebuild request wins
@@ -927,13 +941,16 @@ int linear_solve_impl(pfem_ctx *c, double rtol, int rebuild, int *niter, int *us
     if (niter) *niter = total;
     // policy 7: the solve right after a kept set-up sets the mark; one on the restored
     // levels that needs clearly more iterations makes the next first iteration rebuild
-    if (rebuild == 5) {
+    // (with the learned preconditioner: the AMG phase's kept levels and iterations)
+    const int stash_code = learned ? c->stash_used : rebuild;
+    const int stash_its = learned ? c->amg_phase_iters : total;
+    if (stash_code == 5) {
         AmgStash &st = amg_stash_slot(c->amg.get(), c->stash_slot);
-        st.iters = total;
+        st.iters = stash_its;
         st.stale = false;
-    } else if (rebuild == 4) {
+    } else if (stash_code == 4) {
         AmgStash &st = amg_stash_slot(c->amg.get(), c->stash_slot);
-        if (total > st.iters + std::max(1, st.iters / 16)) st.stale = true;
+        if (stash_its > st.iters + std::max(1, st.iters / 16)) st.stale = true;
     }
     if (used) *used = which;
     c->st.t_solve += now() - t1;
