@@ -332,6 +332,7 @@ def main():
                     help="a cut increment doubles again only after two converged increments that each took at "
                          "most this many Newton iterations")
     ap.add_argument("--verbose", type=int, default=0)
+    ap.add_argument("--progress", action="store_true", help="one stderr line per load increment")
     ap.add_argument("--show-residuals", action="store_true",
                     help="print each increment's Newton residual history in the per-load-step lines")
     ap.add_argument("--vcycle-eager", type=int, default=0, choices=[0, 1],
@@ -441,7 +442,7 @@ def main():
     def load_step(k):
         # newton! + commit, with the failed increments bisected (Model.advance)
         return m.advance(k / wl.nsteps, wl.nsteps, tol=wl.newton_tol, maxiter=args.maxiter, cutback=args.cutback,
-                         grow_iters=args.grow_iters)
+                         grow_iters=args.grow_iters, verbose=args.progress)
 
     def show(k, recs, t):
         inc = ", ".join(f"{r['lam'] * wl.nsteps:.4g}:{r['iters']}{'' if r['converged'] else 'x'}" for r in recs)

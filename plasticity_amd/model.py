@@ -16,6 +16,7 @@ no CPU path.
 from __future__ import annotations
 
 import ctypes as C
+import sys
 import warnings
 from dataclasses import dataclass, field
 
@@ -419,7 +420,8 @@ class Model:
             out.append(rec)
             if verbose:
                 print(f"  lambda {lam:.6f} (+{lam - lam_c:.3e}): {'ok' if rec['converged'] else 'failed'} "
-                      f"newton={rec['iters']} krylov={rec['krylov']}" + (" inverted" if rec["inverted"] else ""))
+                      f"newton={rec['iters']} krylov={rec['krylov']}" + (" inverted" if rec["inverted"] else ""),
+                      file=sys.stderr, flush=True)
             if rec["converged"]:
                 self.commit()
                 self._lam_c = lam
