@@ -812,6 +812,14 @@ __device__ __forceinline__ void contract_blocks(const double *__restrict__ pe, c
 // lane per block did (LDS bandwidth was the limit).  Each Gauss point's packets and
 // reference gradients (four elements) are staged in a double buffer, the next point's
 // 16-byte loads in flight during the contraction.
+// Gauss-point packets and reference gradients are read once per assembly: non-temporal
+// loads keep them from evicting the K blocks the colour passes revisit.
+typedef double d2v_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 stream_load2(const double2 *p) {
+    const d2v_t v = __builtin_nontemporal_load(reinterpret_cast<const d2v_t *>(p));
+    return double2{v.x, v.y};
+}
+
 template <bool SYM>
 __global__ __launch_bounds__(64) void scatter_kernel(const double *__restrict__ packet,
                                                       const double *__restrict__ geo,
@@ -851,7 +859,7 @@ __global__ __launch_bounds__(64) void scatter_kernel(const double *__restrict__ 
     }
     double2 rb[NCH];
 #pragma unroll
-    for (int k = 0; k < NCH; ++k) rb[k] = cok[k] ? csrc[k][0] : double2{0.0, 0.0};
+    for (int k = 0; k < NCH; ++k) rb[k] = cok[k] ? stream_load2(csrc[k]) : double2{0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < NCH; ++k)
         if (lane + 64 * k < 4 * NC) reinterpret_cast<double2 *>(sp[0])[lane + 64 * k] = rb[k];
@@ -867,7 +875,7 @@ __global__ __launch_bounds__(64) void scatter_kernel(const double *__restrict__ 
         if (g < 7) {  // the next Gauss point, written to the other buffer at the end
 #pragma unroll
             for (int k = 0; k < NCH; ++k)
-                if (cok[k]) rb[k] = csrc[k][(g + 1) * cstride[k]];
+                if (cok[k]) rb[k] = stream_load2(csrc[k] + (g + 1) * cstride[k]);
         }
         const double *pk = sp[g & 1] + el * ST;
         double dNa[3], dNb[4][3];
