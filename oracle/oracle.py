@@ -530,7 +530,9 @@ class OracleModel:
         committed U; the committed history is untouched) and halved, at most `cutback`
         times below 1/nsteps; after `grow_after` converged increments in a row that each took
         at most `grow_iters` Newton iterations it doubles again, never above 1/nsteps, and an
-        increment that would pass the load step's end n/nsteps is cut to end there.  `records` (a list) receives one
+        increment that would pass the load step's end n/nsteps is cut to end there.  Every
+        failure doubles the successes needed before the next doubling (at most
+        8 grow_after); a converged increment of the full nominal size resets them.  `records` (a list) receives one
         (lam, converged, newton iterations) tuple per attempt."""
         self.U[:] = 0.0
         self.U_c = self.U.copy()
@@ -541,7 +543,7 @@ class OracleModel:
             Fext0[d] += v
         iters, res, cgs, allconv = [], [], [], True
         nominal = 1.0 / nsteps
-        dl, streak, lam_c = nominal, 0, 0.0
+        dl, streak, lam_c, need = nominal, 0, 0.0, grow_after
         dl_min = nominal / 2.0 ** cutback
         for n in (range(1, nsteps + 1) if steps is None else steps):
             target = n / nsteps
@@ -563,9 +565,11 @@ class OracleModel:
                     res.append(hist)
                     cgs.append(cg)
                     self.commit()
+                    if lam - lam_c >= nominal * (1.0 - 1e-9):
+                        need = grow_after
                     lam_c = lam
                     streak = streak + 1 if len(hist) <= grow_iters else 0
-                    if streak >= grow_after and dl < nominal:
+                    if streak >= need and dl < nominal:
                         dl, streak = min(2.0 * dl, nominal), 0
                     if lam == target:
                         break
@@ -580,7 +584,7 @@ class OracleModel:
                     ok = False
                     break
                 self.U = self.U_c.copy()
-                dl, streak = half, 0
+                dl, streak, need = half, 0, min(2 * need, 8 * grow_after)
             if not ok:
                 allconv = False
                 break

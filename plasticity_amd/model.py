@@ -155,7 +155,8 @@ class Model:
         self._h = h
         self.dir_dofs, self.dir_vals, self.dir_ramp = [], [], []
         self.neu_dofs, self.neu_vals = [], []
-        self._lam_c, self._dl, self._streak = 0.0, None, 0  # committed load factor, increment (advance)
+        # committed load factor, increment, fast successes in a row, successes needed to grow
+        self._lam_c, self._dl, self._streak, self._need = 0.0, None, 0, None
 
     def close(self):
         if getattr(self, "_h", None):
@@ -215,7 +216,7 @@ class Model:
     # ---------------------------------------------------------------- state
     def reset(self):
         L.check(L.lib().pfem_reset(self._h), "reset")
-        self._lam_c, self._dl, self._streak = 0.0, None, 0
+        self._lam_c, self._dl, self._streak, self._need = 0.0, None, 0, None
         return self
 
     @property
@@ -390,13 +391,18 @@ class Model:
         (pfem_rollback) and halved -- at most `cutback` halvings below the nominal increment
         1/nsteps.  After `grow_after` converged increments in a row that each needed at most
         `grow_iters` Newton iterations the increment doubles again, never above 1/nsteps; an
-        increment that would pass `target` is cut to end on it.
+        increment that would pass `target` is cut to end on it.  Every failure doubles the
+        number of such increments needed before the next doubling (at most 8 grow_after),
+        and a converged increment of the full nominal size brings it back to grow_after, so
+        an increment size that keeps failing is not retried at every load step.
         Returns the list of attempted increments, each a dict (lam, converged, inverted,
         newton iterations, residuals, krylov, times); raises RuntimeError when the increment
         would have to fall below its limit (or at the first failure when cutback = 0)."""
         nominal = 1.0 / nsteps
         if self._dl is None:
             self._dl = nominal
+        if self._need is None:
+            self._need = grow_after
         dl_min = nominal / 2.0 ** cutback
         out = []
         while True:
@@ -425,8 +431,10 @@ class Model:
             if rec["converged"]:
                 self.commit()
                 self._lam_c = lam
+                if lam - lam_c >= nominal * (1.0 - 1e-9):
+                    self._need = grow_after
                 self._streak = self._streak + 1 if rec["iters"] <= grow_iters else 0
-                if self._streak >= grow_after and self._dl < nominal:
+                if self._streak >= self._need and self._dl < nominal:
                     self._dl = min(2.0 * self._dl, nominal)
                     self._streak = 0
                 if lam == target:
@@ -443,6 +451,7 @@ class Model:
             self.rollback()
             self._dl = half
             self._streak = 0
+            self._need = min(2 * self._need, 8 * grow_after)
 
     def solve(self, nsteps=10, tol=1e-8, maxiter=25, verbose=False, callback=None, steps=None, cutback=0,
               grow_after=2, grow_iters=6):

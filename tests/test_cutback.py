@@ -4,8 +4,10 @@ same increments for the same failures.  Newton itself is replaced by a stub that
 whenever the increment exceeds a limit, so the attempted load factors can be written down
 by hand:
 
-  nsteps = 4 (nominal increment 0.25), limit 0.1, grow after 2 successes:
-  0.25 x  0.125 x  0.0625 ok  0.125 ok (grow: 0.125)  0.25 x ... """
+  nsteps = 4 (nominal increment 0.25), limit 0.1, grow after 2 successes, each failure
+  doubling the successes needed (at most 16), a full nominal increment resetting them:
+  0.25 x  0.125 x  0.0625 ok x8 (grow: 0.125)  0.875 x ...
+The stub's own sequence is checked against the hand-worked one below."""
 import numpy as np
 import pytest
 
@@ -17,7 +19,7 @@ def expected(nsteps, limit, cutback, grow_after, nload, iters=2, grow_iters=6):
     """Hand-rolled restatement of the rule, independent of both implementations (every
     converged increment takes `iters` Newton iterations)."""
     nominal = 1.0 / nsteps
-    lam_c, dl, streak, out = 0.0, nominal, 0, []
+    lam_c, dl, streak, need, out = 0.0, nominal, 0, grow_after, []
     for n in range(1, nload + 1):
         target = n / nsteps
         while lam_c < target:
@@ -27,13 +29,15 @@ def expected(nsteps, limit, cutback, grow_after, nload, iters=2, grow_iters=6):
             ok = lam - lam_c <= limit + 1e-15
             out.append((round(lam, 12), ok))
             if ok:
+                if lam - lam_c >= nominal * (1 - 1e-9):  # a full nominal increment: back to grow_after
+                    need = grow_after
                 lam_c, streak = lam, (streak + 1 if iters <= grow_iters else 0)
-                if streak >= grow_after and dl < nominal:
+                if streak >= need and dl < nominal:
                     dl, streak = min(2 * dl, nominal), 0
             else:
                 if (lam - lam_c) / 2 < nominal / 2 ** cutback * (1 - 1e-9):
                     return out, False
-                dl, streak = (lam - lam_c) / 2, 0
+                dl, streak, need = (lam - lam_c) / 2, 0, min(2 * need, 8 * grow_after)  # each failure doubles it
     return out, True
 
 
@@ -42,11 +46,11 @@ class _StubModel(PM.Model):
 
     def __init__(self, limit, iters=2):
         self.limit, self.lam_c_dev, self.calls, self.iters = limit, 0.0, [], iters
-        self._lam_c, self._dl, self._streak = 0.0, None, 0
+        self._lam_c, self._dl, self._streak, self._need = 0.0, None, 0, None
         self.dir_dofs, self.neu_dofs = [], []
 
     def reset(self):
-        self._lam_c, self._dl, self._streak = 0.0, None, 0
+        self._lam_c, self._dl, self._streak, self._need = 0.0, None, 0, None
         self.lam_c_dev = 0.0
         return self
 
@@ -87,12 +91,21 @@ def test_host_driver_follows_the_rule(nsteps, limit, cutback, nload):
 def test_slow_increments_do_not_grow(iters):
     """Converged increments that needed more than grow_iters (6) Newton iterations never
     grow the increment again: no failed attempt after the first cut."""
-    exp, ok = expected(4, 0.1, 4, 2, 2, iters=iters)
+    exp, ok = expected(4, 0.1, 4, 2, 3, iters=iters)
     m = _StubModel(0.1, iters=iters)
-    m.solve(nsteps=4, steps=range(1, 3), cutback=4)
+    m.solve(nsteps=4, steps=range(1, 4), cutback=4)
     assert m.calls == exp
     fails = sum(1 for _, c in exp if not c)
     assert (fails == 2) if iters == 7 else (fails > 2)  # 0.25 x, 0.125 x, then 0.0625 for good
+
+
+def test_hand_worked_sequence():
+    """nsteps 4, limit 0.1: 0.25 x, 0.125 x (8 successes now needed), eight increments of
+    0.0625 up to 0.5, then 0.625 x (0.125 > 0.1; 16 needed), 0.5625 ok ..."""
+    exp, _ = expected(4, 0.1, 4, 2, 3)
+    lams = [round(0.0625 * k, 12) for k in range(1, 9)]
+    assert exp[:11] == [(0.25, False), (0.125, False)] + [(l, True) for l in lams] + [(0.625, False)]
+    assert exp[11] == (0.5625, True)
 
 
 def test_without_cutback_the_first_failure_ends_the_solve():
