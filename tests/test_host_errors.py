@@ -83,6 +83,8 @@ def test_entry_points_reject_bad_arguments_before_touching_a_device():
     d4 = [C.c_double() for _ in range(4)]
     assert lib.pfem_bench_vcycle(None, 0, 5, *[C.byref(v) for v in d4]) == BAD
     assert lib.pfem_bench_spmv(None, 0, 5, C.byref(d4[0])) == BAD
+    xp = nodes.ctypes.data_as(L._dp)
+    assert lib.pfem_apply_K(None, 0, xp, xp) == BAD
     assert lib.pfem_net_set_loss(None, 1) == BAD and lib.pfem_net_set_conv_kernel(None, 1) == BAD
     # stand-alone entry points
     assert lib.pfem_box_mesh(1.0, 1.0, 1.0, 0, 2, 2, 0.1, 0, L.ptr(nodes), L.ptr(conn)) == BAD
