@@ -662,6 +662,12 @@ void cheb_update(pfem_ctx *c, AmgLevel &L, int l, T *&d, T *&t, T *r, T *x, doub
         if (!last) std::swap(d, t);  // the new direction is in t; the old one is free
         return;
     }
+    if (l > 0 && L.bs == 6 && !c->opt.smoother_unfused) {  // coarse level: the same fusion
+        bsr_cheb<6, T, T>(L.nb, L.rowptr.p, L.colind.p, Vec<T>::A(L), d, Vec<T>::Dinv(L), r, last ? nullptr : t, x,
+                          c1, c2, c->stream);
+        if (!last) std::swap(d, t);
+        return;
+    }
     level_matvec<T>(c, L, l, d, t);
     if (L.bs == 3) cheb_step<3, T, T>(L.nb, Vec<T>::Dinv(L), t, r, d, x, c1, c2, c->stream);
     else cheb_step<6, T, T>(L.nb, Vec<T>::Dinv(L), t, r, d, x, c1, c2, c->stream);
@@ -806,7 +812,8 @@ double op_bytes(pfem_ctx *c, const AmgLevel &L, int l, double s) {
 double smooth_bytes(pfem_ctx *c, Amg *a, int l, double s, bool zero_init) {
     const AmgLevel &L = a->L[l];
     const double A = op_bytes(c, L, l, s), v = (double)L.n * s, D = (double)L.nb * L.bs * L.bs * s;
-    const bool fused = l == 0 && !c->opt.smoother_unfused && !(s == 4.0 && c->half_fine);
+    const bool fused = l == 0 ? !c->opt.smoother_unfused && !(s == 4.0 && c->half_fine)
+                              : L.bs == 6 && !c->opt.smoother_unfused;
     const int deg = a->degree_at(l);
     if (a->smoother == PFEM_SMOOTH_JACOBI) {
         double b = 0;

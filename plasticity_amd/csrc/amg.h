@@ -174,6 +174,11 @@ void diag_first(int64_t nb, const DT *dinv, const XT *v, XT *r, XT *d, XT *x, do
 template <int BS, class DT, class XT>
 void cheb_step(int64_t nb, const DT *dinv, const XT *Ad, XT *r, XT *d, XT *x, double c1, double c2,
                hipStream_t s);  // r -= Ad; d = c1 d + c2 Dinv r; x += d
+// coarse-level Chebyshev step with its product: t = A d; r -= t; d' = c1 d + c2 Dinv r;
+// x += d' (d' into d_out; d_out == null: the last step, r and d' not stored)
+template <int BS, class DT, class XT>
+void bsr_cheb(int64_t nb, const int32_t *rowptr, const int32_t *colind, const DT *vals, const XT *d, const DT *dinv,
+              XT *r, XT *d_out, XT *x, double c1, double c2, hipStream_t s);
 template <class XT>
 void dense_gemv(int64_t n, const double *Ainv, const XT *b, XT *x, hipStream_t s);
 void scale_by_rnorm(int64_t n, const double *nrm2, double *x, hipStream_t s);

@@ -448,6 +448,47 @@ __global__ void bsr_residual_kernel(int64_t nb, const int32_t *__restrict__ rowp
     r[o] = (XT)((VT)b[o] - s);
 }
 
+// One Chebyshev step of a coarse-level smoother with its product in the same pass: the
+// thread of row rr of block row i forms (A d)_i,rr (as bsr_spmv_kernel), then the block
+// row's BS threads exchange r - A d through LDS and each forms its component of
+// d' = c1 d + c2 Dinv (r - A d) and x += d' -- cheb_step_kernel's arithmetic in the same
+// order, so results are bitwise equal.  d is read by other block rows' products, so d'
+// goes to d_out; d_out == null (the last step of a smoothing) skips the r and d' stores.
+// Workgroups of 42 block rows (252 threads) keep every block row's threads together.
+constexpr int CHEB_ROWS = 42;
+template <int BS, class DT, class XT>
+__global__ __launch_bounds__(CHEB_ROWS * BS) void bsr_cheb_kernel(int64_t nb, const int32_t *__restrict__ rowptr,
+                                                                  const int32_t *__restrict__ colind,
+                                                                  const DT *__restrict__ vals,
+                                                                  const XT *__restrict__ d,
+                                                                  const DT *__restrict__ dinv, XT *__restrict__ r,
+                                                                  XT *__restrict__ d_out, XT *__restrict__ x,
+                                                                  double c1, double c2) {
+    __shared__ DT srv[CHEB_ROWS * BS];
+    const int64_t t = (int64_t)blockIdx.x * (CHEB_ROWS * BS) + threadIdx.x;
+    const int64_t i = t / BS;
+    const int rr = (int)(t % BS);
+    const bool act = i < nb;
+    const int64_t o = (int64_t)BS * i + rr;
+    DT rv = 0;
+    if (act) {
+        const DT s = block_row_dot<BS, BS, DT, XT>(rowptr[i], rowptr[i + 1], rr, colind, vals, d);
+        rv = (DT)r[o] - (DT)(XT)(double)s;  // bsr_spmv_kernel stores (XT)(double)s
+        if (d_out) r[o] = (XT)rv;
+    }
+    srv[threadIdx.x] = rv;
+    __syncthreads();
+    if (!act) return;
+    const DT *m = dinv + (int64_t)BS * BS * i;
+    const DT *rw = srv + (threadIdx.x - rr);
+    DT s2 = 0;
+#pragma unroll
+    for (int c = 0; c < BS; ++c) s2 = fma(m[BS * rr + c], rw[c], s2);
+    const double dn = fma(c1, (double)d[o], c2 * (double)s2);
+    if (d_out) d_out[o] = (XT)dn;
+    x[o] = (XT)((double)x[o] + dn);
+}
+
 template <int BS, class DT, class XT>
 __global__ void block_diag_apply_kernel(int64_t nb, const DT *__restrict__ dinv, const XT *__restrict__ r,
                                         XT *__restrict__ z, double scale) {
@@ -918,6 +959,14 @@ void block_diag_apply(int64_t nb, const DT *dinv, const XT *r, XT *z, double sca
     PF_HIP(hipGetLastError());
 }
 template <int BS, class DT, class XT>
+void bsr_cheb(int64_t nb, const int32_t *rowptr, const int32_t *colind, const DT *vals, const XT *d, const DT *dinv,
+              XT *r, XT *d_out, XT *x, double c1, double c2, hipStream_t s) {
+    const int64_t threads = nb * BS;
+    hipLaunchKernelGGL((bsr_cheb_kernel<BS, DT, XT>), dim3((unsigned)((threads + CHEB_ROWS * BS - 1) / (CHEB_ROWS * BS))),
+                       dim3(CHEB_ROWS * BS), 0, s, nb, rowptr, colind, vals, d, dinv, r, d_out, x, c1, c2);
+    PF_HIP(hipGetLastError());
+}
+template <int BS, class DT, class XT>
 void cheb_step(int64_t nb, const DT *dinv, const XT *Ad, XT *r, XT *d, XT *x, double c1, double c2, hipStream_t s) {
     hipLaunchKernelGGL((cheb_step_kernel<BS, DT, XT>), dim3(grid_for(nb, 256)), dim3(256), 0, s, nb, dinv, Ad, r, d,
                        x, c1, c2);
@@ -1001,7 +1050,9 @@ template void amg_transpose_P<6>(int64_t, const int32_t *, const double *, doubl
     template void cheb_step<3, VT, XT>(int64_t, const VT *, const XT *, XT *, XT *, XT *, double, double,          \
                                        hipStream_t);                                                               \
     template void cheb_step<6, VT, XT>(int64_t, const VT *, const XT *, XT *, XT *, XT *, double, double,          \
-                                       hipStream_t);
+                                       hipStream_t);                                                               \
+    template void bsr_cheb<6, VT, XT>(int64_t, const int32_t *, const int32_t *, const VT *, const XT *,          \
+                                      const VT *, XT *, XT *, XT *, double, double, hipStream_t);
 PF_OPS(double, double)
 PF_OPS(float, float)
 #undef PF_OPS
