@@ -137,8 +137,9 @@ typedef struct pfem_options {
                                    chunk k on a second stream, and the packets live in two chunk-sized
                                    buffers instead of one per Gauss point of the mesh.  0: the Gauss-point
                                    kernel over the whole mesh, then the scatter colour by colour. */
-    int32_t smoother_unfused;   /* 1 = the fine-level Chebyshev steps as a product launch followed by a vector
-                                   launch; 0 = one launch that does both (default; same results bit for bit) */
+    int32_t smoother_unfused;   /* 1 = the Chebyshev steps (fine and coarse levels) as a product launch followed
+                                   by a vector launch; 0 = one launch that does both (default; same results bit
+                                   for bit) */
     int32_t coarse_degree;      /* Chebyshev degree / Jacobi sweeps on the levels below the finest (<= 0: the
                                    fine level's) */
     int32_t krylov_sync;        /* 1 = PCG / flexible CG read each iteration's scalars on the host before they

@@ -1,6 +1,7 @@
-"""Fine-level Chebyshev steps fused with their product (sell_cheb_kernel) against the
-separate product + step launches (option smoother_unfused = 1): the same arithmetic in
-the same order, so the preconditioned solves agree bit for bit -- for both precisions,
+"""Chebyshev steps fused with their product (sell_cheb_kernel on the fine level,
+bsr_cheb_kernel on the 6x6 coarse levels) against the separate product + step launches
+(option smoother_unfused = 1): the same arithmetic in the same order, so the
+preconditioned solves agree bit for bit -- for both precisions,
 the first- and fourth-kind Chebyshev smoothers, and degree 2 (one step per smoothing:
 the fused launch skips the dead r and d stores) as well as degrees 3 and 4 (the new
 direction goes to the spare buffer and the two are swapped)."""
